@@ -1,0 +1,308 @@
+/*
+ * merian_amd.h — C ABI of the MI355X (gfx950) path-tracing library behind the
+ * "Render (Path-traced)" node.
+ *
+ * Conventions
+ *   - Every entry point returns mrt_status (0 = ok, < 0 = error). No C++ exception
+ *     crosses this boundary; the reference raises node_error / compilation_failed
+ *     inside the graph (src/merian-graph/graph/graph.cpp:385-395) and the node host
+ *     maps a negative status onto that.
+ *   - Host buffers are owned by the caller; device memory behind the opaque handles is
+ *     owned by the library. Entry points suffixed _dev take device pointers that the
+ *     caller owns and an optional hipStream_t (NULL = the context's stream).
+ *   - One host thread per context (the reference graph is single-threaded,
+ *     src/merian-graph/graph/graph.cpp:68-253). Work is asynchronous on the context's
+ *     stream; host-buffer entry points synchronise before returning.
+ *   - mrt_last_error(ctx) describes the most recent failure on that context.
+ *
+ * Layouts mirror the reference's packed device structs where a layout is fixed there
+ * (PackedVertexData 24 B, PackedHitInfo 16 B); everything else is this library's own.
+ */
+#ifndef MERIAN_AMD_H
+#define MERIAN_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__)
+#define MRT_API __attribute__((visibility("default")))
+#else
+#define MRT_API
+#endif
+
+typedef int32_t mrt_status;
+enum {
+    MRT_OK = 0,
+    MRT_ERR_INVALID = -1,     /* bad argument (null, size, unknown enum) */
+    MRT_ERR_DEVICE = -2,      /* HIP runtime error */
+    MRT_ERR_BUDGET = -3,      /* HBM budget of the context would be exceeded */
+    MRT_ERR_UNSUPPORTED = -4, /* a feature the reference has but this build does not */
+    MRT_ERR_NOT_READY = -5,   /* job not finished (mrt_job_poll) */
+    MRT_ERR_CANCELLED = -6,   /* job was cancelled */
+    MRT_ERR_ALLOC = -7        /* device allocation failed */
+};
+
+typedef struct mrt_ctx mrt_ctx;
+typedef struct mrt_scene mrt_scene;
+
+/* ------------------------------------------------------------------------------------
+ * Scene description (host side, copied by mrt_scene_create)
+ * ---------------------------------------------------------------------------------- */
+
+/* = PackedVertexData (include/merian-shaders/scene/scene-data.slangh:75-101): 24 bytes. */
+typedef struct mrt_vertex {
+    float position[3];
+    uint32_t encoded_normal;  /* packSnorm2x16(octahedral_project(n)) */
+    uint16_t uv[2];           /* IEEE half bits */
+    uint32_t encoded_tangent; /* LSB = handedness, upper bits = oct direction */
+} mrt_vertex;
+
+/* GeometryDataFlags (scene-data.slangh:13-41). */
+enum {
+    MRT_GEOM_PRETRANSFORMED = 0x1,
+    MRT_GEOM_FLIP_FACING = 0x8,
+    MRT_GEOM_INDEX_UINT32 = 0x40, /* else sequential indices 3i, 3i+1, 3i+2 */
+    MRT_GEOM_FLAT_SHADING = 0x80,
+    MRT_GEOM_HAS_TANGENTS = 0x100,
+    MRT_GEOM_USE_ENV_MAP = 0x200,
+    MRT_GEOM_TWO_SIDED = 0x400
+};
+
+/* GeometryData (scene-data.slangh:119-129) with buffer addresses replaced by offsets
+ * into the scene's shared vertex / index arrays (the reference suballocates shared
+ * buffers the same way, src/merian-shaders/scene/scene.cpp:1942-1972). All meshes are
+ * world-space (pretransformed); instance transforms are baked at scene build. */
+typedef struct mrt_geometry {
+    uint32_t vertex_offset;   /* first vertex in mrt_scene_desc.vertices */
+    uint32_t index_offset;    /* first uint32 in mrt_scene_desc.indices (3 per triangle) */
+    uint32_t primitive_count; /* triangles */
+    uint16_t material_id;
+    uint16_t flags;           /* MRT_GEOM_* */
+    uint32_t instance_mask;   /* 8-bit TLAS instance mask (scene.cpp:1825) */
+} mrt_geometry;
+
+/* Material models (material_system.cpp:124-195 registers type ids; here fixed). */
+enum { MRT_MAT_DIFFUSE = 0, MRT_MAT_OPENPBR = 1 };
+/* OpenPBRMaterial<TRANSMISSION, CLEARCOAT, SHEEN, VOLUME> feature bits
+ * (include/merian-shaders/shading/materials/openpbr-material.slang:51-76). */
+enum {
+    MRT_PBR_TRANSMISSION = 0x1,
+    MRT_PBR_CLEARCOAT = 0x2,
+    MRT_PBR_SHEEN = 0x4,
+    MRT_PBR_VOLUME = 0x8
+};
+
+/* One material entry, 128 bytes. DiffuseMaterial uses base_color only
+ * (diffuse-material.slang:7-43); OpenPBR uses every field its features enable. */
+typedef struct mrt_material {
+    uint32_t model;    /* MRT_MAT_* */
+    uint32_t features; /* MRT_PBR_* */
+    float base_color[3];
+    float opacity;
+    float metalness;
+    float specular_alpha[2]; /* GGX alpha per axis (tangent, bitangent) */
+    float specular_weight;
+    float specular_ior;
+    float emission[3];
+    float transmission_weight;
+    float transmission_color[3];
+    float coat_weight;
+    float coat_alpha;
+    float coat_ior;
+    float sheen_weight;
+    float sheen_color[3];
+    float sheen_alpha;
+    float absorption[3];
+    float _pad[3];
+} mrt_material;
+
+/* Environment maps (environment-map.slang:13-68). */
+enum { MRT_ENV_NONE = 0, MRT_ENV_LATLONG = 1, MRT_ENV_EQUAL_AREA_OCT = 2 };
+enum { MRT_WRAP_REPEAT = 0, MRT_WRAP_CLAMP = 1 };
+typedef struct mrt_env {
+    int32_t type;      /* MRT_ENV_* */
+    int32_t wrap;      /* MRT_WRAP_*: lat-long uses repeat (scene.cpp:785-795), pbrt's
+                          equal-area map clamp-to-edge (pbrt_scene.cpp:924-969) */
+    uint32_t width;
+    uint32_t height;
+    const float* rgba; /* width*height*4 floats, row-major, row 0 = v 0 */
+    float intensity;
+    float to_local[9]; /* row-major 3x3, applied to the world direction */
+} mrt_env;
+
+typedef struct mrt_scene_desc {
+    const mrt_vertex* vertices;
+    uint32_t vertex_count;
+    const uint32_t* indices;
+    uint32_t index_count;
+    const mrt_geometry* geometries;
+    uint32_t geometry_count;
+    const mrt_material* materials;
+    uint32_t material_count;
+    mrt_env env;
+} mrt_scene_desc;
+
+/* Camera as the shader sees it (camera.slang:6-93; written by Camera::write_to,
+ * src/merian/utils/camera/camera.cpp:349-362). */
+typedef struct mrt_camera {
+    float position[3];
+    float U[3];
+    float V[3];
+    float W[3];
+    float near_plane;
+    float far_plane;
+    float jitter[2];
+} mrt_camera;
+
+/* Jitter sequences (camera.cpp:318-339). */
+enum { MRT_JITTER_NONE = 0, MRT_JITTER_R2 = 1 };
+
+/* ------------------------------------------------------------------------------------
+ * Rays and hits
+ * ---------------------------------------------------------------------------------- */
+
+typedef struct mrt_ray {
+    float origin[3];
+    float tmin;
+    float direction[3];
+    float tmax;
+} mrt_ray;
+
+/* Closest hit: triangle id (global, in input order; 0xFFFFFFFF = miss), t, barycentrics
+ * of v1 and v2 (the ray-query convention, acceleration-structure.slang:30-35). */
+typedef struct mrt_hit {
+    uint32_t triangle;
+    float t;
+    float u;
+    float v;
+} mrt_hit;
+
+/* = PackedHitInfo (include/merian-shaders/gbuffer.slang:40-61): the "gbuffer" hit
+ * texel, 16 bytes. Miss: instance_id = primitive_id = 0xFFFFFFFF. */
+typedef struct mrt_packed_hit {
+    uint32_t instance_id;
+    uint32_t primitive_id;
+    uint16_t geometry_index;
+    uint16_t instance_index;
+    uint16_t barycentrics[2]; /* IEEE half bits, round-to-nearest-even */
+} mrt_packed_hit;
+
+/* ------------------------------------------------------------------------------------
+ * Render parameters: the "Render (Path-traced)" properties
+ * (src/merian-graph/nodes/render_pt/render_pt.cpp:147-187) plus build-side modes.
+ * ---------------------------------------------------------------------------------- */
+enum { MRT_MODE_BSDF = 0, MRT_MODE_NEE = 1, MRT_MODE_GUIDED = 2 };
+
+typedef struct mrt_pt_params {
+    int32_t spp;                 /* "samples per pixel" */
+    int32_t max_path_length;     /* "max path length" (segments incl. the primary hit) */
+    int32_t emission_on_primary; /* "emission on primary" */
+    int32_t demodulate_albedo;   /* "demodulate albedo" (must be 0: unsupported) */
+    uint32_t instance_mask;      /* bits "0".."7" */
+    int32_t mode;                /* MRT_MODE_*: BSDF = the reference estimator */
+    int32_t record_signature;    /* write a per-pixel hash of the hit ids (parity aid) */
+    int32_t _reserved;
+} mrt_pt_params;
+
+/* Per-frame statistics (counters kept by the device queues). */
+typedef struct mrt_pt_stats {
+    uint64_t primary_segments;      /* closest-hit queries of the gbuffer pass */
+    uint64_t continuation_segments; /* closest-hit queries of the bounce loop */
+    uint64_t shadow_rays;           /* NEE visibility queries (MRT_MODE_NEE/GUIDED) */
+    float ms_gbuffer;               /* device time of the gbuffer pass */
+    float ms_trace;                 /* device time spent in the intersect kernel */
+    float ms_shade;                 /* device time in shading / sampling kernels */
+    float ms_total;                 /* device time of the whole frame */
+} mrt_pt_stats;
+
+/* ------------------------------------------------------------------------------------
+ * Context, scene
+ * ---------------------------------------------------------------------------------- */
+
+/* Library version / build identity ("gfx950"). */
+MRT_API const char* mrt_version(void);
+
+/* Create a context on HIP device `device`. hbm_budget bytes (0 = 90% of free memory)
+ * bound every allocation the context makes, as the serving loop's admission limit. */
+MRT_API mrt_status mrt_create(int32_t device, uint64_t hbm_budget, mrt_ctx** out);
+MRT_API mrt_status mrt_destroy(mrt_ctx* ctx);
+MRT_API const char* mrt_last_error(mrt_ctx* ctx);
+/* Bytes of device memory the context currently holds. */
+MRT_API uint64_t mrt_memory_in_use(mrt_ctx* ctx);
+/* Waits for all work on the context's stream. */
+MRT_API mrt_status mrt_synchronize(mrt_ctx* ctx);
+
+/* Builds the BVH (binned SAH, 4-wide) and uploads geometry, materials and the
+ * environment map; replaces the driver BLAS/TLAS build (scene.cpp:1672-1892). */
+MRT_API mrt_status mrt_scene_create(mrt_ctx* ctx, const mrt_scene_desc* desc, mrt_scene** out);
+MRT_API mrt_status mrt_scene_destroy(mrt_ctx* ctx, mrt_scene* scene);
+/* Scene statistics: triangles, BVH nodes, device bytes. */
+MRT_API mrt_status mrt_scene_info(mrt_scene* scene, uint64_t* triangles, uint64_t* nodes,
+                                  uint64_t* device_bytes);
+/* Geometry-only scene for ray queries: vertex positions float[3*n_vertices] and
+ * uint32 triangle indices; two_sided != 0 disables back-face culling. */
+MRT_API mrt_status mrt_scene_create_triangles(mrt_ctx* ctx, const float* positions,
+                                              uint32_t vertex_count, const uint32_t* indices,
+                                              uint32_t triangle_count, int32_t two_sided,
+                                              mrt_scene** out);
+
+/* Camera from look-at parameters, as Camera(...) + write_to + advance_jitter
+ * (camera.cpp:10-40, 263-266, 318-362). vfov in radians. */
+MRT_API mrt_status mrt_camera_look_at(const float eye[3], const float target[3],
+                                      const float up[3], float vfov, float aspect,
+                                      float near_plane, float far_plane, int32_t jitter,
+                                      uint32_t frame, mrt_camera* out);
+
+/* ------------------------------------------------------------------------------------
+ * Ray queries: Scene.trace_ray (include/merian-shaders/scene/scene.slang:62-76) with
+ * back-face culling and ties at equal t broken by the lowest triangle id.
+ * ---------------------------------------------------------------------------------- */
+MRT_API mrt_status mrt_trace_closest(mrt_ctx* ctx, mrt_scene* scene, const mrt_ray* rays,
+                                     uint64_t n, mrt_hit* hits);
+MRT_API mrt_status mrt_trace_closest_dev(mrt_ctx* ctx, mrt_scene* scene, const mrt_ray* d_rays,
+                                         uint64_t n, mrt_hit* d_hits, void* stream);
+/* Pinhole rays of a w x h image on the device (camera.slang:45-56), for coherent
+ * ray-query workloads. */
+MRT_API mrt_status mrt_camera_rays_dev(mrt_ctx* ctx, const mrt_camera* cam, uint32_t width,
+                                       uint32_t height, mrt_ray* d_rays, void* stream);
+
+/* ------------------------------------------------------------------------------------
+ * GBuffer (Raytraced) + Render (Path-traced)
+ * ---------------------------------------------------------------------------------- */
+
+/* Primary rays + closest hit into PackedHitInfo (gbuffer_rt.slang:18-94). */
+MRT_API mrt_status mrt_gbuffer_dev(mrt_ctx* ctx, mrt_scene* scene, const mrt_camera* cam,
+                                   uint32_t width, uint32_t height, uint32_t instance_mask,
+                                   mrt_packed_hit* d_gbuffer, void* stream);
+
+/* One frame of pt (render_pt.slang:92-130): reads the gbuffer, writes irradiance
+ * float4(mean rgb, mean yuv-luminance^2) per pixel, row-major. d_signature (optional,
+ * uint32 per pixel) receives a hash of every path's hit sequence when
+ * params->record_signature is set. stats may be NULL. */
+MRT_API mrt_status mrt_pt_render_dev(mrt_ctx* ctx, mrt_scene* scene, const mrt_pt_params* params,
+                                     const mrt_camera* cam, uint32_t frame, uint32_t width,
+                                     uint32_t height, const mrt_packed_hit* d_gbuffer,
+                                     float* d_irradiance, uint32_t* d_signature,
+                                     mrt_pt_stats* stats, void* stream);
+
+/* Host-buffer convenience: gbuffer + pt for one frame, results copied to the host. */
+MRT_API mrt_status mrt_render_frame(mrt_ctx* ctx, mrt_scene* scene, const mrt_pt_params* params,
+                                    const mrt_camera* cam, uint32_t frame, uint32_t width,
+                                    uint32_t height, float* irradiance, uint32_t* signature,
+                                    mrt_packed_hit* gbuffer, mrt_pt_stats* stats);
+
+/* ------------------------------------------------------------------------------------
+ * Node plugin ABI (include/merian-graph/plugin/node_plugin.hpp:17-29). The registry
+ * argument is this library's node registry (merian_amd/host), not the Vulkan one.
+ * ---------------------------------------------------------------------------------- */
+MRT_API uint32_t merian_node_plugin_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MERIAN_AMD_H */

@@ -1,0 +1,173 @@
+"""ctypes mirror of include/merian_amd.h and the loader for the HIP library.
+
+The library is built in-tree (merian_amd/libmerian_amd.so) by ``merian_amd.build``;
+loading fails loudly when it is missing -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG_DIR = Path(__file__).resolve().parent
+LIB_PATH = PKG_DIR / "libmerian_amd.so"
+
+MRT_OK = 0
+STATUS_NAMES = {0: "ok", -1: "invalid", -2: "device", -3: "budget", -4: "unsupported",
+                -5: "not_ready", -6: "cancelled", -7: "alloc"}
+
+# GeometryDataFlags (reference scene-data.slangh:13-41)
+GEOM_PRETRANSFORMED = 0x1
+GEOM_FLIP_FACING = 0x8
+GEOM_INDEX_UINT32 = 0x40
+GEOM_FLAT_SHADING = 0x80
+GEOM_HAS_TANGENTS = 0x100
+GEOM_USE_ENV_MAP = 0x200
+GEOM_TWO_SIDED = 0x400
+
+MAT_DIFFUSE = 0
+MAT_OPENPBR = 1
+PBR_TRANSMISSION = 0x1
+PBR_CLEARCOAT = 0x2
+PBR_SHEEN = 0x4
+PBR_VOLUME = 0x8
+
+ENV_NONE = 0
+ENV_LATLONG = 1
+ENV_EQUAL_AREA_OCT = 2
+WRAP_REPEAT = 0
+WRAP_CLAMP = 1
+JITTER_NONE = 0
+JITTER_R2 = 1
+MODE_BSDF = 0
+MODE_NEE = 1
+MODE_GUIDED = 2
+
+# numpy layouts of the POD arrays handed across the ABI
+VERTEX_DTYPE = np.dtype([("position", "<f4", (3,)), ("encoded_normal", "<u4"),
+                         ("uv", "<u2", (2,)), ("encoded_tangent", "<u4")])
+assert VERTEX_DTYPE.itemsize == 24
+GEOMETRY_DTYPE = np.dtype([("vertex_offset", "<u4"), ("index_offset", "<u4"),
+                           ("primitive_count", "<u4"), ("material_id", "<u2"),
+                           ("flags", "<u2"), ("instance_mask", "<u4")])
+assert GEOMETRY_DTYPE.itemsize == 20
+MATERIAL_DTYPE = np.dtype([("model", "<u4"), ("features", "<u4"), ("base_color", "<f4", (3,)),
+                           ("opacity", "<f4"), ("metalness", "<f4"),
+                           ("specular_alpha", "<f4", (2,)), ("specular_weight", "<f4"),
+                           ("specular_ior", "<f4"), ("emission", "<f4", (3,)),
+                           ("transmission_weight", "<f4"), ("transmission_color", "<f4", (3,)),
+                           ("coat_weight", "<f4"), ("coat_alpha", "<f4"), ("coat_ior", "<f4"),
+                           ("sheen_weight", "<f4"), ("sheen_color", "<f4", (3,)),
+                           ("sheen_alpha", "<f4"), ("absorption", "<f4", (3,)),
+                           ("_pad", "<f4", (3,))])
+assert MATERIAL_DTYPE.itemsize == 128
+RAY_DTYPE = np.dtype([("origin", "<f4", (3,)), ("tmin", "<f4"),
+                      ("direction", "<f4", (3,)), ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("triangle", "<u4"), ("t", "<f4"), ("u", "<f4"), ("v", "<f4")])
+PACKED_HIT_DTYPE = np.dtype([("instance_id", "<u4"), ("primitive_id", "<u4"),
+                             ("geometry_index", "<u2"), ("instance_index", "<u2"),
+                             ("barycentrics", "<u2", (2,))])
+assert PACKED_HIT_DTYPE.itemsize == 16
+
+
+class Env(C.Structure):
+    _fields_ = [("type", C.c_int32), ("wrap", C.c_int32), ("width", C.c_uint32),
+                ("height", C.c_uint32), ("rgba", C.c_void_p), ("intensity", C.c_float),
+                ("to_local", C.c_float * 9)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("vertices", C.c_void_p), ("vertex_count", C.c_uint32),
+                ("indices", C.c_void_p), ("index_count", C.c_uint32),
+                ("geometries", C.c_void_p), ("geometry_count", C.c_uint32),
+                ("materials", C.c_void_p), ("material_count", C.c_uint32),
+                ("env", Env)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("U", C.c_float * 3), ("V", C.c_float * 3),
+                ("W", C.c_float * 3), ("near_plane", C.c_float), ("far_plane", C.c_float),
+                ("jitter", C.c_float * 2)]
+
+    def as_dict(self):
+        return {k: (list(getattr(self, k)) if k not in ("near_plane", "far_plane") else getattr(self, k))
+                for k, _ in self._fields_}
+
+
+class PtParams(C.Structure):
+    _fields_ = [("spp", C.c_int32), ("max_path_length", C.c_int32),
+                ("emission_on_primary", C.c_int32), ("demodulate_albedo", C.c_int32),
+                ("instance_mask", C.c_uint32), ("mode", C.c_int32),
+                ("record_signature", C.c_int32), ("_reserved", C.c_int32)]
+
+
+class PtStats(C.Structure):
+    _fields_ = [("primary_segments", C.c_uint64), ("continuation_segments", C.c_uint64),
+                ("shadow_rays", C.c_uint64), ("ms_gbuffer", C.c_float), ("ms_trace", C.c_float),
+                ("ms_shade", C.c_float), ("ms_total", C.c_float)]
+
+
+assert C.sizeof(SceneDesc) == 128 and C.sizeof(Camera) == 64 and C.sizeof(PtParams) == 32
+
+
+def ptr(a: np.ndarray) -> C.c_void_p:
+    """Address of a C-contiguous numpy array."""
+    assert a.flags["C_CONTIGUOUS"]
+    return C.c_void_p(a.ctypes.data)
+
+
+_LIB = None
+
+# (name, restype, argtypes) of every exported entry point of include/merian_amd.h
+EXPORTS = [
+    ("mrt_version", C.c_char_p, []),
+    ("mrt_create", C.c_int32, [C.c_int32, C.c_uint64, C.POINTER(C.c_void_p)]),
+    ("mrt_destroy", C.c_int32, [C.c_void_p]),
+    ("mrt_last_error", C.c_char_p, [C.c_void_p]),
+    ("mrt_memory_in_use", C.c_uint64, [C.c_void_p]),
+    ("mrt_synchronize", C.c_int32, [C.c_void_p]),
+    ("mrt_scene_create", C.c_int32, [C.c_void_p, C.POINTER(SceneDesc), C.POINTER(C.c_void_p)]),
+    ("mrt_scene_destroy", C.c_int32, [C.c_void_p, C.c_void_p]),
+    ("mrt_scene_info", C.c_int32, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                   C.POINTER(C.c_uint64)]),
+    ("mrt_scene_create_triangles", C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
+                                               C.c_uint32, C.c_int32, C.POINTER(C.c_void_p)]),
+    ("mrt_camera_look_at", C.c_int32, [C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                       C.POINTER(C.c_float), C.c_float, C.c_float, C.c_float,
+                                       C.c_float, C.c_int32, C.c_uint32, C.POINTER(Camera)]),
+    ("mrt_trace_closest", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
+    ("mrt_trace_closest_dev", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64,
+                                          C.c_void_p, C.c_void_p]),
+    ("mrt_camera_rays_dev", C.c_int32, [C.c_void_p, C.POINTER(Camera), C.c_uint32, C.c_uint32,
+                                        C.c_void_p, C.c_void_p]),
+    ("mrt_gbuffer_dev", C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(Camera), C.c_uint32,
+                                    C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
+    ("mrt_pt_render_dev", C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(PtParams),
+                                      C.POINTER(Camera), C.c_uint32, C.c_uint32, C.c_uint32,
+                                      C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(PtStats),
+                                      C.c_void_p]),
+    ("mrt_render_frame", C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(PtParams),
+                                     C.POINTER(Camera), C.c_uint32, C.c_uint32, C.c_uint32,
+                                     C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(PtStats)]),
+    ("merian_node_plugin_abi_version", C.c_uint32, []),
+]
+
+
+def load_library(path: os.PathLike | None = None) -> C.CDLL:
+    """Load libmerian_amd.so; raises if it has not been built (no fallback path)."""
+    global _LIB
+    if _LIB is not None and path is None:
+        return _LIB
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(f"{p} is missing: run __graft_entry__.build() (hipcc, gfx950) first")
+    lib = C.CDLL(str(p), mode=C.RTLD_GLOBAL)
+    for name, res, args in EXPORTS:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _LIB = lib
+    return lib

@@ -1,0 +1,335 @@
+// Binned-SAH BVH2 build, collapsed into the 128-byte BVH4 layout of bvh.h.
+//
+// The reference leaves BVH construction to the Vulkan driver (PreferFastTrace for static mesh
+// groups, src/merian-shaders/scene/scene.cpp:1116-1123). This builder is the project's own:
+// 32-bin SAH per axis over centroid bounds, leaves of at most MAX_LEAF triangles, top levels
+// built in parallel threads. Child boxes are rounded outward and padded so that the slab test
+// (one FMA per plane) can never reject a box containing a hit the triangle test reports.
+#include "bvh.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <thread>
+
+namespace mrt {
+namespace {
+
+struct AABB {
+    float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX};
+    float hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    void grow(const AABB& b) {
+        for (int k = 0; k < 3; k++) {
+            lo[k] = std::min(lo[k], b.lo[k]);
+            hi[k] = std::max(hi[k], b.hi[k]);
+        }
+    }
+    void grow(const float* p) {
+        for (int k = 0; k < 3; k++) {
+            lo[k] = std::min(lo[k], p[k]);
+            hi[k] = std::max(hi[k], p[k]);
+        }
+    }
+    float area() const {
+        const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+        if (dx < 0.f) return 0.f;
+        return 2.f * (dx * dy + dy * dz + dz * dx);
+    }
+};
+
+struct Prim {
+    AABB box;
+    float c[3];
+};
+
+struct Node2 {
+    AABB box;
+    int32_t left = -1, right = -1;
+    uint32_t first = 0, count = 0;
+};
+
+constexpr int BINS = 32;
+
+struct Builder {
+    const std::vector<Prim>& prims;
+    std::vector<uint32_t> idx;
+    std::vector<Node2> nodes;
+    std::atomic<uint32_t> next{1};
+    int par_depth;
+
+    Builder(const std::vector<Prim>& p, std::vector<uint32_t> order, int threads)
+        : prims(p), idx(std::move(order)), nodes(std::max<size_t>(1, 2 * idx.size())) {
+        par_depth = 0;
+        while ((1 << par_depth) < threads && par_depth < 6) par_depth++;
+    }
+
+    void make_leaf(Node2& n, uint32_t b, uint32_t e) {
+        n.left = n.right = -1;
+        n.first = b;
+        n.count = e - b;
+    }
+
+    void build(uint32_t ni, uint32_t b, uint32_t e, int depth) {
+        Node2& node = nodes[ni];
+        AABB box, cbox;
+        for (uint32_t i = b; i < e; i++) {
+            box.grow(prims[idx[i]].box);
+            cbox.grow(prims[idx[i]].c);
+        }
+        node.box = box;
+        const uint32_t n = e - b;
+        if (n <= 1) {
+            make_leaf(node, b, e);
+            return;
+        }
+        // binned SAH over centroid bounds
+        float best_cost = FLT_MAX;
+        int best_axis = -1, best_split = 0;
+        for (int ax = 0; ax < 3; ax++) {
+            const float ext = cbox.hi[ax] - cbox.lo[ax];
+            if (!(ext > 0.f)) continue;
+            const float scale = float(BINS) / ext;
+            AABB bb[BINS];
+            uint32_t cnt[BINS] = {};
+            for (uint32_t i = b; i < e; i++) {
+                const Prim& p = prims[idx[i]];
+                int k = int((p.c[ax] - cbox.lo[ax]) * scale);
+                k = std::min(std::max(k, 0), BINS - 1);
+                cnt[k]++;
+                bb[k].grow(p.box);
+            }
+            float right_area[BINS];
+            uint32_t right_cnt[BINS];
+            AABB acc;
+            uint32_t c = 0;
+            for (int k = BINS - 1; k > 0; k--) {
+                acc.grow(bb[k]);
+                c += cnt[k];
+                right_area[k] = acc.area();
+                right_cnt[k] = c;
+            }
+            AABB lacc;
+            uint32_t lc = 0;
+            for (int k = 1; k < BINS; k++) {
+                lacc.grow(bb[k - 1]);
+                lc += cnt[k - 1];
+                if (lc == 0 || right_cnt[k] == 0) continue;
+                const float cost = lacc.area() * float(lc) + right_area[k] * float(right_cnt[k]);
+                if (cost < best_cost) {
+                    best_cost = cost;
+                    best_axis = ax;
+                    best_split = k;
+                }
+            }
+        }
+        const float leaf_cost = box.area() * float(n);
+        uint32_t mid;
+        if (best_axis < 0) {
+            if (n <= uint32_t(MAX_LEAF)) {
+                make_leaf(node, b, e);
+                return;
+            }
+            mid = b + n / 2; // coincident centroids: split by index
+        } else {
+            if (n <= uint32_t(MAX_LEAF) && best_cost + box.area() >= leaf_cost) {
+                make_leaf(node, b, e);
+                return;
+            }
+            const float ext = cbox.hi[best_axis] - cbox.lo[best_axis];
+            const float scale = float(BINS) / ext;
+            const float lo = cbox.lo[best_axis];
+            auto it = std::partition(idx.begin() + b, idx.begin() + e, [&](uint32_t pi) {
+                int k = int((prims[pi].c[best_axis] - lo) * scale);
+                k = std::min(std::max(k, 0), BINS - 1);
+                return k < best_split;
+            });
+            mid = uint32_t(it - idx.begin());
+            if (mid == b || mid == e) mid = b + n / 2;
+        }
+        const uint32_t l = next.fetch_add(2);
+        const uint32_t r = l + 1;
+        node.left = int32_t(l);
+        node.right = int32_t(r);
+        node.count = 0;
+        if (depth < par_depth && n > 65536) {
+            std::thread t([&, l, b, mid, depth] { build(l, b, mid, depth + 1); });
+            build(r, mid, e, depth + 1);
+            t.join();
+        } else {
+            build(l, b, mid, depth + 1);
+            build(r, mid, e, depth + 1);
+        }
+    }
+};
+
+// pad a child box outward: rounding of the FMA slab test and of the triangle test
+void padded(const AABB& b, float mag, float lo[3], float hi[3]) {
+    float ext = 0.f;
+    for (int k = 0; k < 3; k++) ext = std::max(ext, b.hi[k] - b.lo[k]);
+    const float pad = ext * (1.0f / 1024.0f) + mag * (1.0f / 65536.0f) + 1e-30f;
+    for (int k = 0; k < 3; k++) {
+        lo[k] = b.lo[k] - pad;
+        hi[k] = b.hi[k] + pad;
+    }
+}
+
+struct Collapser {
+    const std::vector<Node2>& n2;
+    BVH4& out;
+    float mag;
+
+    uint32_t collapse(int32_t ni, uint32_t depth) {
+        int32_t kids[4];
+        int nk = 0;
+        const Node2& root = n2[ni];
+        kids[nk++] = root.left;
+        kids[nk++] = root.right;
+        while (nk < 4) {
+            int best = -1;
+            float best_area = -1.f;
+            for (int i = 0; i < nk; i++) {
+                const Node2& c = n2[kids[i]];
+                if (c.left >= 0 && c.box.area() > best_area) {
+                    best_area = c.box.area();
+                    best = i;
+                }
+            }
+            if (best < 0) break;
+            const Node2& c = n2[kids[best]];
+            kids[best] = c.left;
+            kids[nk++] = c.right;
+        }
+        const uint32_t me = uint32_t(out.nodes.size());
+        out.nodes.emplace_back();
+        out.depth = std::max(out.depth, depth);
+        BVH4Node node;
+        std::memset(&node, 0, sizeof(node));
+        uint32_t counts = 0;
+        for (int i = 0; i < 4; i++) {
+            if (i >= nk) {
+                node.lox[i] = node.loy[i] = node.loz[i] = 1e30f;
+                node.hix[i] = node.hiy[i] = node.hiz[i] = -1e30f;
+                node.child[i] = -1;
+                continue;
+            }
+            const Node2& c = n2[kids[i]];
+            float lo[3], hi[3];
+            padded(c.box, mag, lo, hi);
+            node.lox[i] = lo[0];
+            node.loy[i] = lo[1];
+            node.loz[i] = lo[2];
+            node.hix[i] = hi[0];
+            node.hiy[i] = hi[1];
+            node.hiz[i] = hi[2];
+            if (c.left < 0) {
+                node.child[i] = int32_t(c.first);
+                counts |= (c.count & 0xFFu) << (8 * i);
+            } else {
+                node.child[i] = int32_t(collapse(kids[i], depth + 1));
+            }
+        }
+        node.counts = counts;
+        out.nodes[me] = node;
+        return me;
+    }
+};
+
+float down(double v) {
+    float f = float(v);
+    if (double(f) > v) f = std::nextafter(f, -INFINITY);
+    return f;
+}
+float up(double v) {
+    float f = float(v);
+    if (double(f) < v) f = std::nextafter(f, INFINITY);
+    return f;
+}
+
+} // namespace
+
+BVH4 build_bvh4(const float* v0, const float* e1, const float* e2, const uint32_t* flags, uint32_t n,
+                int threads) {
+    if (threads <= 0) threads = int(std::max(1u, std::thread::hardware_concurrency()));
+    std::vector<Prim> prims(n);
+    std::vector<uint32_t> order;
+    order.reserve(n);
+    float mag = 0.f;
+    for (uint32_t i = 0; i < n; i++) {
+        Prim& p = prims[i];
+        bool ok = true;
+        for (int k = 0; k < 3; k++) {
+            // vertices as the intersection test sees them: v0, v0 + e1, v0 + e2 (exact in double)
+            const double a = v0[3 * i + k], b = a + double(e1[3 * i + k]), c = a + double(e2[3 * i + k]);
+            if (!std::isfinite(a) || !std::isfinite(b) || !std::isfinite(c)) ok = false;
+            p.box.lo[k] = down(std::min(a, std::min(b, c)));
+            p.box.hi[k] = up(std::max(a, std::max(b, c)));
+            p.c[k] = float((std::min(a, std::min(b, c)) + std::max(a, std::max(b, c))) * 0.5);
+        }
+        if (!ok) continue; // a non-finite triangle can never be hit
+        for (int k = 0; k < 3; k++) mag = std::max(mag, std::max(std::fabs(p.box.lo[k]), std::fabs(p.box.hi[k])));
+        order.push_back(i);
+    }
+    BVH4 out;
+    const uint32_t m = uint32_t(order.size());
+    Builder bld(prims, std::move(order), threads);
+    if (m > 0) bld.build(0, 0, m, 0);
+    // triangle records in leaf order
+    out.tris.resize(m);
+    for (uint32_t k = 0; k < m; k++) {
+        const uint32_t i = bld.idx[k];
+        TriRecord& t = out.tris[k];
+        for (int c = 0; c < 3; c++) {
+            t.a[c] = v0[3 * i + c];
+            t.b[c] = e1[3 * i + c];
+            t.c[c] = e2[3 * i + c];
+        }
+        std::memcpy(&t.a[3], &i, 4);
+        std::memcpy(&t.b[3], &flags[i], 4);
+        t.c[3] = 0.f;
+    }
+    if (m == 0) {
+        // empty scene: one node with four empty children
+        BVH4Node node;
+        std::memset(&node, 0, sizeof(node));
+        for (int i = 0; i < 4; i++) {
+            node.lox[i] = node.loy[i] = node.loz[i] = 1e30f;
+            node.hix[i] = node.hiy[i] = node.hiz[i] = -1e30f;
+            node.child[i] = -1;
+        }
+        out.nodes.push_back(node);
+        out.depth = 1;
+        return out;
+    }
+    const Node2& root = bld.nodes[0];
+    if (root.left < 0) {
+        // single leaf: wrap it in a root node with one leaf child
+        BVH4Node node;
+        std::memset(&node, 0, sizeof(node));
+        for (int i = 0; i < 4; i++) {
+            node.lox[i] = node.loy[i] = node.loz[i] = 1e30f;
+            node.hix[i] = node.hiy[i] = node.hiz[i] = -1e30f;
+            node.child[i] = -1;
+        }
+        float lo[3], hi[3];
+        padded(root.box, mag, lo, hi);
+        node.lox[0] = lo[0];
+        node.loy[0] = lo[1];
+        node.loz[0] = lo[2];
+        node.hix[0] = hi[0];
+        node.hiy[0] = hi[1];
+        node.hiz[0] = hi[2];
+        node.child[0] = int32_t(root.first);
+        node.counts = root.count & 0xFFu;
+        out.nodes.push_back(node);
+        out.depth = 1;
+        return out;
+    }
+    out.nodes.reserve(bld.next.load() / 2 + 2);
+    Collapser col{bld.nodes, out, mag};
+    col.collapse(0, 1);
+    return out;
+}
+
+} // namespace mrt

@@ -1,0 +1,542 @@
+// gfx950 kernels of the path tracer: ray queries, camera rays, the GBuffer primary pass and
+// the wavefront pt loop (init -> [trace -> shade] x bounces -> accumulate).
+//
+// The reference runs pt as one ray-generation megakernel per pixel
+// (include/merian-graph/nodes/render_pt/render_pt.slang:92-130) over driver ray queries. Here
+// each bounce is two kernels over a device queue of live paths: a lean traversal kernel (LDS
+// stack, low VGPR count, high occupancy) and a shading kernel that samples the material and
+// appends the continuation ray with one wave-aggregated atomic per wavefront.
+#include "mrt_kernels.h"
+#include "mrt_trace.h"
+
+#include <vector>
+
+namespace mrt {
+
+namespace {
+
+constexpr int TRACE_BLOCK = 128;
+constexpr int SHADE_BLOCK = 256;
+constexpr float SCENE_RAY_TMAX = 1e7f;     // scene/hit.slang:8
+constexpr float ENV_MAP_DISTANCE = 1e7f;   // scene/environment-map.slang:11
+constexpr float THROUGHPUT_THRESHOLD = 1e-5f; // scene/scene.slang:24
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// -----------------------------------------------------------------------------------------
+// camera (camera.slang:45-56)
+
+struct RayF {
+    f3 o, d;
+    float tmin, tmax;
+};
+
+__device__ __forceinline__ RayF camera_ray(const mrt_camera& c, uint32_t px, uint32_t py, uint32_t w, uint32_t h) {
+    const float p_x = ((float(px) + 0.5f) + c.jitter[0]) / float(w);
+    const float p_y = ((float(py) + 0.5f) + c.jitter[1]) / float(h);
+    const float ndc_x = 2.0f * p_x + -1.0f;
+    const float ndc_y = -2.0f * p_y + 1.0f;
+    const f3 U = ld3(c.U), V = ld3(c.V), W = ld3(c.W);
+    RayF r;
+    r.o = ld3(c.position);
+    r.d = normalize(ndc_x * U + ndc_y * V + W);
+    const float inv_cos = 1.f / dot(normalize(W), r.d);
+    r.tmin = c.near_plane * inv_cos;
+    r.tmax = c.far_plane * inv_cos;
+    return r;
+}
+
+template <int STACK>
+__device__ __forceinline__ HitRec trace_one(const DeviceScene& S, f3 o, f3 d, float tmin, float tmax, uint32_t mask,
+                                            uint32_t* stack_base) {
+    return traverse<STACK>(DeviceBVH{S.nodes, S.tris}, o, d, tmin, tmax, mask, stack_base + threadIdx.x, blockDim.x);
+}
+
+// -----------------------------------------------------------------------------------------
+// ray queries
+
+template <int STACK>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_trace_rays(DeviceScene S, const mrt_ray* __restrict__ rays,
+                                                            uint64_t n, mrt_hit* __restrict__ hits) {
+    __shared__ uint32_t stack[STACK * TRACE_BLOCK];
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const float4* rp = reinterpret_cast<const float4*>(rays + i);
+        const float4 r0 = rp[0], r1 = rp[1];
+        const HitRec h = trace_one<STACK>(S, f3{r0.x, r0.y, r0.z}, f3{r1.x, r1.y, r1.z}, r0.w, r1.w, 0xFFu, stack);
+        float4 o;
+        o.x = __uint_as_float(h.id);
+        o.y = h.id == TRAVERSAL_DONE ? __builtin_inff() : h.t;
+        o.z = h.id == TRAVERSAL_DONE ? 0.f : h.u;
+        o.w = h.id == TRAVERSAL_DONE ? 0.f : h.v;
+        reinterpret_cast<float4*>(hits)[i] = o;
+    }
+}
+
+__global__ void k_camera_rays(mrt_camera cam, uint32_t w, uint32_t h, mrt_ray* __restrict__ rays) {
+    const uint64_t n = uint64_t(w) * h;
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const RayF r = camera_ray(cam, uint32_t(i % w), uint32_t(i / w), w, h);
+        float4* o = reinterpret_cast<float4*>(rays + i);
+        o[0] = make_float4(r.o.x, r.o.y, r.o.z, r.tmin);
+        o[1] = make_float4(r.d.x, r.d.y, r.d.z, r.tmax);
+    }
+}
+
+// -----------------------------------------------------------------------------------------
+// GBuffer (Raytraced): primary ray + closest hit packed as PackedHitInfo
+// (gbuffer_rt.slang:18-94, gbuffer.slang:175-196)
+
+template <int STACK>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_gbuffer(DeviceScene S, mrt_camera cam, uint32_t w, uint32_t h,
+                                                         uint32_t mask, mrt_packed_hit* __restrict__ out) {
+    __shared__ uint32_t stack[STACK * TRACE_BLOCK];
+    const uint64_t n = uint64_t(w) * h;
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const RayF r = camera_ray(cam, uint32_t(i % w), uint32_t(i / w), w, h);
+        const HitRec hr = trace_one<STACK>(S, r.o, r.d, r.tmin, r.tmax, mask, stack);
+        uint4 p;
+        if (hr.id == TRAVERSAL_DONE) {
+            p = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u);
+        } else {
+            const uint32_t g = S.tri_verts[hr.id].w;
+            p.x = 0u;                          // instance_id: the one pretransformed group
+            p.y = hr.id - S.geom_first[g];     // primitive_id within the geometry
+            p.z = (g & 0xFFFFu);               // geometry_index | instance_index << 16
+            p.w = uint32_t(float_to_half(hr.u)) | (uint32_t(float_to_half(hr.v)) << 16);
+        }
+        reinterpret_cast<uint4*>(out)[i] = p;
+    }
+}
+
+// -----------------------------------------------------------------------------------------
+// environment (environment-map.slang:13-68, utils/math.slang:243-249)
+
+__device__ __forceinline__ f3 env_texel(const DeviceScene& S, int x, int y) {
+    const int w = int(S.env_w), h = int(S.env_h);
+    if (S.env_wrap == MRT_WRAP_REPEAT) {
+        x = ((x % w) + w) % w;
+        y = ((y % h) + h) % h;
+    } else {
+        x = min(max(x, 0), w - 1);
+        y = min(max(y, 0), h - 1);
+    }
+    const float4 t = S.env[size_t(y) * w + x];
+    return {t.x, t.y, t.z};
+}
+__device__ __forceinline__ float tlerp(float x, float y, float t) { return x + t * (y - x); }
+__device__ __forceinline__ f3 tlerp(f3 x, f3 y, float t) {
+    return {tlerp(x.x, y.x, t), tlerp(x.y, y.y, t), tlerp(x.z, y.z, t)};
+}
+__device__ __forceinline__ f3 env_eval(const DeviceScene& S, f3 dir) {
+    if (S.env_type == MRT_ENV_NONE) return {0, 0, 0};
+    const float* m = S.env_to_local;
+    const f3 d{m[0] * dir.x + m[1] * dir.y + m[2] * dir.z, m[3] * dir.x + m[4] * dir.y + m[5] * dir.z,
+               m[6] * dir.x + m[7] * dir.y + m[8] * dir.z};
+    f2 uv;
+    if (S.env_type == MRT_ENV_LATLONG) {
+        const f3 p = normalize(d);
+        uv.x = matan2(p.x, -p.z) * INV_2PI + 0.5f;
+        uv.y = macos(p.y) * M_1_PI_F;
+    } else {
+        const f3 a{fabsf(d.x), fabsf(d.y), fabsf(d.z)};
+        const float r = sqrtf(fmaxf(0.f, 1.f - a.z));
+        const float mm = fmaxf(a.x, a.y);
+        const float b = mm == 0.f ? 0.f : fminf(a.x, a.y) / mm;
+        float phi = matan(b) * (2.f / PI);
+        if (a.x < a.y) phi = 1.f - phi;
+        float v = phi * r;
+        float u = r - v;
+        if (d.z < 0.f) {
+            const float mu = u;
+            u = 1.f - v;
+            v = 1.f - mu;
+        }
+        u = d.x < 0.f ? -u : u;
+        v = d.y < 0.f ? -v : v;
+        uv = {0.5f * (u + 1.f), 0.5f * (v + 1.f)};
+    }
+    const float fx = uv.x * float(S.env_w) - 0.5f;
+    const float fy = uv.y * float(S.env_h) - 0.5f;
+    const float x0f = floorf(fx), y0f = floorf(fy);
+    const float ax = fx - x0f, ay = fy - y0f;
+    const int x0 = int(x0f), y0 = int(y0f);
+    const f3 c = tlerp(tlerp(env_texel(S, x0, y0), env_texel(S, x0 + 1, y0), ax),
+                       tlerp(env_texel(S, x0, y0 + 1), env_texel(S, x0 + 1, y0 + 1), ax), ay);
+    return S.env_intensity * c;
+}
+
+// -----------------------------------------------------------------------------------------
+// shading points (scene.slang:312-498, hit.slang:106-147)
+
+struct ShadingPoint {
+    f3 pos;
+    uint32_t enc_face_normal;
+    f3 emission;
+    Frame frame;
+    bool has_material;
+    MaterialSample mat;
+};
+
+__device__ __forceinline__ f3 load_pos(const mrt_vertex& v) { return {v.position[0], v.position[1], v.position[2]}; }
+
+__device__ __forceinline__ void shading_point(const DeviceScene& S, f3 ro, f3 rd, uint32_t gid, float bu, float bv, float ior,
+                              bool with_material, ShadingPoint& sp) {
+    uint4 tv = make_uint4(0, 0, 0, 0);
+    bool surface = gid != TRAVERSAL_DONE;
+    if (surface) {
+        tv = S.tri_verts[gid];
+        surface = (S.geoms[tv.w].flags & MRT_GEOM_USE_ENV_MAP) == 0;
+    }
+    if (!surface) {
+        sp.pos = ro + rd * ENV_MAP_DISTANCE;
+        sp.enc_face_normal = encode_normal(-rd);
+        sp.emission = env_eval(S, -(-rd));
+        sp.frame = frame_from_normal(-rd);
+        sp.has_material = false;
+        return;
+    }
+    const mrt_geometry G = S.geoms[tv.w];
+    const mrt_vertex v0 = S.verts[tv.x], v1 = S.verts[tv.y], v2 = S.verts[tv.z];
+    const f3 b{(1.0f - bu) - bv, bu, bv};
+    const f3 p0 = load_pos(v0), p1 = load_pos(v1), p2 = load_pos(v2);
+    const f3 pos = b.x * p0 + b.y * p1 + b.z * p2;
+    const f3 e01 = p1 - p0, e02 = p2 - p0;
+    f3 face_normal = normalize(cross(e01, e02));
+    if (G.flags & MRT_GEOM_FLIP_FACING) face_normal = -face_normal;
+    f3 vn;
+    if (G.flags & MRT_GEOM_FLAT_SHADING) {
+        vn = face_normal;
+    } else {
+        vn = normalize(b.x * decode_normal(v0.encoded_normal) + b.y * decode_normal(v1.encoded_normal) +
+                       b.z * decode_normal(v2.encoded_normal));
+    }
+    const f2 uv0{half_to_float(v0.uv[0]), half_to_float(v0.uv[1])};
+    const f2 uv1{half_to_float(v1.uv[0]), half_to_float(v1.uv[1])};
+    const f2 uv2{half_to_float(v2.uv[0]), half_to_float(v2.uv[1])};
+    Frame frame;
+    if (G.flags & MRT_GEOM_HAS_TANGENTS) {
+        const float w0 = (v0.encoded_tangent & 1u) ? -1.f : 1.f;
+        const float w1 = (v1.encoded_tangent & 1u) ? -1.f : 1.f;
+        const float w2 = (v2.encoded_tangent & 1u) ? -1.f : 1.f;
+        const f3 txyz = b.x * decode_normal(v0.encoded_tangent & ~1u) + b.y * decode_normal(v1.encoded_tangent & ~1u) +
+                        b.z * decode_normal(v2.encoded_tangent & ~1u);
+        const float tw = b.x * w0 + b.y * w1 + b.z * w2;
+        frame = frame_from_nt(vn, normalize(txyz), signf(tw));
+    } else {
+        const f2 d1{uv1.x - uv0.x, uv1.y - uv0.y};
+        const f2 d2{uv2.x - uv0.x, uv2.y - uv0.y};
+        const float det = d1.x * d2.y - d2.x * d1.y;
+        if (fabsf(det) > 1e-8f) {
+            const float ws = (G.flags & MRT_GEOM_FLIP_FACING) ? 1.f : -1.f;
+            frame = frame_from_nt(vn, (d2.y * e01 - d1.y * e02) / det, signf(det) * ws);
+        } else {
+            frame = frame_from_normal(vn);
+        }
+    }
+    bool front = true;
+    if (G.flags & MRT_GEOM_TWO_SIDED) {
+        front = dot(-rd, face_normal) >= 0.f;
+        if (!front) frame.r2 = -frame.r2;
+    }
+    sp.pos = pos;
+    sp.enc_face_normal = encode_normal(face_normal);
+    sp.has_material = with_material;
+    const mrt_material& m = S.mats[G.material_id];
+    if (with_material) {
+        ShadingData sd{frame, ior, front};
+        sp.mat = get_material_sample(m, sd, sp.emission, sp.frame);
+    } else {
+        sp.emission = material_emission(m, front);
+        sp.frame = frame;
+    }
+}
+
+// -----------------------------------------------------------------------------------------
+// wavefront pt
+
+__global__ void k_zero_u32(uint32_t* p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// Seeds one path per (pixel, sample) of the chunk with the primary ray and gbuffer hit.
+__global__ void k_pt_init(DeviceScene S, PtArgs A, const mrt_packed_hit* __restrict__ gbuffer, PathBuffers P,
+                          uint32_t* count0) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *count0 = A.n_paths;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n_paths; i += stride) {
+        const uint32_t p = i % A.npix, sl = i / A.npix;
+        const uint32_t x = p % A.width, y = p / A.width;
+        const uint32_t s = A.s0 + sl;
+        const RayF r = camera_ray(A.cam, x, y, A.width, A.height);
+        const uint4 gb = reinterpret_cast<const uint4*>(gbuffer)[p];
+        uint32_t gid = TRAVERSAL_DONE;
+        float bu = 0.f, bv = 0.f;
+        if (gb.x != 0xFFFFFFFFu) {
+            const uint32_t geometry = gb.x + (gb.z & 0xFFFFu); // GeometryID = instance_id + geometry_index
+            // hits of a stale or foreign gbuffer that name no triangle of this scene shade as misses
+            if (geometry < S.n_geoms && gb.y < S.geoms[geometry].primitive_count) gid = S.geom_first[geometry] + gb.y;
+            bu = half_to_float(uint16_t(gb.w & 0xFFFFu));
+            bv = half_to_float(uint16_t(gb.w >> 16));
+        }
+        P.q_o[0][i] = make_float4(r.o.x, r.o.y, r.o.z, __uint_as_float(i));
+        P.q_d[0][i] = make_float4(r.d.x, r.d.y, r.d.z, 0.f);
+        P.hits[i] = make_float4(0.f, bu, bv, __uint_as_float(gid));
+        P.state_a[i] = make_float4(1.f, 1.f, 1.f, 1.f);
+        P.state_b[i] = make_float4(0.f, 0.f, 0.f, __uint_as_float(pcg4d16(x, y, A.frame, s)));
+        if (A.record_signature) P.sig[i] = sig_step(0x9E3779B9u * (s + 1u), gid);
+    }
+}
+
+template <int STACK>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_trace_queue(DeviceScene S, const float4* __restrict__ qo,
+                                                             const float4* __restrict__ qd,
+                                                             const uint32_t* __restrict__ count,
+                                                             float4* __restrict__ hits, uint32_t mask) {
+    __shared__ uint32_t stack[STACK * TRACE_BLOCK];
+    const uint32_t n = *count;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const float4 o = qo[i], d = qd[i];
+        const HitRec h = trace_one<STACK>(S, f3{o.x, o.y, o.z}, f3{d.x, d.y, d.z}, 0.f, SCENE_RAY_TMAX, mask, stack);
+        hits[i] = make_float4(h.t, h.u, h.v, __uint_as_float(h.id));
+    }
+}
+
+// Shades vertex `bounce` of every queued path and appends its continuation ray
+// (trace_path, render_pt.slang:24-90).
+__global__ __launch_bounds__(SHADE_BLOCK) void k_shade(DeviceScene S, PtArgs A, int bounce, PathBuffers P,
+                                                       const float4* __restrict__ qo, const float4* __restrict__ qd,
+                                                       const uint32_t* __restrict__ count_in, float4* __restrict__ no,
+                                                       float4* __restrict__ nd, uint32_t* __restrict__ count_out) {
+    const uint32_t n = *count_in;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t lane = lane_id();
+    const bool last = bounce + 1 >= A.max_path_length;
+    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < n; base += stride) {
+        const uint32_t i = base + lane;
+        bool push = false;
+        f3 next_o{0, 0, 0}, next_d{0, 0, 0};
+        uint32_t pid = 0;
+        if (i < n) {
+            const float4 o4 = qo[i], d4 = qd[i], h4 = P.hits[i];
+            pid = __float_as_uint(o4.w);
+            const f3 ro{o4.x, o4.y, o4.z}, rd{d4.x, d4.y, d4.z};
+            const uint32_t gid = __float_as_uint(h4.w);
+            float4 sa = P.state_a[pid], sb = P.state_b[pid];
+            f3 thr{sa.x, sa.y, sa.z};
+            float ior = sa.w;
+            f3 rad{sb.x, sb.y, sb.z};
+            PCG rng{__float_as_uint(sb.w)};
+            ShadingPoint sp;
+            shading_point(S, ro, rd, gid, h4.y, h4.z, ior, bounce == 0 || !last, sp);
+            if (bounce == 0) {
+                rad = A.emission_on_primary ? thr * sp.emission : f3{0.f, 0.f, 0.f};
+            } else {
+                rad = rad + thr * sp.emission;
+                if (A.record_signature) P.sig[pid] = sig_step(P.sig[pid], gid);
+            }
+            if (!last && sp.has_material &&
+                !(thr.x < THROUGHPUT_THRESHOLD && thr.y < THROUGHPUT_THRESHOLD && thr.z < THROUGHPUT_THRESHOLD)) {
+                const f3 wi_local = sp.frame.to_local(-rd);
+                const Sample s = sp.mat.sample_eval(wi_local, rng);
+                if (s.valid) {
+                    const f3 wo = sp.frame.to_world(s.wo);
+                    thr = thr * s.weight;
+                    const bool transmission = wi_local.z * s.wo.z < 0.f;
+                    if (transmission) ior /= sp.mat.eta;
+                    // ShadingPoint.make_ray (hit.slang:134-146)
+                    const f3 ng = decode_normal(sp.enc_face_normal);
+                    const float wo_ng = dot(wo, ng);
+                    if (((dot(-rd, ng) < 0.f) != (wo_ng < 0.f)) == transmission) {
+                        next_o = offset_ray_origin(sp.pos, wo_ng < 0.f ? -ng : ng);
+                        next_d = wo;
+                        push = true;
+                    }
+                }
+            }
+            P.state_a[pid] = make_float4(thr.x, thr.y, thr.z, ior);
+            P.state_b[pid] = make_float4(rad.x, rad.y, rad.z, __uint_as_float(rng.state));
+        }
+        const uint64_t ballot = __ballot(push);
+        if (ballot) {
+            const uint32_t leader = __ffsll((unsigned long long)ballot) - 1;
+            uint32_t slot = 0;
+            if (lane == leader) slot = atomicAdd(count_out, uint32_t(__popcll(ballot)));
+            slot = __shfl(slot, leader);
+            if (push) {
+                const uint32_t k = slot + uint32_t(__popcll(ballot & ((1ull << lane) - 1ull)));
+                no[k] = make_float4(next_o.x, next_o.y, next_o.z, __uint_as_float(pid));
+                nd[k] = make_float4(next_d.x, next_d.y, next_d.z, 0.f);
+            }
+        }
+    }
+}
+
+// Sums the chunk's samples per pixel in sample order with the NaN/Inf rejection of
+// render_pt.slang:116-127; the last chunk divides by spp and writes the output.
+__global__ void k_accumulate(PtArgs A, PathBuffers P, int first_chunk, int last_chunk, float4* __restrict__ out,
+                             uint32_t* __restrict__ sig_out) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < A.npix; p += stride) {
+        float4 acc = first_chunk ? make_float4(0.f, 0.f, 0.f, 0.f) : P.accum[p];
+        uint32_t sa = first_chunk ? 0u : P.sig_accum[p];
+        f3 irr{acc.x, acc.y, acc.z};
+        float m2 = acc.w;
+        for (uint32_t sl = 0; sl < A.chunk; sl++) {
+            const uint32_t i = sl * A.npix + p;
+            const float4 b = P.state_b[i];
+            const f3 c{b.x, b.y, b.z};
+            if (A.record_signature) sa += P.sig[i];
+            if (finite3(c)) {
+                irr = irr + c;
+                const float l = yuv_luminance(c);
+                m2 += l * l;
+            }
+        }
+        if (last_chunk) {
+            if (A.spp > 0) {
+                irr = irr / float(A.spp);
+                m2 /= float(A.spp);
+            }
+            out[p] = make_float4(irr.x, irr.y, irr.z, m2);
+            if (sig_out) sig_out[p] = sa;
+        } else {
+            P.accum[p] = make_float4(irr.x, irr.y, irr.z, m2);
+            P.sig_accum[p] = sa;
+        }
+    }
+}
+
+int grid_for(uint64_t n, int block, int max_blocks) {
+    const uint64_t b = (n + block - 1) / block;
+    return int(b < uint64_t(max_blocks) ? (b > 0 ? b : 1) : max_blocks);
+}
+
+constexpr int MAX_BLOCKS = 256 * 16; // 256 CUs
+
+} // namespace
+
+// ---------------------------------------------------------------------------------------------
+
+#define MRT_DISPATCH_STACK(S, CALL)                                                  \
+    switch ((S).stack_kind) {                                                        \
+    case 0: CALL(32); break;                                                         \
+    case 1: CALL(64); break;                                                         \
+    default: CALL(128); break;                                                       \
+    }
+
+hipError_t launch_trace_rays(const DeviceScene& s, const mrt_ray* rays, uint64_t n, mrt_hit* hits, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const int g = grid_for(n, TRACE_BLOCK, MAX_BLOCKS);
+#define CALL(K) hipLaunchKernelGGL(k_trace_rays<K>, dim3(g), dim3(TRACE_BLOCK), 0, st, s, rays, n, hits)
+    MRT_DISPATCH_STACK(s, CALL)
+#undef CALL
+    return hipGetLastError();
+}
+
+hipError_t launch_camera_rays(const mrt_camera& cam, uint32_t w, uint32_t h, mrt_ray* rays, hipStream_t st) {
+    const uint64_t n = uint64_t(w) * h;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_camera_rays, dim3(grid_for(n, 256, MAX_BLOCKS)), dim3(256), 0, st, cam, w, h, rays);
+    return hipGetLastError();
+}
+
+hipError_t launch_gbuffer(const DeviceScene& s, const mrt_camera& cam, uint32_t w, uint32_t h, uint32_t mask,
+                          mrt_packed_hit* out, hipStream_t st) {
+    const uint64_t n = uint64_t(w) * h;
+    if (n == 0) return hipSuccess;
+    const int g = grid_for(n, TRACE_BLOCK, MAX_BLOCKS);
+#define CALL(K) hipLaunchKernelGGL(k_gbuffer<K>, dim3(g), dim3(TRACE_BLOCK), 0, st, s, cam, w, h, mask, out)
+    MRT_DISPATCH_STACK(s, CALL)
+#undef CALL
+    return hipGetLastError();
+}
+
+hipError_t launch_pt_frame(const DeviceScene& s, PtArgs a, const mrt_packed_hit* gbuffer, PathBuffers& pb,
+                           float4* out, uint32_t* sig_out, hipStream_t st, FrameTiming* timing,
+                           uint64_t* continuation_segments) {
+    const int L = a.max_path_length;
+    const uint32_t spp = uint32_t(a.spp > 0 ? a.spp : 0);
+    if (spp == 0 || a.npix == 0) {
+        // render_pt.slang:124-129 with spp = 0: irradiance stays 0
+        hipMemsetAsync(out, 0, sizeof(float4) * a.npix, st);
+        if (sig_out) hipMemsetAsync(sig_out, 0, sizeof(uint32_t) * a.npix, st);
+        return hipGetLastError();
+    }
+    const uint32_t chunk = uint32_t(pb.capacity / a.npix) < spp ? uint32_t(pb.capacity / a.npix) : spp;
+    if (chunk == 0) return hipErrorInvalidValue;
+    const uint32_t n_chunks = (spp + chunk - 1) / chunk;
+    const uint32_t per_chunk = uint32_t(L + 1);
+    hipEvent_t ev[4] = {};
+    if (timing) {
+        for (auto& e : ev) hipEventCreate(&e);
+    }
+    float ms_trace = 0.f, ms_shade = 0.f;
+    if (timing) hipEventRecord(ev[0], st);
+    const uint32_t nz = n_chunks * per_chunk;
+    hipLaunchKernelGGL(k_zero_u32, dim3((nz + 255) / 256), dim3(256), 0, st, pb.counters, nz);
+    for (uint32_t c = 0; c < n_chunks; c++) {
+        a.s0 = c * chunk;
+        a.chunk = (a.s0 + chunk <= spp) ? chunk : spp - a.s0;
+        a.n_paths = a.npix * a.chunk;
+        uint32_t* cnt = pb.counters + c * per_chunk;
+        hipLaunchKernelGGL(k_pt_init, dim3(grid_for(a.n_paths, 256, MAX_BLOCKS)), dim3(256), 0, st, s, a, gbuffer,
+                           pb, cnt);
+        for (int b = 0; b < L; b++) {
+            const int qi = b & 1;
+            if (b > 0) {
+                if (timing) hipEventRecord(ev[1], st);
+                const int g = grid_for(a.n_paths, TRACE_BLOCK, MAX_BLOCKS);
+#define CALL(K)                                                                                                  \
+    hipLaunchKernelGGL(k_trace_queue<K>, dim3(g), dim3(TRACE_BLOCK), 0, st, s, pb.q_o[qi], pb.q_d[qi], cnt + b, \
+                       pb.hits, a.instance_mask)
+                MRT_DISPATCH_STACK(s, CALL)
+#undef CALL
+                if (timing) {
+                    hipEventRecord(ev[2], st);
+                    hipEventSynchronize(ev[2]);
+                    float ms = 0.f;
+                    hipEventElapsedTime(&ms, ev[1], ev[2]);
+                    ms_trace += ms;
+                }
+            }
+            if (timing) hipEventRecord(ev[1], st);
+            hipLaunchKernelGGL(k_shade, dim3(grid_for(a.n_paths, SHADE_BLOCK, MAX_BLOCKS)), dim3(SHADE_BLOCK), 0, st,
+                               s, a, b, pb, pb.q_o[qi], pb.q_d[qi], cnt + b, pb.q_o[qi ^ 1], pb.q_d[qi ^ 1],
+                               cnt + b + 1);
+            if (timing) {
+                hipEventRecord(ev[2], st);
+                hipEventSynchronize(ev[2]);
+                float ms = 0.f;
+                hipEventElapsedTime(&ms, ev[1], ev[2]);
+                ms_shade += ms;
+            }
+        }
+        hipLaunchKernelGGL(k_accumulate, dim3(grid_for(a.npix, 256, MAX_BLOCKS)), dim3(256), 0, st, a, pb,
+                           int(c == 0), int(c + 1 == n_chunks), out, sig_out);
+    }
+    if (timing) {
+        hipEventRecord(ev[3], st);
+        hipEventSynchronize(ev[3]);
+        hipEventElapsedTime(&timing->ms_total, ev[0], ev[3]);
+        timing->ms_trace = ms_trace;
+        timing->ms_shade = ms_shade;
+        for (auto& e : ev) hipEventDestroy(e);
+    }
+    if (continuation_segments) {
+        std::vector<uint32_t> h(nz);
+        hipMemcpyAsync(h.data(), pb.counters, sizeof(uint32_t) * nz, hipMemcpyDeviceToHost, st);
+        hipStreamSynchronize(st);
+        uint64_t segs = 0;
+        for (uint32_t c = 0; c < n_chunks; c++)
+            for (int b = 1; b < L; b++) segs += h[c * per_chunk + b];
+        *continuation_segments = segs;
+    }
+    return hipGetLastError();
+}
+
+} // namespace mrt

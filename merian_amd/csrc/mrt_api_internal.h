@@ -1,0 +1,40 @@
+// Internal state behind the opaque handles of include/merian_amd.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/merian_amd.h"
+#include "mrt_kernels.h"
+
+struct mrt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0;  // bytes the context may hold
+    uint64_t in_use = 0;  // bytes it holds
+    uint64_t max_paths = 1ull << 24; // paths in flight per wavefront pass
+    std::unordered_map<void*, size_t> sizes;
+    std::string err;
+    mrt::PathBuffers paths{};
+    uint64_t paths_npix = 0, paths_counters = 0;
+};
+
+struct mrt_scene {
+    mrt::DeviceScene ds{};
+    std::vector<void*> allocs;
+    uint64_t bytes = 0, n_tris = 0, n_nodes = 0;
+    uint32_t depth = 0;
+};
+
+namespace mrt {
+mrt_status fail(mrt_ctx* c, mrt_status s, const std::string& msg);
+mrt_status hip_fail(mrt_ctx* c, hipError_t e, const char* what);
+void* dev_alloc(mrt_ctx* c, size_t bytes, mrt_status* st);
+void dev_free(mrt_ctx* c, void* p);
+hipStream_t pick_stream(mrt_ctx* c, void* stream);
+mrt_status ensure_path_buffers(mrt_ctx* c, uint64_t paths, uint64_t npix, uint64_t counters);
+mrt_status check_params(mrt_ctx* c, const mrt_pt_params* p);
+} // namespace mrt

@@ -1,0 +1,72 @@
+// Host <-> device interface of the gfx950 kernels (kernels.hip): the device-resident scene
+// view, the per-frame arguments of the wavefront path tracer and the launch entry points
+// called by the C ABI (api.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "bvh.h"
+#include "../../include/merian_amd.h"
+
+namespace mrt {
+
+// Scene as the kernels see it (the reference's Scene parameter block, scene.slang:30-54).
+struct DeviceScene {
+    const BVH4Node* nodes;
+    const float4* tris;          // TriRecord stream, leaf order
+    const uint4* tri_verts;      // per global triangle id: 3 global vertex indices, geometry index
+    const mrt_vertex* verts;     // PackedVertexData
+    const mrt_geometry* geoms;   // GeometryData
+    const uint32_t* geom_first;  // global id of each geometry's first triangle
+    const mrt_material* mats;    // material entries
+    const float4* env;           // environment texels (RGBA32F), row-major
+    int32_t env_type, env_wrap;
+    uint32_t env_w, env_h;
+    float env_intensity;
+    float env_to_local[9];
+    uint32_t n_tris;
+    uint32_t n_geoms;
+    int32_t stack_kind;          // 0: 32-entry traversal stack, 1: 64, 2: 128
+};
+
+// Frame-constant arguments of the wavefront pt kernels.
+struct PtArgs {
+    mrt_camera cam;
+    uint32_t width, height, npix, frame;
+    int32_t spp, max_path_length, emission_on_primary, record_signature;
+    uint32_t instance_mask;
+    uint32_t s0;      // first sample index of this chunk
+    uint32_t chunk;   // samples in this chunk
+    uint32_t n_paths; // npix * chunk
+};
+
+// Device buffers of one chunk of paths (structure of arrays).
+struct PathBuffers {
+    float4* state_a;   // throughput.xyz, current ior
+    float4* state_b;   // radiance.xyz, PCG state bits
+    uint32_t* sig;     // hit-sequence hash per path
+    float4* q_o[2];    // ray queue: origin.xyz, path index bits
+    float4* q_d[2];    // ray queue: direction.xyz
+    float4* hits;      // t, u, v, triangle id bits; indexed like the queue being shaded
+    uint32_t* counters;// per bounce queue sizes, max_path_length + 1 per chunk
+    float4* accum;     // per pixel running sum: radiance.xyz, luminance^2
+    uint32_t* sig_accum;
+    uint64_t capacity; // paths
+};
+
+// Launch entry points (kernels.hip). All asynchronous on `stream`.
+hipError_t launch_trace_rays(const DeviceScene& s, const mrt_ray* rays, uint64_t n, mrt_hit* hits, hipStream_t st);
+hipError_t launch_camera_rays(const mrt_camera& cam, uint32_t w, uint32_t h, mrt_ray* rays, hipStream_t st);
+hipError_t launch_gbuffer(const DeviceScene& s, const mrt_camera& cam, uint32_t w, uint32_t h, uint32_t mask,
+                          mrt_packed_hit* out, hipStream_t st);
+
+struct FrameTiming {
+    float ms_trace = 0.f, ms_shade = 0.f, ms_total = 0.f;
+};
+// One frame of pt over `gbuffer`; out = float4 irradiance per pixel; sig_out optional.
+// counters_host (optional) receives every chunk's queue sizes after the frame completes.
+hipError_t launch_pt_frame(const DeviceScene& s, PtArgs a, const mrt_packed_hit* gbuffer, PathBuffers& pb,
+                           float4* out, uint32_t* sig_out, hipStream_t st, FrameTiming* timing,
+                           uint64_t* continuation_segments);
+
+} // namespace mrt

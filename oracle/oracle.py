@@ -1,0 +1,164 @@
+"""ctypes wrapper of oracle/liboracle.so -- TEST INFRASTRUCTURE ONLY.
+
+The CPU oracle restates the reference's pt path (render_pt.slang and the Slang library it
+imports). Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this
+module, and only as the checker / CPU baseline; the product path never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+from merian_amd import abi
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB
+
+
+class BsdfCfg(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("albedo", C.c_float * 3), ("alpha", C.c_float * 2), ("f0", C.c_float),
+                ("eta", C.c_float), ("front_facing", C.c_int32), ("ior", C.c_float),
+                ("mat", C.c_uint8 * 128)]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        L = C.CDLL(str(LIB))
+        vp, u32, u64, i32, f32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int32, C.c_float
+        sig = {
+            "orc_pcg4d16": (u32, [u32, u32, u32, u32]),
+            "orc_pcg_stream": (None, [u32, u32, vp, vp]),
+            "orc_encode_normal": (u32, [vp]),
+            "orc_decode_normal": (None, [u32, vp]),
+            "orc_offset_ray_origin": (None, [vp, vp, vp]),
+            "orc_f32_to_f16": (C.c_uint16, [f32]),
+            "orc_f16_to_f32": (f32, [C.c_uint16]),
+            "orc_camera_look_at": (None, [vp, vp, vp, f32, f32, f32, f32, i32, u32, C.POINTER(abi.Camera)]),
+            "orc_camera_rays": (None, [C.POINTER(abi.Camera), u32, u32, vp]),
+            "orc_scene_create": (vp, [C.POINTER(abi.SceneDesc), i32]),
+            "orc_scene_create_triangles": (vp, [vp, u32, vp, u32, i32, i32]),
+            "orc_scene_destroy": (None, [vp]),
+            "orc_trace_closest": (None, [vp, vp, u64, vp, i32]),
+            "orc_gbuffer": (None, [vp, C.POINTER(abi.Camera), u32, u32, u32, vp, i32]),
+            "orc_pt_render": (u64, [vp, C.POINTER(abi.PtParams), C.POINTER(abi.Camera), u32, u32, u32, vp,
+                                    u32, u32, u32, u32, vp, vp, i32]),
+            "orc_bsdf": (None, [C.POINTER(BsdfCfg), i32, vp, vp, u32, u64, vp, vp, vp, vp]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def camera_look_at(spec, frame: int = 0) -> abi.Camera:
+    cam = abi.Camera()
+    f3 = C.c_float * 3
+    lib().orc_camera_look_at(C.cast(f3(*spec.eye), C.c_void_p), C.cast(f3(*spec.target), C.c_void_p),
+                             C.cast(f3(*spec.up), C.c_void_p), spec.vfov, spec.aspect, spec.near, spec.far,
+                             spec.jitter, frame, C.byref(cam))
+    return cam
+
+
+def camera_rays(cam: abi.Camera, w: int, h: int) -> np.ndarray:
+    out = np.zeros(w * h, dtype=abi.RAY_DTYPE)
+    lib().orc_camera_rays(C.byref(cam), w, h, _p(out))
+    return out
+
+
+class OracleScene:
+    def __init__(self, scene=None, *, triangles=None, two_sided=True, use_bvh=True):
+        if scene is not None:
+            self._desc = scene.desc()
+            self.h = lib().orc_scene_create(C.byref(self._desc), int(use_bvh))
+        else:
+            pos, idx = triangles
+            pos = np.ascontiguousarray(pos, dtype=np.float32)
+            idx = np.ascontiguousarray(idx, dtype=np.uint32)
+            self.h = lib().orc_scene_create_triangles(_p(pos), len(pos), _p(idx), len(idx) // 3,
+                                                      int(two_sided), int(use_bvh))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_scene_destroy(self.h)
+            self.h = None
+
+    def trace(self, rays: np.ndarray, threads: int = 0) -> np.ndarray:
+        rays = np.ascontiguousarray(rays)
+        hits = np.zeros(len(rays), dtype=abi.HIT_DTYPE)
+        lib().orc_trace_closest(self.h, _p(rays), len(rays), _p(hits), threads)
+        return hits
+
+    def gbuffer(self, cam, w, h, mask=0xFF, threads: int = 0) -> np.ndarray:
+        out = np.zeros(w * h, dtype=abi.PACKED_HIT_DTYPE)
+        lib().orc_gbuffer(self.h, C.byref(cam), w, h, mask, _p(out), threads)
+        return out
+
+    def render(self, params: abi.PtParams, cam, frame, w, h, gbuffer, rect=None, threads: int = 0):
+        """Returns (irradiance float32[ch, cw, 4], signature uint32[ch, cw], continuation segments)."""
+        x0, y0, cw, ch = rect if rect else (0, 0, w, h)
+        out = np.zeros((ch, cw, 4), dtype=np.float32)
+        sig = np.zeros((ch, cw), dtype=np.uint32)
+        gb = np.ascontiguousarray(gbuffer)
+        segs = lib().orc_pt_render(self.h, C.byref(params), C.byref(cam), frame, w, h, _p(gb), x0, y0, cw, ch,
+                                   _p(out), _p(sig), threads)
+        return out, sig, int(segs)
+
+
+def bsdf(cfg: BsdfCfg, op: str, wi: np.ndarray, wo: np.ndarray | None = None, seed: int = 1):
+    wi = np.ascontiguousarray(wi, dtype=np.float32)
+    n = len(wi)
+    o_wo = np.zeros((n, 3), np.float32)
+    o_pdf = np.zeros(n, np.float32)
+    o_w = np.zeros((n, 3), np.float32)
+    o_v = np.zeros(n, np.int32)
+    wo_a = None if wo is None else np.ascontiguousarray(wo, dtype=np.float32)
+    code = {"sample_eval": 0, "eval": 1, "pdf": 2}[op]
+    lib().orc_bsdf(C.byref(cfg), code, _p(wi), _p(wo_a), seed, n, _p(o_wo), _p(o_pdf), _p(o_w), _p(o_v))
+    if op == "sample_eval":
+        return o_wo, o_pdf, o_w, o_v.astype(bool)
+    if op == "eval":
+        return o_w
+    return o_pdf
+
+
+def bsdf_cfg(kind: int, albedo=(0.8, 0.8, 0.8), alpha=(0.3, 0.3), f0=0.04, eta=1.0 / 1.5, material=None,
+             front_facing=True, ior=1.0) -> BsdfCfg:
+    c = BsdfCfg()
+    c.kind = kind
+    for k in range(3):
+        c.albedo[k] = albedo[k]
+    c.alpha[0], c.alpha[1] = alpha
+    c.f0, c.eta, c.front_facing, c.ior = f0, eta, int(front_facing), ior
+    if material is not None:
+        raw = np.ascontiguousarray(material).view(np.uint8).reshape(-1)[:128]
+        C.memmove(c.mat, raw.ctypes.data, 128)
+    return c
+
+
+def pt_params(spp=4, max_path_length=4, emission_on_primary=True, mask=0xFF, mode=abi.MODE_BSDF,
+              record_signature=True) -> abi.PtParams:
+    p = abi.PtParams()
+    p.spp, p.max_path_length, p.emission_on_primary = spp, max_path_length, int(emission_on_primary)
+    p.demodulate_albedo, p.instance_mask, p.mode, p.record_signature = 0, mask, mode, int(record_signature)
+    return p
