@@ -205,7 +205,7 @@ typedef struct mrt_pt_params {
     uint32_t instance_mask;      /* bits "0".."7" */
     int32_t mode;                /* MRT_MODE_*: BSDF = the reference estimator */
     int32_t record_signature;    /* write a per-pixel hash of the hit ids (parity aid) */
-    int32_t _reserved;
+    int32_t count_traversal;     /* fill node_visits / triangle_tests (slower counting kernels) */
 } mrt_pt_params;
 
 /* Per-frame statistics (counters kept by the device queues). */
@@ -217,6 +217,10 @@ typedef struct mrt_pt_stats {
     float ms_trace;                 /* device time spent in the intersect kernel */
     float ms_shade;                 /* device time in shading / sampling kernels */
     float ms_total;                 /* device time of the whole frame */
+    uint32_t trace_launches;        /* intersect-kernel launches of the frame */
+    uint32_t _pad;
+    uint64_t node_visits;           /* BVH4 node visits (count_traversal only) */
+    uint64_t triangle_tests;        /* ray-triangle tests (count_traversal only) */
 } mrt_pt_stats;
 
 /* ------------------------------------------------------------------------------------
@@ -265,6 +269,10 @@ MRT_API mrt_status mrt_trace_closest(mrt_ctx* ctx, mrt_scene* scene, const mrt_r
                                      uint64_t n, mrt_hit* hits);
 MRT_API mrt_status mrt_trace_closest_dev(mrt_ctx* ctx, mrt_scene* scene, const mrt_ray* d_rays,
                                          uint64_t n, mrt_hit* d_hits, void* stream);
+/* As mrt_trace_closest_dev, with the counting kernel: counts[0] = BVH4 node visits,
+ * counts[1] = ray-triangle tests (host array). Synchronous; for the bandwidth roofline. */
+MRT_API mrt_status mrt_trace_count_dev(mrt_ctx* ctx, mrt_scene* scene, const mrt_ray* d_rays, uint64_t n,
+                                       mrt_hit* d_hits, uint64_t counts[2]);
 /* Pinhole rays of a w x h image on the device (camera.slang:45-56), for coherent
  * ray-query workloads. */
 MRT_API mrt_status mrt_camera_rays_dev(mrt_ctx* ctx, const mrt_camera* cam, uint32_t width,

@@ -100,13 +100,15 @@ class PtParams(C.Structure):
     _fields_ = [("spp", C.c_int32), ("max_path_length", C.c_int32),
                 ("emission_on_primary", C.c_int32), ("demodulate_albedo", C.c_int32),
                 ("instance_mask", C.c_uint32), ("mode", C.c_int32),
-                ("record_signature", C.c_int32), ("_reserved", C.c_int32)]
+                ("record_signature", C.c_int32), ("count_traversal", C.c_int32)]
 
 
 class PtStats(C.Structure):
     _fields_ = [("primary_segments", C.c_uint64), ("continuation_segments", C.c_uint64),
                 ("shadow_rays", C.c_uint64), ("ms_gbuffer", C.c_float), ("ms_trace", C.c_float),
-                ("ms_shade", C.c_float), ("ms_total", C.c_float)]
+                ("ms_shade", C.c_float), ("ms_total", C.c_float),
+                ("trace_launches", C.c_uint32), ("_pad", C.c_uint32), ("node_visits", C.c_uint64),
+                ("triangle_tests", C.c_uint64)]
 
 
 assert C.sizeof(SceneDesc) == 128 and C.sizeof(Camera) == 64 and C.sizeof(PtParams) == 32
@@ -140,6 +142,8 @@ EXPORTS = [
     ("mrt_trace_closest", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
     ("mrt_trace_closest_dev", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64,
                                           C.c_void_p, C.c_void_p]),
+    ("mrt_trace_count_dev", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p,
+                                        C.POINTER(C.c_uint64)]),
     ("mrt_camera_rays_dev", C.c_int32, [C.c_void_p, C.POINTER(Camera), C.c_uint32, C.c_uint32,
                                         C.c_void_p, C.c_void_p]),
     ("mrt_gbuffer_dev", C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(Camera), C.c_uint32,

@@ -380,6 +380,24 @@ MRT_API mrt_status mrt_trace_closest_dev(mrt_ctx* c, mrt_scene* s, const mrt_ray
     return e == hipSuccess ? MRT_OK : hip_fail(c, e, "trace kernel");
 }
 
+MRT_API mrt_status mrt_trace_count_dev(mrt_ctx* c, mrt_scene* s, const mrt_ray* rays, uint64_t n, mrt_hit* hits,
+                                       uint64_t counts[2]) {
+    if (!c || !s || !counts || (n && (!rays || !hits))) return fail(c, MRT_ERR_INVALID, "mrt_trace_count_dev: bad argument");
+    mrt_status st = MRT_OK;
+    void* dc = dev_alloc(c, 16, &st);
+    if (st != MRT_OK) return st;
+    hipError_t e = hipMemsetAsync(dc, 0, 16, c->stream);
+    if (e == hipSuccess) e = launch_trace_rays(s->ds, rays, n, hits, c->stream, static_cast<unsigned long long*>(dc));
+    unsigned long long h[2] = {0, 0};
+    if (e == hipSuccess) e = hipMemcpyAsync(h, dc, 16, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    dev_free(c, dc);
+    if (e != hipSuccess) return hip_fail(c, e, "mrt_trace_count_dev");
+    counts[0] = h[0];
+    counts[1] = h[1];
+    return MRT_OK;
+}
+
 MRT_API mrt_status mrt_trace_closest(mrt_ctx* c, mrt_scene* s, const mrt_ray* rays, uint64_t n, mrt_hit* hits) {
     if (!c || !s || (n && (!rays || !hits))) return fail(c, MRT_ERR_INVALID, "mrt_trace_closest: bad argument");
     if (n == 0) return MRT_OK;
@@ -464,9 +482,21 @@ MRT_API mrt_status mrt_pt_render_dev(mrt_ctx* c, mrt_scene* s, const mrt_pt_para
     FrameTiming t;
     uint64_t segs = 0;
     const hipStream_t strm = pick_stream(c, stream);
+    unsigned long long* tc = nullptr;
+    if (prm->count_traversal && stats) {
+        tc = static_cast<unsigned long long*>(dev_alloc(c, 16, &st));
+        if (st != MRT_OK) return st;
+        hipMemsetAsync(tc, 0, 16, strm);
+    }
     const hipError_t e = launch_pt_frame(s->ds, a, gbuffer, pb, reinterpret_cast<float4*>(irradiance),
                                          a.record_signature ? signature : nullptr, strm, stats ? &t : nullptr,
-                                         stats ? &segs : nullptr);
+                                         stats ? &segs : nullptr, tc);
+    unsigned long long hc[2] = {0, 0};
+    if (tc) {
+        hipMemcpyAsync(hc, tc, 16, hipMemcpyDeviceToHost, strm);
+        hipStreamSynchronize(strm);
+        dev_free(c, tc);
+    }
     if (e != hipSuccess) return hip_fail(c, e, "pt frame");
     if (stats) {
         stats->primary_segments = npix;
@@ -475,6 +505,9 @@ MRT_API mrt_status mrt_pt_render_dev(mrt_ctx* c, mrt_scene* s, const mrt_pt_para
         stats->ms_trace = t.ms_trace;
         stats->ms_shade = t.ms_shade;
         stats->ms_total = t.ms_total;
+        stats->trace_launches = t.trace_launches;
+        stats->node_visits = hc[0];
+        stats->triangle_tests = hc[1];
     }
     return MRT_OK;
 }

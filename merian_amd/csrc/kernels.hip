@@ -46,24 +46,45 @@ __device__ __forceinline__ RayF camera_ray(const mrt_camera& c, uint32_t px, uin
     return r;
 }
 
-template <int STACK>
+template <int STACK, bool COUNT = false>
 __device__ __forceinline__ HitRec trace_one(const DeviceScene& S, f3 o, f3 d, float tmin, float tmax, uint32_t mask,
-                                            uint32_t* stack_base) {
-    return traverse<STACK>(DeviceBVH{S.nodes, S.tris}, o, d, tmin, tmax, mask, stack_base + threadIdx.x, blockDim.x);
+                                            uint32_t* stack_base, TraversalCount* count = nullptr) {
+    return traverse<STACK, COUNT>(DeviceBVH{S.nodes, S.tris}, o, d, tmin, tmax, mask, stack_base + threadIdx.x,
+                                  blockDim.x, count);
+}
+
+// Adds this lane's traversal counts to counters[0] (node visits) and counters[1] (triangle
+// tests) with one atomic per wave.
+__device__ __forceinline__ void flush_counts(const TraversalCount& c, unsigned long long* counters) {
+    unsigned long long n = c.nodes, t = c.tris;
+    for (int off = 32; off > 0; off >>= 1) {
+        n += __shfl_xor(n, off);
+        t += __shfl_xor(t, off);
+    }
+    if (__lane_id() == 0) {
+        atomicAdd(counters + 0, n);
+        atomicAdd(counters + 1, t);
+    }
 }
 
 // -----------------------------------------------------------------------------------------
 // ray queries
 
-template <int STACK>
+template <int STACK, bool COUNT>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_rays(DeviceScene S, const mrt_ray* __restrict__ rays,
-                                                            uint64_t n, mrt_hit* __restrict__ hits) {
+                                                            uint64_t n, mrt_hit* __restrict__ hits,
+                                                            unsigned long long* counters) {
     __shared__ uint32_t stack[STACK * TRACE_BLOCK];
     const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
-    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    TraversalCount cnt;
+    // wave-uniform trip count so the per-wave count reduction sees every lane
+    for (uint64_t base = uint64_t(blockIdx.x) * blockDim.x + (threadIdx.x & ~63u); base < n; base += stride) {
+        const uint64_t i = base + __lane_id();
+        if (i >= n) continue;
         const float4* rp = reinterpret_cast<const float4*>(rays + i);
         const float4 r0 = rp[0], r1 = rp[1];
-        const HitRec h = trace_one<STACK>(S, f3{r0.x, r0.y, r0.z}, f3{r1.x, r1.y, r1.z}, r0.w, r1.w, 0xFFu, stack);
+        const HitRec h =
+            trace_one<STACK, COUNT>(S, f3{r0.x, r0.y, r0.z}, f3{r1.x, r1.y, r1.z}, r0.w, r1.w, 0xFFu, stack, &cnt);
         float4 o;
         o.x = __uint_as_float(h.id);
         o.y = h.id == TRAVERSAL_DONE ? __builtin_inff() : h.t;
@@ -71,6 +92,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_rays(DeviceScene S, const
         o.w = h.id == TRAVERSAL_DONE ? 0.f : h.v;
         reinterpret_cast<float4*>(hits)[i] = o;
     }
+    if constexpr (COUNT) flush_counts(cnt, counters);
 }
 
 __global__ void k_camera_rays(mrt_camera cam, uint32_t w, uint32_t h, mrt_ray* __restrict__ rays) {
@@ -291,19 +313,25 @@ __global__ void k_pt_init(DeviceScene S, PtArgs A, const mrt_packed_hit* __restr
     }
 }
 
-template <int STACK>
+template <int STACK, bool COUNT>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_queue(DeviceScene S, const float4* __restrict__ qo,
                                                              const float4* __restrict__ qd,
                                                              const uint32_t* __restrict__ count,
-                                                             float4* __restrict__ hits, uint32_t mask) {
+                                                             float4* __restrict__ hits, uint32_t mask,
+                                                             unsigned long long* counters) {
     __shared__ uint32_t stack[STACK * TRACE_BLOCK];
     const uint32_t n = *count;
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    TraversalCount cnt;
+    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < n; base += stride) {
+        const uint32_t i = base + __lane_id();
+        if (i >= n) continue;
         const float4 o = qo[i], d = qd[i];
-        const HitRec h = trace_one<STACK>(S, f3{o.x, o.y, o.z}, f3{d.x, d.y, d.z}, 0.f, SCENE_RAY_TMAX, mask, stack);
+        const HitRec h = trace_one<STACK, COUNT>(S, f3{o.x, o.y, o.z}, f3{d.x, d.y, d.z}, 0.f, SCENE_RAY_TMAX, mask,
+                                                 stack, &cnt);
         hits[i] = make_float4(h.t, h.u, h.v, __uint_as_float(h.id));
     }
+    if constexpr (COUNT) flush_counts(cnt, counters);
 }
 
 // Shades vertex `bounce` of every queued path and appends its continuation ray
@@ -429,12 +457,21 @@ constexpr int MAX_BLOCKS = 256 * 16; // 256 CUs
     default: CALL(128); break;                                                       \
     }
 
-hipError_t launch_trace_rays(const DeviceScene& s, const mrt_ray* rays, uint64_t n, mrt_hit* hits, hipStream_t st) {
+hipError_t launch_trace_rays(const DeviceScene& s, const mrt_ray* rays, uint64_t n, mrt_hit* hits, hipStream_t st,
+                             unsigned long long* counters) {
     if (n == 0) return hipSuccess;
     const int g = grid_for(n, TRACE_BLOCK, MAX_BLOCKS);
-#define CALL(K) hipLaunchKernelGGL(k_trace_rays<K>, dim3(g), dim3(TRACE_BLOCK), 0, st, s, rays, n, hits)
-    MRT_DISPATCH_STACK(s, CALL)
+    if (counters) {
+#define CALL(K) \
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_trace_rays<K, true>), dim3(g), dim3(TRACE_BLOCK), 0, st, s, rays, n, hits, counters)
+        MRT_DISPATCH_STACK(s, CALL)
 #undef CALL
+    } else {
+#define CALL(K) \
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_trace_rays<K, false>), dim3(g), dim3(TRACE_BLOCK), 0, st, s, rays, n, hits, nullptr)
+        MRT_DISPATCH_STACK(s, CALL)
+#undef CALL
+    }
     return hipGetLastError();
 }
 
@@ -458,7 +495,7 @@ hipError_t launch_gbuffer(const DeviceScene& s, const mrt_camera& cam, uint32_t 
 
 hipError_t launch_pt_frame(const DeviceScene& s, PtArgs a, const mrt_packed_hit* gbuffer, PathBuffers& pb,
                            float4* out, uint32_t* sig_out, hipStream_t st, FrameTiming* timing,
-                           uint64_t* continuation_segments) {
+                           uint64_t* continuation_segments, unsigned long long* trav_counters) {
     const int L = a.max_path_length;
     const uint32_t spp = uint32_t(a.spp > 0 ? a.spp : 0);
     if (spp == 0 || a.npix == 0) {
@@ -471,12 +508,18 @@ hipError_t launch_pt_frame(const DeviceScene& s, PtArgs a, const mrt_packed_hit*
     if (chunk == 0) return hipErrorInvalidValue;
     const uint32_t n_chunks = (spp + chunk - 1) / chunk;
     const uint32_t per_chunk = uint32_t(L + 1);
-    hipEvent_t ev[4] = {};
-    if (timing) {
-        for (auto& e : ev) hipEventCreate(&e);
-    }
-    float ms_trace = 0.f, ms_shade = 0.f;
-    if (timing) hipEventRecord(ev[0], st);
+    // timing: one event before and after every launch, read once the frame has completed
+    std::vector<hipEvent_t> ev;
+    std::vector<int> kind; // per event pair: 0 trace, 1 shade
+    auto mark = [&](int k) {
+        if (!timing) return;
+        hipEvent_t e;
+        hipEventCreate(&e);
+        hipEventRecord(e, st);
+        ev.push_back(e);
+        if (k >= 0) kind.push_back(k);
+    };
+    mark(-1);
     const uint32_t nz = n_chunks * per_chunk;
     hipLaunchKernelGGL(k_zero_u32, dim3((nz + 255) / 256), dim3(256), 0, st, pb.counters, nz);
     for (uint32_t c = 0; c < n_chunks; c++) {
@@ -489,42 +532,47 @@ hipError_t launch_pt_frame(const DeviceScene& s, PtArgs a, const mrt_packed_hit*
         for (int b = 0; b < L; b++) {
             const int qi = b & 1;
             if (b > 0) {
-                if (timing) hipEventRecord(ev[1], st);
+                mark(-1);
                 const int g = grid_for(a.n_paths, TRACE_BLOCK, MAX_BLOCKS);
-#define CALL(K)                                                                                                  \
-    hipLaunchKernelGGL(k_trace_queue<K>, dim3(g), dim3(TRACE_BLOCK), 0, st, s, pb.q_o[qi], pb.q_d[qi], cnt + b, \
-                       pb.hits, a.instance_mask)
-                MRT_DISPATCH_STACK(s, CALL)
+                if (trav_counters) {
+#define CALL(K)                                                                                                \
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_trace_queue<K, true>), dim3(g), dim3(TRACE_BLOCK), 0, st, s, pb.q_o[qi], \
+                       pb.q_d[qi], cnt + b, pb.hits, a.instance_mask, trav_counters)
+                    MRT_DISPATCH_STACK(s, CALL)
 #undef CALL
-                if (timing) {
-                    hipEventRecord(ev[2], st);
-                    hipEventSynchronize(ev[2]);
-                    float ms = 0.f;
-                    hipEventElapsedTime(&ms, ev[1], ev[2]);
-                    ms_trace += ms;
+                } else {
+#define CALL(K)                                                                                                 \
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_trace_queue<K, false>), dim3(g), dim3(TRACE_BLOCK), 0, st, s, pb.q_o[qi], \
+                       pb.q_d[qi], cnt + b, pb.hits, a.instance_mask, nullptr)
+                    MRT_DISPATCH_STACK(s, CALL)
+#undef CALL
                 }
+                mark(0);
             }
-            if (timing) hipEventRecord(ev[1], st);
+            mark(-1);
             hipLaunchKernelGGL(k_shade, dim3(grid_for(a.n_paths, SHADE_BLOCK, MAX_BLOCKS)), dim3(SHADE_BLOCK), 0, st,
                                s, a, b, pb, pb.q_o[qi], pb.q_d[qi], cnt + b, pb.q_o[qi ^ 1], pb.q_d[qi ^ 1],
                                cnt + b + 1);
-            if (timing) {
-                hipEventRecord(ev[2], st);
-                hipEventSynchronize(ev[2]);
-                float ms = 0.f;
-                hipEventElapsedTime(&ms, ev[1], ev[2]);
-                ms_shade += ms;
-            }
+            mark(1);
         }
         hipLaunchKernelGGL(k_accumulate, dim3(grid_for(a.npix, 256, MAX_BLOCKS)), dim3(256), 0, st, a, pb,
                            int(c == 0), int(c + 1 == n_chunks), out, sig_out);
     }
+    mark(-1);
     if (timing) {
-        hipEventRecord(ev[3], st);
-        hipEventSynchronize(ev[3]);
-        hipEventElapsedTime(&timing->ms_total, ev[0], ev[3]);
+        hipEventSynchronize(ev.back());
+        float ms_trace = 0.f, ms_shade = 0.f;
+        // events: [frame start] then per launch (start, end) pairs, then [frame end]
+        for (size_t k = 0; k < kind.size(); k++) {
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, ev[1 + 2 * k], ev[2 + 2 * k]);
+            (kind[k] == 0 ? ms_trace : ms_shade) += ms;
+        }
+        hipEventElapsedTime(&timing->ms_total, ev.front(), ev.back());
         timing->ms_trace = ms_trace;
         timing->ms_shade = ms_shade;
+        timing->trace_launches = 0;
+        for (int k : kind) timing->trace_launches += (k == 0);
         for (auto& e : ev) hipEventDestroy(e);
     }
     if (continuation_segments) {

@@ -55,18 +55,21 @@ struct PathBuffers {
 };
 
 // Launch entry points (kernels.hip). All asynchronous on `stream`.
-hipError_t launch_trace_rays(const DeviceScene& s, const mrt_ray* rays, uint64_t n, mrt_hit* hits, hipStream_t st);
+// counters (optional, device): node visits and triangle tests are added to counters[0..1]
+hipError_t launch_trace_rays(const DeviceScene& s, const mrt_ray* rays, uint64_t n, mrt_hit* hits, hipStream_t st,
+                             unsigned long long* counters = nullptr);
 hipError_t launch_camera_rays(const mrt_camera& cam, uint32_t w, uint32_t h, mrt_ray* rays, hipStream_t st);
 hipError_t launch_gbuffer(const DeviceScene& s, const mrt_camera& cam, uint32_t w, uint32_t h, uint32_t mask,
                           mrt_packed_hit* out, hipStream_t st);
 
 struct FrameTiming {
     float ms_trace = 0.f, ms_shade = 0.f, ms_total = 0.f;
+    uint32_t trace_launches = 0;
 };
 // One frame of pt over `gbuffer`; out = float4 irradiance per pixel; sig_out optional.
 // counters_host (optional) receives every chunk's queue sizes after the frame completes.
 hipError_t launch_pt_frame(const DeviceScene& s, PtArgs a, const mrt_packed_hit* gbuffer, PathBuffers& pb,
                            float4* out, uint32_t* sig_out, hipStream_t st, FrameTiming* timing,
-                           uint64_t* continuation_segments);
+                           uint64_t* continuation_segments, unsigned long long* trav_counters = nullptr);
 
 } // namespace mrt

@@ -64,10 +64,15 @@ __device__ __forceinline__ void cswap(float& ka, uint32_t& ea, float& kb, uint32
 
 constexpr uint32_t TRAVERSAL_DONE = 0xFFFFFFFFu;
 
+// Work counters of one traversal (COUNT variant only): node visits and triangle tests.
+struct TraversalCount {
+    uint32_t nodes = 0, tris = 0;
+};
+
 // stack: this lane's column base in LDS; stride = threads per block.
-template <int STACK>
+template <int STACK, bool COUNT = false>
 __device__ __forceinline__ HitRec traverse(const DeviceBVH bvh, f3 o, f3 d, float tmin, float tmax, uint32_t ray_mask,
-                                           uint32_t* stack, int stride) {
+                                           uint32_t* stack, int stride, TraversalCount* count = nullptr) {
     HitRec best{TRAVERSAL_DONE, __builtin_inff(), 0.f, 0.f};
     const float dx = fabsf(d.x) < 1e-20f ? copysignf(1e-20f, d.x) : d.x;
     const float dy = fabsf(d.y) < 1e-20f ? copysignf(1e-20f, d.y) : d.y;
@@ -83,8 +88,10 @@ __device__ __forceinline__ HitRec traverse(const DeviceBVH bvh, f3 o, f3 d, floa
             const uint32_t first = (cur & 0x7FFFFFFFu) >> 3;
             const uint32_t cnt = (cur & 7u) + 1u;
             for (uint32_t k = 0; k < cnt; k++) test_triangle(bvh.tris, first + k, o, d, tmin, tmax, ray_mask, best);
+            if constexpr (COUNT) count->tris += cnt;
             cur = sp > 0 ? stack[(--sp) * stride] : TRAVERSAL_DONE;
         } else {
+            if constexpr (COUNT) count->nodes++;
             const float4* np = reinterpret_cast<const float4*>(bvh.nodes + cur);
             const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
             const int4 ch = reinterpret_cast<const int4*>(np)[6];
