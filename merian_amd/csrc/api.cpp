@@ -93,7 +93,7 @@ mrt_status ensure_path_buffers(mrt_ctx* c, uint64_t paths, uint64_t npix, uint64
         p.q_d[k] = static_cast<float4*>(A(16 * paths));
     }
     p.hits = static_cast<float4*>(A(16 * paths));
-    p.counters = static_cast<uint32_t*>(A(4 * counters));
+    p.counters = static_cast<uint32_t*>(A(2 * 4 * counters));
     p.accum = static_cast<float4*>(A(16 * npix));
     p.sig_accum = static_cast<uint32_t*>(A(4 * npix));
     if (st != MRT_OK) {
@@ -105,6 +105,29 @@ mrt_status ensure_path_buffers(mrt_ctx* c, uint64_t paths, uint64_t npix, uint64
     p.capacity = paths;
     c->paths_npix = npix;
     c->paths_counters = counters;
+    return MRT_OK;
+}
+
+// Global part of the traversal stacks, sized for the deepest scene traced so far.
+mrt_status ensure_trace_scratch(mrt_ctx* c, const mrt_scene* s) {
+    const uint32_t need = s->ds.stack_need > uint32_t(TRACE_LDS_STACK) ? s->ds.stack_need - TRACE_LDS_STACK : 1u;
+    TraceScratch& t = c->trace;
+    if (t.work && t.entries >= need) return MRT_OK;
+    if (t.ovf) dev_free(c, t.ovf);
+    if (t.work) dev_free(c, t.work);
+    t = TraceScratch{};
+    const uint32_t threads = trace_max_threads();
+    mrt_status st = MRT_OK;
+    t.ovf = static_cast<uint32_t*>(dev_alloc(c, size_t(threads) * need * 4, &st));
+    if (st != MRT_OK) return st;
+    t.work = static_cast<uint32_t*>(dev_alloc(c, 64, &st));
+    if (st != MRT_OK) {
+        dev_free(c, t.ovf);
+        t = TraceScratch{};
+        return st;
+    }
+    t.threads = threads;
+    t.entries = need;
     return MRT_OK;
 }
 
@@ -232,13 +255,8 @@ MRT_API mrt_status mrt_scene_create(mrt_ctx* c, const mrt_scene_desc* d, mrt_sce
         }
     }
     const BVH4 bvh = build_bvh4(v0.data(), e1.data(), e2.data(), flags.data(), uint32_t(n_tris), 0);
-    const uint32_t need = 3u * bvh.depth + 4u;
     mrt_scene* s = new mrt_scene();
-    s->ds.stack_kind = need <= 32 ? 0 : need <= 64 ? 1 : need <= 128 ? 2 : 3;
-    if (s->ds.stack_kind == 3) {
-        delete s;
-        return fail(c, MRT_ERR_UNSUPPORTED, "BVH deeper than the 128-entry traversal stack allows");
-    }
+    s->ds.stack_need = 3u * bvh.depth + 4u;
     s->n_tris = n_tris;
     s->n_nodes = bvh.nodes.size();
     s->depth = bvh.depth;
@@ -376,18 +394,22 @@ MRT_API mrt_status mrt_camera_look_at(const float eye[3], const float target[3],
 MRT_API mrt_status mrt_trace_closest_dev(mrt_ctx* c, mrt_scene* s, const mrt_ray* rays, uint64_t n, mrt_hit* hits,
                                          void* stream) {
     if (!c || !s || (n && (!rays || !hits))) return fail(c, MRT_ERR_INVALID, "mrt_trace_closest_dev: bad argument");
-    const hipError_t e = launch_trace_rays(s->ds, rays, n, hits, pick_stream(c, stream));
+    mrt_status st = ensure_trace_scratch(c, s);
+    if (st != MRT_OK) return st;
+    const hipError_t e = launch_trace_rays(s->ds, c->trace, rays, n, hits, pick_stream(c, stream));
     return e == hipSuccess ? MRT_OK : hip_fail(c, e, "trace kernel");
 }
 
 MRT_API mrt_status mrt_trace_count_dev(mrt_ctx* c, mrt_scene* s, const mrt_ray* rays, uint64_t n, mrt_hit* hits,
                                        uint64_t counts[2]) {
     if (!c || !s || !counts || (n && (!rays || !hits))) return fail(c, MRT_ERR_INVALID, "mrt_trace_count_dev: bad argument");
-    mrt_status st = MRT_OK;
+    mrt_status st = ensure_trace_scratch(c, s);
+    if (st != MRT_OK) return st;
     void* dc = dev_alloc(c, 16, &st);
     if (st != MRT_OK) return st;
     hipError_t e = hipMemsetAsync(dc, 0, 16, c->stream);
-    if (e == hipSuccess) e = launch_trace_rays(s->ds, rays, n, hits, c->stream, static_cast<unsigned long long*>(dc));
+    if (e == hipSuccess)
+        e = launch_trace_rays(s->ds, c->trace, rays, n, hits, c->stream, static_cast<unsigned long long*>(dc));
     unsigned long long h[2] = {0, 0};
     if (e == hipSuccess) e = hipMemcpyAsync(h, dc, 16, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -401,7 +423,8 @@ MRT_API mrt_status mrt_trace_count_dev(mrt_ctx* c, mrt_scene* s, const mrt_ray* 
 MRT_API mrt_status mrt_trace_closest(mrt_ctx* c, mrt_scene* s, const mrt_ray* rays, uint64_t n, mrt_hit* hits) {
     if (!c || !s || (n && (!rays || !hits))) return fail(c, MRT_ERR_INVALID, "mrt_trace_closest: bad argument");
     if (n == 0) return MRT_OK;
-    mrt_status st = MRT_OK;
+    mrt_status st = ensure_trace_scratch(c, s);
+    if (st != MRT_OK) return st;
     void* dr = dev_alloc(c, n * sizeof(mrt_ray), &st);
     if (st != MRT_OK) return st;
     void* dh = dev_alloc(c, n * sizeof(mrt_hit), &st);
@@ -410,7 +433,8 @@ MRT_API mrt_status mrt_trace_closest(mrt_ctx* c, mrt_scene* s, const mrt_ray* ra
         return st;
     }
     hipError_t e = hipMemcpyAsync(dr, rays, n * sizeof(mrt_ray), hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) e = launch_trace_rays(s->ds, static_cast<mrt_ray*>(dr), n, static_cast<mrt_hit*>(dh), c->stream);
+    if (e == hipSuccess)
+        e = launch_trace_rays(s->ds, c->trace, static_cast<mrt_ray*>(dr), n, static_cast<mrt_hit*>(dh), c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(hits, dh, n * sizeof(mrt_hit), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     dev_free(c, dr);
@@ -428,7 +452,9 @@ MRT_API mrt_status mrt_camera_rays_dev(mrt_ctx* c, const mrt_camera* cam, uint32
 MRT_API mrt_status mrt_gbuffer_dev(mrt_ctx* c, mrt_scene* s, const mrt_camera* cam, uint32_t w, uint32_t h,
                                    uint32_t mask, mrt_packed_hit* out, void* stream) {
     if (!c || !s || !cam || (uint64_t(w) * h && !out)) return fail(c, MRT_ERR_INVALID, "mrt_gbuffer_dev: bad argument");
-    const hipError_t e = launch_gbuffer(s->ds, *cam, w, h, mask & 0xFFu, out, pick_stream(c, stream));
+    const mrt_status st = ensure_trace_scratch(c, s);
+    if (st != MRT_OK) return st;
+    const hipError_t e = launch_gbuffer(s->ds, c->trace, *cam, w, h, mask & 0xFFu, out, pick_stream(c, stream));
     return e == hipSuccess ? MRT_OK : hip_fail(c, e, "gbuffer kernel");
 }
 
@@ -465,6 +491,8 @@ MRT_API mrt_status mrt_pt_render_dev(mrt_ctx* c, mrt_scene* s, const mrt_pt_para
     const uint64_t n_chunks = spp ? (spp + chunk - 1) / chunk : 1;
     st = ensure_path_buffers(c, npix * chunk, npix, n_chunks * uint64_t(prm->max_path_length + 1));
     if (st != MRT_OK) return st;
+    st = ensure_trace_scratch(c, s);
+    if (st != MRT_OK) return st;
     // capacity may exceed this frame's need; the frame uses exactly `chunk` samples per pass
     PathBuffers pb = c->paths;
     pb.capacity = npix * chunk;
@@ -488,7 +516,7 @@ MRT_API mrt_status mrt_pt_render_dev(mrt_ctx* c, mrt_scene* s, const mrt_pt_para
         if (st != MRT_OK) return st;
         hipMemsetAsync(tc, 0, 16, strm);
     }
-    const hipError_t e = launch_pt_frame(s->ds, a, gbuffer, pb, reinterpret_cast<float4*>(irradiance),
+    const hipError_t e = launch_pt_frame(s->ds, c->trace, a, gbuffer, pb, reinterpret_cast<float4*>(irradiance),
                                          a.record_signature ? signature : nullptr, strm, stats ? &t : nullptr,
                                          stats ? &segs : nullptr, tc);
     unsigned long long hc[2] = {0, 0};

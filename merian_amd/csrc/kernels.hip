@@ -23,6 +23,18 @@ constexpr float THROUGHPUT_THRESHOLD = 1e-5f; // scene/scene.slang:24
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
+// Streamed path data (queues, hits, path state) is touched once per bounce; non-temporal
+// accesses keep it from evicting BVH nodes and triangles from L2 and the Infinity Cache.
+typedef float v4f __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 ld_stream(const float4* p) {
+    const v4f v = __builtin_nontemporal_load(reinterpret_cast<const v4f*>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void st_stream(float4* p, float4 v) {
+    const v4f t = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(t, reinterpret_cast<v4f*>(p));
+}
+
 // -----------------------------------------------------------------------------------------
 // camera (camera.slang:45-56)
 
@@ -46,12 +58,72 @@ __device__ __forceinline__ RayF camera_ray(const mrt_camera& c, uint32_t px, uin
     return r;
 }
 
-template <int STACK, bool COUNT = false>
-__device__ __forceinline__ HitRec trace_one(const DeviceScene& S, f3 o, f3 d, float tmin, float tmax, uint32_t mask,
-                                            uint32_t* stack_base, TraversalCount* count = nullptr) {
-    return traverse<STACK, COUNT>(DeviceBVH{S.nodes, S.tris}, o, d, tmin, tmax, mask, stack_base + threadIdx.x,
-                                  blockDim.x, count);
-}
+// Ray sources and hit sinks of the persistent traversal loop.
+struct SrcRays { // mrt_ray array: (origin, tmin), (direction, tmax)
+    const mrt_ray* rays;
+    __device__ __forceinline__ void load(uint32_t i, f3& o, f3& d, float& tmin, float& tmax) const {
+        const float4* rp = reinterpret_cast<const float4*>(rays + i);
+        const float4 a = rp[0], b = rp[1];
+        o = f3{a.x, a.y, a.z};
+        d = f3{b.x, b.y, b.z};
+        tmin = a.w;
+        tmax = b.w;
+    }
+};
+struct SrcQueue { // path queue: (origin, path id), (direction, -); continuation rays [0, 1e7]
+    const float4* qo;
+    const float4* qd;
+    __device__ __forceinline__ void load(uint32_t i, f3& o, f3& d, float& tmin, float& tmax) const {
+        const float4 a = ld_stream(qo + i), b = ld_stream(qd + i);
+        o = f3{a.x, a.y, a.z};
+        d = f3{b.x, b.y, b.z};
+        tmin = 0.f;
+        tmax = SCENE_RAY_TMAX;
+    }
+};
+struct SrcCamera { // primary rays of a w x h image
+    mrt_camera cam;
+    uint32_t w, h;
+    __device__ __forceinline__ void load(uint32_t i, f3& o, f3& d, float& tmin, float& tmax) const {
+        const RayF r = camera_ray(cam, i % w, i / w, w, h);
+        o = r.o;
+        d = r.d;
+        tmin = r.tmin;
+        tmax = r.tmax;
+    }
+};
+struct SinkHits { // mrt_hit array
+    mrt_hit* hits;
+    __device__ __forceinline__ void store(uint32_t i, const HitRec& h) const {
+        const bool miss = h.id == TRAVERSAL_DONE;
+        reinterpret_cast<float4*>(hits)[i] = make_float4(__uint_as_float(h.id), miss ? __builtin_inff() : h.t,
+                                                         miss ? 0.f : h.u, miss ? 0.f : h.v);
+    }
+};
+struct SinkQueue { // hits of the path queue: (t, u, v, id)
+    float4* hits;
+    __device__ __forceinline__ void store(uint32_t i, const HitRec& h) const {
+        st_stream(hits + i, make_float4(h.t, h.u, h.v, __uint_as_float(h.id)));
+    }
+};
+struct SinkGBuffer { // PackedHitInfo (gbuffer.slang:175-196); one pretransformed group
+    const uint4* tri_verts;
+    const uint32_t* geom_first;
+    mrt_packed_hit* out;
+    __device__ __forceinline__ void store(uint32_t i, const HitRec& h) const {
+        uint4 p;
+        if (h.id == TRAVERSAL_DONE) {
+            p = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u);
+        } else {
+            const uint32_t g = tri_verts[h.id].w;
+            p.x = 0u;                           // instance_id
+            p.y = h.id - geom_first[g];         // primitive_id within the geometry
+            p.z = (g & 0xFFFFu);                // geometry_index | instance_index << 16
+            p.w = uint32_t(float_to_half(h.u)) | (uint32_t(float_to_half(h.v)) << 16);
+        }
+        reinterpret_cast<uint4*>(out)[i] = p;
+    }
+};
 
 // Adds this lane's traversal counts to counters[0] (node visits) and counters[1] (triangle
 // tests) with one atomic per wave.
@@ -70,28 +142,20 @@ __device__ __forceinline__ void flush_counts(const TraversalCount& c, unsigned l
 // -----------------------------------------------------------------------------------------
 // ray queries
 
-template <int STACK, bool COUNT>
-__global__ __launch_bounds__(TRACE_BLOCK) void k_trace_rays(DeviceScene S, const mrt_ray* __restrict__ rays,
-                                                            uint64_t n, mrt_hit* __restrict__ hits,
-                                                            unsigned long long* counters) {
-    __shared__ uint32_t stack[STACK * TRACE_BLOCK];
-    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+// Closest-hit kernel for every ray stream (ray queries, the GBuffer's primary rays, the path
+// queue): a persistent grid, one ray per lane with refill, LDS stack + global overflow.
+// Ray count: *count_ptr when given (device-side queue size), else n_fixed.
+template <bool COUNT, class Src, class Sink>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DeviceScene S, Src src, Sink sink,
+                                                       const uint32_t* __restrict__ count_ptr, uint32_t n_fixed,
+                                                       uint32_t* __restrict__ work, uint32_t mask, TraceScratch ts,
+                                                       unsigned long long* counters) {
+    __shared__ uint32_t stack[TRACE_LDS_STACK * TRACE_BLOCK];
+    const uint32_t n = count_ptr ? *count_ptr : n_fixed;
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const LaneStack stk{stack + threadIdx.x, blockDim.x, ts.ovf + gtid, gridDim.x * blockDim.x};
     TraversalCount cnt;
-    // wave-uniform trip count so the per-wave count reduction sees every lane
-    for (uint64_t base = uint64_t(blockIdx.x) * blockDim.x + (threadIdx.x & ~63u); base < n; base += stride) {
-        const uint64_t i = base + __lane_id();
-        if (i >= n) continue;
-        const float4* rp = reinterpret_cast<const float4*>(rays + i);
-        const float4 r0 = rp[0], r1 = rp[1];
-        const HitRec h =
-            trace_one<STACK, COUNT>(S, f3{r0.x, r0.y, r0.z}, f3{r1.x, r1.y, r1.z}, r0.w, r1.w, 0xFFu, stack, &cnt);
-        float4 o;
-        o.x = __uint_as_float(h.id);
-        o.y = h.id == TRAVERSAL_DONE ? __builtin_inff() : h.t;
-        o.z = h.id == TRAVERSAL_DONE ? 0.f : h.u;
-        o.w = h.id == TRAVERSAL_DONE ? 0.f : h.v;
-        reinterpret_cast<float4*>(hits)[i] = o;
-    }
+    trace_persistent<TRACE_LDS_STACK, COUNT>(DeviceBVH{S.nodes, S.tris}, src, sink, n, work, mask, stk, cnt);
     if constexpr (COUNT) flush_counts(cnt, counters);
 }
 
@@ -103,33 +167,6 @@ __global__ void k_camera_rays(mrt_camera cam, uint32_t w, uint32_t h, mrt_ray* _
         float4* o = reinterpret_cast<float4*>(rays + i);
         o[0] = make_float4(r.o.x, r.o.y, r.o.z, r.tmin);
         o[1] = make_float4(r.d.x, r.d.y, r.d.z, r.tmax);
-    }
-}
-
-// -----------------------------------------------------------------------------------------
-// GBuffer (Raytraced): primary ray + closest hit packed as PackedHitInfo
-// (gbuffer_rt.slang:18-94, gbuffer.slang:175-196)
-
-template <int STACK>
-__global__ __launch_bounds__(TRACE_BLOCK) void k_gbuffer(DeviceScene S, mrt_camera cam, uint32_t w, uint32_t h,
-                                                         uint32_t mask, mrt_packed_hit* __restrict__ out) {
-    __shared__ uint32_t stack[STACK * TRACE_BLOCK];
-    const uint64_t n = uint64_t(w) * h;
-    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
-    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const RayF r = camera_ray(cam, uint32_t(i % w), uint32_t(i / w), w, h);
-        const HitRec hr = trace_one<STACK>(S, r.o, r.d, r.tmin, r.tmax, mask, stack);
-        uint4 p;
-        if (hr.id == TRAVERSAL_DONE) {
-            p = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u);
-        } else {
-            const uint32_t g = S.tri_verts[hr.id].w;
-            p.x = 0u;                          // instance_id: the one pretransformed group
-            p.y = hr.id - S.geom_first[g];     // primitive_id within the geometry
-            p.z = (g & 0xFFFFu);               // geometry_index | instance_index << 16
-            p.w = uint32_t(float_to_half(hr.u)) | (uint32_t(float_to_half(hr.v)) << 16);
-        }
-        reinterpret_cast<uint4*>(out)[i] = p;
     }
 }
 
@@ -304,34 +341,13 @@ __global__ void k_pt_init(DeviceScene S, PtArgs A, const mrt_packed_hit* __restr
             bu = half_to_float(uint16_t(gb.w & 0xFFFFu));
             bv = half_to_float(uint16_t(gb.w >> 16));
         }
-        P.q_o[0][i] = make_float4(r.o.x, r.o.y, r.o.z, __uint_as_float(i));
-        P.q_d[0][i] = make_float4(r.d.x, r.d.y, r.d.z, 0.f);
-        P.hits[i] = make_float4(0.f, bu, bv, __uint_as_float(gid));
-        P.state_a[i] = make_float4(1.f, 1.f, 1.f, 1.f);
-        P.state_b[i] = make_float4(0.f, 0.f, 0.f, __uint_as_float(pcg4d16(x, y, A.frame, s)));
+        st_stream(P.q_o[0] + i, make_float4(r.o.x, r.o.y, r.o.z, __uint_as_float(i)));
+        st_stream(P.q_d[0] + i, make_float4(r.d.x, r.d.y, r.d.z, 0.f));
+        st_stream(P.hits + i, make_float4(0.f, bu, bv, __uint_as_float(gid)));
+        st_stream(P.state_a + i, make_float4(1.f, 1.f, 1.f, 1.f));
+        st_stream(P.state_b + i, make_float4(0.f, 0.f, 0.f, __uint_as_float(pcg4d16(x, y, A.frame, s))));
         if (A.record_signature) P.sig[i] = sig_step(0x9E3779B9u * (s + 1u), gid);
     }
-}
-
-template <int STACK, bool COUNT>
-__global__ __launch_bounds__(TRACE_BLOCK) void k_trace_queue(DeviceScene S, const float4* __restrict__ qo,
-                                                             const float4* __restrict__ qd,
-                                                             const uint32_t* __restrict__ count,
-                                                             float4* __restrict__ hits, uint32_t mask,
-                                                             unsigned long long* counters) {
-    __shared__ uint32_t stack[STACK * TRACE_BLOCK];
-    const uint32_t n = *count;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    TraversalCount cnt;
-    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < n; base += stride) {
-        const uint32_t i = base + __lane_id();
-        if (i >= n) continue;
-        const float4 o = qo[i], d = qd[i];
-        const HitRec h = trace_one<STACK, COUNT>(S, f3{o.x, o.y, o.z}, f3{d.x, d.y, d.z}, 0.f, SCENE_RAY_TMAX, mask,
-                                                 stack, &cnt);
-        hits[i] = make_float4(h.t, h.u, h.v, __uint_as_float(h.id));
-    }
-    if constexpr (COUNT) flush_counts(cnt, counters);
 }
 
 // Shades vertex `bounce` of every queued path and appends its continuation ray
@@ -350,11 +366,11 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(DeviceScene S, PtArgs A, 
         f3 next_o{0, 0, 0}, next_d{0, 0, 0};
         uint32_t pid = 0;
         if (i < n) {
-            const float4 o4 = qo[i], d4 = qd[i], h4 = P.hits[i];
+            const float4 o4 = ld_stream(qo + i), d4 = ld_stream(qd + i), h4 = ld_stream(P.hits + i);
             pid = __float_as_uint(o4.w);
             const f3 ro{o4.x, o4.y, o4.z}, rd{d4.x, d4.y, d4.z};
             const uint32_t gid = __float_as_uint(h4.w);
-            float4 sa = P.state_a[pid], sb = P.state_b[pid];
+            float4 sa = ld_stream(P.state_a + pid), sb = ld_stream(P.state_b + pid);
             f3 thr{sa.x, sa.y, sa.z};
             float ior = sa.w;
             f3 rad{sb.x, sb.y, sb.z};
@@ -386,8 +402,8 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(DeviceScene S, PtArgs A, 
                     }
                 }
             }
-            P.state_a[pid] = make_float4(thr.x, thr.y, thr.z, ior);
-            P.state_b[pid] = make_float4(rad.x, rad.y, rad.z, __uint_as_float(rng.state));
+            st_stream(P.state_a + pid, make_float4(thr.x, thr.y, thr.z, ior));
+            st_stream(P.state_b + pid, make_float4(rad.x, rad.y, rad.z, __uint_as_float(rng.state)));
         }
         const uint64_t ballot = __ballot(push);
         if (ballot) {
@@ -397,8 +413,8 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(DeviceScene S, PtArgs A, 
             slot = __shfl(slot, leader);
             if (push) {
                 const uint32_t k = slot + uint32_t(__popcll(ballot & ((1ull << lane) - 1ull)));
-                no[k] = make_float4(next_o.x, next_o.y, next_o.z, __uint_as_float(pid));
-                nd[k] = make_float4(next_d.x, next_d.y, next_d.z, 0.f);
+                st_stream(no + k, make_float4(next_o.x, next_o.y, next_o.z, __uint_as_float(pid)));
+                st_stream(nd + k, make_float4(next_d.x, next_d.y, next_d.z, 0.f));
             }
         }
     }
@@ -416,7 +432,7 @@ __global__ void k_accumulate(PtArgs A, PathBuffers P, int first_chunk, int last_
         float m2 = acc.w;
         for (uint32_t sl = 0; sl < A.chunk; sl++) {
             const uint32_t i = sl * A.npix + p;
-            const float4 b = P.state_b[i];
+            const float4 b = ld_stream(P.state_b + i);
             const f3 c{b.x, b.y, b.z};
             if (A.record_signature) sa += P.sig[i];
             if (finite3(c)) {
@@ -446,33 +462,56 @@ int grid_for(uint64_t n, int block, int max_blocks) {
 
 constexpr int MAX_BLOCKS = 256 * 16; // 256 CUs
 
+// Blocks of k_trace<...> that fit on the device at once (the persistent grid).
+template <bool COUNT, class Src, class Sink> int trace_resident_blocks() {
+    static int blocks = 0;
+    if (blocks == 0) {
+        int per_cu = 0, dev = 0, cus = 0;
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<COUNT, Src, Sink>, TRACE_BLOCK, 0);
+        hipGetDevice(&dev);
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        blocks = per_cu * cus > 0 ? per_cu * cus : 1;
+    }
+    return blocks;
+}
+
+// Launches the persistent trace kernel over at most n_max rays (the actual count is *count_ptr
+// when given). `work` must be zero on entry.
+template <bool COUNT, class Src, class Sink>
+hipError_t launch_trace(const DeviceScene& s, const TraceScratch& ts, const Src& src, const Sink& sink,
+                        const uint32_t* count_ptr, uint32_t n_max, uint32_t* work, uint32_t mask,
+                        unsigned long long* counters, hipStream_t st) {
+    if (n_max == 0) return hipSuccess;
+    if (s.stack_need > TRACE_LDS_STACK + ts.entries) return hipErrorInvalidValue; // overflow area too small
+    int g = grid_for(n_max, TRACE_BLOCK, trace_resident_blocks<COUNT, Src, Sink>());
+    if (uint64_t(g) * TRACE_BLOCK > ts.threads) g = int(ts.threads / TRACE_BLOCK);
+    if (g <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_trace<COUNT, Src, Sink>), dim3(g), dim3(TRACE_BLOCK), 0, st, s, src, sink,
+                       count_ptr, n_max, work, mask, ts, counters);
+    return hipGetLastError();
+}
+
 } // namespace
 
 // ---------------------------------------------------------------------------------------------
 
-#define MRT_DISPATCH_STACK(S, CALL)                                                  \
-    switch ((S).stack_kind) {                                                        \
-    case 0: CALL(32); break;                                                         \
-    case 1: CALL(64); break;                                                         \
-    default: CALL(128); break;                                                       \
-    }
+uint32_t trace_max_threads() {
+    int dev = 0, cus = 0;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    // at most 32 waves of 64 lanes per CU
+    return uint32_t(cus > 0 ? cus : 256) * 32u * 64u;
+}
 
-hipError_t launch_trace_rays(const DeviceScene& s, const mrt_ray* rays, uint64_t n, mrt_hit* hits, hipStream_t st,
-                             unsigned long long* counters) {
+hipError_t launch_trace_rays(const DeviceScene& s, const TraceScratch& ts, const mrt_ray* rays, uint64_t n,
+                             mrt_hit* hits, hipStream_t st, unsigned long long* counters) {
     if (n == 0) return hipSuccess;
-    const int g = grid_for(n, TRACE_BLOCK, MAX_BLOCKS);
-    if (counters) {
-#define CALL(K) \
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_trace_rays<K, true>), dim3(g), dim3(TRACE_BLOCK), 0, st, s, rays, n, hits, counters)
-        MRT_DISPATCH_STACK(s, CALL)
-#undef CALL
-    } else {
-#define CALL(K) \
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_trace_rays<K, false>), dim3(g), dim3(TRACE_BLOCK), 0, st, s, rays, n, hits, nullptr)
-        MRT_DISPATCH_STACK(s, CALL)
-#undef CALL
-    }
-    return hipGetLastError();
+    if (n > 0xFFFFFFFFull) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, ts.work, 1u);
+    if (counters)
+        return launch_trace<true>(s, ts, SrcRays{rays}, SinkHits{hits}, nullptr, uint32_t(n), ts.work, 0xFFu, counters,
+                                  st);
+    return launch_trace<false>(s, ts, SrcRays{rays}, SinkHits{hits}, nullptr, uint32_t(n), ts.work, 0xFFu, nullptr, st);
 }
 
 hipError_t launch_camera_rays(const mrt_camera& cam, uint32_t w, uint32_t h, mrt_ray* rays, hipStream_t st) {
@@ -482,19 +521,18 @@ hipError_t launch_camera_rays(const mrt_camera& cam, uint32_t w, uint32_t h, mrt
     return hipGetLastError();
 }
 
-hipError_t launch_gbuffer(const DeviceScene& s, const mrt_camera& cam, uint32_t w, uint32_t h, uint32_t mask,
-                          mrt_packed_hit* out, hipStream_t st) {
+hipError_t launch_gbuffer(const DeviceScene& s, const TraceScratch& ts, const mrt_camera& cam, uint32_t w, uint32_t h,
+                          uint32_t mask, mrt_packed_hit* out, hipStream_t st) {
     const uint64_t n = uint64_t(w) * h;
     if (n == 0) return hipSuccess;
-    const int g = grid_for(n, TRACE_BLOCK, MAX_BLOCKS);
-#define CALL(K) hipLaunchKernelGGL(k_gbuffer<K>, dim3(g), dim3(TRACE_BLOCK), 0, st, s, cam, w, h, mask, out)
-    MRT_DISPATCH_STACK(s, CALL)
-#undef CALL
-    return hipGetLastError();
+    if (n > 0xFFFFFFFFull) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, ts.work, 1u);
+    return launch_trace<false>(s, ts, SrcCamera{cam, w, h}, SinkGBuffer{s.tri_verts, s.geom_first, out}, nullptr,
+                               uint32_t(n), ts.work, mask, nullptr, st);
 }
 
-hipError_t launch_pt_frame(const DeviceScene& s, PtArgs a, const mrt_packed_hit* gbuffer, PathBuffers& pb,
-                           float4* out, uint32_t* sig_out, hipStream_t st, FrameTiming* timing,
+hipError_t launch_pt_frame(const DeviceScene& s, const TraceScratch& ts, PtArgs a, const mrt_packed_hit* gbuffer,
+                           PathBuffers& pb, float4* out, uint32_t* sig_out, hipStream_t st, FrameTiming* timing,
                            uint64_t* continuation_segments, unsigned long long* trav_counters) {
     const int L = a.max_path_length;
     const uint32_t spp = uint32_t(a.spp > 0 ? a.spp : 0);
@@ -520,33 +558,28 @@ hipError_t launch_pt_frame(const DeviceScene& s, PtArgs a, const mrt_packed_hit*
         if (k >= 0) kind.push_back(k);
     };
     mark(-1);
+    // queue sizes [chunk][bounce], then the trace kernels' ray cursors [chunk][bounce]
     const uint32_t nz = n_chunks * per_chunk;
-    hipLaunchKernelGGL(k_zero_u32, dim3((nz + 255) / 256), dim3(256), 0, st, pb.counters, nz);
-    for (uint32_t c = 0; c < n_chunks; c++) {
+    hipLaunchKernelGGL(k_zero_u32, dim3((2 * nz + 255) / 256), dim3(256), 0, st, pb.counters, 2 * nz);
+    hipError_t err = hipSuccess;
+    for (uint32_t c = 0; c < n_chunks && err == hipSuccess; c++) {
         a.s0 = c * chunk;
         a.chunk = (a.s0 + chunk <= spp) ? chunk : spp - a.s0;
         a.n_paths = a.npix * a.chunk;
         uint32_t* cnt = pb.counters + c * per_chunk;
+        uint32_t* work = pb.counters + nz + c * per_chunk;
         hipLaunchKernelGGL(k_pt_init, dim3(grid_for(a.n_paths, 256, MAX_BLOCKS)), dim3(256), 0, st, s, a, gbuffer,
                            pb, cnt);
-        for (int b = 0; b < L; b++) {
+        for (int b = 0; b < L && err == hipSuccess; b++) {
             const int qi = b & 1;
             if (b > 0) {
                 mark(-1);
-                const int g = grid_for(a.n_paths, TRACE_BLOCK, MAX_BLOCKS);
-                if (trav_counters) {
-#define CALL(K)                                                                                                \
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_trace_queue<K, true>), dim3(g), dim3(TRACE_BLOCK), 0, st, s, pb.q_o[qi], \
-                       pb.q_d[qi], cnt + b, pb.hits, a.instance_mask, trav_counters)
-                    MRT_DISPATCH_STACK(s, CALL)
-#undef CALL
-                } else {
-#define CALL(K)                                                                                                 \
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_trace_queue<K, false>), dim3(g), dim3(TRACE_BLOCK), 0, st, s, pb.q_o[qi], \
-                       pb.q_d[qi], cnt + b, pb.hits, a.instance_mask, nullptr)
-                    MRT_DISPATCH_STACK(s, CALL)
-#undef CALL
-                }
+                const SrcQueue src{pb.q_o[qi], pb.q_d[qi]};
+                const SinkQueue sink{pb.hits};
+                err = trav_counters ? launch_trace<true>(s, ts, src, sink, cnt + b, a.n_paths, work + b,
+                                                         a.instance_mask, trav_counters, st)
+                                    : launch_trace<false>(s, ts, src, sink, cnt + b, a.n_paths, work + b,
+                                                          a.instance_mask, nullptr, st);
                 mark(0);
             }
             mark(-1);
@@ -575,6 +608,7 @@ hipError_t launch_pt_frame(const DeviceScene& s, PtArgs a, const mrt_packed_hit*
         for (int k : kind) timing->trace_launches += (k == 0);
         for (auto& e : ev) hipEventDestroy(e);
     }
+    if (err != hipSuccess) return err;
     if (continuation_segments) {
         std::vector<uint32_t> h(nz);
         hipMemcpyAsync(h.data(), pb.counters, sizeof(uint32_t) * nz, hipMemcpyDeviceToHost, st);

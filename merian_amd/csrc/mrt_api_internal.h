@@ -20,6 +20,7 @@ struct mrt_ctx {
     std::string err;
     mrt::PathBuffers paths{};
     uint64_t paths_npix = 0, paths_counters = 0;
+    mrt::TraceScratch trace{};
 };
 
 struct mrt_scene {
@@ -37,4 +38,5 @@ void dev_free(mrt_ctx* c, void* p);
 hipStream_t pick_stream(mrt_ctx* c, void* stream);
 mrt_status ensure_path_buffers(mrt_ctx* c, uint64_t paths, uint64_t npix, uint64_t counters);
 mrt_status check_params(mrt_ctx* c, const mrt_pt_params* p);
+mrt_status ensure_trace_scratch(mrt_ctx* c, const mrt_scene* s);
 } // namespace mrt

@@ -26,8 +26,23 @@ struct DeviceScene {
     float env_to_local[9];
     uint32_t n_tris;
     uint32_t n_geoms;
-    int32_t stack_kind;          // 0: 32-entry traversal stack, 1: 64, 2: 128
+    uint32_t stack_need;         // traversal stack entries the tree can need (3 per level + slack)
 };
+
+// Entries of the per-lane traversal stack kept in LDS; deeper entries go to TraceScratch.ovf.
+constexpr int TRACE_LDS_STACK = 16;
+
+// Per-context scratch of the trace kernels: the global part of the traversal stacks (`entries`
+// per thread, `threads` columns) and a ray cursor for launches that do not bring their own.
+struct TraceScratch {
+    uint32_t* ovf;
+    uint32_t threads;
+    uint32_t entries;
+    uint32_t* work;
+};
+
+// Most trace-kernel threads resident on the current device (the persistent grid's upper bound).
+uint32_t trace_max_threads();
 
 // Frame-constant arguments of the wavefront pt kernels.
 struct PtArgs {
@@ -48,7 +63,7 @@ struct PathBuffers {
     float4* q_o[2];    // ray queue: origin.xyz, path index bits
     float4* q_d[2];    // ray queue: direction.xyz
     float4* hits;      // t, u, v, triangle id bits; indexed like the queue being shaded
-    uint32_t* counters;// per bounce queue sizes, max_path_length + 1 per chunk
+    uint32_t* counters;// per chunk and bounce: queue sizes, then the trace kernels' ray cursors
     float4* accum;     // per pixel running sum: radiance.xyz, luminance^2
     uint32_t* sig_accum;
     uint64_t capacity; // paths
@@ -56,11 +71,11 @@ struct PathBuffers {
 
 // Launch entry points (kernels.hip). All asynchronous on `stream`.
 // counters (optional, device): node visits and triangle tests are added to counters[0..1]
-hipError_t launch_trace_rays(const DeviceScene& s, const mrt_ray* rays, uint64_t n, mrt_hit* hits, hipStream_t st,
-                             unsigned long long* counters = nullptr);
+hipError_t launch_trace_rays(const DeviceScene& s, const TraceScratch& ts, const mrt_ray* rays, uint64_t n,
+                             mrt_hit* hits, hipStream_t st, unsigned long long* counters = nullptr);
 hipError_t launch_camera_rays(const mrt_camera& cam, uint32_t w, uint32_t h, mrt_ray* rays, hipStream_t st);
-hipError_t launch_gbuffer(const DeviceScene& s, const mrt_camera& cam, uint32_t w, uint32_t h, uint32_t mask,
-                          mrt_packed_hit* out, hipStream_t st);
+hipError_t launch_gbuffer(const DeviceScene& s, const TraceScratch& ts, const mrt_camera& cam, uint32_t w, uint32_t h,
+                          uint32_t mask, mrt_packed_hit* out, hipStream_t st);
 
 struct FrameTiming {
     float ms_trace = 0.f, ms_shade = 0.f, ms_total = 0.f;
@@ -68,8 +83,8 @@ struct FrameTiming {
 };
 // One frame of pt over `gbuffer`; out = float4 irradiance per pixel; sig_out optional.
 // counters_host (optional) receives every chunk's queue sizes after the frame completes.
-hipError_t launch_pt_frame(const DeviceScene& s, PtArgs a, const mrt_packed_hit* gbuffer, PathBuffers& pb,
-                           float4* out, uint32_t* sig_out, hipStream_t st, FrameTiming* timing,
+hipError_t launch_pt_frame(const DeviceScene& s, const TraceScratch& ts, PtArgs a, const mrt_packed_hit* gbuffer,
+                           PathBuffers& pb, float4* out, uint32_t* sig_out, hipStream_t st, FrameTiming* timing,
                            uint64_t* continuation_segments, unsigned long long* trav_counters = nullptr);
 
 } // namespace mrt

@@ -69,70 +69,159 @@ struct TraversalCount {
     uint32_t nodes = 0, tris = 0;
 };
 
-// stack: this lane's column base in LDS; stride = threads per block.
-template <int STACK, bool COUNT = false>
-__device__ __forceinline__ HitRec traverse(const DeviceBVH bvh, f3 o, f3 d, float tmin, float tmax, uint32_t ray_mask,
-                                           uint32_t* stack, int stride, TraversalCount* count = nullptr) {
-    HitRec best{TRAVERSAL_DONE, __builtin_inff(), 0.f, 0.f};
+// Traversal stack of one lane: the first SL entries in an LDS column (stride = threads per
+// block, so a wave's 64 lanes touch 64 distinct banks), deeper entries in a per-lane column of a
+// global overflow area (stride = threads of the grid). Near-first traversal rarely goes past SL.
+struct LaneStack {
+    uint32_t* lds;
+    uint32_t lds_stride;
+    uint32_t* ovf;
+    uint32_t ovf_stride;
+};
+template <int SL> __device__ __forceinline__ void stack_push(const LaneStack& s, int& sp, uint32_t e) {
+    if (sp < SL) {
+        s.lds[sp * s.lds_stride] = e;
+    } else {
+        s.ovf[size_t(sp - SL) * s.ovf_stride] = e;
+    }
+    sp++;
+}
+template <int SL> __device__ __forceinline__ uint32_t stack_pop(const LaneStack& s, int& sp) {
+    --sp;
+    return sp < SL ? s.lds[sp * s.lds_stride] : s.ovf[size_t(sp - SL) * s.ovf_stride];
+}
+
+// Per-lane traversal state between steps.
+struct RayState {
+    f3 o, d;
+    float ix, iy, iz, ox, oy, oz;
+    float t0, tmin, tmax;
+    uint32_t neg; // bit k: direction component k is negative
+    HitRec best;
+    int sp;
+    uint32_t cur; // next stack entry to process; TRAVERSAL_DONE when finished
+};
+
+__device__ __forceinline__ void ray_begin(RayState& r, f3 o, f3 d, float tmin, float tmax) {
+    r.o = o;
+    r.d = d;
     const float dx = fabsf(d.x) < 1e-20f ? copysignf(1e-20f, d.x) : d.x;
     const float dy = fabsf(d.y) < 1e-20f ? copysignf(1e-20f, d.y) : d.y;
     const float dz = fabsf(d.z) < 1e-20f ? copysignf(1e-20f, d.z) : d.z;
-    const float ix = 1.0f / dx, iy = 1.0f / dy, iz = 1.0f / dz;
-    const float ox = o.x * ix, oy = o.y * iy, oz = o.z * iz;
-    const bool nx = ix < 0.f, ny = iy < 0.f, nz = iz < 0.f;
-    const float t0 = tmin - fabsf(tmin) * 2.4e-7f;
-    int sp = 0;
-    uint32_t cur = 0u; // root node
-    while (true) {
-        if (cur & LEAF_FLAG) {
-            const uint32_t first = (cur & 0x7FFFFFFFu) >> 3;
-            const uint32_t cnt = (cur & 7u) + 1u;
-            for (uint32_t k = 0; k < cnt; k++) test_triangle(bvh.tris, first + k, o, d, tmin, tmax, ray_mask, best);
-            if constexpr (COUNT) count->tris += cnt;
-            cur = sp > 0 ? stack[(--sp) * stride] : TRAVERSAL_DONE;
-        } else {
-            if constexpr (COUNT) count->nodes++;
-            const float4* np = reinterpret_cast<const float4*>(bvh.nodes + cur);
-            const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-            const int4 ch = reinterpret_cast<const int4*>(np)[6];
-            const uint32_t counts = reinterpret_cast<const uint32_t*>(np)[28];
-            const float tlim = fminf(tmax, best.t) * 1.00000024f;
-            float key[4];
-            uint32_t ent[4];
-#define MRT_CHILD(c, LX, HX, LY, HY, LZ, HZ, CH)                                                        \
-    {                                                                                                   \
-        const float tnx = __builtin_fmaf(nx ? HX : LX, ix, -ox), tfx = __builtin_fmaf(nx ? LX : HX, ix, -ox); \
-        const float tny = __builtin_fmaf(ny ? HY : LY, iy, -oy), tfy = __builtin_fmaf(ny ? LY : HY, iy, -oy); \
-        const float tnz = __builtin_fmaf(nz ? HZ : LZ, iz, -oz), tfz = __builtin_fmaf(nz ? LZ : HZ, iz, -oz); \
-        const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, t0));                                       \
-        const float tf = fminf(fminf(tfx, tfy), fminf(tfz, tlim));                                     \
-        const uint32_t cc = (counts >> (8 * c)) & 0xFFu;                                                \
-        key[c] = (tn <= tf) ? tn : __builtin_inff();                                                    \
-        ent[c] = cc ? (LEAF_FLAG | (uint32_t(CH) << 3) | (cc - 1u)) : uint32_t(CH);                     \
+    r.ix = 1.0f / dx;
+    r.iy = 1.0f / dy;
+    r.iz = 1.0f / dz;
+    r.ox = o.x * r.ix;
+    r.oy = o.y * r.iy;
+    r.oz = o.z * r.iz;
+    r.neg = (r.ix < 0.f ? 1u : 0u) | (r.iy < 0.f ? 2u : 0u) | (r.iz < 0.f ? 4u : 0u);
+    r.tmin = tmin;
+    r.tmax = tmax;
+    r.t0 = tmin - fabsf(tmin) * 2.4e-7f;
+    r.best = HitRec{TRAVERSAL_DONE, __builtin_inff(), 0.f, 0.f};
+    r.sp = 0;
+    r.cur = 0u; // root node
+}
+
+// One traversal step: one inner node (test its four children, push the hit ones far to near,
+// continue with the nearest) or one leaf (test its triangles, pop).
+template <int SL, bool COUNT>
+__device__ __forceinline__ void ray_step(const DeviceBVH bvh, RayState& r, const LaneStack& stk, uint32_t ray_mask,
+                                         TraversalCount& count) {
+    if (r.cur & LEAF_FLAG) {
+        const uint32_t first = (r.cur & 0x7FFFFFFFu) >> 3;
+        const uint32_t cnt = (r.cur & 7u) + 1u;
+        for (uint32_t k = 0; k < cnt; k++) test_triangle(bvh.tris, first + k, r.o, r.d, r.tmin, r.tmax, ray_mask, r.best);
+        if constexpr (COUNT) count.tris += cnt;
+        r.cur = r.sp > 0 ? stack_pop<SL>(stk, r.sp) : TRAVERSAL_DONE;
+        return;
     }
-            MRT_CHILD(0, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, ch.x)
-            MRT_CHILD(1, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, ch.y)
-            MRT_CHILD(2, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, ch.z)
-            MRT_CHILD(3, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, ch.w)
+    if constexpr (COUNT) count.nodes++;
+    const float4* np = reinterpret_cast<const float4*>(bvh.nodes + r.cur);
+    const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+    const int4 ch = reinterpret_cast<const int4*>(np)[6];
+    const uint32_t counts = reinterpret_cast<const uint32_t*>(np)[28];
+    const float tlim = fminf(r.tmax, r.best.t) * 1.00000024f;
+    const bool nx = r.neg & 1u, ny = r.neg & 2u, nz = r.neg & 4u;
+    float key[4];
+    uint32_t ent[4];
+#define MRT_CHILD(c, LX, HX, LY, HY, LZ, HZ, CH)                                                               \
+    {                                                                                                          \
+        const float tnx = __builtin_fmaf(nx ? HX : LX, r.ix, -r.ox), tfx = __builtin_fmaf(nx ? LX : HX, r.ix, -r.ox); \
+        const float tny = __builtin_fmaf(ny ? HY : LY, r.iy, -r.oy), tfy = __builtin_fmaf(ny ? LY : HY, r.iy, -r.oy); \
+        const float tnz = __builtin_fmaf(nz ? HZ : LZ, r.iz, -r.oz), tfz = __builtin_fmaf(nz ? LZ : HZ, r.iz, -r.oz); \
+        const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, r.t0));                                            \
+        const float tf = fminf(fminf(tfx, tfy), fminf(tfz, tlim));                                            \
+        const uint32_t cc = (counts >> (8 * c)) & 0xFFu;                                                       \
+        key[c] = (tn <= tf) ? tn : __builtin_inff();                                                           \
+        ent[c] = cc ? (LEAF_FLAG | (uint32_t(CH) << 3) | (cc - 1u)) : uint32_t(CH);                            \
+    }
+    MRT_CHILD(0, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, ch.x)
+    MRT_CHILD(1, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, ch.y)
+    MRT_CHILD(2, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, ch.z)
+    MRT_CHILD(3, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, ch.w)
 #undef MRT_CHILD
-            cswap(key[0], ent[0], key[1], ent[1]);
-            cswap(key[2], ent[2], key[3], ent[3]);
-            cswap(key[0], ent[0], key[2], ent[2]);
-            cswap(key[1], ent[1], key[3], ent[3]);
-            cswap(key[1], ent[1], key[2], ent[2]);
-            const float inf = __builtin_inff();
-            if (key[3] != inf) stack[(sp++) * stride] = ent[3];
-            if (key[2] != inf) stack[(sp++) * stride] = ent[2];
-            if (key[1] != inf) stack[(sp++) * stride] = ent[1];
-            if (key[0] != inf) {
-                cur = ent[0];
-            } else {
-                cur = sp > 0 ? stack[(--sp) * stride] : TRAVERSAL_DONE;
+    cswap(key[0], ent[0], key[1], ent[1]);
+    cswap(key[2], ent[2], key[3], ent[3]);
+    cswap(key[0], ent[0], key[2], ent[2]);
+    cswap(key[1], ent[1], key[3], ent[3]);
+    cswap(key[1], ent[1], key[2], ent[2]);
+    const float inf = __builtin_inff();
+    if (key[3] != inf) stack_push<SL>(stk, r.sp, ent[3]);
+    if (key[2] != inf) stack_push<SL>(stk, r.sp, ent[2]);
+    if (key[1] != inf) stack_push<SL>(stk, r.sp, ent[1]);
+    if (key[0] != inf) {
+        r.cur = ent[0];
+    } else {
+        r.cur = r.sp > 0 ? stack_pop<SL>(stk, r.sp) : TRAVERSAL_DONE;
+    }
+}
+
+// Lanes fetch new rays once at least this many of a wave's 64 lanes are idle (one atomic each).
+constexpr int REFILL_LANES = 16;
+
+// Persistent closest-hit loop of one wave: every lane runs one traversal step per iteration and,
+// when its ray is done, hands the hit to `sink` and later takes a fresh ray from `src`, so lanes
+// with short rays do not idle while others finish long ones. `work` is the launch's ray cursor.
+template <int SL, bool COUNT, class Src, class Sink>
+__device__ __forceinline__ void trace_persistent(const DeviceBVH bvh, const Src& src, const Sink& sink, uint32_t n,
+                                                 uint32_t* work, uint32_t ray_mask, const LaneStack& stk,
+                                                 TraversalCount& count) {
+    const uint32_t lane = __lane_id();
+    RayState r;
+    r.cur = TRAVERSAL_DONE;
+    bool active = false, exhausted = false;
+    uint32_t idx = 0;
+    while (true) {
+        const uint64_t idle = __ballot(!active);
+        if (!exhausted && __popcll(idle) >= REFILL_LANES) {
+            const uint32_t take = uint32_t(__popcll(idle));
+            const uint32_t leader = uint32_t(__ffsll((unsigned long long)idle) - 1);
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(work, take);
+            base = __shfl(base, int(leader));
+            if (base + take >= n) exhausted = true;
+            if (!active) {
+                const uint32_t my = base + uint32_t(__popcll(idle & ((1ull << lane) - 1ull)));
+                if (my < n) {
+                    idx = my;
+                    f3 o, d;
+                    float tmin, tmax;
+                    src.load(idx, o, d, tmin, tmax);
+                    ray_begin(r, o, d, tmin, tmax);
+                    active = true;
+                }
             }
         }
-        if (cur == TRAVERSAL_DONE) break;
+        if (__ballot(active) == 0) break;
+        if (active) {
+            ray_step<SL, COUNT>(bvh, r, stk, ray_mask, count);
+            if (r.cur == TRAVERSAL_DONE) {
+                sink.store(idx, r.best);
+                active = false;
+            }
+        }
     }
-    return best;
 }
 
 } // namespace mrt
