@@ -58,6 +58,7 @@ def lib() -> C.CDLL:
             "orc_pt_render": (u64, [vp, C.POINTER(abi.PtParams), C.POINTER(abi.Camera), u32, u32, u32, vp,
                                     u32, u32, u32, u32, vp, vp, i32]),
             "orc_bsdf": (None, [C.POINTER(BsdfCfg), i32, vp, vp, u32, u64, vp, vp, vp, vp]),
+            "orc_math": (None, [i32, vp, vp, u64, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -123,6 +124,48 @@ class OracleScene:
         segs = lib().orc_pt_render(self.h, C.byref(params), C.byref(cam), frame, w, h, _p(gb), x0, y0, cw, ch,
                                    _p(out), _p(sig), threads)
         return out, sig, int(segs)
+
+
+MATH_OPS = {"sin": 0, "cos": 1, "atan": 2, "atan2": 3, "acos": 4, "log2": 5, "exp2": 6, "exp": 7, "pow": 8}
+
+
+def math(op: str, x, y=None) -> np.ndarray:
+    """The oracle's elementary functions (atan2(x, y) and pow(x, y) take two arguments)."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    yy = None if y is None else np.ascontiguousarray(np.broadcast_to(y, x.shape), dtype=np.float32)
+    out = np.zeros_like(x)
+    lib().orc_math(MATH_OPS[op], _p(x), _p(yy), x.size, _p(out))
+    return out
+
+
+def encode_normal(n) -> int:
+    a = np.ascontiguousarray(n, dtype=np.float32)
+    return int(lib().orc_encode_normal(_p(a)))
+
+
+def decode_normal(e: int) -> np.ndarray:
+    out = np.zeros(3, np.float32)
+    lib().orc_decode_normal(int(e), _p(out))
+    return out
+
+
+def offset_ray_origin(p, n) -> np.ndarray:
+    a = np.ascontiguousarray(p, dtype=np.float32)
+    b = np.ascontiguousarray(n, dtype=np.float32)
+    out = np.zeros(3, np.float32)
+    lib().orc_offset_ray_origin(_p(a), _p(b), _p(out))
+    return out
+
+
+def pcg4d16(x, y, z, w) -> int:
+    return int(lib().orc_pcg4d16(x, y, z, w))
+
+
+def pcg_stream(seed: int, n: int):
+    u = np.zeros(n, np.uint32)
+    f = np.zeros(n, np.float32)
+    lib().orc_pcg_stream(seed, n, _p(u), _p(f))
+    return u, f
 
 
 def bsdf(cfg: BsdfCfg, op: str, wi: np.ndarray, wo: np.ndarray | None = None, seed: int = 1):
