@@ -68,7 +68,8 @@ hipStream_t pick_stream(mrt_ctx* c, void* stream) { return stream ? static_cast<
 void free_path_buffers(mrt_ctx* c) {
     PathBuffers& p = c->paths;
     for (void* q : {(void*)p.state_a, (void*)p.state_b, (void*)p.sig, (void*)p.q_o[0], (void*)p.q_o[1], (void*)p.q_d[0],
-                    (void*)p.q_d[1], (void*)p.hits, (void*)p.counters, (void*)p.accum, (void*)p.sig_accum})
+                    (void*)p.q_d[1], (void*)p.hits, (void*)p.aux, (void*)p.sq_o, (void*)p.sq_d, (void*)p.sq_c,
+                    (void*)p.counters, (void*)p.accum, (void*)p.sig_accum})
         dev_free(c, q);
     p = PathBuffers{};
     c->paths_npix = 0;
@@ -93,7 +94,12 @@ mrt_status ensure_path_buffers(mrt_ctx* c, uint64_t paths, uint64_t npix, uint64
         p.q_d[k] = static_cast<float4*>(A(16 * paths));
     }
     p.hits = static_cast<float4*>(A(16 * paths));
-    p.counters = static_cast<uint32_t*>(A(2 * 4 * counters));
+    p.aux = static_cast<float*>(A(4 * paths));
+    p.sq_o = static_cast<float4*>(A(16 * paths));
+    p.sq_d = static_cast<float4*>(A(16 * paths));
+    p.sq_c = static_cast<float4*>(A(16 * paths));
+    // per chunk and bounce: queue sizes, trace cursors, shadow-queue sizes, shadow cursors
+    p.counters = static_cast<uint32_t*>(A(4 * 4 * counters));
     p.accum = static_cast<float4*>(A(16 * npix));
     p.sig_accum = static_cast<uint32_t*>(A(4 * npix));
     if (st != MRT_OK) {
@@ -200,6 +206,57 @@ static mrt_status upload(mrt_ctx* c, mrt_scene* s, const void* src, size_t bytes
     return MRT_OK;
 }
 
+// Importance-sampling table of the environment for NEE: per texel (prob, alias, p, 0) of a
+// Vose alias table over weights = mean(rgb) x solid-angle factor (sin(theta) at the texel centre
+// for the lat-long map; constant for the equal-area map), p = normalised weight.
+static std::vector<float> build_env_sampling(const mrt_env& e) {
+    const size_t n = size_t(e.width) * e.height;
+    std::vector<double> w(n);
+    double total = 0.0;
+    for (size_t i = 0; i < n; i++) {
+        const float* t = e.rgba + 4 * i;
+        double v = std::max(0.0, (double(t[0]) + double(t[1]) + double(t[2])) / 3.0);
+        if (!std::isfinite(v)) v = 0.0;
+        if (e.type == MRT_ENV_LATLONG) v *= std::sin((double(i / e.width) + 0.5) / double(e.height) * M_PI);
+        w[i] = v;
+        total += v;
+    }
+    std::vector<float> out(4 * n, 0.f);
+    if (!(total > 0.0)) {
+        for (size_t i = 0; i < n; i++) {
+            out[4 * i] = 1.f;
+            uint32_t a = uint32_t(i);
+            std::memcpy(&out[4 * i + 1], &a, 4);
+        }
+        return out;
+    }
+    std::vector<double> q(n);
+    std::vector<uint32_t> small, large, alias(n);
+    std::vector<double> prob(n, 1.0);
+    for (size_t i = 0; i < n; i++) {
+        q[i] = w[i] / total * double(n);
+        (q[i] < 1.0 ? small : large).push_back(uint32_t(i));
+        alias[i] = uint32_t(i);
+    }
+    while (!small.empty() && !large.empty()) {
+        const uint32_t l = small.back(), g = large.back();
+        small.pop_back();
+        prob[l] = q[l];
+        alias[l] = g;
+        q[g] = (q[g] + q[l]) - 1.0;
+        if (q[g] < 1.0) {
+            large.pop_back();
+            small.push_back(g);
+        }
+    }
+    for (size_t i = 0; i < n; i++) {
+        out[4 * i] = float(prob[i]);
+        std::memcpy(&out[4 * i + 1], &alias[i], 4);
+        out[4 * i + 2] = float(w[i] / total);
+    }
+    return out;
+}
+
 static void scene_free(mrt_ctx* c, mrt_scene* s) {
     if (!s) return;
     for (void* p : s->allocs) dev_free(c, p);
@@ -284,8 +341,13 @@ MRT_API mrt_status mrt_scene_create(mrt_ctx* c, const mrt_scene_desc* d, mrt_sce
     if (e.type != MRT_ENV_NONE) {
         if (!e.rgba || e.width == 0 || e.height == 0) st = fail(c, MRT_ERR_INVALID, "environment without texels");
         UP(e.rgba, size_t(e.width) * e.height * 16, env, const float4*)
+        if (st == MRT_OK) {
+            const std::vector<float> samp = build_env_sampling(e);
+            UP(samp.data(), samp.size() * 4, env_samp, const float4*)
+        }
     } else {
         s->ds.env = nullptr;
+        s->ds.env_samp = nullptr;
     }
 #undef UP
     s->ds.n_tris = uint32_t(n_tris);
@@ -467,7 +529,8 @@ mrt_status check_params(mrt_ctx* c, const mrt_pt_params* p) {
     if (p->max_path_length < 1 || p->max_path_length > 64)
         return fail(c, MRT_ERR_INVALID, "max path length outside [1, 64]");
     if (p->demodulate_albedo) return fail(c, MRT_ERR_UNSUPPORTED, "demodulate albedo is not supported");
-    if (p->mode != MRT_MODE_BSDF) return fail(c, MRT_ERR_UNSUPPORTED, "render mode not supported by this entry point");
+    if (p->mode != MRT_MODE_BSDF && p->mode != MRT_MODE_NEE)
+        return fail(c, MRT_ERR_UNSUPPORTED, "render mode not supported by this entry point");
     return MRT_OK;
 }
 } // namespace mrt
@@ -507,6 +570,7 @@ MRT_API mrt_status mrt_pt_render_dev(mrt_ctx* c, mrt_scene* s, const mrt_pt_para
     a.emission_on_primary = prm->emission_on_primary;
     a.record_signature = prm->record_signature && signature ? 1 : 0;
     a.instance_mask = prm->instance_mask & 0xFFu;
+    a.mode = prm->mode;
     FrameTiming t;
     uint64_t segs = 0;
     const hipStream_t strm = pick_stream(c, stream);
@@ -529,7 +593,7 @@ MRT_API mrt_status mrt_pt_render_dev(mrt_ctx* c, mrt_scene* s, const mrt_pt_para
     if (stats) {
         stats->primary_segments = npix;
         stats->continuation_segments = segs;
-        stats->shadow_rays = 0;
+        stats->shadow_rays = t.shadow_rays;
         stats->ms_trace = t.ms_trace;
         stats->ms_shade = t.ms_shade;
         stats->ms_total = t.ms_total;

@@ -125,6 +125,33 @@ struct SinkGBuffer { // PackedHitInfo (gbuffer.slang:175-196); one pretransforme
     }
 };
 
+struct SrcShadow { // NEE shadow queue: (origin, path id), (direction, -); [0, 1e7] like a continuation ray
+    const float4* so;
+    const float4* sd;
+    __device__ __forceinline__ void load(uint32_t i, f3& o, f3& d, float& tmin, float& tmax) const {
+        const float4 a = ld_stream(so + i), b = ld_stream(sd + i);
+        o = f3{a.x, a.y, a.z};
+        d = f3{b.x, b.y, b.z};
+        tmin = 0.f;
+        tmax = SCENE_RAY_TMAX;
+    }
+};
+struct SinkShadow { // unoccluded: add the queued contribution to the path's radiance
+    const float4* so;
+    const float4* sc;
+    float4* state_b;
+    __device__ __forceinline__ void store(uint32_t i, const HitRec& h) const {
+        if (h.id != TRAVERSAL_DONE) return;
+        const uint32_t pid = __float_as_uint(so[i].w);
+        const float4 c = ld_stream(sc + i);
+        float4 b = state_b[pid];
+        b.x = b.x + c.x;
+        b.y = b.y + c.y;
+        b.z = b.z + c.z;
+        state_b[pid] = b;
+    }
+};
+
 // Adds this lane's traversal counts to counters[0] (node visits) and counters[1] (triangle
 // tests) with one atomic per wave.
 __device__ __forceinline__ void flush_counts(const TraversalCount& c, unsigned long long* counters) {
@@ -145,7 +172,8 @@ __device__ __forceinline__ void flush_counts(const TraversalCount& c, unsigned l
 // Closest-hit kernel for every ray stream (ray queries, the GBuffer's primary rays, the path
 // queue): a persistent grid, one ray per lane with refill, LDS stack + global overflow.
 // Ray count: *count_ptr when given (device-side queue size), else n_fixed.
-template <bool COUNT, class Src, class Sink>
+// ANY: visibility queries (first hit ends the traversal).
+template <bool COUNT, class Src, class Sink, bool ANY = false>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DeviceScene S, Src src, Sink sink,
                                                        const uint32_t* __restrict__ count_ptr, uint32_t n_fixed,
                                                        uint32_t* __restrict__ work, uint32_t mask, TraceScratch ts,
@@ -155,7 +183,8 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DeviceScene S, Src src, S
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const LaneStack stk{stack + threadIdx.x, blockDim.x, ts.ovf + gtid, gridDim.x * blockDim.x};
     TraversalCount cnt;
-    trace_persistent<TRACE_LDS_STACK, COUNT>(DeviceBVH{S.nodes, S.tris}, src, sink, n, work, mask, stk, cnt);
+    trace_persistent<TRACE_LDS_STACK, COUNT, Src, Sink, ANY>(DeviceBVH{S.nodes, S.tris}, src, sink, n, work, mask,
+                                                             stk, cnt);
     if constexpr (COUNT) flush_counts(cnt, counters);
 }
 
@@ -228,11 +257,110 @@ __device__ __forceinline__ f3 env_eval(const DeviceScene& S, f3 dir) {
 }
 
 // -----------------------------------------------------------------------------------------
+// environment importance sampling for NEE (no counterpart in the reference's pt): texel by the
+// alias table, uniform position inside the texel, mapped to a direction. Solid-angle pdf:
+//   lat-long   p_texel * W * H / (2 pi^2 sin(theta))   (d omega = 2 pi^2 sin(theta) du dv)
+//   equal-area p_texel * W * H / (4 pi)                (area preserving)
+
+// Clarberg's equal-area square -> sphere map (pbrt-v4 EqualAreaSquareToSphere); inverse of the
+// lookup in env_eval.
+__device__ __forceinline__ f3 equal_area_square_to_sphere(float pu, float pv) {
+    const float u = 2.f * pu - 1.f, v = 2.f * pv - 1.f;
+    const float up = fabsf(u), vp = fabsf(v);
+    const float sd = 1.f - (up + vp);
+    const float r = 1.f - fabsf(sd);
+    const float phi = (r == 0.f ? 1.f : (vp - up) / r + 1.f) * (PI / 4.f);
+    const float z = copysignf(1.f - r * r, sd);
+    float sn, cs;
+    msincos(phi, sn, cs);
+    const float k = r * sqrtf(fmaxf(0.f, 2.f - r * r));
+    return {copysignf(cs, u) * k, copysignf(sn, v) * k, z};
+}
+
+__device__ __forceinline__ f3 env_to_world(const DeviceScene& S, f3 l) { // to_local^T (a rotation)
+    const float* m = S.env_to_local;
+    return {m[0] * l.x + m[3] * l.y + m[6] * l.z, m[1] * l.x + m[4] * l.y + m[7] * l.z,
+            m[2] * l.x + m[5] * l.y + m[8] * l.z};
+}
+
+// Solid-angle density with which env_sample produces world direction `dir`.
+__device__ float env_pdf(const DeviceScene& S, f3 dir) {
+    if (S.env_type == MRT_ENV_NONE) return 0.f;
+    const float* m = S.env_to_local;
+    const f3 d = normalize(f3{m[0] * dir.x + m[1] * dir.y + m[2] * dir.z, m[3] * dir.x + m[4] * dir.y + m[5] * dir.z,
+                              m[6] * dir.x + m[7] * dir.y + m[8] * dir.z});
+    const float n = float(S.env_w) * float(S.env_h);
+    if (S.env_type == MRT_ENV_LATLONG) {
+        const float u = matan2(d.x, -d.z) * INV_2PI + 0.5f;
+        const float v = macos(d.y) * M_1_PI_F;
+        const int x = min(max(int(u * float(S.env_w)), 0), int(S.env_w) - 1);
+        const int y = min(max(int(v * float(S.env_h)), 0), int(S.env_h) - 1);
+        const float sin_t = sqrtf(fmaxf(0.f, 1.f - d.y * d.y));
+        if (sin_t <= 0.f) return 0.f;
+        return S.env_samp[size_t(y) * S.env_w + x].z * n / (2.f * PI * PI * sin_t);
+    }
+    const f3 a{fabsf(d.x), fabsf(d.y), fabsf(d.z)};
+    const float r = sqrtf(fmaxf(0.f, 1.f - a.z));
+    const float mm = fmaxf(a.x, a.y);
+    const float b = mm == 0.f ? 0.f : fminf(a.x, a.y) / mm;
+    float phi = matan(b) * (2.f / PI);
+    if (a.x < a.y) phi = 1.f - phi;
+    float v = phi * r;
+    float u = r - v;
+    if (d.z < 0.f) {
+        const float mu = u;
+        u = 1.f - v;
+        v = 1.f - mu;
+    }
+    u = 0.5f * ((d.x < 0.f ? -u : u) + 1.f);
+    v = 0.5f * ((d.y < 0.f ? -v : v) + 1.f);
+    const int x = min(max(int(u * float(S.env_w)), 0), int(S.env_w) - 1);
+    const int y = min(max(int(v * float(S.env_h)), 0), int(S.env_h) - 1);
+    return S.env_samp[size_t(y) * S.env_w + x].z * n / (4.f * PI);
+}
+
+// Samples a world direction toward the environment; returns false for an empty environment.
+__device__ bool env_sample(const DeviceScene& S, PCG& rng, f3& dir, float& pdf) {
+    if (S.env_type == MRT_ENV_NONE) return false;
+    const uint32_t n = S.env_w * S.env_h;
+    const float r0 = rng.next_float(), r1 = rng.next_float();
+    const float r2 = rng.next_float(), r3 = rng.next_float();
+    uint32_t i = min(uint32_t(r0 * float(n)), n - 1u);
+    const float4 e = S.env_samp[i];
+    if (r1 >= e.x) i = __float_as_uint(e.y);
+    const float p_texel = S.env_samp[i].z;
+    if (!(p_texel > 0.f)) return false;
+    const uint32_t x = i % S.env_w, y = i / S.env_w;
+    const float u = (float(x) + r2) / float(S.env_w);
+    const float v = (float(y) + r3) / float(S.env_h);
+    f3 l;
+    if (S.env_type == MRT_ENV_LATLONG) {
+        float st, ct, sp, cp;
+        msincos(v * PI, st, ct);
+        msincos((u - 0.5f) * TWO_PI, sp, cp);
+        if (st <= 0.f) return false;
+        l = f3{st * sp, ct, -(st * cp)};
+        pdf = p_texel * float(n) / (2.f * PI * PI * st);
+    } else {
+        l = equal_area_square_to_sphere(u, v);
+        pdf = p_texel * float(n) / (4.f * PI);
+    }
+    dir = normalize(env_to_world(S, l));
+    return pdf > 0.f;
+}
+
+__device__ __forceinline__ float power_heuristic(float a, float b) {
+    const float a2 = a * a;
+    return a2 / (a2 + b * b);
+}
+
+// -----------------------------------------------------------------------------------------
 // shading points (scene.slang:312-498, hit.slang:106-147)
 
 struct ShadingPoint {
     f3 pos;
     uint32_t enc_face_normal;
+    bool env; // the ray left the scene (or hit a UseEnvMap mesh)
     f3 emission;
     Frame frame;
     bool has_material;
@@ -255,8 +383,10 @@ __device__ __forceinline__ void shading_point(const DeviceScene& S, f3 ro, f3 rd
         sp.emission = env_eval(S, -(-rd));
         sp.frame = frame_from_normal(-rd);
         sp.has_material = false;
+        sp.env = true;
         return;
     }
+    sp.env = false;
     const mrt_geometry G = S.geoms[tv.w];
     const mrt_vertex v0 = S.verts[tv.x], v1 = S.verts[tv.y], v2 = S.verts[tv.z];
     const f3 b{(1.0f - bu) - bv, bu, bv};
@@ -347,23 +477,42 @@ __global__ void k_pt_init(DeviceScene S, PtArgs A, const mrt_packed_hit* __restr
         st_stream(P.state_a + i, make_float4(1.f, 1.f, 1.f, 1.f));
         st_stream(P.state_b + i, make_float4(0.f, 0.f, 0.f, __uint_as_float(pcg4d16(x, y, A.frame, s))));
         if (A.record_signature) P.sig[i] = sig_step(0x9E3779B9u * (s + 1u), gid);
+        if (A.mode != MRT_MODE_BSDF) P.aux[i] = 0.f;
     }
 }
 
+// Appends one entry per pushing lane with one atomic per wave; returns the lane's slot.
+__device__ __forceinline__ uint32_t wave_append(bool push, uint32_t* counter) {
+    const uint64_t ballot = __ballot(push);
+    if (!ballot) return 0u;
+    const uint32_t lane = __lane_id();
+    const uint32_t leader = __ffsll((unsigned long long)ballot) - 1;
+    uint32_t slot = 0;
+    if (lane == leader) slot = atomicAdd(counter, uint32_t(__popcll(ballot)));
+    slot = __shfl(slot, leader);
+    return slot + uint32_t(__popcll(ballot & ((1ull << lane) - 1ull)));
+}
+
 // Shades vertex `bounce` of every queued path and appends its continuation ray
-// (trace_path, render_pt.slang:24-90).
+// (trace_path, render_pt.slang:24-90). In MRT_MODE_NEE it also samples the environment
+// (one shadow ray per vertex, power-heuristic MIS with the BSDF sample; an escaping BSDF-sampled
+// ray then carries the complementary weight).
+template <bool NEE>
 __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(DeviceScene S, PtArgs A, int bounce, PathBuffers P,
                                                        const float4* __restrict__ qo, const float4* __restrict__ qd,
                                                        const uint32_t* __restrict__ count_in, float4* __restrict__ no,
-                                                       float4* __restrict__ nd, uint32_t* __restrict__ count_out) {
+                                                       float4* __restrict__ nd, uint32_t* __restrict__ count_out,
+                                                       uint32_t* __restrict__ shadow_count) {
     const uint32_t n = *count_in;
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t lane = lane_id();
     const bool last = bounce + 1 >= A.max_path_length;
+    constexpr bool nee = NEE;
     for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < n; base += stride) {
         const uint32_t i = base + lane;
-        bool push = false;
+        bool push = false, shadow = false;
         f3 next_o{0, 0, 0}, next_d{0, 0, 0};
+        f3 sh_o{0, 0, 0}, sh_d{0, 0, 0}, sh_c{0, 0, 0};
         uint32_t pid = 0;
         if (i < n) {
             const float4 o4 = ld_stream(qo + i), d4 = ld_stream(qd + i), h4 = ld_stream(P.hits + i);
@@ -377,15 +526,39 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(DeviceScene S, PtArgs A, 
             PCG rng{__float_as_uint(sb.w)};
             ShadingPoint sp;
             shading_point(S, ro, rd, gid, h4.y, h4.z, ior, bounce == 0 || !last, sp);
+            f3 emission = sp.emission;
+            if (nee && bounce > 0 && sp.env) {
+                // the environment was also reachable by NEE from the previous vertex
+                emission = emission * power_heuristic(P.aux[pid], env_pdf(S, rd));
+            }
             if (bounce == 0) {
-                rad = A.emission_on_primary ? thr * sp.emission : f3{0.f, 0.f, 0.f};
+                rad = A.emission_on_primary ? thr * emission : f3{0.f, 0.f, 0.f};
             } else {
-                rad = rad + thr * sp.emission;
+                rad = rad + thr * emission;
                 if (A.record_signature) P.sig[pid] = sig_step(P.sig[pid], gid);
             }
             if (!last && sp.has_material &&
                 !(thr.x < THROUGHPUT_THRESHOLD && thr.y < THROUGHPUT_THRESHOLD && thr.z < THROUGHPUT_THRESHOLD)) {
                 const f3 wi_local = sp.frame.to_local(-rd);
+                const f3 ng = decode_normal(sp.enc_face_normal);
+                const bool wi_below = dot(-rd, ng) < 0.f;
+                if (nee) {
+                    f3 wl;
+                    float pdf_l;
+                    if (env_sample(S, rng, wl, pdf_l)) {
+                        const f3 wl_local = sp.frame.to_local(wl);
+                        const f3 f = sp.mat.eval(wi_local, wl_local);
+                        const float wl_ng = dot(wl, ng);
+                        const bool trans = wi_local.z * wl_local.z < 0.f;
+                        if ((f.x > 0.f || f.y > 0.f || f.z > 0.f) && ((wi_below != (wl_ng < 0.f)) == trans)) {
+                            const float w = power_heuristic(env_pdf(S, wl), sp.mat.pdf(wi_local, wl_local));
+                            sh_c = ((thr * f) * env_eval(S, wl)) * (w / pdf_l);
+                            sh_o = offset_ray_origin(sp.pos, wl_ng < 0.f ? -ng : ng);
+                            sh_d = wl;
+                            shadow = finite3(sh_c) && (sh_c.x > 0.f || sh_c.y > 0.f || sh_c.z > 0.f);
+                        }
+                    }
+                }
                 const Sample s = sp.mat.sample_eval(wi_local, rng);
                 if (s.valid) {
                     const f3 wo = sp.frame.to_world(s.wo);
@@ -393,28 +566,29 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(DeviceScene S, PtArgs A, 
                     const bool transmission = wi_local.z * s.wo.z < 0.f;
                     if (transmission) ior /= sp.mat.eta;
                     // ShadingPoint.make_ray (hit.slang:134-146)
-                    const f3 ng = decode_normal(sp.enc_face_normal);
                     const float wo_ng = dot(wo, ng);
-                    if (((dot(-rd, ng) < 0.f) != (wo_ng < 0.f)) == transmission) {
+                    if ((wi_below != (wo_ng < 0.f)) == transmission) {
                         next_o = offset_ray_origin(sp.pos, wo_ng < 0.f ? -ng : ng);
                         next_d = wo;
                         push = true;
+                        if (nee) P.aux[pid] = s.pdf;
                     }
                 }
             }
             st_stream(P.state_a + pid, make_float4(thr.x, thr.y, thr.z, ior));
             st_stream(P.state_b + pid, make_float4(rad.x, rad.y, rad.z, __uint_as_float(rng.state)));
         }
-        const uint64_t ballot = __ballot(push);
-        if (ballot) {
-            const uint32_t leader = __ffsll((unsigned long long)ballot) - 1;
-            uint32_t slot = 0;
-            if (lane == leader) slot = atomicAdd(count_out, uint32_t(__popcll(ballot)));
-            slot = __shfl(slot, leader);
-            if (push) {
-                const uint32_t k = slot + uint32_t(__popcll(ballot & ((1ull << lane) - 1ull)));
-                st_stream(no + k, make_float4(next_o.x, next_o.y, next_o.z, __uint_as_float(pid)));
-                st_stream(nd + k, make_float4(next_d.x, next_d.y, next_d.z, 0.f));
+        const uint32_t k = wave_append(push, count_out);
+        if (push) {
+            st_stream(no + k, make_float4(next_o.x, next_o.y, next_o.z, __uint_as_float(pid)));
+            st_stream(nd + k, make_float4(next_d.x, next_d.y, next_d.z, 0.f));
+        }
+        if (nee) {
+            const uint32_t q = wave_append(shadow, shadow_count);
+            if (shadow) {
+                st_stream(P.sq_o + q, make_float4(sh_o.x, sh_o.y, sh_o.z, __uint_as_float(pid)));
+                st_stream(P.sq_d + q, make_float4(sh_d.x, sh_d.y, sh_d.z, 0.f));
+                st_stream(P.sq_c + q, make_float4(sh_c.x, sh_c.y, sh_c.z, 0.f));
             }
         }
     }
@@ -463,11 +637,11 @@ int grid_for(uint64_t n, int block, int max_blocks) {
 constexpr int MAX_BLOCKS = 256 * 16; // 256 CUs
 
 // Blocks of k_trace<...> that fit on the device at once (the persistent grid).
-template <bool COUNT, class Src, class Sink> int trace_resident_blocks() {
+template <bool COUNT, class Src, class Sink, bool ANY> int trace_resident_blocks() {
     static int blocks = 0;
     if (blocks == 0) {
         int per_cu = 0, dev = 0, cus = 0;
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<COUNT, Src, Sink>, TRACE_BLOCK, 0);
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<COUNT, Src, Sink, ANY>, TRACE_BLOCK, 0);
         hipGetDevice(&dev);
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         blocks = per_cu * cus > 0 ? per_cu * cus : 1;
@@ -477,17 +651,17 @@ template <bool COUNT, class Src, class Sink> int trace_resident_blocks() {
 
 // Launches the persistent trace kernel over at most n_max rays (the actual count is *count_ptr
 // when given). `work` must be zero on entry.
-template <bool COUNT, class Src, class Sink>
+template <bool COUNT, class Src, class Sink, bool ANY = false>
 hipError_t launch_trace(const DeviceScene& s, const TraceScratch& ts, const Src& src, const Sink& sink,
                         const uint32_t* count_ptr, uint32_t n_max, uint32_t* work, uint32_t mask,
                         unsigned long long* counters, hipStream_t st) {
     if (n_max == 0) return hipSuccess;
     if (s.stack_need > TRACE_LDS_STACK + ts.entries) return hipErrorInvalidValue; // overflow area too small
-    int g = grid_for(n_max, TRACE_BLOCK, trace_resident_blocks<COUNT, Src, Sink>());
+    int g = grid_for(n_max, TRACE_BLOCK, trace_resident_blocks<COUNT, Src, Sink, ANY>());
     if (uint64_t(g) * TRACE_BLOCK > ts.threads) g = int(ts.threads / TRACE_BLOCK);
     if (g <= 0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_trace<COUNT, Src, Sink>), dim3(g), dim3(TRACE_BLOCK), 0, st, s, src, sink,
-                       count_ptr, n_max, work, mask, ts, counters);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_trace<COUNT, Src, Sink, ANY>), dim3(g), dim3(TRACE_BLOCK), 0, st, s, src,
+                       sink, count_ptr, n_max, work, mask, ts, counters);
     return hipGetLastError();
 }
 
@@ -558,9 +732,10 @@ hipError_t launch_pt_frame(const DeviceScene& s, const TraceScratch& ts, PtArgs 
         if (k >= 0) kind.push_back(k);
     };
     mark(-1);
-    // queue sizes [chunk][bounce], then the trace kernels' ray cursors [chunk][bounce]
+    // [chunk][bounce] arrays: queue sizes, trace cursors, shadow-queue sizes, shadow cursors
     const uint32_t nz = n_chunks * per_chunk;
-    hipLaunchKernelGGL(k_zero_u32, dim3((2 * nz + 255) / 256), dim3(256), 0, st, pb.counters, 2 * nz);
+    hipLaunchKernelGGL(k_zero_u32, dim3((4 * nz + 255) / 256), dim3(256), 0, st, pb.counters, 4 * nz);
+    const bool nee = a.mode != MRT_MODE_BSDF;
     hipError_t err = hipSuccess;
     for (uint32_t c = 0; c < n_chunks && err == hipSuccess; c++) {
         a.s0 = c * chunk;
@@ -568,6 +743,8 @@ hipError_t launch_pt_frame(const DeviceScene& s, const TraceScratch& ts, PtArgs 
         a.n_paths = a.npix * a.chunk;
         uint32_t* cnt = pb.counters + c * per_chunk;
         uint32_t* work = pb.counters + nz + c * per_chunk;
+        uint32_t* scnt = pb.counters + 2 * nz + c * per_chunk;
+        uint32_t* swork = pb.counters + 3 * nz + c * per_chunk;
         hipLaunchKernelGGL(k_pt_init, dim3(grid_for(a.n_paths, 256, MAX_BLOCKS)), dim3(256), 0, st, s, a, gbuffer,
                            pb, cnt);
         for (int b = 0; b < L && err == hipSuccess; b++) {
@@ -583,10 +760,23 @@ hipError_t launch_pt_frame(const DeviceScene& s, const TraceScratch& ts, PtArgs 
                 mark(0);
             }
             mark(-1);
-            hipLaunchKernelGGL(k_shade, dim3(grid_for(a.n_paths, SHADE_BLOCK, MAX_BLOCKS)), dim3(SHADE_BLOCK), 0, st,
-                               s, a, b, pb, pb.q_o[qi], pb.q_d[qi], cnt + b, pb.q_o[qi ^ 1], pb.q_d[qi ^ 1],
-                               cnt + b + 1);
+            if (nee) {
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<true>), dim3(grid_for(a.n_paths, SHADE_BLOCK, MAX_BLOCKS)),
+                                   dim3(SHADE_BLOCK), 0, st, s, a, b, pb, pb.q_o[qi], pb.q_d[qi], cnt + b,
+                                   pb.q_o[qi ^ 1], pb.q_d[qi ^ 1], cnt + b + 1, scnt + b);
+            } else {
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<false>), dim3(grid_for(a.n_paths, SHADE_BLOCK, MAX_BLOCKS)),
+                                   dim3(SHADE_BLOCK), 0, st, s, a, b, pb, pb.q_o[qi], pb.q_d[qi], cnt + b,
+                                   pb.q_o[qi ^ 1], pb.q_d[qi ^ 1], cnt + b + 1, scnt + b);
+            }
             mark(1);
+            if (nee && b + 1 < L && err == hipSuccess) {
+                mark(-1);
+                err = launch_trace<false, SrcShadow, SinkShadow, true>(
+                    s, ts, SrcShadow{pb.sq_o, pb.sq_d}, SinkShadow{pb.sq_o, pb.sq_c, pb.state_b}, scnt + b, a.n_paths,
+                    swork + b, a.instance_mask, nullptr, st);
+                mark(0);
+            }
         }
         hipLaunchKernelGGL(k_accumulate, dim3(grid_for(a.npix, 256, MAX_BLOCKS)), dim3(256), 0, st, a, pb,
                            int(c == 0), int(c + 1 == n_chunks), out, sig_out);
@@ -617,6 +807,11 @@ hipError_t launch_pt_frame(const DeviceScene& s, const TraceScratch& ts, PtArgs 
         for (uint32_t c = 0; c < n_chunks; c++)
             for (int b = 1; b < L; b++) segs += h[c * per_chunk + b];
         *continuation_segments = segs;
+        if (timing && nee) {
+            std::vector<uint32_t> hs(nz);
+            hipMemcpy(hs.data(), pb.counters + 2 * nz, sizeof(uint32_t) * nz, hipMemcpyDeviceToHost);
+            for (uint32_t v : hs) timing->shadow_rays += v;
+        }
     }
     return hipGetLastError();
 }

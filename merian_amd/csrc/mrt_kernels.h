@@ -20,6 +20,7 @@ struct DeviceScene {
     const uint32_t* geom_first;  // global id of each geometry's first triangle
     const mrt_material* mats;    // material entries
     const float4* env;           // environment texels (RGBA32F), row-major
+    const float4* env_samp;      // per texel: alias-table probability, alias index bits, pmf, 0
     int32_t env_type, env_wrap;
     uint32_t env_w, env_h;
     float env_intensity;
@@ -50,6 +51,7 @@ struct PtArgs {
     uint32_t width, height, npix, frame;
     int32_t spp, max_path_length, emission_on_primary, record_signature;
     uint32_t instance_mask;
+    int32_t mode;     // MRT_MODE_*
     uint32_t s0;      // first sample index of this chunk
     uint32_t chunk;   // samples in this chunk
     uint32_t n_paths; // npix * chunk
@@ -64,6 +66,10 @@ struct PathBuffers {
     float4* q_d[2];    // ray queue: direction.xyz
     float4* hits;      // t, u, v, triangle id bits; indexed like the queue being shaded
     uint32_t* counters;// per chunk and bounce: queue sizes, then the trace kernels' ray cursors
+    float* aux;        // pdf of the last direction sampled (MIS weight of an escaping ray)
+    float4* sq_o;      // shadow queue: origin.xyz, path index bits
+    float4* sq_d;      // shadow queue: direction.xyz
+    float4* sq_c;      // shadow queue: unoccluded contribution.rgb
     float4* accum;     // per pixel running sum: radiance.xyz, luminance^2
     uint32_t* sig_accum;
     uint64_t capacity; // paths
@@ -80,6 +86,7 @@ hipError_t launch_gbuffer(const DeviceScene& s, const TraceScratch& ts, const mr
 struct FrameTiming {
     float ms_trace = 0.f, ms_shade = 0.f, ms_total = 0.f;
     uint32_t trace_launches = 0;
+    uint64_t shadow_rays = 0;
 };
 // One frame of pt over `gbuffer`; out = float4 irradiance per pixel; sig_out optional.
 // counters_host (optional) receives every chunk's queue sizes after the frame completes.

@@ -124,8 +124,9 @@ __device__ __forceinline__ void ray_begin(RayState& r, f3 o, f3 d, float tmin, f
 }
 
 // One traversal step: one inner node (test its four children, push the hit ones far to near,
-// continue with the nearest) or one leaf (test its triangles, pop).
-template <int SL, bool COUNT>
+// continue with the nearest) or one leaf (test its triangles, pop). ANY: stop at the first hit
+// (visibility queries, Scene.trace_visibility_ray, scene.slang:82-90).
+template <int SL, bool COUNT, bool ANY = false>
 __device__ __forceinline__ void ray_step(const DeviceBVH bvh, RayState& r, const LaneStack& stk, uint32_t ray_mask,
                                          TraversalCount& count) {
     if (r.cur & LEAF_FLAG) {
@@ -133,6 +134,12 @@ __device__ __forceinline__ void ray_step(const DeviceBVH bvh, RayState& r, const
         const uint32_t cnt = (r.cur & 7u) + 1u;
         for (uint32_t k = 0; k < cnt; k++) test_triangle(bvh.tris, first + k, r.o, r.d, r.tmin, r.tmax, ray_mask, r.best);
         if constexpr (COUNT) count.tris += cnt;
+        if constexpr (ANY) {
+            if (r.best.id != TRAVERSAL_DONE) {
+                r.cur = TRAVERSAL_DONE;
+                return;
+            }
+        }
         r.cur = r.sp > 0 ? stack_pop<SL>(stk, r.sp) : TRAVERSAL_DONE;
         return;
     }
@@ -183,7 +190,7 @@ constexpr int REFILL_LANES = 16;
 // Persistent closest-hit loop of one wave: every lane runs one traversal step per iteration and,
 // when its ray is done, hands the hit to `sink` and later takes a fresh ray from `src`, so lanes
 // with short rays do not idle while others finish long ones. `work` is the launch's ray cursor.
-template <int SL, bool COUNT, class Src, class Sink>
+template <int SL, bool COUNT, class Src, class Sink, bool ANY = false>
 __device__ __forceinline__ void trace_persistent(const DeviceBVH bvh, const Src& src, const Sink& sink, uint32_t n,
                                                  uint32_t* work, uint32_t ray_mask, const LaneStack& stk,
                                                  TraversalCount& count) {
@@ -215,7 +222,7 @@ __device__ __forceinline__ void trace_persistent(const DeviceBVH bvh, const Src&
         }
         if (__ballot(active) == 0) break;
         if (active) {
-            ray_step<SL, COUNT>(bvh, r, stk, ray_mask, count);
+            ray_step<SL, COUNT, ANY>(bvh, r, stk, ray_mask, count);
             if (r.cur == TRAVERSAL_DONE) {
                 sink.store(idx, r.best);
                 active = false;
