@@ -25,6 +25,9 @@ HBM_PEAK_GBS = 8000.0
 NODE_BYTES = 128      # BVH4Node
 TRI_BYTES = 48        # TriRecord
 RAY_IO_BYTES = 32 + 16  # queued ray in, hit out
+# HBM bytes per dispatch of the intersect kernel, measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+# passes over this workload and calibrated against a plain copy (python -m merian_amd.profiling traffic)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_trace_queue_full_1080p.traffic.json")
 
 
 def log(*a):
@@ -42,14 +45,17 @@ def cpu_baseline(scene, spec, frame, target_s: float, threads: int):
     t_gb = time.perf_counter() - t0
     p = O.pt_params(64, 8, record_signature=False)
     cw, ch = 32, 18
-    t0 = time.perf_counter()
-    _, _, segs = osc.render(p, cam, frame, W, H, gb, rect=((W - cw) // 2, (H - ch) // 2, cw, ch), threads=threads)
-    t_cal = time.perf_counter() - t0
-    scale = max(1.0, (target_s / max(t_cal, 1e-3)) ** 0.5)
-    cw, ch = min(W, int(cw * scale) // 2 * 2), min(H, int(ch * scale) // 2 * 2)
-    t0 = time.perf_counter()
-    _, _, segs = osc.render(p, cam, frame, W, H, gb, rect=((W - cw) // 2, (H - ch) // 2, cw, ch), threads=threads)
-    t_pt = time.perf_counter() - t0
+    # grow the centre crop until the timed render takes at least half of target_s
+    t_pt = 0.0
+    for _ in range(4):
+        t0 = time.perf_counter()
+        _, _, segs = osc.render(p, cam, frame, W, H, gb, rect=((W - cw) // 2, (H - ch) // 2, cw, ch),
+                                threads=threads)
+        t_pt = time.perf_counter() - t0
+        if t_pt >= 0.5 * target_s or (cw == W and ch == H):
+            break
+        scale = min(4.0, max(1.2, (target_s / max(t_pt, 1e-3)) ** 0.5))
+        cw, ch = min(W, int(cw * scale) // 2 * 2), min(H, int(ch * scale) // 2 * 2)
     npx = cw * ch
     t = t_pt + t_gb * npx / (W * H)
     return {"value": round((segs + npx) / t / 1e6, 4), "unit": "Mpath-segments/s", "cores": threads,
@@ -127,8 +133,13 @@ def main():
     bytes_per_launch = bytes_per_seg * cont_timed / max(1, launches)
     avg_ms = ms_trace / max(1, launches)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            traffic = json.load(f)["traffic_bytes_per_dispatch"]
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "algorithmic_bytes_per_launch": round(bytes_per_launch),
             "kernel": "k_trace_queue", "avg_launch_ms": round(avg_ms, 4), "launches": launches,
             "bytes_per_segment": round(bytes_per_seg, 1),
             "node_visits_per_segment": round(st_c.node_visits / cont, 2),
