@@ -304,6 +304,182 @@ MRT_API mrt_status mrt_render_frame(mrt_ctx* ctx, mrt_scene* scene, const mrt_pt
                                     mrt_packed_hit* gbuffer, mrt_pt_stats* stats);
 
 /* ------------------------------------------------------------------------------------
+ * Serving loop: continuous batching of render jobs (gbuffer + pt of one frame each).
+ *
+ * No counterpart in the reference, which renders one frame per Graph::run
+ * (src/merian-graph/graph/graph.cpp:68-253). A server owns a scheduler thread, up to
+ * max_slots jobs in flight (one HIP stream each, so their kernels overlap on the
+ * device) and an HBM budget. Jobs are admitted in submission order as soon as a slot
+ * is free and their buffers fit the budget; jobs that arrive while others run join
+ * without waiting for them. Each job's launches are captured in a hipGraph, replayed
+ * through the slot's executable graph (updated in place when the shape repeats).
+ * Per-job RNG depends on (pixel, frame, sample) only, so a job's result is the same
+ * alone and in any mix, bit for bit.
+ * ---------------------------------------------------------------------------------- */
+typedef struct mrt_server mrt_server;
+
+typedef struct mrt_server_config {
+    uint64_t hbm_budget;         /* bytes for the buffers of jobs in flight; reserved from the
+                                    context's budget at creation (0 = half of what is left) */
+    uint32_t max_slots;          /* jobs in flight at once (0 = 4, at most 16) */
+    uint32_t max_paths_per_job;  /* paths in flight per job (0 = 2^22); more spp -> chunks */
+    int32_t use_graphs;          /* 1: capture each job in a hipGraph; 0: launch directly */
+    uint32_t _pad;
+} mrt_server_config;
+
+typedef struct mrt_job_desc {
+    mrt_scene* scene;            /* resident scene (must outlive the job) */
+    mrt_camera camera;
+    mrt_pt_params params;        /* mode BSDF or NEE; record_signature/count_traversal ignored */
+    uint32_t width, height;
+    uint32_t frame;              /* RNG frame index of this job */
+    uint32_t _pad;
+    float* irradiance;           /* host float4[width*height], written when the job is done */
+} mrt_job_desc;
+
+enum {
+    MRT_JOB_QUEUED = 0,
+    MRT_JOB_RUNNING = 1,
+    MRT_JOB_DONE = 2,
+    MRT_JOB_CANCELLED = 3,
+    MRT_JOB_FAILED = 4
+};
+
+typedef struct mrt_job_status {
+    int32_t state;               /* MRT_JOB_* */
+    mrt_status error;            /* status of a failed job */
+    uint64_t segments;           /* closest-hit segments traced (primary + continuation) */
+    uint64_t device_bytes;       /* buffers the job held while in flight */
+    double queued_ms;            /* submit -> admitted */
+    double latency_ms;           /* submit -> result in the caller's buffer */
+} mrt_job_status;
+
+typedef struct mrt_server_stats {
+    uint64_t submitted, completed, cancelled, failed;
+    uint64_t segments;           /* over completed jobs */
+    double p50_ms, p99_ms, max_ms, mean_ms; /* latency of completed jobs (nearest rank) */
+    double wall_ms;              /* first submit -> last completion */
+    uint64_t peak_bytes;         /* most buffer bytes held at once */
+    uint32_t peak_in_flight;     /* most jobs in flight at once */
+    uint32_t graph_instantiations, graph_updates, _pad;
+} mrt_server_stats;
+
+MRT_API mrt_status mrt_server_create(mrt_ctx* ctx, const mrt_server_config* config,
+                                     mrt_server** out);
+/* Cancels queued jobs, waits for jobs in flight, releases the reservation. */
+MRT_API mrt_status mrt_server_destroy(mrt_server* server);
+MRT_API const char* mrt_server_last_error(mrt_server* server);
+/* Device bytes a job needs while in flight (its admission charge). */
+MRT_API mrt_status mrt_job_bytes(mrt_server* server, const mrt_job_desc* job, uint64_t* bytes);
+/* Queues a job; thread-safe. MRT_ERR_BUDGET if it could never fit the server's budget. */
+MRT_API mrt_status mrt_job_submit(mrt_server* server, const mrt_job_desc* job, uint64_t* job_id);
+/* MRT_OK with the status filled in (any state); MRT_ERR_INVALID for an unknown id. */
+MRT_API mrt_status mrt_job_poll(mrt_server* server, uint64_t job_id, mrt_job_status* status);
+/* Waits until the job is done, cancelled or failed, or timeout_ms passes (< 0: no limit;
+ * MRT_ERR_NOT_READY on time-out). */
+MRT_API mrt_status mrt_job_wait(mrt_server* server, uint64_t job_id, int32_t timeout_ms,
+                                mrt_job_status* status);
+/* A queued job is dropped; a job in flight finishes on the device but its result is not
+ * written. MRT_ERR_INVALID if the job is unknown or already finished. */
+MRT_API mrt_status mrt_job_cancel(mrt_server* server, uint64_t job_id);
+MRT_API mrt_status mrt_server_stats_get(mrt_server* server, mrt_server_stats* stats);
+/* Clears the latency history and counters (jobs in flight keep running). */
+MRT_API mrt_status mrt_server_stats_reset(mrt_server* server);
+
+/* ------------------------------------------------------------------------------------
+ * Learned path guiding (MRT_MODE_GUIDED).
+ *
+ * No counterpart in the reference's pt; the closest analogues are the MCPG renderer's
+ * guided one-sample MIS (render_pt_mcpg.slang:104-136,181) and its von Mises-Fisher lobes
+ * (von-mises-fisher.slang:29-45). A network maps a shading position and the outgoing
+ * direction to an 8-lobe vMF mixture over incident directions:
+ *   input   64 = multiresolution hash grid (16 levels x 2 features, fp16 tables)
+ *                + 16 real spherical harmonics (degree <= 3) of the direction to the viewer
+ *                + 16 zeros
+ *   layers  4 hidden layers of width 64 (ReLU), linear output of 40 = 8 x (logit, log kappa,
+ *           mu.xyz); no biases; fp16 weights and activations, fp32 accumulation (MFMA)
+ * It is trained online on the frame's own radiance samples (loss: -sum w log q(dir) /
+ * sum w, w = incident luminance / sampling pdf) with Adam. At an eligible vertex the
+ * renderer samples the mixture with probability guide_fraction, else the BSDF, and weighs
+ * by the one-sample MIS pdf, so the estimator stays unbiased.
+ * ---------------------------------------------------------------------------------- */
+typedef struct mrt_guide mrt_guide;
+
+enum { MRT_GUIDE_LEVELS = 16, MRT_GUIDE_WIDTH = 64, MRT_GUIDE_LAYERS = 5, MRT_GUIDE_OUTPUTS = 40,
+       MRT_GUIDE_LOBES = 8 };
+
+typedef struct mrt_guide_config {
+    float bbox_min[3], bbox_max[3]; /* position normalisation box (all zero: scene bounds) */
+    uint32_t seed;                  /* weight initialisation */
+    uint32_t log2_table;            /* hash-table entries per level (0 = 19) */
+    uint32_t base_resolution;       /* coarsest grid resolution (0 = 16) */
+    uint32_t max_resolution;        /* finest grid resolution (0 = 2048) */
+    uint32_t train_batch;           /* records per training step (0 = 65536) */
+    uint32_t train_steps;           /* training steps per rendered frame (0 = 4) */
+    uint32_t max_records;           /* training records collected per frame (0 = 2^20) */
+    float learning_rate;            /* Adam step size (0 = 5e-3) */
+    float guide_fraction;           /* probability of sampling the guide (0 = 0.5) */
+    uint32_t _pad;
+} mrt_guide_config;
+
+/* Per-level grid resolutions and table size the guide uses (host-side computation). */
+MRT_API mrt_status mrt_guide_layout(const mrt_guide_config* config, uint32_t resolutions[16],
+                                    uint32_t* table_entries);
+MRT_API mrt_status mrt_guide_create(mrt_ctx* ctx, mrt_scene* scene, const mrt_guide_config* config,
+                                    mrt_guide** out);
+MRT_API mrt_status mrt_guide_destroy(mrt_ctx* ctx, mrt_guide* guide);
+/* Position normalisation in use: p' = clamp((p - offset) * scale, 0, 1) per axis. */
+MRT_API mrt_status mrt_guide_normalisation(mrt_guide* guide, float offset[3], float scale[3]);
+/* Parameters as fp32 (host arrays): weights [5][64][64] (row = output; layer 5 rows >= 40 are
+ * zero), grid [16][table][2]. Either pointer may be NULL. */
+MRT_API mrt_status mrt_guide_get_params(mrt_ctx* ctx, mrt_guide* guide, float* weights, float* grid);
+/* Replaces the parameters (rounded to fp16 for the kernels) and clears the Adam state. */
+MRT_API mrt_status mrt_guide_set_params(mrt_ctx* ctx, mrt_guide* guide, const float* weights,
+                                        const float* grid);
+/* Network outputs (40 fp32 per query) for host positions/directions (float[3*n] each). */
+MRT_API mrt_status mrt_guide_forward(mrt_ctx* ctx, mrt_guide* guide, const float* positions,
+                                     const float* directions, uint32_t n, float* outputs);
+/* Training records: float[12*n] = (position.xyz, weight, wo.xyz, 0, dir.xyz, 0); weight =
+ * incident luminance / sampling pdf. Gradients of the loss (sum over records, scaled by
+ * n / sum of weights) without an update: weights [5][64][64], grid [16][table][2] (host
+ * fp32, either may be NULL); *loss = mean weighted -log q. */
+MRT_API mrt_status mrt_guide_gradient(mrt_ctx* ctx, mrt_guide* guide, const float* records,
+                                      uint32_t n, float* d_weights, float* d_grid, float* loss);
+/* `steps` Adam steps over consecutive batches of the host records; *loss = last step's loss. */
+MRT_API mrt_status mrt_guide_train(mrt_ctx* ctx, mrt_guide* guide, const float* records,
+                                   uint32_t n, uint32_t steps, float* loss);
+/* One guided frame (MRT_MODE_GUIDED): as mrt_pt_render_dev, sampling the guide at eligible
+ * vertices; with train != 0 the frame's records then train the guide (train_steps steps). */
+MRT_API mrt_status mrt_guided_render_dev(mrt_ctx* ctx, mrt_scene* scene, mrt_guide* guide,
+                                         const mrt_pt_params* params, const mrt_camera* cam,
+                                         uint32_t frame, uint32_t width, uint32_t height,
+                                         const mrt_packed_hit* d_gbuffer, float* d_irradiance,
+                                         int32_t train, mrt_pt_stats* stats, void* stream);
+
+/* ------------------------------------------------------------------------------------
+ * Graph descriptions (host only, no device work): merian's graph JSON (schema v4, v1-v3
+ * read), fragment merge, includes and command-line overrides, as GraphDescription does
+ * (src/merian-graph/graph/graph_description.cpp:119-857). Strings are UTF-8 JSON; every
+ * returned string is allocated by the library and released with mrt_string_free.
+ * search_dirs is a ':'-separated list (may be NULL). errors (may be NULL) receives the
+ * messages of a failed call, one per line.
+ * ---------------------------------------------------------------------------------- */
+/* GraphDescription::apply_cli: MRT_OK when every override applied, MRT_ERR_INVALID with
+ * the (partially) modified config otherwise. */
+MRT_API mrt_status mrt_graph_apply_cli(const char* config_json, int32_t argc, const char* const* argv,
+                                       const char* search_dirs, char** out_json, char** errors);
+/* GraphDescription::resolve_includes. */
+MRT_API mrt_status mrt_graph_resolve_includes(const char* config_json, const char* search_dirs,
+                                              char** out_json, char** errors);
+/* GraphDescription::merge_into(base, overwrite). */
+MRT_API mrt_status mrt_graph_merge(const char* base_json, const char* overwrite_json, char** out_json);
+/* GraphDescription::from_json + to_json: any schema version in, v4 out. */
+MRT_API mrt_status mrt_graph_store(const char* graph_json, char** out_json, char** errors);
+/* GraphDescription::cli_help. */
+MRT_API mrt_status mrt_graph_cli_help(const char* config_json, const char* search_dirs, char** out_text);
+MRT_API void mrt_string_free(char* s);
+
+/* ------------------------------------------------------------------------------------
  * Node plugin ABI (include/merian-graph/plugin/node_plugin.hpp:17-29). The registry
  * argument is this library's node registry (merian_amd/host), not the Vulkan one.
  * ---------------------------------------------------------------------------------- */

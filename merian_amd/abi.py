@@ -111,7 +111,49 @@ class PtStats(C.Structure):
                 ("triangle_tests", C.c_uint64)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("max_slots", C.c_uint32), ("max_paths_per_job", C.c_uint32),
+                ("use_graphs", C.c_int32), ("_pad", C.c_uint32)]
+
+
+class JobDesc(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("camera", Camera), ("params", PtParams), ("width", C.c_uint32),
+                ("height", C.c_uint32), ("frame", C.c_uint32), ("_pad", C.c_uint32),
+                ("irradiance", C.c_void_p)]
+
+
+JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_CANCELLED, JOB_FAILED = 0, 1, 2, 3, 4
+JOB_STATE_NAMES = {0: "queued", 1: "running", 2: "done", 3: "cancelled", 4: "failed"}
+
+
+class JobStatus(C.Structure):
+    _fields_ = [("state", C.c_int32), ("error", C.c_int32), ("segments", C.c_uint64),
+                ("device_bytes", C.c_uint64), ("queued_ms", C.c_double), ("latency_ms", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("cancelled", C.c_uint64),
+                ("failed", C.c_uint64), ("segments", C.c_uint64), ("p50_ms", C.c_double),
+                ("p99_ms", C.c_double), ("max_ms", C.c_double), ("mean_ms", C.c_double),
+                ("wall_ms", C.c_double), ("peak_bytes", C.c_uint64), ("peak_in_flight", C.c_uint32),
+                ("graph_instantiations", C.c_uint32), ("graph_updates", C.c_uint32), ("_pad", C.c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("_")}
+
+
+class GuideConfig(C.Structure):
+    _fields_ = [("bbox_min", C.c_float * 3), ("bbox_max", C.c_float * 3), ("seed", C.c_uint32),
+                ("log2_table", C.c_uint32), ("base_resolution", C.c_uint32), ("max_resolution", C.c_uint32),
+                ("train_batch", C.c_uint32), ("train_steps", C.c_uint32), ("max_records", C.c_uint32),
+                ("learning_rate", C.c_float), ("guide_fraction", C.c_float), ("_pad", C.c_uint32)]
+
+
+GUIDE_LEVELS, GUIDE_WIDTH, GUIDE_LAYERS, GUIDE_OUTPUTS, GUIDE_LOBES = 16, 64, 5, 40, 8
+
 assert C.sizeof(SceneDesc) == 128 and C.sizeof(Camera) == 64 and C.sizeof(PtParams) == 32
+assert C.sizeof(ServerConfig) == 24 and C.sizeof(JobDesc) == 128 and C.sizeof(JobStatus) == 40
+assert C.sizeof(ServerStats) == 104 and C.sizeof(GuideConfig) == 64
 
 
 def ptr(a: np.ndarray) -> C.c_void_p:
@@ -155,6 +197,38 @@ EXPORTS = [
     ("mrt_render_frame", C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(PtParams),
                                      C.POINTER(Camera), C.c_uint32, C.c_uint32, C.c_uint32,
                                      C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(PtStats)]),
+    ("mrt_server_create", C.c_int32, [C.c_void_p, C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
+    ("mrt_server_destroy", C.c_int32, [C.c_void_p]),
+    ("mrt_server_last_error", C.c_char_p, [C.c_void_p]),
+    ("mrt_job_bytes", C.c_int32, [C.c_void_p, C.POINTER(JobDesc), C.POINTER(C.c_uint64)]),
+    ("mrt_job_submit", C.c_int32, [C.c_void_p, C.POINTER(JobDesc), C.POINTER(C.c_uint64)]),
+    ("mrt_job_poll", C.c_int32, [C.c_void_p, C.c_uint64, C.POINTER(JobStatus)]),
+    ("mrt_job_wait", C.c_int32, [C.c_void_p, C.c_uint64, C.c_int32, C.POINTER(JobStatus)]),
+    ("mrt_job_cancel", C.c_int32, [C.c_void_p, C.c_uint64]),
+    ("mrt_server_stats_get", C.c_int32, [C.c_void_p, C.POINTER(ServerStats)]),
+    ("mrt_server_stats_reset", C.c_int32, [C.c_void_p]),
+    ("mrt_guide_layout", C.c_int32, [C.POINTER(GuideConfig), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    ("mrt_guide_create", C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(GuideConfig), C.POINTER(C.c_void_p)]),
+    ("mrt_guide_destroy", C.c_int32, [C.c_void_p, C.c_void_p]),
+    ("mrt_guide_normalisation", C.c_int32, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    ("mrt_guide_get_params", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("mrt_guide_set_params", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("mrt_guide_forward", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+    ("mrt_guide_gradient", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
+                                       C.POINTER(C.c_float)]),
+    ("mrt_guide_train", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
+                                    C.POINTER(C.c_float)]),
+    ("mrt_guided_render_dev", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(PtParams),
+                                          C.POINTER(Camera), C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
+                                          C.c_void_p, C.c_int32, C.POINTER(PtStats), C.c_void_p]),
+    ("mrt_graph_apply_cli", C.c_int32, [C.c_char_p, C.c_int32, C.POINTER(C.c_char_p), C.c_char_p,
+                                        C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
+    ("mrt_graph_resolve_includes", C.c_int32, [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p),
+                                               C.POINTER(C.c_void_p)]),
+    ("mrt_graph_merge", C.c_int32, [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p)]),
+    ("mrt_graph_store", C.c_int32, [C.c_char_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
+    ("mrt_graph_cli_help", C.c_int32, [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p)]),
+    ("mrt_string_free", None, [C.c_void_p]),
     ("merian_node_plugin_abi_version", C.c_uint32, []),
 ]
 

@@ -17,7 +17,8 @@ OBJ = PKG / "_build"
 LIB = PKG / "libmerian_amd.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["kernels.hip", "api.cpp", "bvh.cpp", "serve.cpp", "guide.hip", "nee.hip", "graph_host.cpp"]
+SOURCES = ["csrc/kernels.hip", "csrc/api.cpp", "csrc/bvh.cpp", "csrc/serve.cpp", "csrc/guide.hip",
+           "csrc/guide_api.cpp", "host/graph_description.cpp", "host/graph_abi.cpp"]
 
 # -ffp-contract=off: expressions round as written (the CPU oracle uses the same rule).
 # -packed-fp32-ops off: no v_pk_*_f32 generation.
@@ -40,7 +41,7 @@ def _compile(src: Path) -> Path:
 
 def build(jobs: int = 8) -> Path:
     OBJ.mkdir(exist_ok=True)
-    srcs = [CSRC / s for s in SOURCES if (CSRC / s).exists()]
+    srcs = [PKG / s for s in SOURCES]
     with ThreadPoolExecutor(max_workers=max(1, min(jobs, 16))) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB.with_suffix(".so.tmp")

@@ -313,6 +313,18 @@ MRT_API mrt_status mrt_scene_create(mrt_ctx* c, const mrt_scene_desc* d, mrt_sce
     }
     const BVH4 bvh = build_bvh4(v0.data(), e1.data(), e2.data(), flags.data(), uint32_t(n_tris), 0);
     mrt_scene* s = new mrt_scene();
+    if (n_tris) {
+        for (int k = 0; k < 3; k++) {
+            s->bbox_min[k] = v0[k];
+            s->bbox_max[k] = v0[k];
+        }
+        for (uint64_t t = 0; t < n_tris; t++)
+            for (int k = 0; k < 3; k++) {
+                const float a = v0[3 * t + k], b = a + e1[3 * t + k], cc = a + e2[3 * t + k];
+                s->bbox_min[k] = std::min(s->bbox_min[k], std::min(a, std::min(b, cc)));
+                s->bbox_max[k] = std::max(s->bbox_max[k], std::max(a, std::max(b, cc)));
+            }
+    }
     s->ds.stack_need = 3u * bvh.depth + 4u;
     s->n_tris = n_tris;
     s->n_nodes = bvh.nodes.size();
@@ -535,14 +547,12 @@ mrt_status check_params(mrt_ctx* c, const mrt_pt_params* p) {
 }
 } // namespace mrt
 
-extern "C" {
-
-MRT_API mrt_status mrt_pt_render_dev(mrt_ctx* c, mrt_scene* s, const mrt_pt_params* prm, const mrt_camera* cam,
-                                     uint32_t frame, uint32_t w, uint32_t h, const mrt_packed_hit* gbuffer,
-                                     float* irradiance, uint32_t* signature, mrt_pt_stats* stats, void* stream) {
-    if (!c || !s || !cam) return fail(c, MRT_ERR_INVALID, "mrt_pt_render_dev: null argument");
-    mrt_status st = check_params(c, prm);
-    if (st != MRT_OK) return st;
+namespace mrt {
+mrt_status render_frame_dev(mrt_ctx* c, mrt_scene* s, const mrt_pt_params* prm, const mrt_camera* cam, uint32_t frame,
+                            uint32_t w, uint32_t h, const mrt_packed_hit* gbuffer, float* irradiance,
+                            uint32_t* signature, mrt_pt_stats* stats, void* stream, mrt_guide* guide) {
+    if (!c || !s || !cam) return fail(c, MRT_ERR_INVALID, "pt frame: null argument");
+    mrt_status st = MRT_OK;
     const uint64_t npix = uint64_t(w) * h;
     if (npix == 0) return MRT_OK;
     if (!gbuffer || !irradiance) return fail(c, MRT_ERR_INVALID, "mrt_pt_render_dev: null buffer");
@@ -571,6 +581,11 @@ MRT_API mrt_status mrt_pt_render_dev(mrt_ctx* c, mrt_scene* s, const mrt_pt_para
     a.record_signature = prm->record_signature && signature ? 1 : 0;
     a.instance_mask = prm->instance_mask & 0xFFu;
     a.mode = prm->mode;
+    const GuideNet* net = nullptr;
+    if (guide) {
+        st = guide_prepare_frame(c, guide, npix * chunk, pb, a.guide_fraction, &net);
+        if (st != MRT_OK) return st;
+    }
     FrameTiming t;
     uint64_t segs = 0;
     const hipStream_t strm = pick_stream(c, stream);
@@ -582,7 +597,7 @@ MRT_API mrt_status mrt_pt_render_dev(mrt_ctx* c, mrt_scene* s, const mrt_pt_para
     }
     const hipError_t e = launch_pt_frame(s->ds, c->trace, a, gbuffer, pb, reinterpret_cast<float4*>(irradiance),
                                          a.record_signature ? signature : nullptr, strm, stats ? &t : nullptr,
-                                         stats ? &segs : nullptr, tc);
+                                         stats ? &segs : nullptr, tc, net);
     unsigned long long hc[2] = {0, 0};
     if (tc) {
         hipMemcpyAsync(hc, tc, 16, hipMemcpyDeviceToHost, strm);
@@ -590,6 +605,10 @@ MRT_API mrt_status mrt_pt_render_dev(mrt_ctx* c, mrt_scene* s, const mrt_pt_para
         dev_free(c, tc);
     }
     if (e != hipSuccess) return hip_fail(c, e, "pt frame");
+    if (guide) {
+        st = guide_train_frame(c, guide, strm);
+        if (st != MRT_OK) return st;
+    }
     if (stats) {
         stats->primary_segments = npix;
         stats->continuation_segments = segs;
@@ -602,6 +621,17 @@ MRT_API mrt_status mrt_pt_render_dev(mrt_ctx* c, mrt_scene* s, const mrt_pt_para
         stats->triangle_tests = hc[1];
     }
     return MRT_OK;
+}
+} // namespace mrt
+
+extern "C" {
+
+MRT_API mrt_status mrt_pt_render_dev(mrt_ctx* c, mrt_scene* s, const mrt_pt_params* prm, const mrt_camera* cam,
+                                     uint32_t frame, uint32_t w, uint32_t h, const mrt_packed_hit* gbuffer,
+                                     float* irradiance, uint32_t* signature, mrt_pt_stats* stats, void* stream) {
+    const mrt_status st = check_params(c, prm);
+    if (st != MRT_OK) return st;
+    return render_frame_dev(c, s, prm, cam, frame, w, h, gbuffer, irradiance, signature, stats, stream, nullptr);
 }
 
 MRT_API mrt_status mrt_render_frame(mrt_ctx* c, mrt_scene* s, const mrt_pt_params* prm, const mrt_camera* cam,

@@ -6,6 +6,7 @@
 // each bounce is two kernels over a device queue of live paths: a lean traversal kernel (LDS
 // stack, low VGPR count, high occupancy) and a shading kernel that samples the material and
 // appends the continuation ray with one wave-aggregated atomic per wavefront.
+#include "mrt_guide.h"
 #include "mrt_kernels.h"
 #include "mrt_trace.h"
 
@@ -359,6 +360,7 @@ __device__ __forceinline__ float power_heuristic(float a, float b) {
 
 struct ShadingPoint {
     f3 pos;
+    uint32_t material; // material entry of a surface hit
     uint32_t enc_face_normal;
     bool env; // the ray left the scene (or hit a UseEnvMap mesh)
     f3 emission;
@@ -431,6 +433,7 @@ __device__ __forceinline__ void shading_point(const DeviceScene& S, f3 ro, f3 rd
         if (!front) frame.r2 = -frame.r2;
     }
     sp.pos = pos;
+    sp.material = G.material_id;
     sp.enc_face_normal = encode_normal(face_normal);
     sp.has_material = with_material;
     const mrt_material& m = S.mats[G.material_id];
@@ -447,8 +450,8 @@ __device__ __forceinline__ void shading_point(const DeviceScene& S, f3 ro, f3 rd
 // wavefront pt
 
 __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0u;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0u;
 }
 
 // Seeds one path per (pixel, sample) of the chunk with the primary ray and gbuffer hit.
@@ -493,11 +496,22 @@ __device__ __forceinline__ uint32_t wave_append(bool push, uint32_t* counter) {
     return slot + uint32_t(__popcll(ballot & ((1ull << lane) - 1ull)));
 }
 
+// Vertices where the guide is sampled: opaque surfaces that are not near-specular (the
+// mixture cannot follow a near-delta lobe; the BSDF alone samples those).
+__device__ __forceinline__ bool guide_eligible(const mrt_material& m) {
+    if (m.model == MRT_MAT_DIFFUSE) return true;
+    if (m.transmission_weight > 0.f) return false;
+    return !(m.metalness > 0.f && m.specular_alpha[0] * m.specular_alpha[1] < 1e-4f);
+}
+
 // Shades vertex `bounce` of every queued path and appends its continuation ray
 // (trace_path, render_pt.slang:24-90). In MRT_MODE_NEE it also samples the environment
 // (one shadow ray per vertex, power-heuristic MIS with the BSDF sample; an escaping BSDF-sampled
-// ray then carries the complementary weight).
-template <bool NEE>
+// ray then carries the complementary weight). In MRT_MODE_GUIDED an eligible vertex samples
+// the guide's lobes with probability A.guide_fraction, else the BSDF, and weighs the direction
+// by f / (a q + (1 - a) p_bsdf) (one-sample MIS); sample-0 paths of the first chunk append a
+// training record per guided vertex.
+template <int MODE>
 __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(DeviceScene S, PtArgs A, int bounce, PathBuffers P,
                                                        const float4* __restrict__ qo, const float4* __restrict__ qd,
                                                        const uint32_t* __restrict__ count_in, float4* __restrict__ no,
@@ -507,11 +521,14 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(DeviceScene S, PtArgs A, 
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t lane = lane_id();
     const bool last = bounce + 1 >= A.max_path_length;
-    constexpr bool nee = NEE;
+    constexpr bool nee = MODE == MRT_MODE_NEE;
+    constexpr bool guided = MODE == MRT_MODE_GUIDED;
     for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < n; base += stride) {
         const uint32_t i = base + lane;
-        bool push = false, shadow = false;
+        bool push = false, shadow = false, record = false;
         f3 next_o{0, 0, 0}, next_d{0, 0, 0};
+        f3 rec_pos{0, 0, 0}, rec_rad{0, 0, 0}, rd_keep{0, 0, 0}, thr_keep{0, 0, 0};
+        float rec_pdf = 0.f;
         f3 sh_o{0, 0, 0}, sh_d{0, 0, 0}, sh_c{0, 0, 0};
         uint32_t pid = 0;
         if (i < n) {
@@ -559,19 +576,64 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(DeviceScene S, PtArgs A, 
                         }
                     }
                 }
-                const Sample s = sp.mat.sample_eval(wi_local, rng);
-                if (s.valid) {
-                    const f3 wo = sp.frame.to_world(s.wo);
-                    thr = thr * s.weight;
-                    const bool transmission = wi_local.z * s.wo.z < 0.f;
-                    if (transmission) ior /= sp.mat.eta;
-                    // ShadingPoint.make_ray (hit.slang:134-146)
-                    const float wo_ng = dot(wo, ng);
-                    if ((wi_below != (wo_ng < 0.f)) == transmission) {
-                        next_o = offset_ray_origin(sp.pos, wo_ng < 0.f ? -ng : ng);
-                        next_d = wo;
-                        push = true;
-                        if (nee) P.aux[pid] = s.pdf;
+                float alpha = 0.f;
+                GuideLobes L;
+                if (guided && guide_eligible(S.mats[sp.material])) {
+                    unpack_lobes(P.lobes + 4 * size_t(i), L);
+                    if (L.pi_sum > 0.f) alpha = A.guide_fraction;
+                }
+                if (guided && alpha > 0.f) {
+                    // one-sample MIS of the guide's mixture and the BSDF
+                    bool ok = true;
+                    f3 wo_l;
+                    if (rng.next_float() < alpha) {
+                        const float u0 = rng.next_float();
+                        const f2 u12 = rng.next_float2();
+                        wo_l = sp.frame.to_local(lobes_sample(L, u0, u12.x, u12.y));
+                    } else {
+                        const Sample s = sp.mat.sample_eval(wi_local, rng);
+                        ok = s.valid;
+                        wo_l = s.wo;
+                    }
+                    if (ok) {
+                        const f3 wo = sp.frame.to_world(wo_l);
+                        const f3 f = sp.mat.eval(wi_local, wo_l);
+                        const float p = alpha * lobes_pdf(L, wo) + (1.f - alpha) * sp.mat.pdf(wi_local, wo_l);
+                        const float wo_ng = dot(wo, ng);
+                        const bool transmission = wi_local.z * wo_l.z < 0.f;
+                        if (p > 0.f && (f.x > 0.f || f.y > 0.f || f.z > 0.f) &&
+                            (wi_below != (wo_ng < 0.f)) == transmission) {
+                            thr = thr * (f / p);
+                            next_o = offset_ray_origin(sp.pos, wo_ng < 0.f ? -ng : ng);
+                            next_d = wo;
+                            push = finite3(thr);
+                            record = push && A.s0 == 0 && pid < A.npix;
+                            rec_pos = sp.pos;
+                            rec_rad = rad;
+                            rec_pdf = p;
+                            rd_keep = rd;
+                            thr_keep = thr;
+                        } else {
+                            thr = f3{0.f, 0.f, 0.f};
+                        }
+                    } else {
+                        thr = f3{0.f, 0.f, 0.f};
+                    }
+                } else {
+                    const Sample s = sp.mat.sample_eval(wi_local, rng);
+                    if (s.valid) {
+                        const f3 wo = sp.frame.to_world(s.wo);
+                        thr = thr * s.weight;
+                        const bool transmission = wi_local.z * s.wo.z < 0.f;
+                        if (transmission) ior /= sp.mat.eta;
+                        // ShadingPoint.make_ray (hit.slang:134-146)
+                        const float wo_ng = dot(wo, ng);
+                        if ((wi_below != (wo_ng < 0.f)) == transmission) {
+                            next_o = offset_ray_origin(sp.pos, wo_ng < 0.f ? -ng : ng);
+                            next_d = wo;
+                            push = true;
+                            if (nee) P.aux[pid] = s.pdf;
+                        }
                     }
                 }
             }
@@ -583,6 +645,17 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(DeviceScene S, PtArgs A, 
             st_stream(no + k, make_float4(next_o.x, next_o.y, next_o.z, __uint_as_float(pid)));
             st_stream(nd + k, make_float4(next_d.x, next_d.y, next_d.z, 0.f));
         }
+        if (guided) {
+            const uint32_t r = wave_append(record, P.grec_count);
+            if (record && r < P.grec_cap) {
+                float4* g = P.grec + 5 * size_t(r);
+                g[0] = make_float4(rec_pos.x, rec_pos.y, rec_pos.z, __uint_as_float(pid));
+                g[1] = make_float4(-rd_keep.x, -rd_keep.y, -rd_keep.z, rec_pdf);
+                g[2] = make_float4(next_d.x, next_d.y, next_d.z, 0.f);
+                g[3] = make_float4(rec_rad.x, rec_rad.y, rec_rad.z, 0.f);
+                g[4] = make_float4(thr_keep.x, thr_keep.y, thr_keep.z, 0.f);
+            }
+        }
         if (nee) {
             const uint32_t q = wave_append(shadow, shadow_count);
             if (shadow) {
@@ -591,6 +664,27 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(DeviceScene S, PtArgs A, 
                 st_stream(P.sq_c + q, make_float4(sh_c.x, sh_c.y, sh_c.z, 0.f));
             }
         }
+    }
+}
+
+// Training records of the guide from the first chunk's sample-0 paths, once their radiance
+// is final: incident radiance along the sampled direction = (final radiance - radiance before
+// the vertex's continuation) / throughput after it; weight = its luminance / sampling pdf.
+__global__ void k_guide_targets(PathBuffers P) {
+    const uint32_t n = min(*P.grec_count, P.grec_cap);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride) {
+        const float4* g = P.grec + 5 * size_t(r);
+        const float4 a = g[0], b = g[1], c = g[2], rb = g[3], th = g[4];
+        const float4 rf = P.state_b[__float_as_uint(a.w)];
+        const f3 L{th.x > 0.f ? fmaxf(rf.x - rb.x, 0.f) / th.x : 0.f, th.y > 0.f ? fmaxf(rf.y - rb.y, 0.f) / th.y : 0.f,
+                   th.z > 0.f ? fmaxf(rf.z - rb.z, 0.f) / th.z : 0.f};
+        float w = b.w > 0.f ? yuv_luminance(L) / b.w : 0.f;
+        if (!(w >= 0.f && w < 3.0e38f)) w = 0.f;
+        float4* t = P.gtrain + 3 * size_t(r);
+        t[0] = make_float4(a.x, a.y, a.z, w);
+        t[1] = make_float4(b.x, b.y, b.z, 0.f);
+        t[2] = make_float4(c.x, c.y, c.z, 0.f);
     }
 }
 
@@ -638,14 +732,14 @@ constexpr int MAX_BLOCKS = 256 * 16; // 256 CUs
 
 // Blocks of k_trace<...> that fit on the device at once (the persistent grid).
 template <bool COUNT, class Src, class Sink, bool ANY> int trace_resident_blocks() {
-    static int blocks = 0;
-    if (blocks == 0) {
+    // computed once per process (thread-safe static initialisation)
+    static const int blocks = [] {
         int per_cu = 0, dev = 0, cus = 0;
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<COUNT, Src, Sink, ANY>, TRACE_BLOCK, 0);
         hipGetDevice(&dev);
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        blocks = per_cu * cus > 0 ? per_cu * cus : 1;
-    }
+        return per_cu * cus > 0 ? per_cu * cus : 1;
+    }();
     return blocks;
 }
 
@@ -668,6 +762,14 @@ hipError_t launch_trace(const DeviceScene& s, const TraceScratch& ts, const Src&
 } // namespace
 
 // ---------------------------------------------------------------------------------------------
+
+void trace_prepare() {
+    // evaluates the cached occupancy queries of every trace-kernel variant the frame launches,
+    // so that no device query runs while a stream is being captured
+    (void)trace_resident_blocks<false, SrcCamera, SinkGBuffer, false>();
+    (void)trace_resident_blocks<false, SrcQueue, SinkQueue, false>();
+    (void)trace_resident_blocks<false, SrcShadow, SinkShadow, true>();
+}
 
 uint32_t trace_max_threads() {
     int dev = 0, cus = 0;
@@ -707,13 +809,22 @@ hipError_t launch_gbuffer(const DeviceScene& s, const TraceScratch& ts, const mr
 
 hipError_t launch_pt_frame(const DeviceScene& s, const TraceScratch& ts, PtArgs a, const mrt_packed_hit* gbuffer,
                            PathBuffers& pb, float4* out, uint32_t* sig_out, hipStream_t st, FrameTiming* timing,
-                           uint64_t* continuation_segments, unsigned long long* trav_counters) {
+                           uint64_t* continuation_segments, unsigned long long* trav_counters, const GuideNet* guide) {
     const int L = a.max_path_length;
+    const bool guided = a.mode == MRT_MODE_GUIDED;
+    if (guided && (!guide || !pb.lobes || !pb.grec || !pb.gtrain || !pb.grec_count)) return hipErrorInvalidValue;
+    if (guided) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, pb.grec_count, 1u);
     const uint32_t spp = uint32_t(a.spp > 0 ? a.spp : 0);
     if (spp == 0 || a.npix == 0) {
-        // render_pt.slang:124-129 with spp = 0: irradiance stays 0
-        hipMemsetAsync(out, 0, sizeof(float4) * a.npix, st);
-        if (sig_out) hipMemsetAsync(sig_out, 0, sizeof(uint32_t) * a.npix, st);
+        // render_pt.slang:124-129 with spp = 0: irradiance stays 0 (fill kernels, no memset
+        // nodes, so the frame can be captured in a graph)
+        if (a.npix) {
+            hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(4ull * a.npix, 256, MAX_BLOCKS)), dim3(256), 0, st,
+                               reinterpret_cast<uint32_t*>(out), 4u * a.npix);
+            if (sig_out)
+                hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(a.npix, 256, MAX_BLOCKS)), dim3(256), 0, st, sig_out,
+                                   a.npix);
+        }
         return hipGetLastError();
     }
     const uint32_t chunk = uint32_t(pb.capacity / a.npix) < spp ? uint32_t(pb.capacity / a.npix) : spp;
@@ -735,7 +846,7 @@ hipError_t launch_pt_frame(const DeviceScene& s, const TraceScratch& ts, PtArgs 
     // [chunk][bounce] arrays: queue sizes, trace cursors, shadow-queue sizes, shadow cursors
     const uint32_t nz = n_chunks * per_chunk;
     hipLaunchKernelGGL(k_zero_u32, dim3((4 * nz + 255) / 256), dim3(256), 0, st, pb.counters, 4 * nz);
-    const bool nee = a.mode != MRT_MODE_BSDF;
+    const bool nee = a.mode == MRT_MODE_NEE;
     hipError_t err = hipSuccess;
     for (uint32_t c = 0; c < n_chunks && err == hipSuccess; c++) {
         a.s0 = c * chunk;
@@ -759,15 +870,26 @@ hipError_t launch_pt_frame(const DeviceScene& s, const TraceScratch& ts, PtArgs 
                                                           a.instance_mask, nullptr, st);
                 mark(0);
             }
+            if (guided && b + 1 < L && err == hipSuccess) {
+                mark(-1);
+                err = launch_guide_infer_queue(*guide, s, pb.q_o[qi], pb.q_d[qi], pb.hits, cnt + b, a.n_paths,
+                                               pb.lobes, st);
+                mark(1);
+            }
             mark(-1);
+            const dim3 sg(grid_for(a.n_paths, SHADE_BLOCK, MAX_BLOCKS));
             if (nee) {
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<true>), dim3(grid_for(a.n_paths, SHADE_BLOCK, MAX_BLOCKS)),
-                                   dim3(SHADE_BLOCK), 0, st, s, a, b, pb, pb.q_o[qi], pb.q_d[qi], cnt + b,
-                                   pb.q_o[qi ^ 1], pb.q_d[qi ^ 1], cnt + b + 1, scnt + b);
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<MRT_MODE_NEE>), sg, dim3(SHADE_BLOCK), 0, st, s, a, b, pb,
+                                   pb.q_o[qi], pb.q_d[qi], cnt + b, pb.q_o[qi ^ 1], pb.q_d[qi ^ 1], cnt + b + 1,
+                                   scnt + b);
+            } else if (guided) {
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<MRT_MODE_GUIDED>), sg, dim3(SHADE_BLOCK), 0, st, s, a, b, pb,
+                                   pb.q_o[qi], pb.q_d[qi], cnt + b, pb.q_o[qi ^ 1], pb.q_d[qi ^ 1], cnt + b + 1,
+                                   scnt + b);
             } else {
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<false>), dim3(grid_for(a.n_paths, SHADE_BLOCK, MAX_BLOCKS)),
-                                   dim3(SHADE_BLOCK), 0, st, s, a, b, pb, pb.q_o[qi], pb.q_d[qi], cnt + b,
-                                   pb.q_o[qi ^ 1], pb.q_d[qi ^ 1], cnt + b + 1, scnt + b);
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<MRT_MODE_BSDF>), sg, dim3(SHADE_BLOCK), 0, st, s, a, b, pb,
+                                   pb.q_o[qi], pb.q_d[qi], cnt + b, pb.q_o[qi ^ 1], pb.q_d[qi ^ 1], cnt + b + 1,
+                                   scnt + b);
             }
             mark(1);
             if (nee && b + 1 < L && err == hipSuccess) {
@@ -778,6 +900,8 @@ hipError_t launch_pt_frame(const DeviceScene& s, const TraceScratch& ts, PtArgs 
                 mark(0);
             }
         }
+        if (guided && c == 0)
+            hipLaunchKernelGGL(k_guide_targets, dim3(grid_for(pb.grec_cap, 256, MAX_BLOCKS)), dim3(256), 0, st, pb);
         hipLaunchKernelGGL(k_accumulate, dim3(grid_for(a.npix, 256, MAX_BLOCKS)), dim3(256), 0, st, a, pb,
                            int(c == 0), int(c + 1 == n_chunks), out, sig_out);
     }

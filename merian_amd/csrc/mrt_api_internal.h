@@ -28,6 +28,7 @@ struct mrt_scene {
     std::vector<void*> allocs;
     uint64_t bytes = 0, n_tris = 0, n_nodes = 0;
     uint32_t depth = 0;
+    float bbox_min[3] = {0.f, 0.f, 0.f}, bbox_max[3] = {0.f, 0.f, 0.f}; // bounds of the triangles
 };
 
 namespace mrt {
@@ -39,4 +40,13 @@ hipStream_t pick_stream(mrt_ctx* c, void* stream);
 mrt_status ensure_path_buffers(mrt_ctx* c, uint64_t paths, uint64_t npix, uint64_t counters);
 mrt_status check_params(mrt_ctx* c, const mrt_pt_params* p);
 mrt_status ensure_trace_scratch(mrt_ctx* c, const mrt_scene* s);
+// One pt frame on the device (mrt_pt_render_dev and, with a guide, mrt_guided_render_dev).
+mrt_status render_frame_dev(mrt_ctx* c, mrt_scene* s, const mrt_pt_params* prm, const mrt_camera* cam, uint32_t frame,
+                            uint32_t w, uint32_t h, const mrt_packed_hit* gbuffer, float* irradiance,
+                            uint32_t* signature, mrt_pt_stats* stats, void* stream, mrt_guide* guide);
+// Guide hooks of render_frame_dev (guide_api.cpp): buffers for `paths` paths in flight, and the
+// training of the frame's records after it was enqueued.
+mrt_status guide_prepare_frame(mrt_ctx* c, mrt_guide* g, uint64_t paths, PathBuffers& pb, float& fraction,
+                               const GuideNet** net);
+mrt_status guide_train_frame(mrt_ctx* c, mrt_guide* g, hipStream_t st);
 } // namespace mrt

@@ -44,6 +44,8 @@ struct TraceScratch {
 
 // Most trace-kernel threads resident on the current device (the persistent grid's upper bound).
 uint32_t trace_max_threads();
+// Runs the one-time occupancy queries of the frame's trace kernels (call before capturing).
+void trace_prepare();
 
 // Frame-constant arguments of the wavefront pt kernels.
 struct PtArgs {
@@ -52,6 +54,7 @@ struct PtArgs {
     int32_t spp, max_path_length, emission_on_primary, record_signature;
     uint32_t instance_mask;
     int32_t mode;     // MRT_MODE_*
+    float guide_fraction; // MRT_MODE_GUIDED: probability of sampling the guide at an eligible vertex
     uint32_t s0;      // first sample index of this chunk
     uint32_t chunk;   // samples in this chunk
     uint32_t n_paths; // npix * chunk
@@ -72,8 +75,16 @@ struct PathBuffers {
     float4* sq_c;      // shadow queue: unoccluded contribution.rgb
     float4* accum;     // per pixel running sum: radiance.xyz, luminance^2
     uint32_t* sig_accum;
+    // MRT_MODE_GUIDED only
+    uint4* lobes;      // packed guide lobes per queue entry (4 x uint4)
+    float4* grec;      // raw training records: (pos, path), (wo, pdf), (dir, -), (radiance before, -), (throughput after, -)
+    float4* gtrain;    // training records: (pos, weight), (wo, -), (dir, -)
+    uint32_t* grec_count;
+    uint32_t grec_cap;
     uint64_t capacity; // paths
 };
+
+struct GuideNet;
 
 // Launch entry points (kernels.hip). All asynchronous on `stream`.
 // counters (optional, device): node visits and triangle tests are added to counters[0..1]
@@ -90,8 +101,11 @@ struct FrameTiming {
 };
 // One frame of pt over `gbuffer`; out = float4 irradiance per pixel; sig_out optional.
 // counters_host (optional) receives every chunk's queue sizes after the frame completes.
+// guide (MRT_MODE_GUIDED): the network whose lobes are inferred for every bounce's queue; the
+// first chunk's sample-0 paths write training records (pb.grec / pb.gtrain).
 hipError_t launch_pt_frame(const DeviceScene& s, const TraceScratch& ts, PtArgs a, const mrt_packed_hit* gbuffer,
                            PathBuffers& pb, float4* out, uint32_t* sig_out, hipStream_t st, FrameTiming* timing,
-                           uint64_t* continuation_segments, unsigned long long* trav_counters = nullptr);
+                           uint64_t* continuation_segments, unsigned long long* trav_counters = nullptr,
+                           const GuideNet* guide = nullptr);
 
 } // namespace mrt

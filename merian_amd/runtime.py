@@ -136,3 +136,169 @@ def pt_params(spp=4, max_path_length=4, emission_on_primary=True, mask=0xFF, mod
     p.spp, p.max_path_length, p.emission_on_primary = spp, max_path_length, int(emission_on_primary)
     p.demodulate_albedo, p.instance_mask, p.mode, p.record_signature = 0, mask, mode, int(record_signature)
     return p
+
+
+class Job:
+    """A submitted render job; keeps its host output buffer alive until it is done."""
+
+    def __init__(self, server: "Server", job_id: int, irradiance: np.ndarray, desc: abi.JobDesc):
+        self.server, self.id, self.irradiance, self.desc = server, job_id, irradiance, desc
+
+    def poll(self) -> abi.JobStatus:
+        st = abi.JobStatus()
+        self.server._check(self.server.lib.mrt_job_poll(self.server.h, self.id, C.byref(st)))
+        return st
+
+    def wait(self, timeout_ms: int = -1) -> abi.JobStatus:
+        st = abi.JobStatus()
+        r = self.server.lib.mrt_job_wait(self.server.h, self.id, timeout_ms, C.byref(st))
+        if r not in (abi.MRT_OK, abi.MRT_ERR_NOT_READY):
+            self.server._check(r)
+        return st
+
+    def cancel(self) -> bool:
+        return self.server.lib.mrt_job_cancel(self.server.h, self.id) == abi.MRT_OK
+
+
+class Server:
+    """Continuous-batching render server over mrt_server_* (one scheduler thread in the library)."""
+
+    def __init__(self, ctx: Context, hbm_budget: int = 0, max_slots: int = 4, max_paths_per_job: int = 0,
+                 use_graphs: bool = True):
+        self.ctx, self.lib = ctx, ctx.lib
+        cfg = abi.ServerConfig(hbm_budget, max_slots, max_paths_per_job, int(use_graphs), 0)
+        h = C.c_void_p()
+        ctx._check(self.lib.mrt_server_create(ctx.h, C.byref(cfg), C.byref(h)))
+        self.h = h
+
+    def _check(self, st: int):
+        if st != abi.MRT_OK:
+            raise MrtError(st, self.lib.mrt_server_last_error(self.h).decode())
+
+    def close(self):
+        if self.h:
+            self.lib.mrt_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @staticmethod
+    def desc(scene: DeviceScene, params: abi.PtParams, cam: abi.Camera, frame: int, w: int, h: int,
+             out: np.ndarray | None = None) -> abi.JobDesc:
+        d = abi.JobDesc()
+        d.scene, d.camera, d.params, d.width, d.height, d.frame = scene.h, cam, params, w, h, frame
+        d.irradiance = out.ctypes.data if out is not None else None
+        return d
+
+    def job_bytes(self, scene, params, cam, frame, w, h) -> int:
+        out = np.zeros(4, np.float32)
+        d = self.desc(scene, params, cam, frame, w, h, out)
+        b = C.c_uint64()
+        self._check(self.lib.mrt_job_bytes(self.h, C.byref(d), C.byref(b)))
+        return b.value
+
+    def submit(self, scene: DeviceScene, params: abi.PtParams, cam: abi.Camera, frame: int, w: int, h: int) -> Job:
+        out = np.zeros((h, w, 4), np.float32)
+        d = self.desc(scene, params, cam, frame, w, h, out)
+        jid = C.c_uint64()
+        self._check(self.lib.mrt_job_submit(self.h, C.byref(d), C.byref(jid)))
+        return Job(self, jid.value, out, d)
+
+    def stats(self) -> abi.ServerStats:
+        s = abi.ServerStats()
+        self._check(self.lib.mrt_server_stats_get(self.h, C.byref(s)))
+        return s
+
+    def reset_stats(self):
+        self._check(self.lib.mrt_server_stats_reset(self.h))
+
+
+def guide_config(seed: int = 1, log2_table: int = 0, base_resolution: int = 0, max_resolution: int = 0,
+                 train_batch: int = 0, train_steps: int = 0, max_records: int = 0, learning_rate: float = 0.0,
+                 guide_fraction: float = 0.0, bbox=None) -> abi.GuideConfig:
+    c = abi.GuideConfig()
+    c.seed, c.log2_table, c.base_resolution, c.max_resolution = seed, log2_table, base_resolution, max_resolution
+    c.train_batch, c.train_steps, c.max_records = train_batch, train_steps, max_records
+    c.learning_rate, c.guide_fraction = learning_rate, guide_fraction
+    if bbox is not None:
+        for k in range(3):
+            c.bbox_min[k], c.bbox_max[k] = bbox[0][k], bbox[1][k]
+    return c
+
+
+def guide_layout(cfg: abi.GuideConfig):
+    """(per-level resolutions, table entries) of a guide configuration (host computation)."""
+    lib = abi.load_library()
+    res = (C.c_uint32 * 16)()
+    t = C.c_uint32()
+    st = lib.mrt_guide_layout(C.byref(cfg), res, C.byref(t))
+    if st != abi.MRT_OK:
+        raise MrtError(st, "mrt_guide_layout: configuration out of range")
+    return list(res), t.value
+
+
+class Guide:
+    """Learned guide over mrt_guide_* (network parameters and training state on the device)."""
+
+    def __init__(self, ctx: Context, scene: DeviceScene, cfg: abi.GuideConfig | None = None):
+        self.ctx, self.lib = ctx, ctx.lib
+        self.cfg = cfg if cfg is not None else guide_config()
+        h = C.c_void_p()
+        ctx._check(self.lib.mrt_guide_create(ctx.h, scene.h, C.byref(self.cfg), C.byref(h)))
+        self.h = h
+        self.resolutions, self.table = guide_layout(self.cfg)
+
+    def close(self):
+        if self.h:
+            self.lib.mrt_guide_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def normalisation(self):
+        o, s = (C.c_float * 3)(), (C.c_float * 3)()
+        self.ctx._check(self.lib.mrt_guide_normalisation(self.h, o, s))
+        return list(o), list(s)
+
+    def params(self):
+        W = np.zeros((abi.GUIDE_LAYERS, 64, 64), np.float32)
+        G = np.zeros((abi.GUIDE_LEVELS, self.table, 2), np.float32)
+        self.ctx._check(self.lib.mrt_guide_get_params(self.ctx.h, self.h, _p(W), _p(G)))
+        return W, G
+
+    def set_params(self, W: np.ndarray | None, G: np.ndarray | None):
+        W = None if W is None else np.ascontiguousarray(W, np.float32)
+        G = None if G is None else np.ascontiguousarray(G, np.float32)
+        self.ctx._check(self.lib.mrt_guide_set_params(self.ctx.h, self.h, _p(W), _p(G)))
+
+    def forward(self, pos: np.ndarray, wo: np.ndarray) -> np.ndarray:
+        P = np.ascontiguousarray(pos, np.float32)
+        D = np.ascontiguousarray(wo, np.float32)
+        out = np.zeros((len(P), abi.GUIDE_OUTPUTS), np.float32)
+        self.ctx._check(self.lib.mrt_guide_forward(self.ctx.h, self.h, _p(P), _p(D), len(P), _p(out)))
+        return out
+
+    def gradient(self, records: np.ndarray):
+        R = np.ascontiguousarray(records, np.float32)
+        dW = np.zeros((abi.GUIDE_LAYERS, 64, 64), np.float32)
+        dG = np.zeros((abi.GUIDE_LEVELS, self.table, 2), np.float32)
+        loss = C.c_float()
+        self.ctx._check(self.lib.mrt_guide_gradient(self.ctx.h, self.h, _p(R), len(R), _p(dW), _p(dG),
+                                                    C.byref(loss)))
+        return loss.value, dW, dG
+
+    def train(self, records: np.ndarray, steps: int) -> float:
+        R = np.ascontiguousarray(records, np.float32)
+        loss = C.c_float()
+        self.ctx._check(self.lib.mrt_guide_train(self.ctx.h, self.h, _p(R), len(R), steps, C.byref(loss)))
+        return loss.value
+
+    def render_dev(self, scene: DeviceScene, params: abi.PtParams, cam: abi.Camera, frame: int, w: int, h: int,
+                   d_gbuffer: int, d_out: int, train: bool = True, stats: bool = False):
+        st = abi.PtStats() if stats else None
+        self.ctx._check(self.lib.mrt_guided_render_dev(self.ctx.h, scene.h, self.h, C.byref(params), C.byref(cam),
+                                                       frame, w, h, C.c_void_p(d_gbuffer), C.c_void_p(d_out),
+                                                       int(train), C.byref(st) if st is not None else None, None))
+        return st

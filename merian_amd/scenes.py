@@ -86,6 +86,11 @@ class Scene:
     def triangle_count(self) -> int:
         return int(self.geometries["primitive_count"].sum())
 
+    def bounds(self):
+        """(min xyz, max xyz) of the vertex positions."""
+        p = self.vertices["position"]
+        return p.min(0), p.max(0)
+
     def desc(self) -> abi.SceneDesc:
         """mrt_scene_desc pointing into this scene's arrays (kept alive by the scene)."""
         d = abi.SceneDesc()

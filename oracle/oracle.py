@@ -30,6 +30,12 @@ class BsdfCfg(C.Structure):
                 ("mat", C.c_uint8 * 128)]
 
 
+class GuideSpec(C.Structure):
+    """Guide encoding layout: per-level grid resolutions, table entries, position normalisation."""
+    _fields_ = [("res", C.c_uint32 * 16), ("table", C.c_uint32), ("offset", C.c_float * 3),
+                ("scale", C.c_float * 3)]
+
+
 _lib = None
 
 
@@ -59,6 +65,11 @@ def lib() -> C.CDLL:
                                     u32, u32, u32, u32, vp, vp, i32]),
             "orc_bsdf": (None, [C.POINTER(BsdfCfg), i32, vp, vp, u32, u64, vp, vp, vp, vp]),
             "orc_math": (None, [i32, vp, vp, u64, vp]),
+            "orc_guide_forward": (None, [C.POINTER(GuideSpec), vp, vp, vp, vp, u32, vp, vp, i32]),
+            "orc_guide_gradient": (f32, [C.POINTER(GuideSpec), vp, vp, vp, u32, vp, vp, i32]),
+            "orc_guide_sh": (None, [vp, vp]),
+            "orc_guide_mixture_pdf": (C.c_double, [vp, vp, vp, vp, vp]),
+            "orc_vmf_pdf": (C.c_double, [C.c_double, C.c_double]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -205,3 +216,61 @@ def pt_params(spp=4, max_path_length=4, emission_on_primary=True, mask=0xFF, mod
     p.spp, p.max_path_length, p.emission_on_primary = spp, max_path_length, int(emission_on_primary)
     p.demodulate_albedo, p.instance_mask, p.mode, p.record_signature = 0, mask, mode, int(record_signature)
     return p
+
+
+# ------------------------------------------------------------------------------------------
+# learned guide (oracle_guide.cpp)
+
+def guide_spec(res, table: int, offset, scale) -> GuideSpec:
+    g = GuideSpec()
+    for k in range(16):
+        g.res[k] = int(res[k])
+    g.table = int(table)
+    for a in range(3):
+        g.offset[a], g.scale[a] = float(offset[a]), float(scale[a])
+    return g
+
+
+def guide_forward(spec: GuideSpec, weights: np.ndarray, grid: np.ndarray, pos: np.ndarray, wo: np.ndarray,
+                  exact: bool = False, encoding: bool = False):
+    """Network outputs [n, 40] (and the encodings [n, 64]); mirror rounding unless exact."""
+    W = np.ascontiguousarray(weights, np.float32)
+    G = np.ascontiguousarray(grid, np.float32)
+    P = np.ascontiguousarray(pos, np.float32)
+    D = np.ascontiguousarray(wo, np.float32)
+    n = len(P)
+    out = np.zeros((n, 40), np.float32)
+    enc = np.zeros((n, 64), np.float32) if encoding else None
+    lib().orc_guide_forward(C.byref(spec), _p(W), _p(G), _p(P), _p(D), n, _p(out), _p(enc), int(exact))
+    return (out, enc) if encoding else out
+
+
+def guide_gradient(spec: GuideSpec, weights: np.ndarray, grid: np.ndarray, records: np.ndarray, exact: bool = False):
+    """(loss, dW [5, 64, 64], dgrid) of the training loss over records [n, 12]."""
+    W = np.ascontiguousarray(weights, np.float32)
+    G = np.ascontiguousarray(grid, np.float32)
+    R = np.ascontiguousarray(records, np.float32)
+    dW = np.zeros_like(W)
+    dG = np.zeros_like(G)
+    loss = lib().orc_guide_gradient(C.byref(spec), _p(W), _p(G), _p(R), len(R), _p(dW), _p(dG), int(exact))
+    return loss, dW, dG
+
+
+def guide_sh(d) -> np.ndarray:
+    d = np.ascontiguousarray(d, np.float32)
+    o = np.zeros(16, np.float32)
+    lib().orc_guide_sh(_p(d), _p(o))
+    return o
+
+
+def guide_mixture(out40: np.ndarray, d) -> tuple[float, np.ndarray, np.ndarray, np.ndarray]:
+    """(q(d), pi[8], kappa[8], mu[8, 3]) of one output vector."""
+    o = np.ascontiguousarray(out40, np.float32)
+    dd = np.ascontiguousarray(d, np.float32)
+    pi, ka, mu = np.zeros(8, np.float32), np.zeros(8, np.float32), np.zeros((8, 3), np.float32)
+    q = lib().orc_guide_mixture_pdf(_p(o), _p(dd), _p(pi), _p(ka), _p(mu))
+    return q, pi, ka, mu
+
+
+def vmf_pdf(kappa: float, cos_mu: float) -> float:
+    return lib().orc_vmf_pdf(kappa, cos_mu)

@@ -1,0 +1,212 @@
+"""Learned guide on the GPU (mrt_guide_*, MRT_MODE_GUIDED) against the CPU oracle and the plain pt.
+
+- forward outputs and loss gradients (weights and hash grid) of the MFMA kernels equal the
+  oracle's mirror mode (the kernels' fp16 rounding points, fp32 accumulation) within tolerance;
+- training on records drawn from a position-dependent vMF lowers the loss toward its optimum;
+- guided rendering is unbiased: block means agree with the BSDF-only reference;
+- at equal wall time the trained guide beats the plain pt in relMSE against a converged render.
+"""
+import math
+import time
+
+import numpy as np
+import pytest
+
+from merian_amd import abi, scenes
+from merian_amd.runtime import Guide, guide_config, pt_params
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SMALL = dict(log2_table=12, base_resolution=2, max_resolution=64)
+
+
+@pytest.fixture(scope="module")
+def box_scene(ctx):
+    sc = scenes.plain_small(seed=3, subdiv=1, grid=2)
+    ds = ctx.scene(sc)
+    yield sc, ds
+    ds.destroy()
+
+
+def _spec(g: Guide):
+    off, sc = g.normalisation()
+    return O.guide_spec(g.resolutions, g.table, off, sc)
+
+
+def _records(n, lo, hi, seed, kappa=None):
+    rng = np.random.default_rng(seed)
+    lo, hi = np.asarray(lo), np.asarray(hi)
+    p = lo + (hi - lo) * rng.uniform(0.02, 0.98, (n, 3))
+    wo = rng.normal(size=(n, 3))
+    wo /= np.linalg.norm(wo, axis=1, keepdims=True)
+    if kappa is None:
+        d = rng.normal(size=(n, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        w = rng.exponential(1.0, n)
+    else:
+        # directions from a vMF whose mean depends on the position
+        mu = np.stack([np.cos(3 * p[:, 0]), np.sin(2 * p[:, 2]), 0.5 + 0 * p[:, 1]], 1)
+        mu /= np.linalg.norm(mu, axis=1, keepdims=True)
+        u1, u2 = rng.uniform(size=n), rng.uniform(size=n)
+        c = 1 + np.log(u1 + (1 - u1) * np.exp(-2 * kappa)) / kappa
+        s = np.sqrt(np.maximum(0, 1 - c * c))
+        t = np.cross(mu, [0.0, 1.0, 0.0])
+        t /= np.linalg.norm(t, axis=1, keepdims=True)
+        b = np.cross(mu, t)
+        d = c[:, None] * mu + s[:, None] * (np.cos(2 * np.pi * u2)[:, None] * t + np.sin(2 * np.pi * u2)[:, None] * b)
+        w = np.ones(n)
+    R = np.zeros((n, 12), np.float32)
+    R[:, 0:3], R[:, 3], R[:, 4:7], R[:, 8:11] = p, w, wo, d
+    return R
+
+
+def _random_params(g, seed, scale=1.0):
+    rng = np.random.default_rng(seed)
+    W = np.zeros((5, 64, 64), np.float32)
+    for k in range(5):
+        n_out = 64 if k < 4 else 40
+        W[k, :n_out] = rng.uniform(-1, 1, (n_out, 64)) * math.sqrt(6 / (64 + n_out)) * scale
+    G = rng.uniform(-0.5, 0.5, (16, g.table, 2)).astype(np.float32)
+    return W, G
+
+
+def test_forward_matches_oracle(ctx, box_scene):
+    sc, ds = box_scene
+    g = Guide(ctx, ds, guide_config(seed=1, **SMALL))
+    try:
+        W, G = _random_params(g, 11, scale=1.5)
+        g.set_params(W, G)
+        lo, hi = sc.bounds()
+        R = _records(3000, lo, hi, 12)
+        gpu = g.forward(R[:, 0:3], R[:, 4:7])
+        cpu = O.guide_forward(_spec(g), W, G, R[:, 0:3], R[:, 4:7])
+        err = np.abs(gpu - cpu)
+        scale = np.abs(cpu).max()
+        assert err.max() <= 2e-2 * scale, (err.max(), scale)
+        assert err.mean() <= 1e-3 * scale
+        # parameters round-trip
+        W2, G2 = g.params()
+        assert np.array_equal(W2, W) and np.array_equal(G2, G)
+    finally:
+        g.close()
+
+
+def test_gradient_matches_oracle(ctx, box_scene):
+    sc, ds = box_scene
+    g = Guide(ctx, ds, guide_config(seed=2, **SMALL))
+    try:
+        W, G = _random_params(g, 21)
+        g.set_params(W, G)
+        lo, hi = sc.bounds()
+        R = _records(5000, lo, hi, 22)  # > 2 weight-gradient blocks, partial last tile
+        loss, dW, dG = g.gradient(R)
+        l_ref, dW_ref, dG_ref = O.guide_gradient(_spec(g), W, G, R)
+        assert abs(loss - l_ref) <= 1e-3 * abs(l_ref) + 1e-4
+        for k in range(5):
+            rel = np.linalg.norm(dW[k] - dW_ref[k]) / np.linalg.norm(dW_ref[k])
+            assert rel < 2e-2, (k, rel)
+        rel = np.linalg.norm(dG - dG_ref) / np.linalg.norm(dG_ref)
+        assert rel < 2e-2, rel
+        # a second call with fewer records reuses the buffers and clears the grid accumulator
+        loss2, dW2, _ = g.gradient(R[:100])
+        l2, dW2_ref, _ = O.guide_gradient(_spec(g), W, G, R[:100])
+        assert abs(loss2 - l2) <= 1e-3 * abs(l2) + 1e-4
+        assert np.linalg.norm(dW2[4] - dW2_ref[4]) / np.linalg.norm(dW2_ref[4]) < 2e-2
+    finally:
+        g.close()
+
+
+def test_training_lowers_the_loss(ctx, box_scene):
+    sc, ds = box_scene
+    g = Guide(ctx, ds, guide_config(seed=3, train_batch=8192, **SMALL))
+    try:
+        lo, hi = sc.bounds()
+        R = _records(65536, lo, hi, 31, kappa=20.0)
+        l0, _, _ = g.gradient(R[:8192])
+        losses = [g.train(R, 50) for _ in range(6)]
+        l1, _, _ = g.gradient(R[:8192])
+        # uniform = log(4 pi) = 2.53; the generating vMF(20) has entropy -0.16
+        assert l0 > 2.0
+        assert l1 < 0.6, (l0, losses, l1)
+    finally:
+        g.close()
+
+
+@pytest.fixture(scope="module")
+def sky(ctx):
+    sc = scenes.full_scene(seed=5, instances=16, env_size=(256, 128))
+    ds = ctx.scene(sc)
+    yield sc, ds
+    ds.destroy()
+
+
+W_, H_ = 96, 54
+
+
+def _frames(ctx, ds, spec, cam_frame, params, frames, guide=None, train=True):
+    """Mean of `frames` frames (frame index varies the RNG; the camera stays fixed)."""
+    import torch
+    cam = ctx.camera(spec, cam_frame)
+    gb = torch.empty(W_ * H_ * 16, dtype=torch.uint8, device="cuda")
+    out = torch.empty(W_ * H_ * 4, dtype=torch.float32, device="cuda")
+    ctx.gbuffer_dev(ds, cam, W_, H_, gb.data_ptr())
+    acc = np.zeros((H_, W_, 4), np.float64)
+    for f in range(frames):
+        if guide is None:
+            ctx.pt_render_dev(ds, params, cam, 1000 + f, W_, H_, gb.data_ptr(), out.data_ptr())
+        else:
+            guide.render_dev(ds, params, cam, 1000 + f, W_, H_, gb.data_ptr(), out.data_ptr(), train=train)
+        torch.cuda.synchronize()
+        acc += out.cpu().numpy().reshape(H_, W_, 4)
+    return acc / frames
+
+
+def _lum(img):
+    return img[..., 0] * 0.299 + img[..., 1] * 0.587 + img[..., 2] * 0.114
+
+
+def test_guided_is_unbiased_and_wins_at_equal_time(ctx, sky):
+    import torch
+    sc, ds = sky
+    spec = scenes.CameraSpec(sc.camera.eye, sc.camera.target, sc.camera.up, sc.camera.vfov, W_, H_)
+    L = 6
+    ref = _frames(ctx, ds, spec, 0, pt_params(spp=1024, max_path_length=L, record_signature=False), 16)
+    g = Guide(ctx, ds, guide_config(seed=4))
+    try:
+        gp = pt_params(spp=16, max_path_length=L, mode=abi.MODE_GUIDED, record_signature=False)
+        _frames(ctx, ds, spec, 0, gp, 24, guide=g, train=True)  # online training
+        # unbiased: guided (frozen) block means against the reference within their errors
+        gp.spp = 512
+        gimg = _frames(ctx, ds, spec, 0, gp, 4, guide=g, train=False)
+        def blocks(img, spp):
+            l = _lum(img)
+            var = np.maximum(img[..., 3] - l * l, 0) / spp
+            b = lambda a: a.reshape(H_ // 6, 6, W_ // 6, 6).mean((1, 3))
+            return b(l), np.sqrt(b(var) / 36)
+        m_r, s_r = blocks(ref, 1024 * 16)
+        m_g, s_g = blocks(gimg, 512 * 4)
+        z = (m_g - m_r) / np.sqrt(s_r ** 2 + s_g ** 2 + 1e-12)
+        assert np.abs(z).max() < 6.0, np.abs(z).max()
+        assert abs(z.mean()) < 0.6
+        # equal time: plain and guided (training included) get the same wall time
+        def timed(fn):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = fn()
+            torch.cuda.synchronize()
+            return r, time.perf_counter() - t0
+        gp.spp = 16
+        gimg, t_g = timed(lambda: _frames(ctx, ds, spec, 0, gp, 8, guide=g, train=True))
+        pp = pt_params(spp=16, max_path_length=L, record_signature=False)
+        _, t_p1 = timed(lambda: _frames(ctx, ds, spec, 0, pp, 8))
+        frames_p = max(8, int(round(8 * t_g / t_p1)))
+        pimg, t_p = timed(lambda: _frames(ctx, ds, spec, 0, pp, frames_p))
+        def rel_mse(x):
+            return float(np.mean((x[..., :3] - ref[..., :3]) ** 2 / (ref[..., :3] ** 2 + 1e-2)))
+        e_g, e_p = rel_mse(gimg), rel_mse(pimg)
+        print(f"equal time: guided {t_g:.3f} s relMSE {e_g:.4g}; plain {t_p:.3f} s ({frames_p} frames) relMSE {e_p:.4g}")
+        assert t_p >= 0.8 * t_g
+        assert e_g < e_p, (e_g, e_p)
+    finally:
+        g.close()
