@@ -304,6 +304,20 @@ MRT_API mrt_status mrt_render_frame(mrt_ctx* ctx, mrt_scene* scene, const mrt_pt
                                     mrt_packed_hit* gbuffer, mrt_pt_stats* stats);
 
 /* ------------------------------------------------------------------------------------
+ * Device buffers and the accumulation pass, for node hosts that link only this ABI.
+ * ---------------------------------------------------------------------------------- */
+/* Device allocation charged against the context's budget. */
+MRT_API mrt_status mrt_alloc_dev(mrt_ctx* ctx, uint64_t bytes, void** d_ptr);
+MRT_API mrt_status mrt_free_dev(mrt_ctx* ctx, void* d_ptr);
+/* Synchronous copies on the context's stream. */
+MRT_API mrt_status mrt_download(mrt_ctx* ctx, void* host, const void* d_src, uint64_t bytes);
+MRT_API mrt_status mrt_upload(mrt_ctx* ctx, void* d_dst, const void* host, uint64_t bytes);
+/* Running mean of float4 images (Accumulate, accumulate.slang:101 with a plain history):
+ * d_accum = count == 0 ? d_src : d_accum + (d_src - d_accum) / (count + 1), per component. */
+MRT_API mrt_status mrt_accumulate_dev(mrt_ctx* ctx, float* d_accum, const float* d_src, uint32_t pixels,
+                                      uint32_t count, void* stream);
+
+/* ------------------------------------------------------------------------------------
  * Serving loop: continuous batching of render jobs (gbuffer + pt of one frame each).
  *
  * No counterpart in the reference, which renders one frame per Graph::run
@@ -479,11 +493,27 @@ MRT_API mrt_status mrt_graph_store(const char* graph_json, char** out_json, char
 MRT_API mrt_status mrt_graph_cli_help(const char* config_json, const char* search_dirs, char** out_text);
 MRT_API void mrt_string_free(char* s);
 
+/* pbrt-v4 scenes as the "PBRT Scene" node loads them (src/merian-shaders/scene/pbrt_scene.cpp).
+ * mrt_pbrt_inspect (host only): JSON with the geometry, vertex and triangle counts, every
+ * material entry, the environment, the camera (eye, forward, up, vfov radians, width,
+ * height) and the loader's warnings. */
+MRT_API mrt_status mrt_pbrt_inspect(const char* path, char** out_json, char** errors);
+/* Loads and uploads the scene (as mrt_scene_create). */
+MRT_API mrt_status mrt_pbrt_scene_create(mrt_ctx* ctx, const char* path, mrt_scene** out);
+/* The graph runner (merian-graph-run's command line over this library's node host);
+ * returns the process exit code. */
+MRT_API int32_t mrt_graph_run_main(int32_t argc, const char* const* argv);
+
 /* ------------------------------------------------------------------------------------
  * Node plugin ABI (include/merian-graph/plugin/node_plugin.hpp:17-29). The registry
  * argument is this library's node registry (merian_amd/host), not the Vulkan one.
  * ---------------------------------------------------------------------------------- */
 MRT_API uint32_t merian_node_plugin_abi_version(void);
+/* Registers "PBRT Scene", "GBuffer (Raytraced)", "Render (Path-traced)", "Accumulate" and
+ * "Image Write" into a node registry of this library's graph host (mrt::graph::NodeRegistry;
+ * the reference passes merian::NodeRegistry& with the same calling convention). */
+typedef struct mrt_node_registry mrt_node_registry;
+MRT_API void merian_register_nodes(mrt_node_registry* registry);
 
 #ifdef __cplusplus
 }

@@ -229,7 +229,16 @@ EXPORTS = [
     ("mrt_graph_store", C.c_int32, [C.c_char_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     ("mrt_graph_cli_help", C.c_int32, [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p)]),
     ("mrt_string_free", None, [C.c_void_p]),
+    ("mrt_pbrt_inspect", C.c_int32, [C.c_char_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
+    ("mrt_pbrt_scene_create", C.c_int32, [C.c_void_p, C.c_char_p, C.POINTER(C.c_void_p)]),
+    ("mrt_graph_run_main", C.c_int32, [C.c_int32, C.POINTER(C.c_char_p)]),
+    ("mrt_alloc_dev", C.c_int32, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
+    ("mrt_free_dev", C.c_int32, [C.c_void_p, C.c_void_p]),
+    ("mrt_download", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
+    ("mrt_upload", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
+    ("mrt_accumulate_dev", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]),
     ("merian_node_plugin_abi_version", C.c_uint32, []),
+    ("merian_register_nodes", None, [C.c_void_p]),
 ]
 
 

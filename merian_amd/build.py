@@ -18,7 +18,9 @@ LIB = PKG / "libmerian_amd.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["csrc/kernels.hip", "csrc/api.cpp", "csrc/bvh.cpp", "csrc/serve.cpp", "csrc/guide.hip",
-           "csrc/guide_api.cpp", "host/graph_description.cpp", "host/graph_abi.cpp"]
+           "csrc/guide_api.cpp", "host/graph_description.cpp", "host/graph_abi.cpp", "host/pbrt.cpp",
+           "host/graph.cpp", "host/nodes.cpp", "host/graph_run.cpp"]
+RUNNER = PKG / "merian-amd-graph-run"
 
 # -ffp-contract=off: expressions round as written (the CPU oracle uses the same rule).
 # -packed-fp32-ops off: no v_pk_*_f32 generation.
@@ -28,7 +30,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-pthread", "-fvisibility=hidden", "-ffp
 
 
 def _compile(src: Path) -> Path:
-    obj = OBJ / (src.name + ".o")
+    obj = OBJ / (src.parent.name + "_" + src.name + ".o")
     cmd = [HIPCC, *COMMON, "-c", str(src), "-o", str(obj)]
     if src.suffix == ".cpp":
         cmd[1:1] = ["-x", "hip"]
@@ -51,6 +53,16 @@ def build(jobs: int = 8) -> Path:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("link failed")
     os.replace(tmp, LIB)
+    # the runner executable links the library by relative rpath
+    main_obj = _compile(PKG / "host" / "graph_run_main.cpp")
+    tmp = RUNNER.with_suffix(".tmp")
+    cmd = [HIPCC, "--offload-arch=gfx950", "-o", str(tmp), str(main_obj), f"-L{PKG}", "-l:libmerian_amd.so",
+           "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("runner link failed")
+    os.replace(tmp, RUNNER)
     return LIB
 
 

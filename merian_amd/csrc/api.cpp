@@ -184,6 +184,44 @@ MRT_API const char* mrt_last_error(mrt_ctx* c) { return c ? c->err.c_str() : "nu
 
 MRT_API uint64_t mrt_memory_in_use(mrt_ctx* c) { return c ? c->in_use : 0; }
 
+MRT_API mrt_status mrt_alloc_dev(mrt_ctx* c, uint64_t bytes, void** d_ptr) {
+    if (!c || !d_ptr) return MRT_ERR_INVALID;
+    hipSetDevice(c->device);
+    mrt_status st = MRT_OK;
+    *d_ptr = dev_alloc(c, bytes, &st);
+    return st;
+}
+
+MRT_API mrt_status mrt_free_dev(mrt_ctx* c, void* d_ptr) {
+    if (!c) return MRT_ERR_INVALID;
+    if (d_ptr && !c->sizes.count(d_ptr)) return fail(c, MRT_ERR_INVALID, "mrt_free_dev: not a context allocation");
+    hipStreamSynchronize(c->stream);
+    dev_free(c, d_ptr);
+    return MRT_OK;
+}
+
+MRT_API mrt_status mrt_download(mrt_ctx* c, void* host, const void* d_src, uint64_t bytes) {
+    if (!c || (bytes && (!host || !d_src))) return MRT_ERR_INVALID;
+    hipError_t e = hipMemcpyAsync(host, d_src, bytes, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    return e == hipSuccess ? MRT_OK : hip_fail(c, e, "mrt_download");
+}
+
+MRT_API mrt_status mrt_upload(mrt_ctx* c, void* d_dst, const void* host, uint64_t bytes) {
+    if (!c || (bytes && (!host || !d_dst))) return MRT_ERR_INVALID;
+    hipError_t e = hipMemcpyAsync(d_dst, host, bytes, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    return e == hipSuccess ? MRT_OK : hip_fail(c, e, "mrt_upload");
+}
+
+MRT_API mrt_status mrt_accumulate_dev(mrt_ctx* c, float* d_accum, const float* d_src, uint32_t pixels, uint32_t count,
+                                      void* stream) {
+    if (!c || (pixels && (!d_accum || !d_src))) return fail(c, MRT_ERR_INVALID, "mrt_accumulate_dev: null argument");
+    const hipError_t e = launch_running_mean(reinterpret_cast<float4*>(d_accum), reinterpret_cast<const float4*>(d_src),
+                                             pixels, count, pick_stream(c, stream));
+    return e == hipSuccess ? MRT_OK : hip_fail(c, e, "accumulate kernel");
+}
+
 MRT_API mrt_status mrt_synchronize(mrt_ctx* c) {
     if (!c) return MRT_ERR_INVALID;
     const hipError_t e = hipStreamSynchronize(c->stream);

@@ -723,6 +723,22 @@ __global__ void k_accumulate(PtArgs A, PathBuffers P, int first_chunk, int last_
     }
 }
 
+// Running mean over frames: acc = count == 0 ? src : acc + (src - acc) / (count + 1).
+__global__ void k_running_mean(float4* __restrict__ acc, const float4* __restrict__ src, uint32_t n, uint32_t count) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const float w = 1.f / float(count + 1u);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const float4 s = src[i];
+        if (count == 0u) {
+            acc[i] = s;
+        } else {
+            const float4 a = acc[i];
+            acc[i] = make_float4(a.x + (s.x - a.x) * w, a.y + (s.y - a.y) * w, a.z + (s.z - a.z) * w,
+                                 a.w + (s.w - a.w) * w);
+        }
+    }
+}
+
 int grid_for(uint64_t n, int block, int max_blocks) {
     const uint64_t b = (n + block - 1) / block;
     return int(b < uint64_t(max_blocks) ? (b > 0 ? b : 1) : max_blocks);
@@ -788,6 +804,12 @@ hipError_t launch_trace_rays(const DeviceScene& s, const TraceScratch& ts, const
         return launch_trace<true>(s, ts, SrcRays{rays}, SinkHits{hits}, nullptr, uint32_t(n), ts.work, 0xFFu, counters,
                                   st);
     return launch_trace<false>(s, ts, SrcRays{rays}, SinkHits{hits}, nullptr, uint32_t(n), ts.work, 0xFFu, nullptr, st);
+}
+
+hipError_t launch_running_mean(float4* acc, const float4* src, uint32_t n, uint32_t count, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_running_mean, dim3(grid_for(n, 256, MAX_BLOCKS)), dim3(256), 0, st, acc, src, n, count);
+    return hipGetLastError();
 }
 
 hipError_t launch_camera_rays(const mrt_camera& cam, uint32_t w, uint32_t h, mrt_ray* rays, hipStream_t st) {

@@ -91,6 +91,7 @@ struct GuideNet;
 hipError_t launch_trace_rays(const DeviceScene& s, const TraceScratch& ts, const mrt_ray* rays, uint64_t n,
                              mrt_hit* hits, hipStream_t st, unsigned long long* counters = nullptr);
 hipError_t launch_camera_rays(const mrt_camera& cam, uint32_t w, uint32_t h, mrt_ray* rays, hipStream_t st);
+hipError_t launch_running_mean(float4* acc, const float4* src, uint32_t n, uint32_t count, hipStream_t st);
 hipError_t launch_gbuffer(const DeviceScene& s, const TraceScratch& ts, const mrt_camera& cam, uint32_t w, uint32_t h,
                           uint32_t mask, mrt_packed_hit* out, hipStream_t st);
 

@@ -7,6 +7,7 @@
 
 #include "../../include/merian_amd.h"
 #include "graph_description.h"
+#include "pbrt.h"
 
 using namespace mrt;
 using mrt::graph::GraphDescription;
@@ -141,5 +142,79 @@ MRT_API mrt_status mrt_graph_cli_help(const char* config_json, const char* searc
 }
 
 MRT_API void mrt_string_free(char* s) { std::free(s); }
+
+MRT_API mrt_status mrt_pbrt_inspect(const char* path, char** out_json, char** errors) {
+    if (!path) return MRT_ERR_INVALID;
+    try {
+        const pbrt::Scene sc = pbrt::load(path);
+        json::Value j = json::Value::object();
+        j["geometries"] = int64_t(sc.geometries.size());
+        j["vertices"] = int64_t(sc.vertices.size());
+        j["triangles"] = int64_t(sc.indices.size() / 3);
+        json::Value geoms = json::Value::array();
+        for (const mrt_geometry& g : sc.geometries) {
+            json::Value e = json::Value::object();
+            e["material_id"] = int64_t(g.material_id);
+            e["flags"] = int64_t(g.flags);
+            e["primitive_count"] = int64_t(g.primitive_count);
+            geoms.push_back(e);
+        }
+        j["geometry_list"] = geoms;
+        json::Value mats = json::Value::array();
+        for (const mrt_material& m : sc.materials) {
+            json::Value e = json::Value::object();
+            auto v3 = [](const float* f) { return json::Value(json::Array{double(f[0]), double(f[1]), double(f[2])}); };
+            e["model"] = int64_t(m.model);
+            e["features"] = int64_t(m.features);
+            e["base_color"] = v3(m.base_color);
+            e["metalness"] = double(m.metalness);
+            e["specular_alpha"] = json::Value(json::Array{double(m.specular_alpha[0]), double(m.specular_alpha[1])});
+            e["specular_weight"] = double(m.specular_weight);
+            e["specular_ior"] = double(m.specular_ior);
+            e["transmission_weight"] = double(m.transmission_weight);
+            e["emission"] = v3(m.emission);
+            mats.push_back(e);
+        }
+        j["materials"] = mats;
+        json::Value env = json::Value::object();
+        env["width"] = int64_t(sc.env_width);
+        env["height"] = int64_t(sc.env_height);
+        env["scale"] = double(sc.env_scale);
+        if (sc.env_width == 1 && sc.env_height == 1)
+            env["constant"] = json::Value(json::Array{double(sc.env_rgba[0]), double(sc.env_rgba[1]), double(sc.env_rgba[2])});
+        j["env"] = env;
+        json::Value cam = json::Value::object();
+        cam["present"] = sc.camera.present;
+        auto a3 = [](const float* f) { return json::Value(json::Array{double(f[0]), double(f[1]), double(f[2])}); };
+        if (sc.camera.present) {
+            cam["eye"] = a3(sc.camera.eye);
+            cam["forward"] = a3(sc.camera.forward);
+            cam["up"] = a3(sc.camera.up);
+            cam["vfov"] = double(sc.camera.vfov);
+        }
+        cam["width"] = int64_t(sc.camera.width);
+        cam["height"] = int64_t(sc.camera.height);
+        j["camera"] = cam;
+        json::Value w = json::Value::array();
+        for (const std::string& m : sc.warnings) w.push_back(m);
+        j["warnings"] = w;
+        if (out_json) *out_json = dup(json::dump(j, 4));
+        return MRT_OK;
+    } catch (const std::exception& e) {
+        put_errors(errors, {e.what()});
+        return MRT_ERR_INVALID;
+    }
+}
+
+MRT_API mrt_status mrt_pbrt_scene_create(mrt_ctx* ctx, const char* path, mrt_scene** out) {
+    if (!ctx || !path || !out) return MRT_ERR_INVALID;
+    try {
+        const pbrt::Scene sc = pbrt::load(path);
+        const mrt_scene_desc d = sc.desc();
+        return mrt_scene_create(ctx, &d, out);
+    } catch (const std::exception&) {
+        return MRT_ERR_INVALID;
+    }
+}
 
 } // extern "C"

@@ -1,0 +1,107 @@
+"""Graph runner, node host and pbrt loader on the CPU (no device work).
+
+The pbrt mapping follows merian's PBRTScene (src/merian-shaders/scene/pbrt_scene.cpp): diffuse /
+conductor / dielectric onto OpenPBR (:365-487), roughness remapped by sqrt unless
+"remaproughness" is false (:332-364), every mesh two-sided except two-sided emitters (:802-815),
+camera forward/up from the camera-to-world matrix and fov on the shorter axis (:970-1000).
+"""
+import math
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from merian_amd import abi, pbrt_io, scenes
+
+ROOT = Path(__file__).resolve().parent.parent
+GRAPHS = Path(__file__).resolve().parent / "golden" / "graphs"
+RUNNER = ROOT / "merian_amd" / "merian-amd-graph-run"
+
+
+def small_pbrt(tmp_path, **kw):
+    sc = scenes.plain_small(seed=7, subdiv=1, grid=2, res=64)
+    p = tmp_path / "scene.pbrt"
+    scenes.write_pbrt(sc, str(p))
+    return sc, p
+
+
+def test_pbrt_mapping_matches_scene(tmp_path):
+    sc, p = small_pbrt(tmp_path)
+    info = pbrt_io.inspect(p)
+    assert info["triangles"] == sc.triangle_count and info["geometries"] == len(sc.geometries)
+    assert info["warnings"] == []
+    for g, gi in zip(sc.geometries, info["geometry_list"]):
+        m, mi = sc.materials[g["material_id"]], info["materials"][gi["material_id"]]
+        assert np.allclose(mi["base_color"], m["base_color"], atol=1e-6)
+        assert mi["metalness"] == pytest.approx(float(m["metalness"]))
+        assert np.allclose(mi["specular_alpha"], m["specular_alpha"], atol=1e-6)  # remaproughness false
+        assert mi["specular_weight"] == pytest.approx(float(m["specular_weight"]))
+        assert gi["flags"] & abi.GEOM_TWO_SIDED
+    cam = info["camera"]
+    assert cam["present"] and (cam["width"], cam["height"]) == (64, 64)
+    c = sc.camera
+    f = np.subtract(c.target, c.eye)
+    assert np.allclose(cam["eye"], c.eye, atol=1e-5)
+    assert np.allclose(cam["forward"], f / np.linalg.norm(f), atol=1e-5)
+    assert cam["vfov"] == pytest.approx(c.vfov, rel=1e-5)
+    assert np.allclose(info["env"]["constant"], [1, 1, 1])
+
+
+def test_pbrt_statements(tmp_path):
+    p = tmp_path / "s.pbrt"
+    p.write_text('''# comment
+Scale -1 1 1
+LookAt 0 0 5  0 0 0  0 1 0
+Camera "perspective" "float fov" [ 45 ]
+Film "rgb" "integer xresolution" [ 40 ] "integer yresolution" [ 80 ]
+Sampler "zsobol" "integer pixelsamples" 16
+WorldBegin
+LightSource "infinite" "rgb L" [ 0.5 0.25 1 ] "float scale" 2
+MakeNamedMaterial "rough" "string type" "conductor" "float roughness" 0.25 "rgb reflectance" [0.9 0.8 0.7]
+AttributeBegin
+  NamedMaterial "rough"
+  Translate 1 0 0
+  Shape "trianglemesh" "integer indices" [0 1 2] "point3 P" [0 0 0 1 0 0 0 1 0]
+AttributeEnd
+AttributeBegin
+  AreaLightSource "diffuse" "rgb L" [4 4 4] "bool twosided" true
+  Material "diffuse" "rgb reflectance" [0.2 0.3 0.4]
+  Shape "trianglemesh" "point3 P" [0 0 1 1 0 1 0 1 1]
+AttributeEnd
+Material "dielectric" "float eta" 1.33
+Shape "trianglemesh" "integer indices" [0 1 2] "point3 P" [0 0 2 1 0 2 0 1 2]
+Shape "sphere" "float radius" 1
+''')
+    info = pbrt_io.inspect(p)
+    assert info["triangles"] == 4  # the two-sided emitter is two one-sided sheets
+    m = info["materials"]
+    g = info["geometry_list"]
+    assert m[g[0]["material_id"]]["specular_alpha"] == [0.5, 0.5]  # sqrt(0.25), remapped by default
+    assert m[g[0]["material_id"]]["metalness"] == 1.0
+    assert m[g[1]["material_id"]]["emission"] == [4.0, 4.0, 4.0]
+    assert not g[1]["flags"] & abi.GEOM_TWO_SIDED and not g[2]["flags"] & abi.GEOM_TWO_SIDED
+    assert m[g[3]["material_id"]]["transmission_weight"] == 1.0
+    assert m[g[3]["material_id"]]["specular_ior"] == pytest.approx(1.33)
+    assert info["env"]["constant"] == [0.5, 0.25, 1.0] and info["env"]["scale"] == 2.0
+    cam = info["camera"]
+    assert (cam["width"], cam["height"]) == (40, 80)
+    # fov applies to the shorter axis (here x): vertical fov = 2 atan(tan(fov / 2) / aspect)
+    assert cam["vfov"] == pytest.approx(2 * math.atan(math.tan(math.radians(22.5)) / 0.5), rel=1e-6)
+    assert np.allclose(cam["forward"], [0, 0, -1], atol=1e-6)
+    assert any("sphere" in w for w in info["warnings"])
+
+
+def test_runner_help_lists_overrides():
+    r = subprocess.run([str(RUNNER), str(GRAPHS / "headless_pt.json"), "--help"], capture_output=True, text=True)
+    assert r.returncode == 0
+    for s in ("--spp <value>", "--max-path-length <value>", "--renderer <pt|file.json>", "(required)"):
+        assert s in r.stdout, r.stdout
+
+
+def test_runner_rejects_missing_required_scene():
+    r = subprocess.run([str(RUNNER), str(GRAPHS / "headless_pt.json"), "--spp", "4"], capture_output=True, text=True)
+    assert r.returncode == 1 and "missing required positional args" in r.stderr
+    r = subprocess.run([str(RUNNER), "-x"], capture_output=True, text=True)
+    assert r.returncode == 1 and "unknown option" in r.stderr
