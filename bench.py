@@ -64,6 +64,113 @@ def cpu_baseline(scene, spec, frame, target_s: float, threads: int):
                       f"({segs + npx} segments, {t:.1f} s incl. the crop's share of the CPU gbuffer pass)"}
 
 
+def guided_leg(ctx, scene, L: int) -> dict:
+    """Equal-time relMSE of guided vs plain pt on a 480x270 view of the 1080p scene (same camera,
+    lower resolution), against a 16384-spp plain render of that view. The guided side trains
+    online during its frames; both sides get the same wall time (timed around whole frames)."""
+    import numpy as np
+    import torch
+    from merian_amd import abi, scenes
+    from merian_amd.runtime import Guide, guide_config, pt_params
+    w, h = 480, 270
+    spec = scenes.CameraSpec(scene.camera.eye, scene.camera.target, scene.camera.up, scene.camera.vfov, w, h)
+    ds = ctx.scene(scene)
+    cam = ctx.camera(spec, 0)
+    gb = torch.empty(w * h * 16, dtype=torch.uint8, device="cuda")
+    out = torch.empty(w * h * 4, dtype=torch.float32, device="cuda")
+    acc = torch.zeros(w * h * 4, dtype=torch.float64, device="cuda")
+    ctx.gbuffer_dev(ds, cam, w, h, gb.data_ptr())
+
+    def run(frames, params, guide=None, train=True, base=0):
+        acc.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for f in range(frames):
+            if guide is None:
+                ctx.pt_render_dev(ds, params, cam, base + f, w, h, gb.data_ptr(), out.data_ptr())
+            else:
+                guide.render_dev(ds, params, cam, base + f, w, h, gb.data_ptr(), out.data_ptr(), train=train)
+            acc.add_(out)
+        torch.cuda.synchronize()
+        return (acc / frames).cpu().numpy().reshape(h, w, 4), time.perf_counter() - t0
+
+    plain = pt_params(spp=64, max_path_length=L, record_signature=False)
+    ref, t_ref = run(256, plain, base=100000)  # 16384 spp
+    g = Guide(ctx, ds, guide_config(seed=1))
+    gp = pt_params(spp=64, max_path_length=L, mode=abi.MODE_GUIDED, record_signature=False)
+    gimg, t_g = run(8, gp, guide=g)
+    frames_p = 8
+    pimg, t_p = run(frames_p, plain)
+    frames_p = max(1, int(round(frames_p * t_g / t_p)))
+    pimg, t_p = run(frames_p, plain)
+    g.close()
+    ds.destroy()
+    rel = lambda x: float(np.mean((x[..., :3] - ref[..., :3]) ** 2 / (ref[..., :3] ** 2 + 1e-2)))
+    res = {"workload": f"{w}x{h} view of the 1080p scene, L={L}, 64 spp per frame; reference 16384 spp plain",
+           "relmse_guided": rel(gimg), "relmse_plain": rel(pimg), "seconds_guided": round(t_g, 3),
+           "seconds_plain": round(t_p, 3), "frames_guided": 8, "frames_plain": frames_p,
+           "reference_seconds": round(t_ref, 2)}
+    log(f"guided leg: {res}")
+    return res
+
+
+def serving_leg(ctx) -> dict:
+    """Serving mix: Poisson arrivals over 4 resident scenes (82k..1.0M triangles), resolutions
+    {256^2, 512^2, 1024x768, 1920x1080}, spp {1, 4, 16, 64}, L in 4..8; arrival rate 80 % of the
+    capacity measured with a closed batch. Latency = submit -> result in the host buffer."""
+    import numpy as np
+    from merian_amd import scenes
+    from merian_amd.runtime import Server, pt_params
+    rng = np.random.default_rng(3)
+    resident = [scenes.plain_small(seed=101), scenes.full_scene(seed=102, instances=26, env_size=(512, 256)),
+                scenes.plain_small(seed=105, subdiv=4), scenes.full_scene(seed=103, instances=128, env_size=(512, 256))]
+    dss = [ctx.scene(s) for s in resident]
+    res_choices = [(256, 256), (512, 512), (1024, 768), (1920, 1080)]
+
+    def job():
+        k = int(rng.integers(len(dss)))
+        w, h = res_choices[int(rng.integers(4))]
+        spp = [1, 4, 16, 64][int(rng.integers(4))]
+        L = int(rng.integers(4, 9))
+        sc = resident[k]
+        spec = scenes.CameraSpec(sc.camera.eye, sc.camera.target, sc.camera.up, sc.camera.vfov, w, h)
+        f = int(rng.integers(1 << 16))
+        return dss[k], pt_params(spp=spp, max_path_length=L, record_signature=False), ctx.camera(spec, f), f, w, h
+
+    with Server(ctx, max_slots=4) as srv:
+        batch = [job() for _ in range(16)]
+        t0 = time.perf_counter()
+        subs = [srv.submit(*j) for j in batch]
+        for s_ in subs:
+            s_.wait()
+        capacity = len(batch) / (time.perf_counter() - t0)
+        srv.reset_stats()
+        lam = 0.8 * capacity
+        jobs = [job() for _ in range(48)]
+        arrivals = np.cumsum(rng.exponential(1.0 / lam, len(jobs)))
+        subs = []
+        t0 = time.perf_counter()
+        for a, j in zip(arrivals, jobs):
+            dt = a - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            subs.append(srv.submit(*j))
+        for s_ in subs:
+            s_.wait()
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+    for d in dss:
+        d.destroy()
+    res = {"workload": "48 Poisson-arrival jobs over 4 resident scenes (81,922 / 203,120 / 327,682 / "
+                       "999,944 triangles), res {256^2, 512^2, 1024x768, 1920x1080}, spp {1,4,16,64}, L 4..8",
+           "arrival_rate_per_s": round(lam, 2), "capacity_jobs_per_s": round(capacity, 2),
+           "p50_ms": round(st.p50_ms, 2), "p99_ms": round(st.p99_ms, 2), "max_ms": round(st.max_ms, 2),
+           "jobs_per_s": round(st.completed / wall, 2), "mseg_per_s": round(st.segments / wall / 1e6, 1),
+           "peak_in_flight": st.peak_in_flight, "completed": st.completed}
+    log(f"serving leg: {res}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +180,9 @@ def main():
     ap.add_argument("--max-path-length", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--tune-ms", type=float, default=15000.0, help="autotuning budget before timing (0: off)")
+    ap.add_argument("--no-guided", action="store_true", help="skip the equal-time guided vs plain leg")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-mix leg")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; --gpus N > 1 is not supported", file=sys.stderr)
@@ -99,6 +209,11 @@ def main():
     gb = torch.empty(W * H * 16, dtype=torch.uint8, device=dev)
     out = torch.empty(W * H * 4, dtype=torch.float32, device=dev)
     params = pt_params(spp=args.spp, max_path_length=args.max_path_length, record_signature=False)
+    tuning = None
+    if args.tune_ms > 0:
+        t0 = time.perf_counter()
+        tuning = ds.tune(ctx.camera(spec, 0), W, H, params, budget_ms=args.tune_ms).as_dict()
+        log(f"autotuned in {time.perf_counter() - t0:.1f} s: {tuning}")
 
     def frame(f: int, count: bool = False):
         cam = ctx.camera(spec, f)
@@ -152,9 +267,14 @@ def main():
             "config": {"workload": f"full_1080p: {W}x{H}, {args.spp} spp, max path length {args.max_path_length}, "
                                    f"{scene.triangle_count} triangles (256 displaced tori + open room), "
                                    f"{len(scene.materials)} materials, 2048x1024 lat-long sky, BSDF-only pt",
-                       "segments_per_step": segs // args.steps, "trace_ms_per_step": round(ms_trace / args.steps, 2)},
+                       "segments_per_step": segs // args.steps, "trace_ms_per_step": round(ms_trace / args.steps, 2),
+                       "tuning": tuning},
             "roofline": roof}
     ds.destroy()
+    if not args.no_guided:
+        line["guided"] = guided_leg(ctx, scene, args.max_path_length)
+    if not args.no_serving:
+        line["serving"] = serving_leg(ctx)
     ctx.close()
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)

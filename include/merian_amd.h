@@ -304,6 +304,39 @@ MRT_API mrt_status mrt_render_frame(mrt_ctx* ctx, mrt_scene* scene, const mrt_pt
                                     mrt_packed_hit* gbuffer, mrt_pt_stats* stats);
 
 /* ------------------------------------------------------------------------------------
+ * Autotuning (no counterpart in the reference, whose driver picks its own acceleration
+ * structure and dispatch shapes). A scene carries the launch and BVH settings used when
+ * it is rendered; mrt_tune measures candidate settings on the caller's view and keeps
+ * the fastest. No setting changes results: traversal is exact whatever the tree and the
+ * estimator accumulates in sample order whatever the pass size.
+ * ---------------------------------------------------------------------------------- */
+typedef struct mrt_tuning {
+    uint64_t max_paths;       /* paths in flight per wavefront pass (0 = the context's) */
+    uint32_t refill_lanes;    /* idle lanes of 64 before a traversal wave fetches new rays */
+    uint32_t grid_percent;    /* persistent traversal grid, percent of the resident blocks */
+    uint32_t bvh_max_leaf;    /* triangles per BVH leaf, 1..8 */
+    float bvh_traversal_cost; /* SAH cost of a node visit relative to a triangle test */
+    uint32_t bvh_width;       /* BVH node width (4, the layout this build traverses) */
+    float ms_per_frame;       /* frame time measured for these settings (mrt_tune) */
+} mrt_tuning;
+
+typedef struct mrt_tune_request {
+    mrt_camera camera;        /* the view to tune on */
+    uint32_t width, height;
+    mrt_pt_params params;     /* spp is capped at 8 for the trial frames */
+    float time_budget_ms;     /* wall time the search may take (0 = 20000) */
+    uint32_t _pad;
+} mrt_tune_request;
+
+MRT_API mrt_status mrt_scene_get_tuning(mrt_scene* scene, mrt_tuning* out);
+/* Applies settings; the BVH is rebuilt when its leaf size or traversal cost changes. */
+MRT_API mrt_status mrt_scene_set_tuning(mrt_ctx* ctx, mrt_scene* scene, const mrt_tuning* tuning);
+/* Coordinate search over BVH (leaf size, traversal cost), paths in flight, refill
+ * threshold and grid size; the fastest settings stay applied and are returned. */
+MRT_API mrt_status mrt_tune(mrt_ctx* ctx, mrt_scene* scene, const mrt_tune_request* request,
+                            mrt_tuning* best);
+
+/* ------------------------------------------------------------------------------------
  * Device buffers and the accumulation pass, for node hosts that link only this ABI.
  * ---------------------------------------------------------------------------------- */
 /* Device allocation charged against the context's budget. */

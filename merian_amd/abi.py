@@ -15,6 +15,8 @@ PKG_DIR = Path(__file__).resolve().parent
 LIB_PATH = PKG_DIR / "libmerian_amd.so"
 
 MRT_OK = 0
+MRT_ERR_INVALID, MRT_ERR_DEVICE, MRT_ERR_BUDGET, MRT_ERR_UNSUPPORTED = -1, -2, -3, -4
+MRT_ERR_NOT_READY, MRT_ERR_CANCELLED, MRT_ERR_ALLOC = -5, -6, -7
 STATUS_NAMES = {0: "ok", -1: "invalid", -2: "device", -3: "budget", -4: "unsupported",
                 -5: "not_ready", -6: "cancelled", -7: "alloc"}
 
@@ -109,6 +111,20 @@ class PtStats(C.Structure):
                 ("ms_shade", C.c_float), ("ms_total", C.c_float),
                 ("trace_launches", C.c_uint32), ("_pad", C.c_uint32), ("node_visits", C.c_uint64),
                 ("triangle_tests", C.c_uint64)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("max_paths", C.c_uint64), ("refill_lanes", C.c_uint32), ("grid_percent", C.c_uint32),
+                ("bvh_max_leaf", C.c_uint32), ("bvh_traversal_cost", C.c_float), ("bvh_width", C.c_uint32),
+                ("ms_per_frame", C.c_float)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class TuneRequest(C.Structure):
+    _fields_ = [("camera", Camera), ("width", C.c_uint32), ("height", C.c_uint32), ("params", PtParams),
+                ("time_budget_ms", C.c_float), ("_pad", C.c_uint32)]
 
 
 class ServerConfig(C.Structure):
@@ -239,6 +255,9 @@ EXPORTS = [
     ("mrt_accumulate_dev", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]),
     ("merian_node_plugin_abi_version", C.c_uint32, []),
     ("merian_register_nodes", None, [C.c_void_p]),
+    ("mrt_scene_get_tuning", C.c_int32, [C.c_void_p, C.POINTER(Tuning)]),
+    ("mrt_scene_set_tuning", C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(Tuning)]),
+    ("mrt_tune", C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(TuneRequest), C.POINTER(Tuning)]),
 ]
 
 

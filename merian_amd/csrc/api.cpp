@@ -137,6 +137,13 @@ mrt_status ensure_trace_scratch(mrt_ctx* c, const mrt_scene* s) {
     return MRT_OK;
 }
 
+TraceScratch scene_scratch(const mrt_ctx* c, const mrt_scene* s) {
+    TraceScratch t = c ? c->trace : TraceScratch{};
+    t.refill = s->tuning.refill_lanes >= 1 && s->tuning.refill_lanes <= 64 ? s->tuning.refill_lanes : 16u;
+    t.grid_pct = s->tuning.grid_percent >= 1 && s->tuning.grid_percent <= 100 ? s->tuning.grid_percent : 100u;
+    return t;
+}
+
 } // namespace mrt
 
 extern "C" {
@@ -410,6 +417,10 @@ MRT_API mrt_status mrt_scene_create(mrt_ctx* c, const mrt_scene_desc* d, mrt_sce
         scene_free(c, s);
         return st;
     }
+    s->h_v0 = std::move(v0);
+    s->h_e1 = std::move(e1);
+    s->h_e2 = std::move(e2);
+    s->h_flags = std::move(flags);
     *out = s;
     return MRT_OK;
 }
@@ -508,7 +519,7 @@ MRT_API mrt_status mrt_trace_closest_dev(mrt_ctx* c, mrt_scene* s, const mrt_ray
     if (!c || !s || (n && (!rays || !hits))) return fail(c, MRT_ERR_INVALID, "mrt_trace_closest_dev: bad argument");
     mrt_status st = ensure_trace_scratch(c, s);
     if (st != MRT_OK) return st;
-    const hipError_t e = launch_trace_rays(s->ds, c->trace, rays, n, hits, pick_stream(c, stream));
+    const hipError_t e = launch_trace_rays(s->ds, scene_scratch(c, s), rays, n, hits, pick_stream(c, stream));
     return e == hipSuccess ? MRT_OK : hip_fail(c, e, "trace kernel");
 }
 
@@ -521,7 +532,7 @@ MRT_API mrt_status mrt_trace_count_dev(mrt_ctx* c, mrt_scene* s, const mrt_ray* 
     if (st != MRT_OK) return st;
     hipError_t e = hipMemsetAsync(dc, 0, 16, c->stream);
     if (e == hipSuccess)
-        e = launch_trace_rays(s->ds, c->trace, rays, n, hits, c->stream, static_cast<unsigned long long*>(dc));
+        e = launch_trace_rays(s->ds, scene_scratch(c, s), rays, n, hits, c->stream, static_cast<unsigned long long*>(dc));
     unsigned long long h[2] = {0, 0};
     if (e == hipSuccess) e = hipMemcpyAsync(h, dc, 16, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -546,7 +557,8 @@ MRT_API mrt_status mrt_trace_closest(mrt_ctx* c, mrt_scene* s, const mrt_ray* ra
     }
     hipError_t e = hipMemcpyAsync(dr, rays, n * sizeof(mrt_ray), hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess)
-        e = launch_trace_rays(s->ds, c->trace, static_cast<mrt_ray*>(dr), n, static_cast<mrt_hit*>(dh), c->stream);
+        e = launch_trace_rays(s->ds, scene_scratch(c, s), static_cast<mrt_ray*>(dr), n, static_cast<mrt_hit*>(dh),
+                              c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(hits, dh, n * sizeof(mrt_hit), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     dev_free(c, dr);
@@ -566,7 +578,7 @@ MRT_API mrt_status mrt_gbuffer_dev(mrt_ctx* c, mrt_scene* s, const mrt_camera* c
     if (!c || !s || !cam || (uint64_t(w) * h && !out)) return fail(c, MRT_ERR_INVALID, "mrt_gbuffer_dev: bad argument");
     const mrt_status st = ensure_trace_scratch(c, s);
     if (st != MRT_OK) return st;
-    const hipError_t e = launch_gbuffer(s->ds, c->trace, *cam, w, h, mask & 0xFFu, out, pick_stream(c, stream));
+    const hipError_t e = launch_gbuffer(s->ds, scene_scratch(c, s), *cam, w, h, mask & 0xFFu, out, pick_stream(c, stream));
     return e == hipSuccess ? MRT_OK : hip_fail(c, e, "gbuffer kernel");
 }
 
@@ -597,7 +609,8 @@ mrt_status render_frame_dev(mrt_ctx* c, mrt_scene* s, const mrt_pt_params* prm, 
     if (npix > 0xFFFFFFFFull) return fail(c, MRT_ERR_UNSUPPORTED, "image too large");
     hipSetDevice(c->device);
     const uint64_t spp = uint64_t(prm->spp);
-    uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(spp ? spp : 1, c->max_paths / npix));
+    const uint64_t max_paths = s->tuning.max_paths ? s->tuning.max_paths : c->max_paths;
+    uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(spp ? spp : 1, max_paths / npix));
     if (npix * chunk > 0xFFFFFFFFull) chunk = 0xFFFFFFFFull / npix;
     const uint64_t n_chunks = spp ? (spp + chunk - 1) / chunk : 1;
     st = ensure_path_buffers(c, npix * chunk, npix, n_chunks * uint64_t(prm->max_path_length + 1));
@@ -633,7 +646,7 @@ mrt_status render_frame_dev(mrt_ctx* c, mrt_scene* s, const mrt_pt_params* prm, 
         if (st != MRT_OK) return st;
         hipMemsetAsync(tc, 0, 16, strm);
     }
-    const hipError_t e = launch_pt_frame(s->ds, c->trace, a, gbuffer, pb, reinterpret_cast<float4*>(irradiance),
+    const hipError_t e = launch_pt_frame(s->ds, scene_scratch(c, s), a, gbuffer, pb, reinterpret_cast<float4*>(irradiance),
                                          a.record_signature ? signature : nullptr, strm, stats ? &t : nullptr,
                                          stats ? &segs : nullptr, tc, net);
     unsigned long long hc[2] = {0, 0};

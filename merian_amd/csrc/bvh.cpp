@@ -2,7 +2,7 @@
 //
 // The reference leaves BVH construction to the Vulkan driver (PreferFastTrace for static mesh
 // groups, src/merian-shaders/scene/scene.cpp:1116-1123). This builder is the project's own:
-// 32-bin SAH per axis over centroid bounds, leaves of at most MAX_LEAF triangles, top levels
+// 32-bin SAH per axis over centroid bounds, leaves of at most max_leaf (<= MAX_LEAF) triangles, top levels
 // built in parallel threads. Child boxes are rounded outward and padded so that the slab test
 // (one FMA per plane) can never reject a box containing a hit the triangle test reports.
 #include "bvh.h"
@@ -58,6 +58,8 @@ struct Builder {
     std::vector<Node2> nodes;
     std::atomic<uint32_t> next{1};
     int par_depth;
+    uint32_t max_leaf = 4;
+    float traversal_cost = 1.0f;
 
     Builder(const std::vector<Prim>& p, std::vector<uint32_t> order, int threads)
         : prims(p), idx(std::move(order)), nodes(std::max<size_t>(1, 2 * idx.size())) {
@@ -127,13 +129,13 @@ struct Builder {
         const float leaf_cost = box.area() * float(n);
         uint32_t mid;
         if (best_axis < 0) {
-            if (n <= uint32_t(MAX_LEAF)) {
+            if (n <= max_leaf) {
                 make_leaf(node, b, e);
                 return;
             }
             mid = b + n / 2; // coincident centroids: split by index
         } else {
-            if (n <= uint32_t(MAX_LEAF) && best_cost + box.area() >= leaf_cost) {
+            if (n <= max_leaf && best_cost + traversal_cost * box.area() >= leaf_cost) {
                 make_leaf(node, b, e);
                 return;
             }
@@ -250,7 +252,7 @@ float up(double v) {
 } // namespace
 
 BVH4 build_bvh4(const float* v0, const float* e1, const float* e2, const uint32_t* flags, uint32_t n,
-                int threads) {
+                int threads, int max_leaf, float traversal_cost) {
     if (threads <= 0) threads = int(std::max(1u, std::thread::hardware_concurrency()));
     std::vector<Prim> prims(n);
     std::vector<uint32_t> order;
@@ -274,6 +276,8 @@ BVH4 build_bvh4(const float* v0, const float* e1, const float* e2, const uint32_
     BVH4 out;
     const uint32_t m = uint32_t(order.size());
     Builder bld(prims, std::move(order), threads);
+    bld.max_leaf = uint32_t(std::min(std::max(max_leaf, 1), MAX_LEAF));
+    bld.traversal_cost = traversal_cost > 0.f ? traversal_cost : 1.0f;
     if (m > 0) bld.build(0, 0, m, 0);
     // triangle records in leaf order
     out.tris.resize(m);

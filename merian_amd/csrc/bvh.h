@@ -31,7 +31,7 @@ struct alignas(16) TriRecord {
 
 // Leaf stack entries: bit 31 set, first record in bits 3..30, (count - 1) in bits 0..2.
 constexpr uint32_t LEAF_FLAG = 0x80000000u;
-constexpr int MAX_LEAF = 4;
+constexpr int MAX_LEAF = 8; // most triangles per leaf the encoding allows (the build default is 4)
 
 struct BVH4 {
     std::vector<BVH4Node> nodes;   // nodes[0] is the root
@@ -42,7 +42,9 @@ struct BVH4 {
 
 // v0/e1/e2: per-triangle float[3] operands exactly as the intersection test uses them
 // (e1 = v1 - v0, e2 = v2 - v0 rounded in float). flags as in TriRecord.
+// max_leaf: most triangles per leaf (1..MAX_LEAF); traversal_cost: SAH cost of one node
+// visit relative to one triangle test.
 BVH4 build_bvh4(const float* v0, const float* e1, const float* e2, const uint32_t* flags, uint32_t n,
-                int threads);
+                int threads, int max_leaf = 4, float traversal_cost = 1.0f);
 
 } // namespace mrt

@@ -10,6 +10,7 @@
 #include "mrt_kernels.h"
 #include "mrt_trace.h"
 
+#include <algorithm>
 #include <vector>
 
 namespace mrt {
@@ -185,7 +186,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DeviceScene S, Src src, S
     const LaneStack stk{stack + threadIdx.x, blockDim.x, ts.ovf + gtid, gridDim.x * blockDim.x};
     TraversalCount cnt;
     trace_persistent<TRACE_LDS_STACK, COUNT, Src, Sink, ANY>(DeviceBVH{S.nodes, S.tris}, src, sink, n, work, mask,
-                                                             stk, cnt);
+                                                             stk, cnt, ts.refill);
     if constexpr (COUNT) flush_counts(cnt, counters);
 }
 
@@ -511,8 +512,9 @@ __device__ __forceinline__ bool guide_eligible(const mrt_material& m) {
 // the guide's lobes with probability A.guide_fraction, else the BSDF, and weighs the direction
 // by f / (a q + (1 - a) p_bsdf) (one-sample MIS); sample-0 paths of the first chunk append a
 // training record per guided vertex.
+// BSDF mode fits 168 VGPRs without spills: 3 waves per SIMD instead of 2.
 template <int MODE>
-__global__ __launch_bounds__(SHADE_BLOCK) void k_shade(DeviceScene S, PtArgs A, int bounce, PathBuffers P,
+__global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(MODE == MRT_MODE_BSDF ? 3 : 1))) void k_shade(DeviceScene S, PtArgs A, int bounce, PathBuffers P,
                                                        const float4* __restrict__ qo, const float4* __restrict__ qd,
                                                        const uint32_t* __restrict__ count_in, float4* __restrict__ no,
                                                        float4* __restrict__ nd, uint32_t* __restrict__ count_out,
@@ -767,7 +769,9 @@ hipError_t launch_trace(const DeviceScene& s, const TraceScratch& ts, const Src&
                         unsigned long long* counters, hipStream_t st) {
     if (n_max == 0) return hipSuccess;
     if (s.stack_need > TRACE_LDS_STACK + ts.entries) return hipErrorInvalidValue; // overflow area too small
-    int g = grid_for(n_max, TRACE_BLOCK, trace_resident_blocks<COUNT, Src, Sink, ANY>());
+    const int resident = trace_resident_blocks<COUNT, Src, Sink, ANY>();
+    const int cap = std::max(1, int(int64_t(resident) * int64_t(ts.grid_pct) / 100));
+    int g = grid_for(n_max, TRACE_BLOCK, cap);
     if (uint64_t(g) * TRACE_BLOCK > ts.threads) g = int(ts.threads / TRACE_BLOCK);
     if (g <= 0) return hipErrorInvalidValue;
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_trace<COUNT, Src, Sink, ANY>), dim3(g), dim3(TRACE_BLOCK), 0, st, s, src,

@@ -29,6 +29,10 @@ struct mrt_scene {
     uint64_t bytes = 0, n_tris = 0, n_nodes = 0;
     uint32_t depth = 0;
     float bbox_min[3] = {0.f, 0.f, 0.f}, bbox_max[3] = {0.f, 0.f, 0.f}; // bounds of the triangles
+    // triangle operands kept on the host for BVH rebuilds (tuning)
+    std::vector<float> h_v0, h_e1, h_e2;
+    std::vector<uint32_t> h_flags;
+    mrt_tuning tuning{0, 16, 100, 4, 1.0f, 4, 0.f};
 };
 
 namespace mrt {
@@ -40,6 +44,10 @@ hipStream_t pick_stream(mrt_ctx* c, void* stream);
 mrt_status ensure_path_buffers(mrt_ctx* c, uint64_t paths, uint64_t npix, uint64_t counters);
 mrt_status check_params(mrt_ctx* c, const mrt_pt_params* p);
 mrt_status ensure_trace_scratch(mrt_ctx* c, const mrt_scene* s);
+// The context's traversal scratch with the scene's launch settings.
+TraceScratch scene_scratch(const mrt_ctx* c, const mrt_scene* s);
+// Rebuilds and uploads the scene's BVH with the given leaf size and SAH traversal cost.
+mrt_status rebuild_bvh(mrt_ctx* c, mrt_scene* s, uint32_t max_leaf, float traversal_cost);
 // One pt frame on the device (mrt_pt_render_dev and, with a guide, mrt_guided_render_dev).
 mrt_status render_frame_dev(mrt_ctx* c, mrt_scene* s, const mrt_pt_params* prm, const mrt_camera* cam, uint32_t frame,
                             uint32_t w, uint32_t h, const mrt_packed_hit* gbuffer, float* irradiance,

@@ -40,6 +40,8 @@ struct TraceScratch {
     uint32_t threads;
     uint32_t entries;
     uint32_t* work;
+    uint32_t refill = 16;   // idle lanes before a wave refetches rays (1..64)
+    uint32_t grid_pct = 100; // persistent grid size, percent of the resident blocks
 };
 
 // Most trace-kernel threads resident on the current device (the persistent grid's upper bound).

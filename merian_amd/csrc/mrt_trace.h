@@ -184,7 +184,8 @@ __device__ __forceinline__ void ray_step(const DeviceBVH bvh, RayState& r, const
     }
 }
 
-// Lanes fetch new rays once at least this many of a wave's 64 lanes are idle (one atomic each).
+// Default refill threshold: lanes fetch new rays once at least this many of a wave's 64 lanes
+// are idle (one atomic per refill). The autotuner may choose another.
 constexpr int REFILL_LANES = 16;
 
 // Persistent closest-hit loop of one wave: every lane runs one traversal step per iteration and,
@@ -193,7 +194,7 @@ constexpr int REFILL_LANES = 16;
 template <int SL, bool COUNT, class Src, class Sink, bool ANY = false>
 __device__ __forceinline__ void trace_persistent(const DeviceBVH bvh, const Src& src, const Sink& sink, uint32_t n,
                                                  uint32_t* work, uint32_t ray_mask, const LaneStack& stk,
-                                                 TraversalCount& count) {
+                                                 TraversalCount& count, uint32_t refill) {
     const uint32_t lane = __lane_id();
     RayState r;
     r.cur = TRAVERSAL_DONE;
@@ -201,7 +202,7 @@ __device__ __forceinline__ void trace_persistent(const DeviceBVH bvh, const Src&
     uint32_t idx = 0;
     while (true) {
         const uint64_t idle = __ballot(!active);
-        if (!exhausted && __popcll(idle) >= REFILL_LANES) {
+        if (!exhausted && uint32_t(__popcll(idle)) >= refill) {
             const uint32_t take = uint32_t(__popcll(idle));
             const uint32_t leader = uint32_t(__ffsll((unsigned long long)idle) - 1);
             uint32_t base = 0;

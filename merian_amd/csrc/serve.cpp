@@ -174,6 +174,9 @@ hipError_t alloc_job(Slot& sl, const Job& j, const JobShape& sh) {
     b.ts.work = static_cast<uint32_t*>(A(64));
     b.ts.threads = l.threads;
     b.ts.entries = sh.stack_entries;
+    const TraceScratch tuned = scene_scratch(nullptr, j.d.scene);
+    b.ts.refill = tuned.refill;
+    b.ts.grid_pct = tuned.grid_pct;
     return e;
 }
 

@@ -1,0 +1,207 @@
+// Autotuning of include/merian_amd.h (mrt_tune, mrt_scene_get/set_tuning): per-scene BVH build
+// settings (leaf size, SAH traversal cost) and launch settings (paths in flight, refill
+// threshold, persistent grid size), chosen by timing trial frames of the caller's view.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+#include "../../include/merian_amd.h"
+#include "bvh.h"
+#include "mrt_api_internal.h"
+
+using namespace mrt;
+
+namespace mrt {
+
+mrt_status rebuild_bvh(mrt_ctx* c, mrt_scene* s, uint32_t max_leaf, float traversal_cost) {
+    if (s->h_flags.size() != s->n_tris) return fail(c, MRT_ERR_UNSUPPORTED, "scene keeps no triangle copy");
+    const BVH4 bvh = build_bvh4(s->h_v0.data(), s->h_e1.data(), s->h_e2.data(), s->h_flags.data(),
+                                uint32_t(s->n_tris), 0, int(max_leaf), traversal_cost);
+    hipStreamSynchronize(c->stream);
+    mrt_status st = MRT_OK;
+    void* nodes = dev_alloc(c, bvh.nodes.size() * sizeof(BVH4Node), &st);
+    if (st != MRT_OK) return st;
+    void* tris = dev_alloc(c, bvh.tris.size() * sizeof(TriRecord), &st);
+    if (st != MRT_OK) {
+        dev_free(c, nodes);
+        return st;
+    }
+    hipError_t e = hipMemcpyAsync(nodes, bvh.nodes.data(), bvh.nodes.size() * sizeof(BVH4Node),
+                                  hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(tris, bvh.tris.data(), bvh.tris.size() * sizeof(TriRecord), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        dev_free(c, nodes);
+        dev_free(c, tris);
+        return hip_fail(c, e, "BVH upload");
+    }
+    // swap the device trees
+    for (void** old : {(void**)&s->ds.nodes, (void**)&s->ds.tris}) {
+        auto it = std::find(s->allocs.begin(), s->allocs.end(), *old);
+        if (it != s->allocs.end()) s->allocs.erase(it);
+        s->bytes -= c->sizes.count(*old) ? c->sizes[*old] : 0;
+        dev_free(c, *old);
+    }
+    s->ds.nodes = static_cast<const BVH4Node*>(nodes);
+    s->ds.tris = static_cast<const float4*>(tris);
+    s->allocs.push_back(nodes);
+    s->allocs.push_back(tris);
+    s->bytes += bvh.nodes.size() * sizeof(BVH4Node) + bvh.tris.size() * sizeof(TriRecord);
+    s->ds.stack_need = 3u * bvh.depth + 4u;
+    s->n_nodes = bvh.nodes.size();
+    s->depth = bvh.depth;
+    s->tuning.bvh_max_leaf = max_leaf;
+    s->tuning.bvh_traversal_cost = traversal_cost;
+    return MRT_OK;
+}
+
+} // namespace mrt
+
+namespace {
+
+bool valid(const mrt_tuning& t) {
+    return t.refill_lanes >= 1 && t.refill_lanes <= 64 && t.grid_percent >= 1 && t.grid_percent <= 100 &&
+           t.bvh_max_leaf >= 1 && t.bvh_max_leaf <= uint32_t(MAX_LEAF) && t.bvh_traversal_cost > 0.f &&
+           (t.bvh_width == 0 || t.bvh_width == 4);
+}
+
+// Device time of one trial frame (gbuffer + pt) in ms, after one untimed frame.
+mrt_status trial(mrt_ctx* c, mrt_scene* s, const mrt_tune_request& r, float& ms) {
+    mrt_pt_params p = r.params;
+    p.spp = std::min(std::max(p.spp, 1), 8);
+    p.record_signature = 0;
+    p.count_traversal = 0;
+    const uint64_t npix = uint64_t(r.width) * r.height;
+    mrt_status st = MRT_OK;
+    void* gb = dev_alloc(c, npix * sizeof(mrt_packed_hit), &st);
+    if (st != MRT_OK) return st;
+    void* out = dev_alloc(c, npix * 16, &st);
+    if (st != MRT_OK) {
+        dev_free(c, gb);
+        return st;
+    }
+    float best = 1e30f;
+    for (int k = 0; k < 3 && st == MRT_OK; k++) {
+        hipEvent_t e0, e1;
+        hipEventCreate(&e0);
+        hipEventCreate(&e1);
+        hipEventRecord(e0, c->stream);
+        st = mrt_gbuffer_dev(c, s, &r.camera, r.width, r.height, p.instance_mask, static_cast<mrt_packed_hit*>(gb),
+                             nullptr);
+        if (st == MRT_OK)
+            st = render_frame_dev(c, s, &p, &r.camera, uint32_t(k), r.width, r.height,
+                                  static_cast<mrt_packed_hit*>(gb), static_cast<float*>(out), nullptr, nullptr,
+                                  nullptr, nullptr);
+        hipEventRecord(e1, c->stream);
+        hipEventSynchronize(e1);
+        float t = 0.f;
+        hipEventElapsedTime(&t, e0, e1);
+        if (k > 0) best = std::min(best, t); // the first frame may (re)allocate buffers
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+    }
+    dev_free(c, gb);
+    dev_free(c, out);
+    ms = best;
+    return st;
+}
+
+} // namespace
+
+extern "C" {
+
+MRT_API mrt_status mrt_scene_get_tuning(mrt_scene* s, mrt_tuning* out) {
+    if (!s || !out) return MRT_ERR_INVALID;
+    *out = s->tuning;
+    return MRT_OK;
+}
+
+MRT_API mrt_status mrt_scene_set_tuning(mrt_ctx* c, mrt_scene* s, const mrt_tuning* t) {
+    if (!c || !s || !t) return MRT_ERR_INVALID;
+    if (!valid(*t)) return fail(c, MRT_ERR_INVALID, "tuning values out of range");
+    hipSetDevice(c->device);
+    if (t->bvh_max_leaf != s->tuning.bvh_max_leaf || t->bvh_traversal_cost != s->tuning.bvh_traversal_cost) {
+        const mrt_status st = rebuild_bvh(c, s, t->bvh_max_leaf, t->bvh_traversal_cost);
+        if (st != MRT_OK) return st;
+    }
+    s->tuning = *t;
+    s->tuning.bvh_width = 4;
+    return MRT_OK;
+}
+
+MRT_API mrt_status mrt_tune(mrt_ctx* c, mrt_scene* s, const mrt_tune_request* r, mrt_tuning* best_out) {
+    if (!c || !s || !r) return MRT_ERR_INVALID;
+    if (r->width == 0 || r->height == 0) return fail(c, MRT_ERR_INVALID, "mrt_tune: empty image");
+    const mrt_status pc = check_params(c, &r->params);
+    if (pc != MRT_OK) return pc;
+    hipSetDevice(c->device);
+    const auto t0 = std::chrono::steady_clock::now();
+    const double budget = r->time_budget_ms > 0.f ? double(r->time_budget_ms) : 20000.0;
+    auto elapsed = [&] { return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); };
+
+    mrt_tuning best = s->tuning;
+    if (best.max_paths == 0) best.max_paths = c->max_paths;
+    mrt_status st = mrt_scene_set_tuning(c, s, &best);
+    if (st != MRT_OK) return st;
+    st = trial(c, s, *r, best.ms_per_frame);
+    if (st != MRT_OK) return st;
+
+    auto try_one = [&](mrt_tuning cand) -> mrt_status {
+        if (elapsed() > budget) return MRT_OK;
+        mrt_status e = mrt_scene_set_tuning(c, s, &cand);
+        if (e != MRT_OK) return e;
+        float ms = 0.f;
+        e = trial(c, s, *r, ms);
+        if (e == MRT_ERR_BUDGET || e == MRT_ERR_ALLOC) {
+            c->err.clear(); // a candidate that does not fit is skipped
+            return mrt_scene_set_tuning(c, s, &best);
+        }
+        if (e != MRT_OK) return e;
+        cand.ms_per_frame = ms;
+        if (ms < best.ms_per_frame) best = cand;
+        return mrt_scene_set_tuning(c, s, &best);
+    };
+    const uint64_t npix = uint64_t(r->width) * r->height;
+    for (int pass = 0; pass < 2 && elapsed() < budget; pass++) {
+        const mrt_tuning start = best;
+        // BVH build settings
+        const std::pair<uint32_t, float> bvh[] = {{4, 1.0f}, {2, 1.0f}, {8, 1.0f}, {4, 0.5f}, {4, 2.0f}, {8, 2.0f}};
+        for (auto [leaf, cost] : bvh) {
+            if (leaf == best.bvh_max_leaf && cost == best.bvh_traversal_cost) continue;
+            mrt_tuning cand = best;
+            cand.bvh_max_leaf = leaf;
+            cand.bvh_traversal_cost = cost;
+            if ((st = try_one(cand)) != MRT_OK) return st;
+        }
+        // paths in flight (whole samples of the image per pass)
+        for (uint64_t mp : {uint64_t(1) << 21, uint64_t(1) << 22, uint64_t(1) << 23, uint64_t(1) << 24,
+                            uint64_t(1) << 25}) {
+            if (mp == best.max_paths || mp < npix) continue;
+            mrt_tuning cand = best;
+            cand.max_paths = mp;
+            if ((st = try_one(cand)) != MRT_OK) return st;
+        }
+        for (uint32_t rf : {8u, 16u, 24u, 32u, 48u}) {
+            if (rf == best.refill_lanes) continue;
+            mrt_tuning cand = best;
+            cand.refill_lanes = rf;
+            if ((st = try_one(cand)) != MRT_OK) return st;
+        }
+        for (uint32_t gp : {50u, 75u, 100u}) {
+            if (gp == best.grid_percent) continue;
+            mrt_tuning cand = best;
+            cand.grid_percent = gp;
+            if ((st = try_one(cand)) != MRT_OK) return st;
+        }
+        if (std::memcmp(&start, &best, sizeof(best)) == 0) break;
+    }
+    st = mrt_scene_set_tuning(c, s, &best);
+    if (st == MRT_OK && best_out) *best_out = best;
+    return st;
+}
+
+} // extern "C"

@@ -35,6 +35,22 @@ class DeviceScene:
             self.ctx.lib.mrt_scene_destroy(self.ctx.h, self.h)
             self.h = None
 
+    def tuning(self) -> abi.Tuning:
+        t = abi.Tuning()
+        self.ctx._check(self.ctx.lib.mrt_scene_get_tuning(self.h, C.byref(t)))
+        return t
+
+    def set_tuning(self, t: abi.Tuning):
+        self.ctx._check(self.ctx.lib.mrt_scene_set_tuning(self.ctx.h, self.h, C.byref(t)))
+
+    def tune(self, cam: abi.Camera, w: int, h: int, params: abi.PtParams, budget_ms: float = 20000.0) -> abi.Tuning:
+        """Autotunes this scene's BVH and launch settings on the given view (mrt_tune)."""
+        r = abi.TuneRequest()
+        r.camera, r.width, r.height, r.params, r.time_budget_ms = cam, w, h, params, budget_ms
+        t = abi.Tuning()
+        self.ctx._check(self.ctx.lib.mrt_tune(self.ctx.h, self.h, C.byref(r), C.byref(t)))
+        return t
+
 
 class Context:
     def __init__(self, device: int = 0, hbm_budget: int = 0):

@@ -75,3 +75,28 @@ def test_pt_full_scene_crop(ctx):
     ref, ref_sig, _ = osc.render(O.pt_params(4, 8), cam, frame, W, H, gb)
     ds.destroy()
     _compare(img, sig, ref, ref_sig, min_match=0.999)
+
+
+def test_converged_image_relmse_against_16384_spp(ctx):
+    # the converged image (1024 spp, independent RNG streams) against a 16384-spp render of the
+    # same frame: relMSE = mean (x - r)^2 / (r^2 + 1e-2) < 1e-3
+    sc = scenes.plain_small(seed=0, res=128)
+    ds = ctx.scene(sc)
+    try:
+        cam = ctx.camera(sc.camera, 0)
+        ref = np.zeros((128, 128, 4), np.float64)
+        for k in range(4):  # 4 x 4096 spp, distinct frame seeds
+            img, _, _, _ = ctx.render(ds, pt_params(spp=4096, max_path_length=4, record_signature=False), cam,
+                                      1000 + k, 128, 128, signature=False, gbuffer=False)
+            ref += img
+        ref /= 4
+        img, _, _, _ = ctx.render(ds, pt_params(spp=1024, max_path_length=4, record_signature=False), cam, 7,
+                                  128, 128, signature=False, gbuffer=False)
+        rel = float(np.mean((img[..., :3] - ref[..., :3]) ** 2 / (ref[..., :3] ** 2 + 1e-2)))
+        assert rel < 1e-3, rel
+        # and far fewer samples are visibly not converged (the bound is not vacuous)
+        few, _, _, _ = ctx.render(ds, pt_params(spp=4, max_path_length=4, record_signature=False), cam, 9,
+                                  128, 128, signature=False, gbuffer=False)
+        assert float(np.mean((few[..., :3] - ref[..., :3]) ** 2 / (ref[..., :3] ** 2 + 1e-2))) > rel
+    finally:
+        ds.destroy()
