@@ -22,12 +22,14 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mpath-segments/s at 1080p; equal-time relMSE guided vs plain; p99 job latency"
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES = 128      # BVH4Node
+NODE_BYTES = {4: 128, 8: 80}  # BVH4Node (fp32 boxes), BVH8Node (8-bit quantized boxes)
 TRI_BYTES = 48        # TriRecord
 RAY_IO_BYTES = 32 + 16  # queued ray in, hit out
 # HBM bytes per dispatch of the intersect kernel, measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
 # passes over this workload and calibrated against a plain copy (python -m merian_amd.profiling traffic)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_trace_queue_full_1080p.traffic.json")
+# autotune result of the workload, applied before timing (re-run the search with --retune)
+TUNING_CACHE = os.path.join(ROOT, "merian_amd", "tuning", "full_1080p.json")
 
 
 def log(*a):
@@ -180,7 +182,10 @@ def main():
     ap.add_argument("--max-path-length", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--tune-ms", type=float, default=15000.0, help="autotuning budget before timing (0: off)")
+    ap.add_argument("--tune-ms", type=float, default=15000.0, help="autotuning budget of --retune")
+    ap.add_argument("--retune", action="store_true",
+                    help="search the tuning again instead of applying the cached autotune result")
+    ap.add_argument("--default-tuning", action="store_true", help="render with the untuned defaults")
     ap.add_argument("--no-guided", action="store_true", help="skip the equal-time guided vs plain leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-mix leg")
     args = ap.parse_args()
@@ -209,11 +214,22 @@ def main():
     gb = torch.empty(W * H * 16, dtype=torch.uint8, device=dev)
     out = torch.empty(W * H * 4, dtype=torch.float32, device=dev)
     params = pt_params(spp=args.spp, max_path_length=args.max_path_length, record_signature=False)
-    tuning = None
-    if args.tune_ms > 0:
+    tuning, tuning_source = None, "defaults"
+    if args.retune:
         t0 = time.perf_counter()
         tuning = ds.tune(ctx.camera(spec, 0), W, H, params, budget_ms=args.tune_ms).as_dict()
+        tuning_source = f"mrt_tune now ({time.perf_counter() - t0:.1f} s)"
         log(f"autotuned in {time.perf_counter() - t0:.1f} s: {tuning}")
+    elif not args.default_tuning and os.path.exists(TUNING_CACHE) and args.spp == 64 and args.max_path_length == 8:
+        with open(TUNING_CACHE) as f:
+            cached = json.load(f)["tuning"]
+        t = ds.tuning()
+        for k, v in cached.items():
+            setattr(t, k, v)
+        ds.set_tuning(t)
+        tuning = ds.tuning().as_dict()
+        tuning_source = "cached mrt_tune result (merian_amd/tuning/full_1080p.json)"
+        log(f"applied cached tuning {tuning}")
 
     def frame(f: int, count: bool = False):
         cam = ctx.camera(spec, f)
@@ -243,7 +259,8 @@ def main():
     # variant on the first timed frame, scaled to the timed frames' continuation segments
     st_c = frame(args.warmup, count=True)
     cont = max(1, st_c.continuation_segments)
-    bytes_per_seg = (st_c.node_visits * NODE_BYTES + st_c.triangle_tests * TRI_BYTES) / cont + RAY_IO_BYTES
+    node_bytes = NODE_BYTES[int(ds.tuning().bvh_width)]
+    bytes_per_seg = (st_c.node_visits * node_bytes + st_c.triangle_tests * TRI_BYTES) / cont + RAY_IO_BYTES
     cont_timed = segs - args.steps * W * H
     bytes_per_launch = bytes_per_seg * cont_timed / max(1, launches)
     avg_ms = ms_trace / max(1, launches)
@@ -251,12 +268,16 @@ def main():
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
-            traffic = json.load(f)["traffic_bytes_per_dispatch"]
+            tj = json.load(f)
+        # the profile holds for the tuning it was taken with
+        prof_t = tj.get("tuning")
+        if prof_t is None or tuning is None or all(abs(float(prof_t[k]) - float(tuning[k])) < 1e-6 for k in prof_t):
+            traffic = tj["traffic_bytes_per_dispatch"] if (prof_t is None) == (tuning is None) else None
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "algorithmic_bytes_per_launch": round(bytes_per_launch),
             "kernel": "k_trace_queue", "avg_launch_ms": round(avg_ms, 4), "launches": launches,
-            "bytes_per_segment": round(bytes_per_seg, 1),
+            "bytes_per_segment": round(bytes_per_seg, 1), "node_bytes": node_bytes,
             "node_visits_per_segment": round(st_c.node_visits / cont, 2),
             "triangle_tests_per_segment": round(st_c.triangle_tests / cont, 2)}
 
@@ -268,7 +289,7 @@ def main():
                                    f"{scene.triangle_count} triangles (256 displaced tori + open room), "
                                    f"{len(scene.materials)} materials, 2048x1024 lat-long sky, BSDF-only pt",
                        "segments_per_step": segs // args.steps, "trace_ms_per_step": round(ms_trace / args.steps, 2),
-                       "tuning": tuning},
+                       "tuning": tuning, "tuning_source": tuning_source},
             "roofline": roof}
     ds.destroy()
     if not args.no_guided:

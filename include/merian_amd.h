@@ -316,7 +316,8 @@ typedef struct mrt_tuning {
     uint32_t grid_percent;    /* persistent traversal grid, percent of the resident blocks */
     uint32_t bvh_max_leaf;    /* triangles per BVH leaf, 1..8 */
     float bvh_traversal_cost; /* SAH cost of a node visit relative to a triangle test */
-    uint32_t bvh_width;       /* BVH node width (4, the layout this build traverses) */
+    uint32_t bvh_width;       /* BVH node width: 4 (full-precision boxes, 128-byte nodes) or 8
+                                 (8-bit quantised boxes, 80-byte nodes) */
     float ms_per_frame;       /* frame time measured for these settings (mrt_tune) */
 } mrt_tuning;
 
@@ -331,7 +332,7 @@ typedef struct mrt_tune_request {
 MRT_API mrt_status mrt_scene_get_tuning(mrt_scene* scene, mrt_tuning* out);
 /* Applies settings; the BVH is rebuilt when its leaf size or traversal cost changes. */
 MRT_API mrt_status mrt_scene_set_tuning(mrt_ctx* ctx, mrt_scene* scene, const mrt_tuning* tuning);
-/* Coordinate search over BVH (leaf size, traversal cost), paths in flight, refill
+/* Coordinate search over BVH (width, leaf size, traversal cost), paths in flight, refill
  * threshold and grid size; the fastest settings stay applied and are returned. */
 MRT_API mrt_status mrt_tune(mrt_ctx* ctx, mrt_scene* scene, const mrt_tune_request* request,
                             mrt_tuning* best);

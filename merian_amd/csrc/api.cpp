@@ -371,6 +371,8 @@ MRT_API mrt_status mrt_scene_create(mrt_ctx* c, const mrt_scene_desc* d, mrt_sce
             }
     }
     s->ds.stack_need = 3u * bvh.depth + 4u;
+    s->ds.width = 4;
+    s->ds.nodes8 = nullptr;
     s->n_tris = n_tris;
     s->n_nodes = bvh.nodes.size();
     s->depth = bvh.depth;

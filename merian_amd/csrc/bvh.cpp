@@ -251,15 +251,21 @@ float up(double v) {
 
 } // namespace
 
-BVH4 build_bvh4(const float* v0, const float* e1, const float* e2, const uint32_t* flags, uint32_t n,
-                int threads, int max_leaf, float traversal_cost) {
-    if (threads <= 0) threads = int(std::max(1u, std::thread::hardware_concurrency()));
-    std::vector<Prim> prims(n);
+namespace {
+
+// Primitive boxes (as the intersection test sees the vertices) and the binned-SAH BVH2.
+struct Prepared {
+    std::vector<Prim> prims;
     std::vector<uint32_t> order;
-    order.reserve(n);
     float mag = 0.f;
+};
+
+Prepared prepare(const float* v0, const float* e1, const float* e2, uint32_t n) {
+    Prepared pr;
+    pr.prims.resize(n);
+    pr.order.reserve(n);
     for (uint32_t i = 0; i < n; i++) {
-        Prim& p = prims[i];
+        Prim& p = pr.prims[i];
         bool ok = true;
         for (int k = 0; k < 3; k++) {
             // vertices as the intersection test sees them: v0, v0 + e1, v0 + e2 (exact in double)
@@ -270,9 +276,163 @@ BVH4 build_bvh4(const float* v0, const float* e1, const float* e2, const uint32_
             p.c[k] = float((std::min(a, std::min(b, c)) + std::max(a, std::max(b, c))) * 0.5);
         }
         if (!ok) continue; // a non-finite triangle can never be hit
-        for (int k = 0; k < 3; k++) mag = std::max(mag, std::max(std::fabs(p.box.lo[k]), std::fabs(p.box.hi[k])));
-        order.push_back(i);
+        for (int k = 0; k < 3; k++)
+            pr.mag = std::max(pr.mag, std::max(std::fabs(p.box.lo[k]), std::fabs(p.box.hi[k])));
+        pr.order.push_back(i);
     }
+    return pr;
+}
+
+void set_record(TriRecord& t, uint32_t i, const float* v0, const float* e1, const float* e2, const uint32_t* flags) {
+    for (int c = 0; c < 3; c++) {
+        t.a[c] = v0[3 * i + c];
+        t.b[c] = e1[3 * i + c];
+        t.c[c] = e2[3 * i + c];
+    }
+    std::memcpy(&t.a[3], &i, 4);
+    std::memcpy(&t.b[3], &flags[i], 4);
+    t.c[3] = 0.f;
+}
+
+// Children of a BVH2 node for a k-wide node: repeatedly open the largest inner child.
+int gather_children(const std::vector<Node2>& n2, int32_t ni, int width, int32_t* kids) {
+    int nk = 0;
+    kids[nk++] = n2[ni].left;
+    kids[nk++] = n2[ni].right;
+    while (nk < width) {
+        int best = -1;
+        float best_area = -1.f;
+        for (int i = 0; i < nk; i++) {
+            const Node2& c = n2[kids[i]];
+            if (c.left >= 0 && c.box.area() > best_area) {
+                best_area = c.box.area();
+                best = i;
+            }
+        }
+        if (best < 0) break;
+        const Node2 c = n2[kids[best]];
+        kids[best] = c.left;
+        kids[nk++] = c.right;
+    }
+    return nk;
+}
+
+// 8-bit grid of one node: origin, exponent per axis, quantised child planes containing `lo`/`hi`.
+struct Quant {
+    float p[3];
+    int e[3];
+};
+
+Quant node_grid(const float (*lo)[3], const float (*hi)[3], int nk) {
+    Quant q;
+    for (int a = 0; a < 3; a++) {
+        float mn = FLT_MAX, mx = -FLT_MAX;
+        for (int k = 0; k < nk; k++) {
+            mn = std::min(mn, lo[k][a]);
+            mx = std::max(mx, hi[k][a]);
+        }
+        q.p[a] = mn;
+        const double ext = double(mx) - double(mn);
+        int e = -126;
+        if (ext > 0) {
+            e = int(std::ceil(std::log2(ext / 250.0)));
+            while (std::ldexp(250.0, e) < ext) e++;
+        }
+        q.e[a] = std::max(-126, std::min(127, e));
+    }
+    return q;
+}
+
+uint8_t quant_lo(float v, float p, float s) {
+    double q = std::floor((double(v) - double(p)) / double(s));
+    int qi = int(std::max(0.0, std::min(255.0, q)));
+    while (qi > 0 && std::fmaf(float(qi), s, p) > v) qi--;
+    return uint8_t(qi);
+}
+
+uint8_t quant_hi(float v, float p, float s) {
+    double q = std::ceil((double(v) - double(p)) / double(s));
+    int qi = int(std::max(0.0, std::min(255.0, q)));
+    while (qi < 255 && std::fmaf(float(qi), s, p) < v) qi++;
+    return uint8_t(qi);
+}
+
+struct Collapser8 {
+    const std::vector<Node2>& n2;
+    const std::vector<uint32_t>& idx;
+    const float *v0, *e1, *e2;
+    const uint32_t* flags;
+    BVH8& out;
+    float mag;
+
+    // fills out.nodes[me] from BVH2 node ni (or a lone leaf when ni is a leaf)
+    void emit(int32_t ni, uint32_t me, uint32_t depth) {
+        out.depth = std::max(out.depth, depth);
+        int32_t kids[8];
+        int nk;
+        if (n2[ni].left < 0) {
+            kids[0] = ni;
+            nk = 1;
+        } else {
+            nk = gather_children(n2, ni, 8, kids);
+        }
+        float lo[8][3], hi[8][3];
+        for (int k = 0; k < nk; k++) padded(n2[kids[k]].box, mag, lo[k], hi[k]);
+        const Quant g = node_grid(lo, hi, nk);
+        BVH8Node node;
+        std::memset(&node, 0, sizeof(node));
+        node.px = g.p[0];
+        node.py = g.p[1];
+        node.pz = g.p[2];
+        node.ex = int8_t(g.e[0]);
+        node.ey = int8_t(g.e[1]);
+        node.ez = int8_t(g.e[2]);
+        uint8_t* qlo[3] = {node.qlox, node.qloy, node.qloz};
+        uint8_t* qhi[3] = {node.qhix, node.qhiy, node.qhiz};
+        for (int k = 0; k < 8; k++)
+            for (int a = 0; a < 3; a++) {
+                qlo[a][k] = 255;
+                qhi[a][k] = 0;
+            }
+        int n_inner = 0;
+        for (int k = 0; k < nk; k++)
+            if (n2[kids[k]].left >= 0) n_inner++;
+        node.child_base = uint32_t(out.nodes.size());
+        out.nodes.resize(out.nodes.size() + size_t(n_inner));
+        node.tri_base = uint32_t(out.tris.size());
+        for (int k = 0; k < nk; k++) {
+            for (int a = 0; a < 3; a++) {
+                const float sc = std::ldexp(1.0f, g.e[a]);
+                qlo[a][k] = quant_lo(lo[k][a], g.p[a], sc);
+                qhi[a][k] = quant_hi(hi[k][a], g.p[a], sc);
+            }
+            const Node2& c = n2[kids[k]];
+            if (c.left >= 0) {
+                node.inner_mask |= uint8_t(1u << k);
+            } else {
+                node.leaf_counts |= (c.count & 15u) << (4 * k);
+                for (uint32_t t = 0; t < c.count; t++) {
+                    out.tris.emplace_back();
+                    set_record(out.tris.back(), idx[c.first + t], v0, e1, e2, flags);
+                }
+            }
+        }
+        out.nodes[me] = node;
+        uint32_t slot = node.child_base;
+        for (int k = 0; k < nk; k++)
+            if (n2[kids[k]].left >= 0) emit(kids[k], slot++, depth + 1);
+    }
+};
+
+} // namespace
+
+BVH4 build_bvh4(const float* v0, const float* e1, const float* e2, const uint32_t* flags, uint32_t n,
+                int threads, int max_leaf, float traversal_cost) {
+    if (threads <= 0) threads = int(std::max(1u, std::thread::hardware_concurrency()));
+    Prepared pr = prepare(v0, e1, e2, n);
+    const std::vector<Prim>& prims = pr.prims;
+    std::vector<uint32_t>& order = pr.order;
+    const float mag = pr.mag;
     BVH4 out;
     const uint32_t m = uint32_t(order.size());
     Builder bld(prims, std::move(order), threads);
@@ -333,6 +493,36 @@ BVH4 build_bvh4(const float* v0, const float* e1, const float* e2, const uint32_
     out.nodes.reserve(bld.next.load() / 2 + 2);
     Collapser col{bld.nodes, out, mag};
     col.collapse(0, 1);
+    return out;
+}
+
+BVH8 build_bvh8(const float* v0, const float* e1, const float* e2, const uint32_t* flags, uint32_t n,
+                int threads, int max_leaf, float traversal_cost) {
+    if (threads <= 0) threads = int(std::max(1u, std::thread::hardware_concurrency()));
+    Prepared pr = prepare(v0, e1, e2, n);
+    const float mag = pr.mag;
+    const uint32_t m = uint32_t(pr.order.size());
+    Builder bld(pr.prims, std::move(pr.order), threads);
+    bld.max_leaf = uint32_t(std::min(std::max(max_leaf, 1), std::min(MAX_LEAF, 15)));
+    bld.traversal_cost = traversal_cost > 0.f ? traversal_cost : 1.0f;
+    BVH8 out;
+    if (m == 0) {
+        BVH8Node node;
+        std::memset(&node, 0, sizeof(node));
+        std::memset(node.qlox, 255, 8);
+        std::memset(node.qloy, 255, 8);
+        std::memset(node.qloz, 255, 8);
+        node.ex = node.ey = node.ez = 0;
+        out.nodes.push_back(node);
+        out.depth = 1;
+        return out;
+    }
+    bld.build(0, 0, m, 0);
+    out.nodes.reserve(bld.next.load() / 4 + 2);
+    out.tris.reserve(m);
+    out.nodes.emplace_back();
+    Collapser8 col{bld.nodes, bld.idx, v0, e1, e2, flags, out, mag};
+    col.emit(0, 0, 1);
     return out;
 }
 

@@ -42,8 +42,33 @@ struct BVH4 {
 
 // v0/e1/e2: per-triangle float[3] operands exactly as the intersection test uses them
 // (e1 = v1 - v0, e2 = v2 - v0 rounded in float). flags as in TriRecord.
+// 8-wide node with child boxes quantised to 8 bits on a per-node power-of-two grid (80 bytes,
+// five 16-byte loads): child box = fma(q, 2^e, origin) per plane, every decoded box containing
+// the padded child box exactly as the BVH4 stores it. Inner children are stored consecutively
+// from child_base in slot order; the triangles of the leaf children consecutively from tri_base.
+struct alignas(16) BVH8Node {
+    float px, py, pz;       // grid origin
+    int8_t ex, ey, ez;      // grid step 2^e per axis
+    uint8_t inner_mask;     // bit k: slot k is an inner child
+    uint32_t child_base;    // node index of the first inner child
+    uint32_t tri_base;      // first triangle record of the leaf children
+    uint32_t leaf_counts;   // 4 bits per slot: triangles of a leaf child (0: inner or empty)
+    uint32_t _pad;
+    uint8_t qlox[8], qhix[8], qloy[8], qhiy[8], qloz[8], qhiz[8]; // empty slot: lo 255, hi 0
+};
+static_assert(sizeof(BVH8Node) == 80, "BVH8Node must be 80 bytes");
+
+struct BVH8 {
+    std::vector<BVH8Node> nodes; // nodes[0] is the root
+    std::vector<TriRecord> tris; // per node, its leaf children's triangles in slot order
+    uint32_t depth = 0;
+};
+
 // max_leaf: most triangles per leaf (1..MAX_LEAF); traversal_cost: SAH cost of one node
 // visit relative to one triangle test.
+BVH8 build_bvh8(const float* v0, const float* e1, const float* e2, const uint32_t* flags, uint32_t n,
+                int threads, int max_leaf = 4, float traversal_cost = 1.0f);
+
 BVH4 build_bvh4(const float* v0, const float* e1, const float* e2, const uint32_t* flags, uint32_t n,
                 int threads, int max_leaf = 4, float traversal_cost = 1.0f);
 

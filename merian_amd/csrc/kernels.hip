@@ -175,7 +175,8 @@ __device__ __forceinline__ void flush_counts(const TraversalCount& c, unsigned l
 // queue): a persistent grid, one ray per lane with refill, LDS stack + global overflow.
 // Ray count: *count_ptr when given (device-side queue size), else n_fixed.
 // ANY: visibility queries (first hit ends the traversal).
-template <bool COUNT, class Src, class Sink, bool ANY = false>
+// W: BVH node width of the scene (4: BVH4Node, 8: quantised BVH8Node).
+template <bool COUNT, class Src, class Sink, bool ANY = false, int W = 4>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DeviceScene S, Src src, Sink sink,
                                                        const uint32_t* __restrict__ count_ptr, uint32_t n_fixed,
                                                        uint32_t* __restrict__ work, uint32_t mask, TraceScratch ts,
@@ -185,8 +186,13 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DeviceScene S, Src src, S
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const LaneStack stk{stack + threadIdx.x, blockDim.x, ts.ovf + gtid, gridDim.x * blockDim.x};
     TraversalCount cnt;
-    trace_persistent<TRACE_LDS_STACK, COUNT, Src, Sink, ANY>(DeviceBVH{S.nodes, S.tris}, src, sink, n, work, mask,
-                                                             stk, cnt, ts.refill);
+    if constexpr (W == 8) {
+        trace_persistent<TRACE_LDS_STACK, COUNT, Src, Sink, ANY>(DeviceBVH8{S.nodes8, S.tris}, src, sink, n, work,
+                                                                 mask, stk, cnt, ts.refill);
+    } else {
+        trace_persistent<TRACE_LDS_STACK, COUNT, Src, Sink, ANY>(DeviceBVH{S.nodes, S.tris}, src, sink, n, work, mask,
+                                                                 stk, cnt, ts.refill);
+    }
     if constexpr (COUNT) flush_counts(cnt, counters);
 }
 
@@ -749,11 +755,11 @@ int grid_for(uint64_t n, int block, int max_blocks) {
 constexpr int MAX_BLOCKS = 256 * 16; // 256 CUs
 
 // Blocks of k_trace<...> that fit on the device at once (the persistent grid).
-template <bool COUNT, class Src, class Sink, bool ANY> int trace_resident_blocks() {
+template <bool COUNT, class Src, class Sink, bool ANY, int W> int trace_resident_blocks() {
     // computed once per process (thread-safe static initialisation)
     static const int blocks = [] {
         int per_cu = 0, dev = 0, cus = 0;
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<COUNT, Src, Sink, ANY>, TRACE_BLOCK, 0);
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<COUNT, Src, Sink, ANY, W>, TRACE_BLOCK, 0);
         hipGetDevice(&dev);
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         return per_cu * cus > 0 ? per_cu * cus : 1;
@@ -769,13 +775,20 @@ hipError_t launch_trace(const DeviceScene& s, const TraceScratch& ts, const Src&
                         unsigned long long* counters, hipStream_t st) {
     if (n_max == 0) return hipSuccess;
     if (s.stack_need > TRACE_LDS_STACK + ts.entries) return hipErrorInvalidValue; // overflow area too small
-    const int resident = trace_resident_blocks<COUNT, Src, Sink, ANY>();
+    if (s.width == 8 ? !s.nodes8 : !s.nodes) return hipErrorInvalidValue;
+    const int resident = s.width == 8 ? trace_resident_blocks<COUNT, Src, Sink, ANY, 8>()
+                                      : trace_resident_blocks<COUNT, Src, Sink, ANY, 4>();
     const int cap = std::max(1, int(int64_t(resident) * int64_t(ts.grid_pct) / 100));
     int g = grid_for(n_max, TRACE_BLOCK, cap);
     if (uint64_t(g) * TRACE_BLOCK > ts.threads) g = int(ts.threads / TRACE_BLOCK);
     if (g <= 0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_trace<COUNT, Src, Sink, ANY>), dim3(g), dim3(TRACE_BLOCK), 0, st, s, src,
-                       sink, count_ptr, n_max, work, mask, ts, counters);
+    if (s.width == 8) {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_trace<COUNT, Src, Sink, ANY, 8>), dim3(g), dim3(TRACE_BLOCK), 0, st, s,
+                           src, sink, count_ptr, n_max, work, mask, ts, counters);
+    } else {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_trace<COUNT, Src, Sink, ANY, 4>), dim3(g), dim3(TRACE_BLOCK), 0, st, s,
+                           src, sink, count_ptr, n_max, work, mask, ts, counters);
+    }
     return hipGetLastError();
 }
 
@@ -786,9 +799,12 @@ hipError_t launch_trace(const DeviceScene& s, const TraceScratch& ts, const Src&
 void trace_prepare() {
     // evaluates the cached occupancy queries of every trace-kernel variant the frame launches,
     // so that no device query runs while a stream is being captured
-    (void)trace_resident_blocks<false, SrcCamera, SinkGBuffer, false>();
-    (void)trace_resident_blocks<false, SrcQueue, SinkQueue, false>();
-    (void)trace_resident_blocks<false, SrcShadow, SinkShadow, true>();
+    (void)trace_resident_blocks<false, SrcCamera, SinkGBuffer, false, 4>();
+    (void)trace_resident_blocks<false, SrcQueue, SinkQueue, false, 4>();
+    (void)trace_resident_blocks<false, SrcShadow, SinkShadow, true, 4>();
+    (void)trace_resident_blocks<false, SrcCamera, SinkGBuffer, false, 8>();
+    (void)trace_resident_blocks<false, SrcQueue, SinkQueue, false, 8>();
+    (void)trace_resident_blocks<false, SrcShadow, SinkShadow, true, 8>();
 }
 
 uint32_t trace_max_threads() {

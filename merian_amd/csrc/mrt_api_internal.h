@@ -46,8 +46,9 @@ mrt_status check_params(mrt_ctx* c, const mrt_pt_params* p);
 mrt_status ensure_trace_scratch(mrt_ctx* c, const mrt_scene* s);
 // The context's traversal scratch with the scene's launch settings.
 TraceScratch scene_scratch(const mrt_ctx* c, const mrt_scene* s);
-// Rebuilds and uploads the scene's BVH with the given leaf size and SAH traversal cost.
-mrt_status rebuild_bvh(mrt_ctx* c, mrt_scene* s, uint32_t max_leaf, float traversal_cost);
+// Rebuilds and uploads the scene's BVH with the given node width (4 or 8), leaf size and SAH
+// traversal cost.
+mrt_status rebuild_bvh(mrt_ctx* c, mrt_scene* s, uint32_t width, uint32_t max_leaf, float traversal_cost);
 // One pt frame on the device (mrt_pt_render_dev and, with a guide, mrt_guided_render_dev).
 mrt_status render_frame_dev(mrt_ctx* c, mrt_scene* s, const mrt_pt_params* prm, const mrt_camera* cam, uint32_t frame,
                             uint32_t w, uint32_t h, const mrt_packed_hit* gbuffer, float* irradiance,

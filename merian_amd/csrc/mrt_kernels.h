@@ -12,7 +12,9 @@ namespace mrt {
 
 // Scene as the kernels see it (the reference's Scene parameter block, scene.slang:30-54).
 struct DeviceScene {
-    const BVH4Node* nodes;
+    const BVH4Node* nodes;       // width 4
+    const BVH8Node* nodes8;      // width 8 (quantised)
+    uint32_t width;              // BVH node width in use: 4 or 8
     const float4* tris;          // TriRecord stream, leaf order
     const uint4* tri_verts;      // per global triangle id: 3 global vertex indices, geometry index
     const mrt_vertex* verts;     // PackedVertexData
@@ -27,7 +29,7 @@ struct DeviceScene {
     float env_to_local[9];
     uint32_t n_tris;
     uint32_t n_geoms;
-    uint32_t stack_need;         // traversal stack entries the tree can need (3 per level + slack)
+    uint32_t stack_need;         // traversal stack entries the tree can need (width - 1 per level + slack)
 };
 
 // Entries of the per-lane traversal stack kept in LDS; deeper entries go to TraceScratch.ovf.

@@ -123,6 +123,23 @@ __device__ __forceinline__ void ray_begin(RayState& r, f3 o, f3 d, float tmin, f
     r.cur = 0u; // root node
 }
 
+// A leaf entry: test its triangles, then pop (ANY: stop at the first hit).
+template <int SL, bool COUNT, bool ANY>
+__device__ __forceinline__ void leaf_step(const float4* __restrict__ tris, RayState& r, const LaneStack& stk,
+                                          uint32_t ray_mask, TraversalCount& count) {
+    const uint32_t first = (r.cur & 0x7FFFFFFFu) >> 3;
+    const uint32_t cnt = (r.cur & 7u) + 1u;
+    for (uint32_t k = 0; k < cnt; k++) test_triangle(tris, first + k, r.o, r.d, r.tmin, r.tmax, ray_mask, r.best);
+    if constexpr (COUNT) count.tris += cnt;
+    if constexpr (ANY) {
+        if (r.best.id != TRAVERSAL_DONE) {
+            r.cur = TRAVERSAL_DONE;
+            return;
+        }
+    }
+    r.cur = r.sp > 0 ? stack_pop<SL>(stk, r.sp) : TRAVERSAL_DONE;
+}
+
 // One traversal step: one inner node (test its four children, push the hit ones far to near,
 // continue with the nearest) or one leaf (test its triangles, pop). ANY: stop at the first hit
 // (visibility queries, Scene.trace_visibility_ray, scene.slang:82-90).
@@ -130,17 +147,7 @@ template <int SL, bool COUNT, bool ANY = false>
 __device__ __forceinline__ void ray_step(const DeviceBVH bvh, RayState& r, const LaneStack& stk, uint32_t ray_mask,
                                          TraversalCount& count) {
     if (r.cur & LEAF_FLAG) {
-        const uint32_t first = (r.cur & 0x7FFFFFFFu) >> 3;
-        const uint32_t cnt = (r.cur & 7u) + 1u;
-        for (uint32_t k = 0; k < cnt; k++) test_triangle(bvh.tris, first + k, r.o, r.d, r.tmin, r.tmax, ray_mask, r.best);
-        if constexpr (COUNT) count.tris += cnt;
-        if constexpr (ANY) {
-            if (r.best.id != TRAVERSAL_DONE) {
-                r.cur = TRAVERSAL_DONE;
-                return;
-            }
-        }
-        r.cur = r.sp > 0 ? stack_pop<SL>(stk, r.sp) : TRAVERSAL_DONE;
+        leaf_step<SL, COUNT, ANY>(bvh.tris, r, stk, ray_mask, count);
         return;
     }
     if constexpr (COUNT) count.nodes++;
@@ -184,6 +191,75 @@ __device__ __forceinline__ void ray_step(const DeviceBVH bvh, RayState& r, const
     }
 }
 
+struct DeviceBVH8 {
+    const BVH8Node* nodes;
+    const float4* tris; // TriRecord as 3 x float4
+};
+
+// One step over the 8-wide quantised BVH: decode the eight child boxes (fma(q, 2^e, origin),
+// conservative by construction), slab-test them as the BVH4 step does, sort the hit children
+// near to far with a 19-comparator network, push all but the nearest, continue with it.
+template <int SL, bool COUNT, bool ANY = false>
+__device__ __forceinline__ void ray_step(const DeviceBVH8 bvh, RayState& r, const LaneStack& stk, uint32_t ray_mask,
+                                         TraversalCount& count) {
+    if (r.cur & LEAF_FLAG) {
+        leaf_step<SL, COUNT, ANY>(bvh.tris, r, stk, ray_mask, count);
+        return;
+    }
+    if constexpr (COUNT) count.nodes++;
+    const uint4* np = reinterpret_cast<const uint4*>(bvh.nodes + r.cur);
+    const uint4 h0 = np[0], h1 = np[1], qx = np[2], qy = np[3], qz = np[4];
+    const float px = __uint_as_float(h0.x), py = __uint_as_float(h0.y), pz = __uint_as_float(h0.z);
+    const int ex = int(int8_t(h0.w & 0xFFu)), ey = int(int8_t((h0.w >> 8) & 0xFFu)), ez = int(int8_t((h0.w >> 16) & 0xFFu));
+    const uint32_t imask = h0.w >> 24;
+    const float sx = __uint_as_float(uint32_t(ex + 127) << 23);
+    const float sy = __uint_as_float(uint32_t(ey + 127) << 23);
+    const float sz = __uint_as_float(uint32_t(ez + 127) << 23);
+    const float tlim = fminf(r.tmax, r.best.t) * 1.00000024f;
+    const bool nx = r.neg & 1u, ny = r.neg & 2u, nz = r.neg & 4u;
+    const float inf = __builtin_inff();
+    float key[8];
+    uint32_t ent[8];
+    uint32_t inner_next = h1.x, tri_next = h1.y;
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        const uint32_t sh = 8u * uint32_t(c & 3);
+        const uint32_t lxq = ((c < 4 ? qx.x : qx.y) >> sh) & 0xFFu, hxq = ((c < 4 ? qx.z : qx.w) >> sh) & 0xFFu;
+        const uint32_t lyq = ((c < 4 ? qy.x : qy.y) >> sh) & 0xFFu, hyq = ((c < 4 ? qy.z : qy.w) >> sh) & 0xFFu;
+        const uint32_t lzq = ((c < 4 ? qz.x : qz.y) >> sh) & 0xFFu, hzq = ((c < 4 ? qz.z : qz.w) >> sh) & 0xFFu;
+        const float LX = __builtin_fmaf(float(lxq), sx, px), HX = __builtin_fmaf(float(hxq), sx, px);
+        const float LY = __builtin_fmaf(float(lyq), sy, py), HY = __builtin_fmaf(float(hyq), sy, py);
+        const float LZ = __builtin_fmaf(float(lzq), sz, pz), HZ = __builtin_fmaf(float(hzq), sz, pz);
+        const float tnx = __builtin_fmaf(nx ? HX : LX, r.ix, -r.ox), tfx = __builtin_fmaf(nx ? LX : HX, r.ix, -r.ox);
+        const float tny = __builtin_fmaf(ny ? HY : LY, r.iy, -r.oy), tfy = __builtin_fmaf(ny ? LY : HY, r.iy, -r.oy);
+        const float tnz = __builtin_fmaf(nz ? HZ : LZ, r.iz, -r.oz), tfz = __builtin_fmaf(nz ? LZ : HZ, r.iz, -r.oz);
+        const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, r.t0));
+        const float tf = fminf(fminf(tfx, tfy), fminf(tfz, tlim));
+        const uint32_t cnt = (h1.z >> (4 * c)) & 15u;
+        const bool inner = (imask >> c) & 1u;
+        key[c] = (tn <= tf && (inner || cnt)) ? tn : inf;
+        ent[c] = inner ? inner_next : (LEAF_FLAG | (tri_next << 3) | ((cnt - 1u) & 7u));
+        inner_next += inner ? 1u : 0u;
+        tri_next += cnt;
+    }
+#define MRT_CS(a, b) cswap(key[a], ent[a], key[b], ent[b]);
+    MRT_CS(0, 1) MRT_CS(2, 3) MRT_CS(4, 5) MRT_CS(6, 7)
+    MRT_CS(0, 2) MRT_CS(1, 3) MRT_CS(4, 6) MRT_CS(5, 7)
+    MRT_CS(1, 2) MRT_CS(5, 6)
+    MRT_CS(0, 4) MRT_CS(1, 5) MRT_CS(2, 6) MRT_CS(3, 7)
+    MRT_CS(2, 4) MRT_CS(3, 5)
+    MRT_CS(1, 2) MRT_CS(3, 4) MRT_CS(5, 6)
+#undef MRT_CS
+#pragma unroll
+    for (int c = 7; c >= 1; c--)
+        if (key[c] != inf) stack_push<SL>(stk, r.sp, ent[c]);
+    if (key[0] != inf) {
+        r.cur = ent[0];
+    } else {
+        r.cur = r.sp > 0 ? stack_pop<SL>(stk, r.sp) : TRAVERSAL_DONE;
+    }
+}
+
 // Default refill threshold: lanes fetch new rays once at least this many of a wave's 64 lanes
 // are idle (one atomic per refill). The autotuner may choose another.
 constexpr int REFILL_LANES = 16;
@@ -191,8 +267,8 @@ constexpr int REFILL_LANES = 16;
 // Persistent closest-hit loop of one wave: every lane runs one traversal step per iteration and,
 // when its ray is done, hands the hit to `sink` and later takes a fresh ray from `src`, so lanes
 // with short rays do not idle while others finish long ones. `work` is the launch's ray cursor.
-template <int SL, bool COUNT, class Src, class Sink, bool ANY = false>
-__device__ __forceinline__ void trace_persistent(const DeviceBVH bvh, const Src& src, const Sink& sink, uint32_t n,
+template <int SL, bool COUNT, class Src, class Sink, bool ANY = false, class BVH = DeviceBVH>
+__device__ __forceinline__ void trace_persistent(const BVH bvh, const Src& src, const Sink& sink, uint32_t n,
                                                  uint32_t* work, uint32_t ray_mask, const LaneStack& stk,
                                                  TraversalCount& count, uint32_t refill) {
     const uint32_t lane = __lane_id();

@@ -5,6 +5,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -16,44 +18,72 @@ using namespace mrt;
 
 namespace mrt {
 
-mrt_status rebuild_bvh(mrt_ctx* c, mrt_scene* s, uint32_t max_leaf, float traversal_cost) {
+mrt_status rebuild_bvh(mrt_ctx* c, mrt_scene* s, uint32_t width, uint32_t max_leaf, float traversal_cost) {
     if (s->h_flags.size() != s->n_tris) return fail(c, MRT_ERR_UNSUPPORTED, "scene keeps no triangle copy");
-    const BVH4 bvh = build_bvh4(s->h_v0.data(), s->h_e1.data(), s->h_e2.data(), s->h_flags.data(),
-                                uint32_t(s->n_tris), 0, int(max_leaf), traversal_cost);
+    if (width != 4 && width != 8) return fail(c, MRT_ERR_INVALID, "BVH width must be 4 or 8");
+    const float *v0 = s->h_v0.data(), *e1 = s->h_e1.data(), *e2 = s->h_e2.data();
+    const uint32_t* fl = s->h_flags.data();
+    const uint32_t n = uint32_t(s->n_tris);
+    // build on the host, then swap the device tree
+    const void* node_src;
+    const void* tri_src;
+    size_t node_bytes, tri_bytes, n_nodes;
+    uint32_t depth;
+    BVH4 b4;
+    BVH8 b8;
+    if (width == 8) {
+        b8 = build_bvh8(v0, e1, e2, fl, n, 0, int(max_leaf), traversal_cost);
+        node_src = b8.nodes.data();
+        node_bytes = b8.nodes.size() * sizeof(BVH8Node);
+        n_nodes = b8.nodes.size();
+        tri_src = b8.tris.data();
+        tri_bytes = b8.tris.size() * sizeof(TriRecord);
+        depth = b8.depth;
+    } else {
+        b4 = build_bvh4(v0, e1, e2, fl, n, 0, int(max_leaf), traversal_cost);
+        node_src = b4.nodes.data();
+        node_bytes = b4.nodes.size() * sizeof(BVH4Node);
+        n_nodes = b4.nodes.size();
+        tri_src = b4.tris.data();
+        tri_bytes = b4.tris.size() * sizeof(TriRecord);
+        depth = b4.depth;
+    }
     hipStreamSynchronize(c->stream);
     mrt_status st = MRT_OK;
-    void* nodes = dev_alloc(c, bvh.nodes.size() * sizeof(BVH4Node), &st);
+    void* nodes = dev_alloc(c, node_bytes, &st);
     if (st != MRT_OK) return st;
-    void* tris = dev_alloc(c, bvh.tris.size() * sizeof(TriRecord), &st);
+    void* tris = dev_alloc(c, tri_bytes, &st);
     if (st != MRT_OK) {
         dev_free(c, nodes);
         return st;
     }
-    hipError_t e = hipMemcpyAsync(nodes, bvh.nodes.data(), bvh.nodes.size() * sizeof(BVH4Node),
-                                  hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(tris, bvh.tris.data(), bvh.tris.size() * sizeof(TriRecord), hipMemcpyHostToDevice, c->stream);
+    hipError_t e = hipMemcpyAsync(nodes, node_src, node_bytes, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(tris, tri_src, tri_bytes, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
         dev_free(c, nodes);
         dev_free(c, tris);
         return hip_fail(c, e, "BVH upload");
     }
-    // swap the device trees
-    for (void** old : {(void**)&s->ds.nodes, (void**)&s->ds.tris}) {
-        auto it = std::find(s->allocs.begin(), s->allocs.end(), *old);
+    void* olds[3] = {(void*)s->ds.nodes, (void*)s->ds.nodes8, (void*)s->ds.tris};
+    for (void* old : olds) {
+        if (!old) continue;
+        auto it = std::find(s->allocs.begin(), s->allocs.end(), old);
         if (it != s->allocs.end()) s->allocs.erase(it);
-        s->bytes -= c->sizes.count(*old) ? c->sizes[*old] : 0;
-        dev_free(c, *old);
+        s->bytes -= c->sizes.count(old) ? c->sizes[old] : 0;
+        dev_free(c, old);
     }
-    s->ds.nodes = static_cast<const BVH4Node*>(nodes);
+    s->ds.nodes = width == 4 ? static_cast<const BVH4Node*>(nodes) : nullptr;
+    s->ds.nodes8 = width == 8 ? static_cast<const BVH8Node*>(nodes) : nullptr;
+    s->ds.width = width;
     s->ds.tris = static_cast<const float4*>(tris);
     s->allocs.push_back(nodes);
     s->allocs.push_back(tris);
-    s->bytes += bvh.nodes.size() * sizeof(BVH4Node) + bvh.tris.size() * sizeof(TriRecord);
-    s->ds.stack_need = 3u * bvh.depth + 4u;
-    s->n_nodes = bvh.nodes.size();
-    s->depth = bvh.depth;
+    s->bytes += node_bytes + tri_bytes;
+    s->ds.stack_need = (width - 1u) * depth + 4u;
+    s->n_nodes = n_nodes;
+    s->depth = depth;
+    s->tuning.bvh_width = width;
     s->tuning.bvh_max_leaf = max_leaf;
     s->tuning.bvh_traversal_cost = traversal_cost;
     return MRT_OK;
@@ -66,7 +96,7 @@ namespace {
 bool valid(const mrt_tuning& t) {
     return t.refill_lanes >= 1 && t.refill_lanes <= 64 && t.grid_percent >= 1 && t.grid_percent <= 100 &&
            t.bvh_max_leaf >= 1 && t.bvh_max_leaf <= uint32_t(MAX_LEAF) && t.bvh_traversal_cost > 0.f &&
-           (t.bvh_width == 0 || t.bvh_width == 4);
+           (t.bvh_width == 0 || t.bvh_width == 4 || t.bvh_width == 8);
 }
 
 // Device time of one trial frame (gbuffer + pt) in ms, after one untimed frame.
@@ -124,12 +154,14 @@ MRT_API mrt_status mrt_scene_set_tuning(mrt_ctx* c, mrt_scene* s, const mrt_tuni
     if (!c || !s || !t) return MRT_ERR_INVALID;
     if (!valid(*t)) return fail(c, MRT_ERR_INVALID, "tuning values out of range");
     hipSetDevice(c->device);
-    if (t->bvh_max_leaf != s->tuning.bvh_max_leaf || t->bvh_traversal_cost != s->tuning.bvh_traversal_cost) {
-        const mrt_status st = rebuild_bvh(c, s, t->bvh_max_leaf, t->bvh_traversal_cost);
+    const uint32_t width = t->bvh_width ? t->bvh_width : 4u;
+    if (t->bvh_max_leaf != s->tuning.bvh_max_leaf || t->bvh_traversal_cost != s->tuning.bvh_traversal_cost ||
+        width != s->tuning.bvh_width) {
+        const mrt_status st = rebuild_bvh(c, s, width, t->bvh_max_leaf, t->bvh_traversal_cost);
         if (st != MRT_OK) return st;
     }
     s->tuning = *t;
-    s->tuning.bvh_width = 4;
+    s->tuning.bvh_width = width;
     return MRT_OK;
 }
 
@@ -143,6 +175,9 @@ MRT_API mrt_status mrt_tune(mrt_ctx* c, mrt_scene* s, const mrt_tune_request* r,
     const double budget = r->time_budget_ms > 0.f ? double(r->time_budget_ms) : 20000.0;
     auto elapsed = [&] { return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); };
 
+    // MRT_TUNE_VERBOSE=1 prints every trial to stderr
+    const char* ev = std::getenv("MRT_TUNE_VERBOSE");
+    const bool verbose = ev && ev[0] == '1';
     mrt_tuning best = s->tuning;
     if (best.max_paths == 0) best.max_paths = c->max_paths;
     mrt_status st = mrt_scene_set_tuning(c, s, &best);
@@ -162,6 +197,12 @@ MRT_API mrt_status mrt_tune(mrt_ctx* c, mrt_scene* s, const mrt_tune_request* r,
         }
         if (e != MRT_OK) return e;
         cand.ms_per_frame = ms;
+        if (verbose)
+            std::fprintf(stderr,
+                         "mrt_tune: width %u leaf %u cost %.2f paths %llu refill %u grid %u%% -> %.3f ms (best %.3f)\n",
+                         cand.bvh_width, cand.bvh_max_leaf, double(cand.bvh_traversal_cost),
+                         (unsigned long long)cand.max_paths, cand.refill_lanes, cand.grid_percent, double(ms),
+                         double(best.ms_per_frame));
         if (ms < best.ms_per_frame) best = cand;
         return mrt_scene_set_tuning(c, s, &best);
     };
@@ -169,12 +210,18 @@ MRT_API mrt_status mrt_tune(mrt_ctx* c, mrt_scene* s, const mrt_tune_request* r,
     for (int pass = 0; pass < 2 && elapsed() < budget; pass++) {
         const mrt_tuning start = best;
         // BVH build settings
-        const std::pair<uint32_t, float> bvh[] = {{4, 1.0f}, {2, 1.0f}, {8, 1.0f}, {4, 0.5f}, {4, 2.0f}, {8, 2.0f}};
-        for (auto [leaf, cost] : bvh) {
-            if (leaf == best.bvh_max_leaf && cost == best.bvh_traversal_cost) continue;
+        struct B {
+            uint32_t width, leaf;
+            float cost;
+        };
+        const B bvh[] = {{4, 4, 1.0f}, {4, 2, 1.0f}, {4, 8, 1.0f}, {8, 2, 1.0f}, {8, 4, 1.0f},
+                         {8, 1, 1.0f}, {4, 4, 0.5f}, {4, 4, 2.0f}};
+        for (const B& b : bvh) {
+            if (b.width == best.bvh_width && b.leaf == best.bvh_max_leaf && b.cost == best.bvh_traversal_cost) continue;
             mrt_tuning cand = best;
-            cand.bvh_max_leaf = leaf;
-            cand.bvh_traversal_cost = cost;
+            cand.bvh_width = b.width;
+            cand.bvh_max_leaf = b.leaf;
+            cand.bvh_traversal_cost = b.cost;
             if ((st = try_one(cand)) != MRT_OK) return st;
         }
         // paths in flight (whole samples of the image per pass)

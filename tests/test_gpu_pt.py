@@ -31,6 +31,13 @@ def _compare(img, sig, ref, ref_sig, min_match=0.999):
     return frac, worst
 
 
+def _with_tree(ds, width, leaf):
+    t = ds.tuning()
+    t.bvh_width, t.bvh_max_leaf = width, leaf
+    ds.set_tuning(t)
+    return ds
+
+
 @pytest.fixture(scope="module")
 def small(ctx):
     sc = scenes.plain_small(seed=0)
@@ -63,10 +70,29 @@ def test_pt_vs_oracle(ctx, small, spp, L, frame):
     assert abs(int(st.continuation_segments) - segs) <= max(10, segs // 1000)
 
 
-def test_pt_full_scene_crop(ctx):
-    # 256 displaced tori (reduced instance count), dielectrics, lat-long sky; 1080p camera
-    sc = scenes.full_scene(seed=2, instances=32, env_size=(256, 128))
+def test_pt_bvh8_matches_bvh4_bitwise(ctx):
+    # the same frame through the 4-wide and the 8-wide tree: every pixel bit-identical
+    sc = scenes.plain_small(seed=0)
     ds = ctx.scene(sc)
+    try:
+        W = H = 128
+        cam = ctx.camera(sc.camera, 5)
+        p = pt_params(spp=2, max_path_length=6)
+        a, sa, _, _ = ctx.render(ds, p, cam, 5, W, H)
+        _with_tree(ds, 8, 3)
+        b, sb, _, _ = ctx.render(ds, p, cam, 5, W, H)
+        assert np.array_equal(sa, sb)
+        assert a.tobytes() == b.tobytes()
+    finally:
+        ds.destroy()
+
+
+@pytest.mark.parametrize("tree", [(4, 4), (4, 2), (8, 2), (8, 4)])
+def test_pt_full_scene_crop(ctx, tree):
+    # 256 displaced tori (reduced instance count), dielectrics, lat-long sky; 1080p camera.
+    # Every BVH shape the autotuner may pick must give the same image.
+    sc = scenes.full_scene(seed=2, instances=32, env_size=(256, 128))
+    ds = _with_tree(ctx.scene(sc), *tree)
     osc = O.OracleScene(sc)
     W, H = 192, 108
     frame = 1

@@ -12,6 +12,23 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 
+# (BVH width, max leaf size): the default 4-wide tree and the 8-wide tree with quantized
+# child boxes, at leaf sizes the autotuner may choose.
+TREES = [(4, 4), (8, 2), (8, 8)]
+
+
+def _tris(ctx, pos, idx, tree, two_sided=True):
+    ds = ctx.triangles(pos, idx, two_sided=two_sided)
+    width, leaf = tree
+    if (width, leaf) != (4, 4):
+        t = ds.tuning()
+        t.bvh_width, t.bvh_max_leaf = width, leaf
+        ds.set_tuning(t)
+        got = ds.tuning()
+        assert (got.bvh_width, got.bvh_max_leaf) == (width, leaf)
+    return ds
+
+
 def _assert_hits_equal(a, b):
     assert np.array_equal(a["triangle"], b["triangle"]), \
         f"{int((a['triangle'] != b['triangle']).sum())} id mismatches of {len(a)}"
@@ -20,18 +37,20 @@ def _assert_hits_equal(a, b):
         assert np.array_equal(a[k][hit].view(np.uint32), b[k][hit].view(np.uint32)), f"{k} differs"
 
 
-def test_incoherent_vs_bruteforce(ctx):
+@pytest.mark.parametrize("tree", TREES)
+def test_incoherent_vs_bruteforce(ctx, tree):
     pos, idx = scenes.ray_query(seed=7, n_tri=4000)
     rays = scenes.incoherent_rays(60000, seed=8)
     ref = O.OracleScene(triangles=(pos, idx), use_bvh=False).trace(rays)
-    ds = ctx.triangles(pos, idx, two_sided=True)
+    ds = _tris(ctx, pos, idx, tree)
     got = ctx.trace(ds, rays)
     ds.destroy()
     assert (ref["triangle"] != 0xFFFFFFFF).mean() > 0.2
     _assert_hits_equal(got, ref)
 
 
-def test_backface_culling_and_ties(ctx):
+@pytest.mark.parametrize("tree", TREES)
+def test_backface_culling_and_ties(ctx, tree):
     # two coincident quads with opposite winding + duplicated triangles (ties at equal t)
     P = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]], np.float32)
     tris = np.array([[0, 1, 2], [0, 2, 3], [0, 2, 1], [0, 3, 2], [0, 1, 2], [0, 2, 3]], np.uint32)
@@ -48,7 +67,7 @@ def test_backface_culling_and_ties(ctx):
     pos = P
     for two_sided in (True, False):
         ref = O.OracleScene(triangles=(pos, tris.reshape(-1)), two_sided=two_sided, use_bvh=False).trace(r)
-        ds = ctx.triangles(pos, tris.reshape(-1), two_sided=two_sided)
+        ds = _tris(ctx, pos, tris.reshape(-1), tree, two_sided=two_sided)
         got = ctx.trace(ds, r)
         ds.destroy()
         _assert_hits_equal(got, ref)
@@ -60,40 +79,43 @@ def test_backface_culling_and_ties(ctx):
             assert set(np.unique(got["triangle"][h & ~side])) <= {2, 3}
 
 
-def test_tmin_tmax_interval(ctx):
+@pytest.mark.parametrize("tree", TREES)
+def test_tmin_tmax_interval(ctx, tree):
     pos, idx = scenes.ray_query(seed=9, n_tri=2000)
     rays = scenes.incoherent_rays(20000, seed=10)
     rng = np.random.default_rng(0)
     rays["tmin"] = rng.uniform(0, 0.2, len(rays)).astype(np.float32)
     rays["tmax"] = rng.uniform(0.2, 1.0, len(rays)).astype(np.float32)
     ref = O.OracleScene(triangles=(pos, idx), use_bvh=False).trace(rays)
-    ds = ctx.triangles(pos, idx)
+    ds = _tris(ctx, pos, idx, tree)
     got = ctx.trace(ds, rays)
     ds.destroy()
     _assert_hits_equal(got, ref)
 
 
-def test_empty_and_single_triangle(ctx):
+@pytest.mark.parametrize("tree", TREES)
+def test_empty_and_single_triangle(ctx, tree):
     rays = scenes.incoherent_rays(1000, seed=1)
     pos = np.array([[0.2, 0.2, 0.5], [0.8, 0.2, 0.5], [0.5, 0.8, 0.5]], np.float32)
     idx = np.array([0, 1, 2], np.uint32)
     ref = O.OracleScene(triangles=(pos, idx), use_bvh=False).trace(rays)
-    ds = ctx.triangles(pos, idx)
+    ds = _tris(ctx, pos, idx, tree)
     _assert_hits_equal(ctx.trace(ds, rays), ref)
     ds.destroy()
-    ds = ctx.triangles(np.zeros((0, 3), np.float32), np.zeros(0, np.uint32))
+    ds = _tris(ctx, np.zeros((0, 3), np.float32), np.zeros(0, np.uint32), tree)
     got = ctx.trace(ds, rays)
     ds.destroy()
     assert (got["triangle"] == 0xFFFFFFFF).all()
 
 
-def test_coherent_camera_rays_vs_oracle(ctx):
+@pytest.mark.parametrize("tree", TREES)
+def test_coherent_camera_rays_vs_oracle(ctx, tree):
     pos, idx = scenes.ray_query(seed=1, n_tri=50000)
     spec = scenes.ray_query_camera(256)
     cam = O.camera_look_at(spec, 0)
     rays = O.camera_rays(cam, 256, 256)
     ref = O.OracleScene(triangles=(pos, idx), use_bvh=True).trace(rays)
-    ds = ctx.triangles(pos, idx)
+    ds = _tris(ctx, pos, idx, tree)
     got = ctx.trace(ds, rays)
     ds.destroy()
     _assert_hits_equal(got, ref)
