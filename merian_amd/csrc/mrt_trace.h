@@ -266,44 +266,83 @@ constexpr int REFILL_LANES = 16;
 
 // Persistent closest-hit loop of one wave: every lane runs one traversal step per iteration and,
 // when its ray is done, hands the hit to `sink` and later takes a fresh ray from `src`, so lanes
-// with short rays do not idle while others finish long ones. `work` is the launch's ray cursor.
+// with short rays do not idle while others finish long ones.
+//
+// Ray indices are handed out in wave-private chunks, so the single device-wide cursor `work`
+// (zero at launch) sees a few atomics per wave instead of one per refill: every wave first owns
+// a static chunk of n / (2 * waves) rays (rounded down to whole waves of 64), then claims
+// guided chunks of (unclaimed rays) / (2 * waves), 64..1024 rays, from `work`.
+// Refilled lanes issue their ray loads before the still-active lanes' traversal step, so both
+// wait on one memory round trip.
 template <int SL, bool COUNT, class Src, class Sink, bool ANY = false, class BVH = DeviceBVH>
 __device__ __forceinline__ void trace_persistent(const BVH bvh, const Src& src, const Sink& sink, uint32_t n,
                                                  uint32_t* work, uint32_t ray_mask, const LaneStack& stk,
                                                  TraversalCount& count, uint32_t refill) {
     const uint32_t lane = __lane_id();
+    const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t c0 = (n / (2u * waves)) & ~63u;
+    const uint32_t dyn0 = c0 * waves; // <= n / 2: dynamic claims start here
+    uint32_t pool = wave * c0, pool_end = pool + c0;
+    uint32_t seen = dyn0;
+    bool claims_done = dyn0 >= n;
     RayState r;
     r.cur = TRAVERSAL_DONE;
-    bool active = false, exhausted = false;
+    bool active = false;
     uint32_t idx = 0;
     while (true) {
         const uint64_t idle = __ballot(!active);
-        if (!exhausted && uint32_t(__popcll(idle)) >= refill) {
-            const uint32_t take = uint32_t(__popcll(idle));
-            const uint32_t leader = uint32_t(__ffsll((unsigned long long)idle) - 1);
-            uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(work, take);
-            base = __shfl(base, int(leader));
-            if (base + take >= n) exhausted = true;
+        const uint32_t nidle = uint32_t(__popcll(idle));
+        bool fresh = false;
+        if (nidle >= refill && !(claims_done && pool == pool_end)) {
+            const uint32_t avail = pool_end - pool;
+            uint32_t lo = 0, hi = 0;
+            if (avail < nidle && !claims_done) {
+                const uint32_t left = n - (seen < n ? seen : n);
+                const uint32_t chunk = min(max((left / (2u * waves)) & ~63u, 64u), 1024u);
+                const uint32_t leader = uint32_t(__ffsll((unsigned long long)idle) - 1);
+                uint32_t got = 0;
+                if (lane == leader) got = atomicAdd(work, chunk);
+                const uint64_t base = uint64_t(__shfl(got, int(leader))) + dyn0;
+                seen = base < n ? uint32_t(base) : n;
+                if (base < n) {
+                    lo = uint32_t(base);
+                    hi = uint32_t(base + chunk < n ? base + chunk : n);
+                }
+                if (base + chunk >= n) claims_done = true;
+            }
             if (!active) {
-                const uint32_t my = base + uint32_t(__popcll(idle & ((1ull << lane) - 1ull)));
-                if (my < n) {
+                const uint32_t k = uint32_t(__popcll(idle & ((1ull << lane) - 1ull)));
+                uint32_t my = TRAVERSAL_DONE;
+                if (k < avail) {
+                    my = pool + k;
+                } else if (k - avail < hi - lo) {
+                    my = lo + (k - avail);
+                }
+                if (my != TRAVERSAL_DONE) {
                     idx = my;
-                    f3 o, d;
-                    float tmin, tmax;
-                    src.load(idx, o, d, tmin, tmax);
-                    ray_begin(r, o, d, tmin, tmax);
-                    active = true;
+                    src.load(idx, r.o, r.d, r.tmin, r.tmax); // this lane's state is dead until ray_begin
+                    fresh = true;
                 }
             }
+            if (avail >= nidle) {
+                pool += nidle;
+            } else {
+                pool = lo + min(nidle - avail, hi - lo);
+                pool_end = hi;
+            }
         }
-        if (__ballot(active) == 0) break;
+        if (__ballot(active || fresh) == 0) break;
         if (active) {
             ray_step<SL, COUNT, ANY>(bvh, r, stk, ray_mask, count);
             if (r.cur == TRAVERSAL_DONE) {
                 sink.store(idx, r.best);
                 active = false;
             }
+        }
+        if (fresh) {
+            ray_begin(r, r.o, r.d, r.tmin, r.tmax);
+            active = true;
         }
     }
 }

@@ -232,7 +232,7 @@ MRT_API mrt_status mrt_tune(mrt_ctx* c, mrt_scene* s, const mrt_tune_request* r,
             cand.max_paths = mp;
             if ((st = try_one(cand)) != MRT_OK) return st;
         }
-        for (uint32_t rf : {8u, 16u, 24u, 32u, 48u}) {
+        for (uint32_t rf : {4u, 8u, 16u, 24u, 32u, 48u, 64u}) {
             if (rf == best.refill_lanes) continue;
             mrt_tuning cand = best;
             cand.refill_lanes = rf;
