@@ -319,6 +319,9 @@ typedef struct mrt_tuning {
     uint32_t bvh_width;       /* BVH node width: 4 (full-precision boxes, 128-byte nodes) or 8
                                  (8-bit quantised boxes, 80-byte nodes) */
     float ms_per_frame;       /* frame time measured for these settings (mrt_tune) */
+    uint32_t leaf_lanes;      /* lanes of 64 left with nothing but a postponed leaf before a
+                                 traversal wave runs a leaf-test phase, 1..64 (0 = 16) */
+    uint32_t _reserved;       /* zero */
 } mrt_tuning;
 
 typedef struct mrt_tune_request {

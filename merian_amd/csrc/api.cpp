@@ -141,6 +141,7 @@ TraceScratch scene_scratch(const mrt_ctx* c, const mrt_scene* s) {
     TraceScratch t = c ? c->trace : TraceScratch{};
     t.refill = s->tuning.refill_lanes >= 1 && s->tuning.refill_lanes <= 64 ? s->tuning.refill_lanes : 16u;
     t.grid_pct = s->tuning.grid_percent >= 1 && s->tuning.grid_percent <= 100 ? s->tuning.grid_percent : 100u;
+    t.leaf_lanes = s->tuning.leaf_lanes >= 1 && s->tuning.leaf_lanes <= 64 ? s->tuning.leaf_lanes : 16u;
     return t;
 }
 

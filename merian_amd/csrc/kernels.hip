@@ -188,10 +188,10 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DeviceScene S, Src src, S
     TraversalCount cnt;
     if constexpr (W == 8) {
         trace_persistent<TRACE_LDS_STACK, COUNT, Src, Sink, ANY>(DeviceBVH8{S.nodes8, S.tris}, src, sink, n, work,
-                                                                 mask, stk, cnt, ts.refill);
+                                                                 mask, stk, cnt, ts.refill, ts.leaf_lanes);
     } else {
         trace_persistent<TRACE_LDS_STACK, COUNT, Src, Sink, ANY>(DeviceBVH{S.nodes, S.tris}, src, sink, n, work, mask,
-                                                                 stk, cnt, ts.refill);
+                                                                 stk, cnt, ts.refill, ts.leaf_lanes);
     }
     if constexpr (COUNT) flush_counts(cnt, counters);
 }

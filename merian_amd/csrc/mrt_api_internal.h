@@ -32,7 +32,7 @@ struct mrt_scene {
     // triangle operands kept on the host for BVH rebuilds (tuning)
     std::vector<float> h_v0, h_e1, h_e2;
     std::vector<uint32_t> h_flags;
-    mrt_tuning tuning{0, 16, 100, 4, 1.0f, 4, 0.f};
+    mrt_tuning tuning{0, 16, 100, 4, 1.0f, 4, 0.f, 16, 0};
 };
 
 namespace mrt {

@@ -99,8 +99,12 @@ struct RayState {
     uint32_t neg; // bit k: direction component k is negative
     HitRec best;
     int sp;
-    uint32_t cur; // next stack entry to process; TRAVERSAL_DONE when finished
+    uint32_t cur;  // next entry to process; TRAVERSAL_DONE when the stack is exhausted
+    uint32_t leaf; // postponed leaf entry awaiting its triangle tests; TRAVERSAL_DONE when none
 };
+
+__device__ __forceinline__ bool is_inner(uint32_t e) { return !(e & LEAF_FLAG); }
+__device__ __forceinline__ bool is_leaf(uint32_t e) { return e != TRAVERSAL_DONE && (e & LEAF_FLAG); }
 
 __device__ __forceinline__ void ray_begin(RayState& r, f3 o, f3 d, float tmin, float tmax) {
     r.o = o;
@@ -120,36 +124,43 @@ __device__ __forceinline__ void ray_begin(RayState& r, f3 o, f3 d, float tmin, f
     r.t0 = tmin - fabsf(tmin) * 2.4e-7f;
     r.best = HitRec{TRAVERSAL_DONE, __builtin_inff(), 0.f, 0.f};
     r.sp = 0;
-    r.cur = 0u; // root node
+    r.cur = 0u; // root node (always an inner node)
+    r.leaf = TRAVERSAL_DONE;
 }
 
-// A leaf entry: test its triangles, then pop (ANY: stop at the first hit).
+// Tests the triangles of the postponed leaf (ANY: the first hit ends the traversal).
 template <int SL, bool COUNT, bool ANY>
-__device__ __forceinline__ void leaf_step(const float4* __restrict__ tris, RayState& r, const LaneStack& stk,
-                                          uint32_t ray_mask, TraversalCount& count) {
-    const uint32_t first = (r.cur & 0x7FFFFFFFu) >> 3;
-    const uint32_t cnt = (r.cur & 7u) + 1u;
+__device__ __forceinline__ void leaf_test(const float4* __restrict__ tris, RayState& r, uint32_t ray_mask,
+                                          TraversalCount& count) {
+    const uint32_t first = (r.leaf & 0x7FFFFFFFu) >> 3;
+    const uint32_t cnt = (r.leaf & 7u) + 1u;
     for (uint32_t k = 0; k < cnt; k++) test_triangle(tris, first + k, r.o, r.d, r.tmin, r.tmax, ray_mask, r.best);
     if constexpr (COUNT) count.tris += cnt;
+    r.leaf = TRAVERSAL_DONE;
     if constexpr (ANY) {
         if (r.best.id != TRAVERSAL_DONE) {
             r.cur = TRAVERSAL_DONE;
-            return;
+            r.sp = 0;
         }
     }
-    r.cur = r.sp > 0 ? stack_pop<SL>(stk, r.sp) : TRAVERSAL_DONE;
 }
 
-// One traversal step: one inner node (test its four children, push the hit ones far to near,
-// continue with the nearest) or one leaf (test its triangles, pop). ANY: stop at the first hit
-// (visibility queries, Scene.trace_visibility_ray, scene.slang:82-90).
-template <int SL, bool COUNT, bool ANY = false>
-__device__ __forceinline__ void ray_step(const DeviceBVH bvh, RayState& r, const LaneStack& stk, uint32_t ray_mask,
-                                         TraversalCount& count) {
-    if (r.cur & LEAF_FLAG) {
-        leaf_step<SL, COUNT, ANY>(bvh.tris, r, stk, ray_mask, count);
-        return;
+// Moves a leaf entry in `cur` into the (empty) postponed slot and pops the next entry, so the
+// lane can keep stepping through inner nodes while its leaf waits for the wave's next leaf
+// phase (speculative traversal, Aila & Laine 2009).
+template <int SL>
+__device__ __forceinline__ void postpone_leaf(RayState& r, const LaneStack& stk) {
+    if (r.leaf == TRAVERSAL_DONE && is_leaf(r.cur)) {
+        r.leaf = r.cur;
+        r.cur = r.sp > 0 ? stack_pop<SL>(stk, r.sp) : TRAVERSAL_DONE;
     }
+}
+
+// One inner-node step (r.cur is an inner node): test its four children, push the hit ones far
+// to near, continue with the nearest.
+template <int SL, bool COUNT>
+__device__ __forceinline__ void node_step(const DeviceBVH bvh, RayState& r, const LaneStack& stk,
+                                          TraversalCount& count) {
     if constexpr (COUNT) count.nodes++;
     const float4* np = reinterpret_cast<const float4*>(bvh.nodes + r.cur);
     const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
@@ -196,16 +207,13 @@ struct DeviceBVH8 {
     const float4* tris; // TriRecord as 3 x float4
 };
 
-// One step over the 8-wide quantised BVH: decode the eight child boxes (fma(q, 2^e, origin),
-// conservative by construction), slab-test them as the BVH4 step does, sort the hit children
-// near to far with a 19-comparator network, push all but the nearest, continue with it.
-template <int SL, bool COUNT, bool ANY = false>
-__device__ __forceinline__ void ray_step(const DeviceBVH8 bvh, RayState& r, const LaneStack& stk, uint32_t ray_mask,
-                                         TraversalCount& count) {
-    if (r.cur & LEAF_FLAG) {
-        leaf_step<SL, COUNT, ANY>(bvh.tris, r, stk, ray_mask, count);
-        return;
-    }
+// One inner-node step over the 8-wide quantised BVH: decode the eight child boxes
+// (fma(q, 2^e, origin), conservative by construction), slab-test them as the BVH4 step does,
+// sort the hit children near to far with a 19-comparator network, push all but the nearest,
+// continue with it.
+template <int SL, bool COUNT>
+__device__ __forceinline__ void node_step(const DeviceBVH8 bvh, RayState& r, const LaneStack& stk,
+                                          TraversalCount& count) {
     if constexpr (COUNT) count.nodes++;
     const uint4* np = reinterpret_cast<const uint4*>(bvh.nodes + r.cur);
     const uint4 h0 = np[0], h1 = np[1], qx = np[2], qy = np[3], qz = np[4];
@@ -277,7 +285,7 @@ constexpr int REFILL_LANES = 16;
 template <int SL, bool COUNT, class Src, class Sink, bool ANY = false, class BVH = DeviceBVH>
 __device__ __forceinline__ void trace_persistent(const BVH bvh, const Src& src, const Sink& sink, uint32_t n,
                                                  uint32_t* work, uint32_t ray_mask, const LaneStack& stk,
-                                                 TraversalCount& count, uint32_t refill) {
+                                                 TraversalCount& count, uint32_t refill, uint32_t leaf_lanes) {
     const uint32_t lane = __lane_id();
     const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -332,10 +340,24 @@ __device__ __forceinline__ void trace_persistent(const BVH bvh, const Src& src, 
                 pool_end = hi;
             }
         }
-        if (__ballot(active || fresh) == 0) break;
+        // One wave-uniform phase per iteration: an inner-node step for every lane whose current
+        // entry is an inner node, or a leaf test for every lane with a postponed leaf. Leaf
+        // phases run once `leaf_lanes` lanes can do nothing but their leaf (or no lane has an
+        // inner node left), so both code paths run with most lanes active.
+        const bool inner_ok = active && is_inner(r.cur);
+        const bool has_leaf = active && r.leaf != TRAVERSAL_DONE;
+        const uint64_t mi = __ballot(inner_ok);
+        const uint64_t ml = __ballot(has_leaf);
+        if ((mi | ml) == 0 && __ballot(fresh) == 0) break;
+        const bool leaf_phase = mi == 0 || uint32_t(__popcll(ml & ~mi)) >= leaf_lanes;
+        if (leaf_phase) {
+            if (has_leaf) leaf_test<SL, COUNT, ANY>(bvh.tris, r, ray_mask, count);
+        } else {
+            if (inner_ok) node_step<SL, COUNT>(bvh, r, stk, count);
+        }
         if (active) {
-            ray_step<SL, COUNT, ANY>(bvh, r, stk, ray_mask, count);
-            if (r.cur == TRAVERSAL_DONE) {
+            postpone_leaf<SL>(r, stk);
+            if (r.cur == TRAVERSAL_DONE && r.leaf == TRAVERSAL_DONE) {
                 sink.store(idx, r.best);
                 active = false;
             }

@@ -177,6 +177,7 @@ hipError_t alloc_job(Slot& sl, const Job& j, const JobShape& sh) {
     const TraceScratch tuned = scene_scratch(nullptr, j.d.scene);
     b.ts.refill = tuned.refill;
     b.ts.grid_pct = tuned.grid_pct;
+    b.ts.leaf_lanes = tuned.leaf_lanes;
     return e;
 }
 

@@ -96,7 +96,7 @@ namespace {
 bool valid(const mrt_tuning& t) {
     return t.refill_lanes >= 1 && t.refill_lanes <= 64 && t.grid_percent >= 1 && t.grid_percent <= 100 &&
            t.bvh_max_leaf >= 1 && t.bvh_max_leaf <= uint32_t(MAX_LEAF) && t.bvh_traversal_cost > 0.f &&
-           (t.bvh_width == 0 || t.bvh_width == 4 || t.bvh_width == 8);
+           (t.bvh_width == 0 || t.bvh_width == 4 || t.bvh_width == 8) && t.leaf_lanes <= 64;
 }
 
 // Device time of one trial frame (gbuffer + pt) in ms, after one untimed frame.
@@ -162,6 +162,8 @@ MRT_API mrt_status mrt_scene_set_tuning(mrt_ctx* c, mrt_scene* s, const mrt_tuni
     }
     s->tuning = *t;
     s->tuning.bvh_width = width;
+    if (s->tuning.leaf_lanes == 0) s->tuning.leaf_lanes = 16;
+    s->tuning._reserved = 0;
     return MRT_OK;
 }
 
@@ -199,10 +201,11 @@ MRT_API mrt_status mrt_tune(mrt_ctx* c, mrt_scene* s, const mrt_tune_request* r,
         cand.ms_per_frame = ms;
         if (verbose)
             std::fprintf(stderr,
-                         "mrt_tune: width %u leaf %u cost %.2f paths %llu refill %u grid %u%% -> %.3f ms (best %.3f)\n",
+                         "mrt_tune: width %u leaf %u cost %.2f paths %llu refill %u leaf-phase %u grid %u%% -> %.3f ms "
+                         "(best %.3f)\n",
                          cand.bvh_width, cand.bvh_max_leaf, double(cand.bvh_traversal_cost),
-                         (unsigned long long)cand.max_paths, cand.refill_lanes, cand.grid_percent, double(ms),
-                         double(best.ms_per_frame));
+                         (unsigned long long)cand.max_paths, cand.refill_lanes, cand.leaf_lanes, cand.grid_percent,
+                         double(ms), double(best.ms_per_frame));
         if (ms < best.ms_per_frame) best = cand;
         return mrt_scene_set_tuning(c, s, &best);
     };
@@ -236,6 +239,12 @@ MRT_API mrt_status mrt_tune(mrt_ctx* c, mrt_scene* s, const mrt_tune_request* r,
             if (rf == best.refill_lanes) continue;
             mrt_tuning cand = best;
             cand.refill_lanes = rf;
+            if ((st = try_one(cand)) != MRT_OK) return st;
+        }
+        for (uint32_t ll : {4u, 8u, 16u, 24u, 32u, 48u}) {
+            if (ll == best.leaf_lanes) continue;
+            mrt_tuning cand = best;
+            cand.leaf_lanes = ll;
             if ((st = try_one(cand)) != MRT_OK) return st;
         }
         for (uint32_t gp : {50u, 75u, 100u}) {
