@@ -208,12 +208,11 @@ struct Collapser {
         out.depth = std::max(out.depth, depth);
         BVH4Node node;
         std::memset(&node, 0, sizeof(node));
-        uint32_t counts = 0;
         for (int i = 0; i < 4; i++) {
             if (i >= nk) {
                 node.lox[i] = node.loy[i] = node.loz[i] = 1e30f;
                 node.hix[i] = node.hiy[i] = node.hiz[i] = -1e30f;
-                node.child[i] = -1;
+                node.entry[i] = EMPTY_ENTRY;
                 continue;
             }
             const Node2& c = n2[kids[i]];
@@ -226,13 +225,11 @@ struct Collapser {
             node.hiy[i] = hi[1];
             node.hiz[i] = hi[2];
             if (c.left < 0) {
-                node.child[i] = int32_t(c.first);
-                counts |= (c.count & 0xFFu) << (8 * i);
+                node.entry[i] = leaf_entry(c.first, c.count);
             } else {
-                node.child[i] = int32_t(collapse(kids[i], depth + 1));
+                node.entry[i] = collapse(kids[i], depth + 1);
             }
         }
-        node.counts = counts;
         out.nodes[me] = node;
         return me;
     }
@@ -460,7 +457,7 @@ BVH4 build_bvh4(const float* v0, const float* e1, const float* e2, const uint32_
         for (int i = 0; i < 4; i++) {
             node.lox[i] = node.loy[i] = node.loz[i] = 1e30f;
             node.hix[i] = node.hiy[i] = node.hiz[i] = -1e30f;
-            node.child[i] = -1;
+            node.entry[i] = EMPTY_ENTRY;
         }
         out.nodes.push_back(node);
         out.depth = 1;
@@ -474,7 +471,7 @@ BVH4 build_bvh4(const float* v0, const float* e1, const float* e2, const uint32_
         for (int i = 0; i < 4; i++) {
             node.lox[i] = node.loy[i] = node.loz[i] = 1e30f;
             node.hix[i] = node.hiy[i] = node.hiz[i] = -1e30f;
-            node.child[i] = -1;
+            node.entry[i] = EMPTY_ENTRY;
         }
         float lo[3], hi[3];
         padded(root.box, mag, lo, hi);
@@ -484,8 +481,7 @@ BVH4 build_bvh4(const float* v0, const float* e1, const float* e2, const uint32_
         node.hix[0] = hi[0];
         node.hiy[0] = hi[1];
         node.hiz[0] = hi[2];
-        node.child[0] = int32_t(root.first);
-        node.counts = root.count & 0xFFu;
+        node.entry[0] = leaf_entry(root.first, root.count);
         out.nodes.push_back(node);
         out.depth = 1;
         return out;

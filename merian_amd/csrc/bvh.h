@@ -13,12 +13,12 @@
 namespace mrt {
 
 // One node = one 128-byte line: child boxes as structure-of-arrays so a lane reads them with
-// six 16-byte loads, then the four child references and leaf sizes.
+// six 16-byte loads (lo/hi of one axis 16 bytes apart, so the ray's direction octant picks
+// the near and far plane by load address), then the four children as ready traversal entries.
 struct alignas(128) BVH4Node {
     float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
-    int32_t child[4];   // inner child: node index; leaf child: first triangle record; empty: -1
-    uint32_t counts;    // 8 bits per child: triangles in a leaf child, 0 for inner / empty
-    uint32_t _pad[3];
+    uint32_t entry[4];  // inner child: node index; leaf child: leaf entry (below); empty: 0xFFFFFFFF
+    uint32_t _pad[4];
 };
 static_assert(sizeof(BVH4Node) == 128, "BVH4Node must be one 128-byte line");
 
@@ -31,6 +31,8 @@ struct alignas(16) TriRecord {
 
 // Leaf stack entries: bit 31 set, first record in bits 3..30, (count - 1) in bits 0..2.
 constexpr uint32_t LEAF_FLAG = 0x80000000u;
+constexpr uint32_t EMPTY_ENTRY = 0xFFFFFFFFu;
+inline uint32_t leaf_entry(uint32_t first, uint32_t count) { return LEAF_FLAG | (first << 3) | (count - 1u); }
 constexpr int MAX_LEAF = 8; // most triangles per leaf the encoding allows (the build default is 4)
 
 struct BVH4 {

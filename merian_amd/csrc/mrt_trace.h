@@ -96,7 +96,9 @@ struct RayState {
     f3 o, d;
     float ix, iy, iz, ox, oy, oz;
     float t0, tmin, tmax;
+    float tlim;   // far limit of the box tests: min(tmax, best.t) widened by 2 ulp
     uint32_t neg; // bit k: direction component k is negative
+    uint32_t pn, pf; // BVH4: byte offsets of the near / far plane row of axis k in byte k
     HitRec best;
     int sp;
     uint32_t cur;  // next entry to process; TRAVERSAL_DONE when the stack is exhausted
@@ -123,6 +125,11 @@ __device__ __forceinline__ void ray_begin(RayState& r, f3 o, f3 d, float tmin, f
     r.tmax = tmax;
     r.t0 = tmin - fabsf(tmin) * 2.4e-7f;
     r.best = HitRec{TRAVERSAL_DONE, __builtin_inff(), 0.f, 0.f};
+    r.tlim = fminf(tmax, r.best.t) * 1.00000024f;
+    // BVH4Node rows: lox hix loy hiy loz hiz, 16 bytes each
+    const uint32_t nx = r.neg & 1u, ny = (r.neg >> 1) & 1u, nz = (r.neg >> 2) & 1u;
+    r.pn = (nx << 4) | ((32u + (ny << 4)) << 8) | ((64u + (nz << 4)) << 16);
+    r.pf = ((16u - (nx << 4))) | ((48u - (ny << 4)) << 8) | ((80u - (nz << 4)) << 16);
     r.sp = 0;
     r.cur = 0u; // root node (always an inner node)
     r.leaf = TRAVERSAL_DONE;
@@ -137,6 +144,7 @@ __device__ __forceinline__ void leaf_test(const float4* __restrict__ tris, RaySt
     for (uint32_t k = 0; k < cnt; k++) test_triangle(tris, first + k, r.o, r.d, r.tmin, r.tmax, ray_mask, r.best);
     if constexpr (COUNT) count.tris += cnt;
     r.leaf = TRAVERSAL_DONE;
+    r.tlim = fminf(r.tmax, r.best.t) * 1.00000024f;
     if constexpr (ANY) {
         if (r.best.id != TRAVERSAL_DONE) {
             r.cur = TRAVERSAL_DONE;
@@ -162,29 +170,30 @@ template <int SL, bool COUNT>
 __device__ __forceinline__ void node_step(const DeviceBVH bvh, RayState& r, const LaneStack& stk,
                                           TraversalCount& count) {
     if constexpr (COUNT) count.nodes++;
-    const float4* np = reinterpret_cast<const float4*>(bvh.nodes + r.cur);
-    const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-    const int4 ch = reinterpret_cast<const int4*>(np)[6];
-    const uint32_t counts = reinterpret_cast<const uint32_t*>(np)[28];
-    const float tlim = fminf(r.tmax, r.best.t) * 1.00000024f;
-    const bool nx = r.neg & 1u, ny = r.neg & 2u, nz = r.neg & 4u;
+    // near and far plane rows of each axis chosen by address: one FMA per plane, no selects
+    const char* nb = reinterpret_cast<const char*>(bvh.nodes);
+    const uint32_t off = r.cur << 7;
+    auto row = [&](uint32_t o) { return *reinterpret_cast<const float4*>(nb + (off + o)); };
+    const float4 nxr = row(r.pn & 0xFFu), fxr = row(r.pf & 0xFFu);
+    const float4 nyr = row((r.pn >> 8) & 0xFFu), fyr = row((r.pf >> 8) & 0xFFu);
+    const float4 nzr = row(r.pn >> 16), fzr = row(r.pf >> 16);
+    const uint4 ent4 = *reinterpret_cast<const uint4*>(nb + (off + 96u));
+    const float tlim = r.tlim;
     float key[4];
     uint32_t ent[4];
-#define MRT_CHILD(c, LX, HX, LY, HY, LZ, HZ, CH)                                                               \
-    {                                                                                                          \
-        const float tnx = __builtin_fmaf(nx ? HX : LX, r.ix, -r.ox), tfx = __builtin_fmaf(nx ? LX : HX, r.ix, -r.ox); \
-        const float tny = __builtin_fmaf(ny ? HY : LY, r.iy, -r.oy), tfy = __builtin_fmaf(ny ? LY : HY, r.iy, -r.oy); \
-        const float tnz = __builtin_fmaf(nz ? HZ : LZ, r.iz, -r.oz), tfz = __builtin_fmaf(nz ? LZ : HZ, r.iz, -r.oz); \
-        const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, r.t0));                                            \
-        const float tf = fminf(fminf(tfx, tfy), fminf(tfz, tlim));                                            \
-        const uint32_t cc = (counts >> (8 * c)) & 0xFFu;                                                       \
-        key[c] = (tn <= tf) ? tn : __builtin_inff();                                                           \
-        ent[c] = cc ? (LEAF_FLAG | (uint32_t(CH) << 3) | (cc - 1u)) : uint32_t(CH);                            \
+#define MRT_CHILD(c, NX, FX, NY, FY, NZ, FZ, E)                                                              \
+    {                                                                                                        \
+        const float tn = fmaxf(fmaxf(__builtin_fmaf(NX, r.ix, -r.ox), __builtin_fmaf(NY, r.iy, -r.oy)),      \
+                               fmaxf(__builtin_fmaf(NZ, r.iz, -r.oz), r.t0));                               \
+        const float tf = fminf(fminf(__builtin_fmaf(FX, r.ix, -r.ox), __builtin_fmaf(FY, r.iy, -r.oy)),      \
+                               fminf(__builtin_fmaf(FZ, r.iz, -r.oz), tlim));                               \
+        key[c] = (tn <= tf) ? tn : __builtin_inff();                                                         \
+        ent[c] = E;                                                                                          \
     }
-    MRT_CHILD(0, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, ch.x)
-    MRT_CHILD(1, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, ch.y)
-    MRT_CHILD(2, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, ch.z)
-    MRT_CHILD(3, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, ch.w)
+    MRT_CHILD(0, nxr.x, fxr.x, nyr.x, fyr.x, nzr.x, fzr.x, ent4.x)
+    MRT_CHILD(1, nxr.y, fxr.y, nyr.y, fyr.y, nzr.y, fzr.y, ent4.y)
+    MRT_CHILD(2, nxr.z, fxr.z, nyr.z, fyr.z, nzr.z, fzr.z, ent4.z)
+    MRT_CHILD(3, nxr.w, fxr.w, nyr.w, fyr.w, nzr.w, fzr.w, ent4.w)
 #undef MRT_CHILD
     cswap(key[0], ent[0], key[1], ent[1]);
     cswap(key[2], ent[2], key[3], ent[3]);
@@ -223,26 +232,30 @@ __device__ __forceinline__ void node_step(const DeviceBVH8 bvh, RayState& r, con
     const float sx = __uint_as_float(uint32_t(ex + 127) << 23);
     const float sy = __uint_as_float(uint32_t(ey + 127) << 23);
     const float sz = __uint_as_float(uint32_t(ez + 127) << 23);
-    const float tlim = fminf(r.tmax, r.best.t) * 1.00000024f;
+    const float tlim = r.tlim;
     const bool nx = r.neg & 1u, ny = r.neg & 2u, nz = r.neg & 4u;
     const float inf = __builtin_inff();
+    // near / far plane bytes per axis, selected by the ray's octant once per node (children 0-3
+    // in the low word, 4-7 in the high word)
+    const uint32_t nx0 = nx ? qx.z : qx.x, nx1 = nx ? qx.w : qx.y, fx0 = nx ? qx.x : qx.z, fx1 = nx ? qx.y : qx.w;
+    const uint32_t ny0 = ny ? qy.z : qy.x, ny1 = ny ? qy.w : qy.y, fy0 = ny ? qy.x : qy.z, fy1 = ny ? qy.y : qy.w;
+    const uint32_t nz0 = nz ? qz.z : qz.x, nz1 = nz ? qz.w : qz.y, fz0 = nz ? qz.x : qz.z, fz1 = nz ? qz.y : qz.w;
     float key[8];
     uint32_t ent[8];
     uint32_t inner_next = h1.x, tri_next = h1.y;
 #pragma unroll
     for (int c = 0; c < 8; c++) {
         const uint32_t sh = 8u * uint32_t(c & 3);
-        const uint32_t lxq = ((c < 4 ? qx.x : qx.y) >> sh) & 0xFFu, hxq = ((c < 4 ? qx.z : qx.w) >> sh) & 0xFFu;
-        const uint32_t lyq = ((c < 4 ? qy.x : qy.y) >> sh) & 0xFFu, hyq = ((c < 4 ? qy.z : qy.w) >> sh) & 0xFFu;
-        const uint32_t lzq = ((c < 4 ? qz.x : qz.y) >> sh) & 0xFFu, hzq = ((c < 4 ? qz.z : qz.w) >> sh) & 0xFFu;
-        const float LX = __builtin_fmaf(float(lxq), sx, px), HX = __builtin_fmaf(float(hxq), sx, px);
-        const float LY = __builtin_fmaf(float(lyq), sy, py), HY = __builtin_fmaf(float(hyq), sy, py);
-        const float LZ = __builtin_fmaf(float(lzq), sz, pz), HZ = __builtin_fmaf(float(hzq), sz, pz);
-        const float tnx = __builtin_fmaf(nx ? HX : LX, r.ix, -r.ox), tfx = __builtin_fmaf(nx ? LX : HX, r.ix, -r.ox);
-        const float tny = __builtin_fmaf(ny ? HY : LY, r.iy, -r.oy), tfy = __builtin_fmaf(ny ? LY : HY, r.iy, -r.oy);
-        const float tnz = __builtin_fmaf(nz ? HZ : LZ, r.iz, -r.oz), tfz = __builtin_fmaf(nz ? LZ : HZ, r.iz, -r.oz);
-        const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, r.t0));
-        const float tf = fminf(fminf(tfx, tfy), fminf(tfz, tlim));
+        const float NX = __builtin_fmaf(float(((c < 4 ? nx0 : nx1) >> sh) & 0xFFu), sx, px);
+        const float FX = __builtin_fmaf(float(((c < 4 ? fx0 : fx1) >> sh) & 0xFFu), sx, px);
+        const float NY = __builtin_fmaf(float(((c < 4 ? ny0 : ny1) >> sh) & 0xFFu), sy, py);
+        const float FY = __builtin_fmaf(float(((c < 4 ? fy0 : fy1) >> sh) & 0xFFu), sy, py);
+        const float NZ = __builtin_fmaf(float(((c < 4 ? nz0 : nz1) >> sh) & 0xFFu), sz, pz);
+        const float FZ = __builtin_fmaf(float(((c < 4 ? fz0 : fz1) >> sh) & 0xFFu), sz, pz);
+        const float tn = fmaxf(fmaxf(__builtin_fmaf(NX, r.ix, -r.ox), __builtin_fmaf(NY, r.iy, -r.oy)),
+                               fmaxf(__builtin_fmaf(NZ, r.iz, -r.oz), r.t0));
+        const float tf = fminf(fminf(__builtin_fmaf(FX, r.ix, -r.ox), __builtin_fmaf(FY, r.iy, -r.oy)),
+                               fminf(__builtin_fmaf(FZ, r.iz, -r.oz), tlim));
         const uint32_t cnt = (h1.z >> (4 * c)) & 15u;
         const bool inner = (imask >> c) & 1u;
         key[c] = (tn <= tf && (inner || cnt)) ? tn : inf;
