@@ -322,11 +322,22 @@ MRT_API mrt_status mrt_scene_create(mrt_ctx* c, const mrt_scene_desc* d, mrt_sce
     std::vector<float> v0(3 * n_tris), e1(3 * n_tris), e2(3 * n_tris);
     std::vector<uint32_t> flags(n_tris), geom_first(std::max<uint32_t>(1, d->geometry_count));
     std::vector<uint32_t> tri_verts(4 * n_tris);
+    std::vector<uint8_t> tri_class(std::max<uint64_t>(1, n_tris));
     uint64_t gid = 0;
     for (uint32_t g = 0; g < d->geometry_count; g++) {
         const mrt_geometry& G = d->geometries[g];
         if (G.material_id >= d->material_count) return fail(c, MRT_ERR_INVALID, "material id out of range");
         geom_first[g] = uint32_t(gid);
+        const mrt_material& M = d->materials[G.material_id];
+        uint8_t cls = SHADE_CLASS_DIFFUSE;
+        if (G.flags & MRT_GEOM_USE_ENV_MAP) {
+            cls = SHADE_CLASS_ENV;
+        } else if (M.model != MRT_MAT_DIFFUSE) {
+            cls = (M.features & MRT_PBR_TRANSMISSION) ? SHADE_CLASS_TRANSMISSIVE
+                  : M.metalness >= 0.5f              ? SHADE_CLASS_METAL
+                                                     : SHADE_CLASS_DIELECTRIC;
+        }
+        std::fill(tri_class.begin() + gid, tri_class.begin() + gid + G.primitive_count, cls);
         for (uint32_t p = 0; p < G.primitive_count; p++, gid++) {
             uint32_t vi[3];
             for (int k = 0; k < 3; k++) {
@@ -387,6 +398,7 @@ MRT_API mrt_status mrt_scene_create(mrt_ctx* c, const mrt_scene_desc* d, mrt_sce
     UP(bvh.nodes.data(), bvh.nodes.size() * sizeof(BVH4Node), nodes, const BVH4Node*)
     UP(bvh.tris.data(), bvh.tris.size() * sizeof(TriRecord), tris, const float4*)
     UP(tri_verts.data(), tri_verts.size() * 4, tri_verts, const uint4*)
+    UP(tri_class.data(), tri_class.size(), tri_class, const uint8_t*)
     UP(d->vertices, size_t(d->vertex_count) * sizeof(mrt_vertex), verts, const mrt_vertex*)
     UP(d->geometries, size_t(d->geometry_count) * sizeof(mrt_geometry), geoms, const mrt_geometry*)
     UP(geom_first.data(), geom_first.size() * 4, geom_first, const uint32_t*)

@@ -185,10 +185,13 @@ mrt_status guide_train_frame(mrt_ctx* c, mrt_guide* g, hipStream_t st) {
     if (e != hipSuccess) return hip_fail(c, e, "guide record count");
     const uint32_t n = std::min(*g->h_count, g->cfg.max_records);
     if (n == 0) return MRT_OK;
-    const uint32_t batch = std::min(n, g->cfg.train_batch);
+    // the steps take disjoint slices of the (permuted) records while the frame has enough of
+    // them: no record drives more than one step of a frame
+    const uint32_t steps = g->cfg.train_steps;
+    const uint32_t batch = std::min(g->cfg.train_batch, std::max(1u, (n + steps - 1) / steps));
     mrt_status s = ensure_train(c, g, batch);
     if (s != MRT_OK) return s;
-    for (uint32_t k = 0; k < g->cfg.train_steps; k++) {
+    for (uint32_t k = 0; k < steps; k++) {
         const uint32_t off = uint32_t((uint64_t(k) * batch) % n);
         const uint32_t cnt = std::min(batch, n - off);
         s = train_step(c, g, g->gtrain + 3 * size_t(off), cnt, st);

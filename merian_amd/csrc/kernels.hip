@@ -19,6 +19,8 @@ namespace {
 
 constexpr int TRACE_BLOCK = 128;
 constexpr int SHADE_BLOCK = 256;
+constexpr int SHADE_ROUNDS = 4;                          // paths per thread per tile
+constexpr int SHADE_TILE = SHADE_BLOCK * SHADE_ROUNDS;   // paths a block groups by class at once
 constexpr float SCENE_RAY_TMAX = 1e7f;     // scene/hit.slang:8
 constexpr float ENV_MAP_DISTANCE = 1e7f;   // scene/environment-map.slang:11
 constexpr float THROUGHPUT_THRESHOLD = 1e-5f; // scene/scene.slang:24
@@ -503,6 +505,54 @@ __device__ __forceinline__ uint32_t wave_append(bool push, uint32_t* counter) {
     return slot + uint32_t(__popcll(ballot & ((1ull << lane) - 1ull)));
 }
 
+// Groups the tile's paths [tile, tile + SHADE_TILE) by shading class (counting sort in LDS,
+// stable within a class): order[k] = queue slot of the k-th path in class order; slots past
+// the queue end sort last. Ends with the block synchronised.
+__device__ void shade_sort_tile(const DeviceScene& S, const float4* __restrict__ hits, uint32_t tile, uint32_t n,
+                                uint32_t* order, uint32_t* cnt, uint32_t* total) {
+    constexpr uint32_t K = SHADE_CLASSES + 1;
+    constexpr uint32_t WAVES = SHADE_BLOCK / 64;
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+    uint32_t cls[SHADE_ROUNDS], rank[SHADE_ROUNDS];
+#pragma unroll
+    for (int r = 0; r < SHADE_ROUNDS; r++) {
+        const uint32_t j = tile + uint32_t(r) * SHADE_BLOCK + threadIdx.x;
+        uint32_t c = SHADE_CLASSES;
+        if (j < n) {
+            const uint32_t gid = reinterpret_cast<const uint32_t*>(hits + j)[3];
+            c = gid == TRAVERSAL_DONE ? uint32_t(SHADE_CLASS_ENV) : uint32_t(S.tri_class[gid]);
+        }
+        cls[r] = c;
+        rank[r] = 0;
+        for (uint32_t k = 0; k < K; k++) {
+            const uint64_t b = __ballot(c == k);
+            if (c == k) rank[r] = uint32_t(__popcll(b & ((1ull << lane) - 1ull)));
+            if (lane == 0) cnt[(uint32_t(r) * WAVES + wave) * K + k] = uint32_t(__popcll(b));
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < K) {
+        // exclusive prefix of class k over the (round, wave) groups, and the class total
+        const uint32_t k = threadIdx.x;
+        uint32_t run = 0;
+        for (uint32_t g = 0; g < SHADE_ROUNDS * WAVES; g++) {
+            const uint32_t v = cnt[g * K + k];
+            cnt[g * K + k] = run;
+            run += v;
+        }
+        total[k] = run;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < SHADE_ROUNDS; r++) {
+        uint32_t off = 0;
+        for (uint32_t k = 0; k < cls[r]; k++) off += total[k];
+        order[off + cnt[(uint32_t(r) * WAVES + wave) * K + cls[r]] + rank[r]] =
+            tile + uint32_t(r) * SHADE_BLOCK + threadIdx.x;
+    }
+    __syncthreads();
+}
+
 // Vertices where the guide is sampled: opaque surfaces that are not near-specular (the
 // mixture cannot follow a near-delta lobe; the BSDF alone samples those).
 __device__ __forceinline__ bool guide_eligible(const mrt_material& m) {
@@ -525,159 +575,170 @@ __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(MOD
                                                        const uint32_t* __restrict__ count_in, float4* __restrict__ no,
                                                        float4* __restrict__ nd, uint32_t* __restrict__ count_out,
                                                        uint32_t* __restrict__ shadow_count) {
+    __shared__ uint32_t s_order[SHADE_TILE];
+    __shared__ uint32_t s_cnt[SHADE_ROUNDS * (SHADE_BLOCK / 64) * (SHADE_CLASSES + 1)];
+    __shared__ uint32_t s_total[SHADE_CLASSES + 1];
     const uint32_t n = *count_in;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    const uint32_t lane = lane_id();
     const bool last = bounce + 1 >= A.max_path_length;
     constexpr bool nee = MODE == MRT_MODE_NEE;
     constexpr bool guided = MODE == MRT_MODE_GUIDED;
-    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < n; base += stride) {
-        const uint32_t i = base + lane;
-        bool push = false, shadow = false, record = false;
-        f3 next_o{0, 0, 0}, next_d{0, 0, 0};
-        f3 rec_pos{0, 0, 0}, rec_rad{0, 0, 0}, rd_keep{0, 0, 0}, thr_keep{0, 0, 0};
-        float rec_pdf = 0.f;
-        f3 sh_o{0, 0, 0}, sh_d{0, 0, 0}, sh_c{0, 0, 0};
-        uint32_t pid = 0;
-        if (i < n) {
-            const float4 o4 = ld_stream(qo + i), d4 = ld_stream(qd + i), h4 = ld_stream(P.hits + i);
-            pid = __float_as_uint(o4.w);
-            const f3 ro{o4.x, o4.y, o4.z}, rd{d4.x, d4.y, d4.z};
-            const uint32_t gid = __float_as_uint(h4.w);
-            float4 sa = ld_stream(P.state_a + pid), sb = ld_stream(P.state_b + pid);
-            f3 thr{sa.x, sa.y, sa.z};
-            float ior = sa.w;
-            f3 rad{sb.x, sb.y, sb.z};
-            PCG rng{__float_as_uint(sb.w)};
-            ShadingPoint sp;
-            shading_point(S, ro, rd, gid, h4.y, h4.z, ior, bounce == 0 || !last, sp);
-            f3 emission = sp.emission;
-            if (nee && bounce > 0 && sp.env) {
-                // the environment was also reachable by NEE from the previous vertex
-                emission = emission * power_heuristic(P.aux[pid], env_pdf(S, rd));
-            }
-            if (bounce == 0) {
-                rad = A.emission_on_primary ? thr * emission : f3{0.f, 0.f, 0.f};
-            } else {
-                rad = rad + thr * emission;
-                if (A.record_signature) P.sig[pid] = sig_step(P.sig[pid], gid);
-            }
-            if (!last && sp.has_material &&
-                !(thr.x < THROUGHPUT_THRESHOLD && thr.y < THROUGHPUT_THRESHOLD && thr.z < THROUGHPUT_THRESHOLD)) {
-                const f3 wi_local = sp.frame.to_local(-rd);
-                const f3 ng = decode_normal(sp.enc_face_normal);
-                const bool wi_below = dot(-rd, ng) < 0.f;
-                if (nee) {
-                    f3 wl;
-                    float pdf_l;
-                    if (env_sample(S, rng, wl, pdf_l)) {
-                        const f3 wl_local = sp.frame.to_local(wl);
-                        const f3 f = sp.mat.eval(wi_local, wl_local);
-                        const float wl_ng = dot(wl, ng);
-                        const bool trans = wi_local.z * wl_local.z < 0.f;
-                        if ((f.x > 0.f || f.y > 0.f || f.z > 0.f) && ((wi_below != (wl_ng < 0.f)) == trans)) {
-                            const float w = power_heuristic(env_pdf(S, wl), sp.mat.pdf(wi_local, wl_local));
-                            sh_c = ((thr * f) * env_eval(S, wl)) * (w / pdf_l);
-                            sh_o = offset_ray_origin(sp.pos, wl_ng < 0.f ? -ng : ng);
-                            sh_d = wl;
-                            shadow = finite3(sh_c) && (sh_c.x > 0.f || sh_c.y > 0.f || sh_c.z > 0.f);
+    // Continuation hits arrive in arbitrary order: group each tile by shading class so a wave
+    // runs one material's code. Primary hits arrive in pixel order and are coherent already.
+    const bool sort = bounce > 0;
+    for (uint32_t tile = blockIdx.x * SHADE_TILE; tile < n; tile += gridDim.x * SHADE_TILE) {
+        if (sort) shade_sort_tile(S, P.hits, tile, n, s_order, s_cnt, s_total);
+        for (int round = 0; round < SHADE_ROUNDS; round++) {
+            const uint32_t slot = uint32_t(round) * SHADE_BLOCK + threadIdx.x;
+            const uint32_t i = sort ? s_order[slot] : tile + slot;
+            bool push = false, shadow = false, record = false;
+            f3 next_o{0, 0, 0}, next_d{0, 0, 0};
+            f3 rec_pos{0, 0, 0}, rec_rad{0, 0, 0}, rd_keep{0, 0, 0}, thr_keep{0, 0, 0};
+            float rec_pdf = 0.f;
+            f3 sh_o{0, 0, 0}, sh_d{0, 0, 0}, sh_c{0, 0, 0};
+            uint32_t pid = 0;
+            if (i < n) {
+                const float4 o4 = ld_stream(qo + i), d4 = ld_stream(qd + i), h4 = ld_stream(P.hits + i);
+                pid = __float_as_uint(o4.w);
+                const f3 ro{o4.x, o4.y, o4.z}, rd{d4.x, d4.y, d4.z};
+                const uint32_t gid = __float_as_uint(h4.w);
+                float4 sa = ld_stream(P.state_a + pid), sb = ld_stream(P.state_b + pid);
+                f3 thr{sa.x, sa.y, sa.z};
+                float ior = sa.w;
+                f3 rad{sb.x, sb.y, sb.z};
+                PCG rng{__float_as_uint(sb.w)};
+                ShadingPoint sp;
+                shading_point(S, ro, rd, gid, h4.y, h4.z, ior, bounce == 0 || !last, sp);
+                f3 emission = sp.emission;
+                if (nee && bounce > 0 && sp.env) {
+                    // the environment was also reachable by NEE from the previous vertex
+                    emission = emission * power_heuristic(P.aux[pid], env_pdf(S, rd));
+                }
+                if (bounce == 0) {
+                    rad = A.emission_on_primary ? thr * emission : f3{0.f, 0.f, 0.f};
+                } else {
+                    rad = rad + thr * emission;
+                    if (A.record_signature) P.sig[pid] = sig_step(P.sig[pid], gid);
+                }
+                if (!last && sp.has_material &&
+                    !(thr.x < THROUGHPUT_THRESHOLD && thr.y < THROUGHPUT_THRESHOLD && thr.z < THROUGHPUT_THRESHOLD)) {
+                    const f3 wi_local = sp.frame.to_local(-rd);
+                    const f3 ng = decode_normal(sp.enc_face_normal);
+                    const bool wi_below = dot(-rd, ng) < 0.f;
+                    if (nee) {
+                        f3 wl;
+                        float pdf_l;
+                        if (env_sample(S, rng, wl, pdf_l)) {
+                            const f3 wl_local = sp.frame.to_local(wl);
+                            const f3 f = sp.mat.eval(wi_local, wl_local);
+                            const float wl_ng = dot(wl, ng);
+                            const bool trans = wi_local.z * wl_local.z < 0.f;
+                            if ((f.x > 0.f || f.y > 0.f || f.z > 0.f) && ((wi_below != (wl_ng < 0.f)) == trans)) {
+                                const float w = power_heuristic(env_pdf(S, wl), sp.mat.pdf(wi_local, wl_local));
+                                sh_c = ((thr * f) * env_eval(S, wl)) * (w / pdf_l);
+                                sh_o = offset_ray_origin(sp.pos, wl_ng < 0.f ? -ng : ng);
+                                sh_d = wl;
+                                shadow = finite3(sh_c) && (sh_c.x > 0.f || sh_c.y > 0.f || sh_c.z > 0.f);
+                            }
                         }
                     }
-                }
-                float alpha = 0.f;
-                GuideLobes L;
-                if (guided && guide_eligible(S.mats[sp.material])) {
-                    unpack_lobes(P.lobes + 4 * size_t(i), L);
-                    if (L.pi_sum > 0.f) alpha = A.guide_fraction;
-                }
-                if (guided && alpha > 0.f) {
-                    // one-sample MIS of the guide's mixture and the BSDF
-                    bool ok = true;
-                    f3 wo_l;
-                    if (rng.next_float() < alpha) {
-                        const float u0 = rng.next_float();
-                        const f2 u12 = rng.next_float2();
-                        wo_l = sp.frame.to_local(lobes_sample(L, u0, u12.x, u12.y));
-                    } else {
-                        const Sample s = sp.mat.sample_eval(wi_local, rng);
-                        ok = s.valid;
-                        wo_l = s.wo;
+                    float alpha = 0.f;
+                    GuideLobes L;
+                    if (guided && guide_eligible(S.mats[sp.material])) {
+                        unpack_lobes(P.lobes + 4 * size_t(i), L);
+                        if (L.pi_sum > 0.f) alpha = A.guide_fraction;
                     }
-                    if (ok) {
-                        const f3 wo = sp.frame.to_world(wo_l);
-                        const f3 f = sp.mat.eval(wi_local, wo_l);
-                        const float p = alpha * lobes_pdf(L, wo) + (1.f - alpha) * sp.mat.pdf(wi_local, wo_l);
-                        const float wo_ng = dot(wo, ng);
-                        const bool transmission = wi_local.z * wo_l.z < 0.f;
-                        if (p > 0.f && (f.x > 0.f || f.y > 0.f || f.z > 0.f) &&
-                            (wi_below != (wo_ng < 0.f)) == transmission) {
-                            thr = thr * (f / p);
-                            next_o = offset_ray_origin(sp.pos, wo_ng < 0.f ? -ng : ng);
-                            next_d = wo;
-                            push = finite3(thr);
-                            record = push && A.s0 == 0 && pid < A.npix;
-                            rec_pos = sp.pos;
-                            rec_rad = rad;
-                            rec_pdf = p;
-                            rd_keep = rd;
-                            thr_keep = thr;
+                    if (guided && alpha > 0.f) {
+                        // one-sample MIS of the guide's mixture and the BSDF
+                        bool ok = true;
+                        f3 wo_l;
+                        if (rng.next_float() < alpha) {
+                            const float u0 = rng.next_float();
+                            const f2 u12 = rng.next_float2();
+                            wo_l = sp.frame.to_local(lobes_sample(L, u0, u12.x, u12.y));
+                        } else {
+                            const Sample s = sp.mat.sample_eval(wi_local, rng);
+                            ok = s.valid;
+                            wo_l = s.wo;
+                        }
+                        if (ok) {
+                            const f3 wo = sp.frame.to_world(wo_l);
+                            const f3 f = sp.mat.eval(wi_local, wo_l);
+                            const float p = alpha * lobes_pdf(L, wo) + (1.f - alpha) * sp.mat.pdf(wi_local, wo_l);
+                            const float wo_ng = dot(wo, ng);
+                            const bool transmission = wi_local.z * wo_l.z < 0.f;
+                            if (p > 0.f && (f.x > 0.f || f.y > 0.f || f.z > 0.f) &&
+                                (wi_below != (wo_ng < 0.f)) == transmission) {
+                                thr = thr * (f / p);
+                                next_o = offset_ray_origin(sp.pos, wo_ng < 0.f ? -ng : ng);
+                                next_d = wo;
+                                push = finite3(thr);
+                                record = push && A.s0 == 0 && pid < A.npix;
+                                rec_pos = sp.pos;
+                                rec_rad = rad;
+                                rec_pdf = p;
+                                rd_keep = rd;
+                                thr_keep = thr;
+                            } else {
+                                thr = f3{0.f, 0.f, 0.f};
+                            }
                         } else {
                             thr = f3{0.f, 0.f, 0.f};
                         }
                     } else {
-                        thr = f3{0.f, 0.f, 0.f};
-                    }
-                } else {
-                    const Sample s = sp.mat.sample_eval(wi_local, rng);
-                    if (s.valid) {
-                        const f3 wo = sp.frame.to_world(s.wo);
-                        thr = thr * s.weight;
-                        const bool transmission = wi_local.z * s.wo.z < 0.f;
-                        if (transmission) ior /= sp.mat.eta;
-                        // ShadingPoint.make_ray (hit.slang:134-146)
-                        const float wo_ng = dot(wo, ng);
-                        if ((wi_below != (wo_ng < 0.f)) == transmission) {
-                            next_o = offset_ray_origin(sp.pos, wo_ng < 0.f ? -ng : ng);
-                            next_d = wo;
-                            push = true;
-                            if (nee) P.aux[pid] = s.pdf;
+                        const Sample s = sp.mat.sample_eval(wi_local, rng);
+                        if (s.valid) {
+                            const f3 wo = sp.frame.to_world(s.wo);
+                            thr = thr * s.weight;
+                            const bool transmission = wi_local.z * s.wo.z < 0.f;
+                            if (transmission) ior /= sp.mat.eta;
+                            // ShadingPoint.make_ray (hit.slang:134-146)
+                            const float wo_ng = dot(wo, ng);
+                            if ((wi_below != (wo_ng < 0.f)) == transmission) {
+                                next_o = offset_ray_origin(sp.pos, wo_ng < 0.f ? -ng : ng);
+                                next_d = wo;
+                                push = true;
+                                if (nee) P.aux[pid] = s.pdf;
+                            }
                         }
                     }
                 }
+                st_stream(P.state_a + pid, make_float4(thr.x, thr.y, thr.z, ior));
+                st_stream(P.state_b + pid, make_float4(rad.x, rad.y, rad.z, __uint_as_float(rng.state)));
             }
-            st_stream(P.state_a + pid, make_float4(thr.x, thr.y, thr.z, ior));
-            st_stream(P.state_b + pid, make_float4(rad.x, rad.y, rad.z, __uint_as_float(rng.state)));
-        }
-        const uint32_t k = wave_append(push, count_out);
-        if (push) {
-            st_stream(no + k, make_float4(next_o.x, next_o.y, next_o.z, __uint_as_float(pid)));
-            st_stream(nd + k, make_float4(next_d.x, next_d.y, next_d.z, 0.f));
-        }
-        if (guided) {
-            const uint32_t r = wave_append(record, P.grec_count);
-            if (record && r < P.grec_cap) {
-                float4* g = P.grec + 5 * size_t(r);
-                g[0] = make_float4(rec_pos.x, rec_pos.y, rec_pos.z, __uint_as_float(pid));
-                g[1] = make_float4(-rd_keep.x, -rd_keep.y, -rd_keep.z, rec_pdf);
-                g[2] = make_float4(next_d.x, next_d.y, next_d.z, 0.f);
-                g[3] = make_float4(rec_rad.x, rec_rad.y, rec_rad.z, 0.f);
-                g[4] = make_float4(thr_keep.x, thr_keep.y, thr_keep.z, 0.f);
+            const uint32_t k = wave_append(push, count_out);
+            if (push) {
+                st_stream(no + k, make_float4(next_o.x, next_o.y, next_o.z, __uint_as_float(pid)));
+                st_stream(nd + k, make_float4(next_d.x, next_d.y, next_d.z, 0.f));
+            }
+            if (guided) {
+                const uint32_t r = wave_append(record, P.grec_count);
+                if (record && r < P.grec_cap) {
+                    float4* g = P.grec + 5 * size_t(r);
+                    g[0] = make_float4(rec_pos.x, rec_pos.y, rec_pos.z, __uint_as_float(pid));
+                    g[1] = make_float4(-rd_keep.x, -rd_keep.y, -rd_keep.z, rec_pdf);
+                    g[2] = make_float4(next_d.x, next_d.y, next_d.z, 0.f);
+                    g[3] = make_float4(rec_rad.x, rec_rad.y, rec_rad.z, 0.f);
+                    g[4] = make_float4(thr_keep.x, thr_keep.y, thr_keep.z, 0.f);
+                }
+            }
+            if (nee) {
+                const uint32_t q = wave_append(shadow, shadow_count);
+                if (shadow) {
+                    st_stream(P.sq_o + q, make_float4(sh_o.x, sh_o.y, sh_o.z, __uint_as_float(pid)));
+                    st_stream(P.sq_d + q, make_float4(sh_d.x, sh_d.y, sh_d.z, 0.f));
+                    st_stream(P.sq_c + q, make_float4(sh_c.x, sh_c.y, sh_c.z, 0.f));
+                }
             }
         }
-        if (nee) {
-            const uint32_t q = wave_append(shadow, shadow_count);
-            if (shadow) {
-                st_stream(P.sq_o + q, make_float4(sh_o.x, sh_o.y, sh_o.z, __uint_as_float(pid)));
-                st_stream(P.sq_d + q, make_float4(sh_d.x, sh_d.y, sh_d.z, 0.f));
-                st_stream(P.sq_c + q, make_float4(sh_c.x, sh_c.y, sh_c.z, 0.f));
-            }
-        }
+        if (sort) __syncthreads(); // s_order is rewritten by the next tile
     }
 }
 
 // Training records of the guide from the first chunk's sample-0 paths, once their radiance
 // is final: incident radiance along the sampled direction = (final radiance - radiance before
 // the vertex's continuation) / throughput after it; weight = its luminance / sampling pdf.
+// Record r is stored at (r * 2654435761) mod n, a permutation (the multiplier is a prime above
+// any record count), so consecutive training slices mix pixels and bounces.
 __global__ void k_guide_targets(PathBuffers P) {
     const uint32_t n = min(*P.grec_count, P.grec_cap);
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -689,7 +750,7 @@ __global__ void k_guide_targets(PathBuffers P) {
                    th.z > 0.f ? fmaxf(rf.z - rb.z, 0.f) / th.z : 0.f};
         float w = b.w > 0.f ? yuv_luminance(L) / b.w : 0.f;
         if (!(w >= 0.f && w < 3.0e38f)) w = 0.f;
-        float4* t = P.gtrain + 3 * size_t(r);
+        float4* t = P.gtrain + 3 * ((uint64_t(r) * 2654435761ull) % n);
         t[0] = make_float4(a.x, a.y, a.z, w);
         t[1] = make_float4(b.x, b.y, b.z, 0.f);
         t[2] = make_float4(c.x, c.y, c.z, 0.f);
@@ -919,7 +980,7 @@ hipError_t launch_pt_frame(const DeviceScene& s, const TraceScratch& ts, PtArgs 
                 mark(1);
             }
             mark(-1);
-            const dim3 sg(grid_for(a.n_paths, SHADE_BLOCK, MAX_BLOCKS));
+            const dim3 sg(grid_for(a.n_paths, SHADE_TILE, MAX_BLOCKS));
             if (nee) {
                 hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<MRT_MODE_NEE>), sg, dim3(SHADE_BLOCK), 0, st, s, a, b, pb,
                                    pb.q_o[qi], pb.q_d[qi], cnt + b, pb.q_o[qi ^ 1], pb.q_d[qi ^ 1], cnt + b + 1,

@@ -17,6 +17,7 @@ struct DeviceScene {
     uint32_t width;              // BVH node width in use: 4 or 8
     const float4* tris;          // TriRecord stream, leaf order
     const uint4* tri_verts;      // per global triangle id: 3 global vertex indices, geometry index
+    const uint8_t* tri_class;    // per global triangle id: shading class (SHADE_CLASS_*)
     const mrt_vertex* verts;     // PackedVertexData
     const mrt_geometry* geoms;   // GeometryData
     const uint32_t* geom_first;  // global id of each geometry's first triangle
@@ -30,6 +31,17 @@ struct DeviceScene {
     uint32_t n_tris;
     uint32_t n_geoms;
     uint32_t stack_need;         // traversal stack entries the tree can need (width - 1 per level + slack)
+};
+
+// Shading classes: the shade kernel groups a block's paths by class so each wave runs one
+// material code path. Hit triangles carry theirs (DeviceScene.tri_class); misses are ENV.
+enum : uint32_t {
+    SHADE_CLASS_ENV = 0,         // environment (a miss, or a UseEnvMap geometry)
+    SHADE_CLASS_DIFFUSE = 1,     // Lambert
+    SHADE_CLASS_METAL = 2,       // OpenPBR, metalness >= 0.5, no transmission
+    SHADE_CLASS_DIELECTRIC = 3,  // OpenPBR, metalness < 0.5, no transmission
+    SHADE_CLASS_TRANSMISSIVE = 4, // OpenPBR with transmission
+    SHADE_CLASSES = 5
 };
 
 // Entries of the per-lane traversal stack kept in LDS; deeper entries go to TraceScratch.ovf.

@@ -141,7 +141,7 @@ def sky(ctx):
     ds.destroy()
 
 
-W_, H_ = 96, 54
+W_, H_ = 192, 108
 
 
 def _frames(ctx, ds, spec, cam_frame, params, frames, guide=None, train=True):
