@@ -463,6 +463,23 @@ __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0u;
 }
 
+// Global triangle id and barycentrics of a gbuffer texel (PackedHitInfo, gbuffer.slang:40-61);
+// misses, and hits of a stale or foreign gbuffer that name no triangle of this scene, give
+// TRAVERSAL_DONE (shaded as misses).
+__device__ __forceinline__ uint32_t gbuffer_hit(const DeviceScene& S, const mrt_packed_hit* gbuffer, uint32_t p,
+                                                float& bu, float& bv) {
+    const uint4 gb = reinterpret_cast<const uint4*>(gbuffer)[p];
+    uint32_t gid = TRAVERSAL_DONE;
+    bu = bv = 0.f;
+    if (gb.x != 0xFFFFFFFFu) {
+        const uint32_t geometry = gb.x + (gb.z & 0xFFFFu); // GeometryID = instance_id + geometry_index
+        if (geometry < S.n_geoms && gb.y < S.geoms[geometry].primitive_count) gid = S.geom_first[geometry] + gb.y;
+        bu = half_to_float(uint16_t(gb.w & 0xFFFFu));
+        bv = half_to_float(uint16_t(gb.w >> 16));
+    }
+    return gid;
+}
+
 // Seeds one path per (pixel, sample) of the chunk with the primary ray and gbuffer hit.
 __global__ void k_pt_init(DeviceScene S, PtArgs A, const mrt_packed_hit* __restrict__ gbuffer, PathBuffers P,
                           uint32_t* count0) {
@@ -473,16 +490,8 @@ __global__ void k_pt_init(DeviceScene S, PtArgs A, const mrt_packed_hit* __restr
         const uint32_t x = p % A.width, y = p / A.width;
         const uint32_t s = A.s0 + sl;
         const RayF r = camera_ray(A.cam, x, y, A.width, A.height);
-        const uint4 gb = reinterpret_cast<const uint4*>(gbuffer)[p];
-        uint32_t gid = TRAVERSAL_DONE;
-        float bu = 0.f, bv = 0.f;
-        if (gb.x != 0xFFFFFFFFu) {
-            const uint32_t geometry = gb.x + (gb.z & 0xFFFFu); // GeometryID = instance_id + geometry_index
-            // hits of a stale or foreign gbuffer that name no triangle of this scene shade as misses
-            if (geometry < S.n_geoms && gb.y < S.geoms[geometry].primitive_count) gid = S.geom_first[geometry] + gb.y;
-            bu = half_to_float(uint16_t(gb.w & 0xFFFFu));
-            bv = half_to_float(uint16_t(gb.w >> 16));
-        }
+        float bu, bv;
+        const uint32_t gid = gbuffer_hit(S, gbuffer, p, bu, bv);
         st_stream(P.q_o[0] + i, make_float4(r.o.x, r.o.y, r.o.z, __uint_as_float(i)));
         st_stream(P.q_d[0] + i, make_float4(r.d.x, r.d.y, r.d.z, 0.f));
         st_stream(P.hits + i, make_float4(0.f, bu, bv, __uint_as_float(gid)));
@@ -553,6 +562,26 @@ __device__ void shade_sort_tile(const DeviceScene& S, const float4* __restrict__
     __syncthreads();
 }
 
+// Continues a shaded surface vertex with a BSDF sample (trace_path, render_pt.slang:44-75):
+// throughput *= weight, ior follows transmission, and ShadingPoint.make_ray (hit.slang:134-146)
+// gives the next ray unless the geometric side disagrees with the transmission flag.
+// Returns whether the path continues; pdf = the sample's density.
+__device__ __forceinline__ bool bsdf_continue(const ShadingPoint& sp, f3 wi_local, f3 ng, bool wi_below, PCG& rng,
+                                              f3& thr, float& ior, f3& next_o, f3& next_d, float& pdf) {
+    const Sample s = sp.mat.sample_eval(wi_local, rng);
+    if (!s.valid) return false;
+    const f3 wo = sp.frame.to_world(s.wo);
+    thr = thr * s.weight;
+    const bool transmission = wi_local.z * s.wo.z < 0.f;
+    if (transmission) ior /= sp.mat.eta;
+    const float wo_ng = dot(wo, ng);
+    if ((wi_below != (wo_ng < 0.f)) != transmission) return false;
+    next_o = offset_ray_origin(sp.pos, wo_ng < 0.f ? -ng : ng);
+    next_d = wo;
+    pdf = s.pdf;
+    return true;
+}
+
 // Vertices where the guide is sampled: opaque surfaces that are not near-specular (the
 // mixture cannot follow a near-delta lobe; the BSDF alone samples those).
 __device__ __forceinline__ bool guide_eligible(const mrt_material& m) {
@@ -583,8 +612,9 @@ __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(MOD
     constexpr bool nee = MODE == MRT_MODE_NEE;
     constexpr bool guided = MODE == MRT_MODE_GUIDED;
     // Continuation hits arrive in arbitrary order: group each tile by shading class so a wave
-    // runs one material's code. Primary hits arrive in pixel order and are coherent already.
-    const bool sort = bounce > 0;
+    // runs one material's code. Primary hits arrive in pixel order and are coherent already;
+    // the last vertex only adds emission.
+    const bool sort = bounce > 0 && !last;
     for (uint32_t tile = blockIdx.x * SHADE_TILE; tile < n; tile += gridDim.x * SHADE_TILE) {
         if (sort) shade_sort_tile(S, P.hits, tile, n, s_order, s_cnt, s_total);
         for (int round = 0; round < SHADE_ROUNDS; round++) {
@@ -685,21 +715,9 @@ __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(MOD
                             thr = f3{0.f, 0.f, 0.f};
                         }
                     } else {
-                        const Sample s = sp.mat.sample_eval(wi_local, rng);
-                        if (s.valid) {
-                            const f3 wo = sp.frame.to_world(s.wo);
-                            thr = thr * s.weight;
-                            const bool transmission = wi_local.z * s.wo.z < 0.f;
-                            if (transmission) ior /= sp.mat.eta;
-                            // ShadingPoint.make_ray (hit.slang:134-146)
-                            const float wo_ng = dot(wo, ng);
-                            if ((wi_below != (wo_ng < 0.f)) == transmission) {
-                                next_o = offset_ray_origin(sp.pos, wo_ng < 0.f ? -ng : ng);
-                                next_d = wo;
-                                push = true;
-                                if (nee) P.aux[pid] = s.pdf;
-                            }
-                        }
+                        float pdf = 0.f;
+                        push = bsdf_continue(sp, wi_local, ng, wi_below, rng, thr, ior, next_o, next_d, pdf);
+                        if (nee && push) P.aux[pid] = pdf;
                     }
                 }
                 st_stream(P.state_a + pid, make_float4(thr.x, thr.y, thr.z, ior));
@@ -731,6 +749,62 @@ __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(MOD
             }
         }
         if (sort) __syncthreads(); // s_order is rewritten by the next tile
+    }
+}
+
+// Primary vertices of a chunk in MRT_MODE_BSDF, in place of k_pt_init and the bounce-0 shade:
+// the samples of a pixel share its primary ray and gbuffer hit (render_pt.slang:99-103), so a
+// thread shades the pixel's hit once and continues every sample of the chunk from it.
+// Results per path are those of k_pt_init + k_shade<MRT_MODE_BSDF> at bounce 0.
+__global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(2))) void k_shade_primary(
+    DeviceScene S, PtArgs A, const mrt_packed_hit* __restrict__ gbuffer, PathBuffers P, float4* __restrict__ no,
+    float4* __restrict__ nd, uint32_t* __restrict__ count_out) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t lane = lane_id();
+    const bool last = A.max_path_length <= 1;
+    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < A.npix; base += stride) {
+        const uint32_t p = base + lane;
+        const bool valid = p < A.npix;
+        uint32_t x = 0, y = 0, gid = TRAVERSAL_DONE;
+        ShadingPoint sp;
+        sp.has_material = false;
+        sp.emission = f3{0.f, 0.f, 0.f};
+        f3 wi_local{0.f, 0.f, 1.f}, ng{0.f, 0.f, 1.f};
+        bool wi_below = false;
+        if (valid) {
+            x = p % A.width;
+            y = p / A.width;
+            const RayF r = camera_ray(A.cam, x, y, A.width, A.height);
+            float bu, bv;
+            gid = gbuffer_hit(S, gbuffer, p, bu, bv);
+            shading_point(S, r.o, r.d, gid, bu, bv, 1.f, true, sp);
+            wi_local = sp.frame.to_local(-r.d);
+            ng = decode_normal(sp.enc_face_normal);
+            wi_below = dot(-r.d, ng) < 0.f;
+        }
+        const f3 one{1.f, 1.f, 1.f};
+        const f3 rad = A.emission_on_primary ? one * sp.emission : f3{0.f, 0.f, 0.f};
+        const bool cont = valid && !last && sp.has_material;
+        for (uint32_t sl = 0; sl < A.chunk; sl++) {
+            const uint32_t pid = sl * A.npix + p;
+            bool push = false;
+            f3 next_o{0.f, 0.f, 0.f}, next_d{0.f, 0.f, 0.f};
+            if (valid) {
+                const uint32_t s = A.s0 + sl;
+                PCG rng{pcg4d16(x, y, A.frame, s)};
+                f3 thr = one;
+                float ior = 1.f, pdf = 0.f;
+                if (cont) push = bsdf_continue(sp, wi_local, ng, wi_below, rng, thr, ior, next_o, next_d, pdf);
+                st_stream(P.state_a + pid, make_float4(thr.x, thr.y, thr.z, ior));
+                st_stream(P.state_b + pid, make_float4(rad.x, rad.y, rad.z, __uint_as_float(rng.state)));
+                if (A.record_signature) P.sig[pid] = sig_step(0x9E3779B9u * (s + 1u), gid);
+            }
+            const uint32_t k = wave_append(push, count_out);
+            if (push) {
+                st_stream(no + k, make_float4(next_o.x, next_o.y, next_o.z, __uint_as_float(pid)));
+                st_stream(nd + k, make_float4(next_d.x, next_d.y, next_d.z, 0.f));
+            }
+        }
     }
 }
 
@@ -959,9 +1033,19 @@ hipError_t launch_pt_frame(const DeviceScene& s, const TraceScratch& ts, PtArgs 
         uint32_t* work = pb.counters + nz + c * per_chunk;
         uint32_t* scnt = pb.counters + 2 * nz + c * per_chunk;
         uint32_t* swork = pb.counters + 3 * nz + c * per_chunk;
-        hipLaunchKernelGGL(k_pt_init, dim3(grid_for(a.n_paths, 256, MAX_BLOCKS)), dim3(256), 0, st, s, a, gbuffer,
-                           pb, cnt);
-        for (int b = 0; b < L && err == hipSuccess; b++) {
+        // BSDF mode shades the primary vertices per pixel (k_shade_primary); the other modes
+        // seed the path queue and shade bounce 0 like every other bounce
+        const bool fused_primary = a.mode == MRT_MODE_BSDF;
+        if (fused_primary) {
+            mark(-1);
+            hipLaunchKernelGGL(k_shade_primary, dim3(grid_for(a.npix, SHADE_BLOCK, MAX_BLOCKS)), dim3(SHADE_BLOCK), 0,
+                               st, s, a, gbuffer, pb, pb.q_o[1], pb.q_d[1], cnt + 1);
+            mark(1);
+        } else {
+            hipLaunchKernelGGL(k_pt_init, dim3(grid_for(a.n_paths, 256, MAX_BLOCKS)), dim3(256), 0, st, s, a, gbuffer,
+                               pb, cnt);
+        }
+        for (int b = fused_primary ? 1 : 0; b < L && err == hipSuccess; b++) {
             const int qi = b & 1;
             if (b > 0) {
                 mark(-1);
