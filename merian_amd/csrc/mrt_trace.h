@@ -24,11 +24,8 @@ struct DeviceBVH {
 
 // Textbook Moller-Trumbore (1997) in a fixed operation order; no contraction (the file is
 // compiled with -ffp-contract=off), so CPU and GPU round identically.
-__device__ __forceinline__ void test_triangle(const float4* __restrict__ tris, uint32_t rec, f3 o, f3 d, float tmin,
+__device__ __forceinline__ void test_triangle(const float4 A, const float4 B, const float4 Cc, f3 o, f3 d, float tmin,
                                               float tmax, uint32_t ray_mask, HitRec& best) {
-    const float4 A = tris[3 * rec + 0];
-    const float4 B = tris[3 * rec + 1];
-    const float4 Cc = tris[3 * rec + 2];
     const uint32_t flags = __float_as_uint(B.w);
     if (((flags >> 8) & ray_mask) == 0u) return;
     const f3 e1{B.x, B.y, B.z}, e2{Cc.x, Cc.y, Cc.z};
@@ -141,7 +138,14 @@ __device__ __forceinline__ void leaf_test(const float4* __restrict__ tris, RaySt
                                           TraversalCount& count) {
     const uint32_t first = (r.leaf & 0x7FFFFFFFu) >> 3;
     const uint32_t cnt = (r.leaf & 7u) + 1u;
-    for (uint32_t k = 0; k < cnt; k++) test_triangle(tris, first + k, r.o, r.d, r.tmin, r.tmax, ray_mask, r.best);
+    // two records per round, both fetched before either is tested (one memory round trip)
+    for (uint32_t k = 0; k < cnt; k += 2) {
+        const uint32_t a = first + k, b = first + min(k + 1u, cnt - 1u);
+        const float4 A0 = tris[3 * a], B0 = tris[3 * a + 1], C0 = tris[3 * a + 2];
+        const float4 A1 = tris[3 * b], B1 = tris[3 * b + 1], C1 = tris[3 * b + 2];
+        test_triangle(A0, B0, C0, r.o, r.d, r.tmin, r.tmax, ray_mask, r.best);
+        if (k + 1u < cnt) test_triangle(A1, B1, C1, r.o, r.d, r.tmin, r.tmax, ray_mask, r.best);
+    }
     if constexpr (COUNT) count.tris += cnt;
     r.leaf = TRAVERSAL_DONE;
     r.tlim = fminf(r.tmax, r.best.t) * 1.00000024f;
