@@ -514,6 +514,50 @@ __device__ __forceinline__ uint32_t wave_append(bool push, uint32_t* counter) {
     return slot + uint32_t(__popcll(ballot & ((1ull << lane) - 1ull)));
 }
 
+// Block-level append of continuation rays: lanes stage their ray in LDS (one LDS atomic per
+// wave), then the block reserves its range of the queue with one global atomic and copies the
+// staged rays out contiguously. One global atomic per SHADE_TILE rays instead of one per wave
+// keeps the queue counter (a single address every block of the grid appends to) off the
+// critical path.
+struct RayStage {
+    float4* o;      // SHADE_TILE entries each
+    float4* d;
+    uint32_t* meta; // [0] staged count, [1] reserved queue base, [2] count being flushed
+};
+
+__device__ __forceinline__ void stage_push(const RayStage& st, bool push, float4 o, float4 d) {
+    const uint64_t b = __ballot(push);
+    if (!b) return;
+    const uint32_t lane = lane_id();
+    const uint32_t leader = uint32_t(__ffsll((unsigned long long)b) - 1);
+    uint32_t slot = 0;
+    if (lane == leader) slot = atomicAdd(st.meta, uint32_t(__popcll(b)));
+    slot = __shfl(slot, int(leader)) + uint32_t(__popcll(b & ((1ull << lane) - 1ull)));
+    if (push) {
+        st.o[slot] = o;
+        st.d[slot] = d;
+    }
+}
+
+// Every thread of the block calls this; at most SHADE_TILE rays may be staged between calls.
+__device__ __forceinline__ void stage_flush(const RayStage& st, float4* __restrict__ no, float4* __restrict__ nd,
+                                            uint32_t* __restrict__ count_out) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t n = st.meta[0];
+        st.meta[2] = n;
+        st.meta[1] = n ? atomicAdd(count_out, n) : 0u;
+        st.meta[0] = 0u;
+    }
+    __syncthreads();
+    const uint32_t n = st.meta[2], base = st.meta[1];
+    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
+        st_stream(no + base + k, st.o[k]);
+        st_stream(nd + base + k, st.d[k]);
+    }
+    __syncthreads();
+}
+
 // Groups the tile's paths [tile, tile + SHADE_TILE) by shading class (counting sort in LDS,
 // stable within a class): order[k] = queue slot of the k-th path in class order; slots past
 // the queue end sort last. Ends with the block synchronised.
@@ -607,6 +651,11 @@ __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(MOD
     __shared__ uint32_t s_order[SHADE_TILE];
     __shared__ uint32_t s_cnt[SHADE_ROUNDS * (SHADE_BLOCK / 64) * (SHADE_CLASSES + 1)];
     __shared__ uint32_t s_total[SHADE_CLASSES + 1];
+    __shared__ float4 s_stage_o[SHADE_TILE], s_stage_d[SHADE_TILE];
+    __shared__ uint32_t s_meta[3];
+    const RayStage stage{s_stage_o, s_stage_d, s_meta};
+    if (threadIdx.x == 0) s_meta[0] = 0u;
+    __syncthreads();
     const uint32_t n = *count_in;
     const bool last = bounce + 1 >= A.max_path_length;
     constexpr bool nee = MODE == MRT_MODE_NEE;
@@ -723,11 +772,8 @@ __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(MOD
                 st_stream(P.state_a + pid, make_float4(thr.x, thr.y, thr.z, ior));
                 st_stream(P.state_b + pid, make_float4(rad.x, rad.y, rad.z, __uint_as_float(rng.state)));
             }
-            const uint32_t k = wave_append(push, count_out);
-            if (push) {
-                st_stream(no + k, make_float4(next_o.x, next_o.y, next_o.z, __uint_as_float(pid)));
-                st_stream(nd + k, make_float4(next_d.x, next_d.y, next_d.z, 0.f));
-            }
+            stage_push(stage, push, make_float4(next_o.x, next_o.y, next_o.z, __uint_as_float(pid)),
+                       make_float4(next_d.x, next_d.y, next_d.z, 0.f));
             if (guided) {
                 const uint32_t r = wave_append(record, P.grec_count);
                 if (record && r < P.grec_cap) {
@@ -748,7 +794,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(MOD
                 }
             }
         }
-        if (sort) __syncthreads(); // s_order is rewritten by the next tile
+        stage_flush(stage, no, nd, count_out); // also keeps s_order intact until every round read it
     }
 }
 
@@ -759,11 +805,15 @@ __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(MOD
 __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(2))) void k_shade_primary(
     DeviceScene S, PtArgs A, const mrt_packed_hit* __restrict__ gbuffer, PathBuffers P, float4* __restrict__ no,
     float4* __restrict__ nd, uint32_t* __restrict__ count_out) {
+    __shared__ float4 s_stage_o[SHADE_TILE], s_stage_d[SHADE_TILE];
+    __shared__ uint32_t s_meta[3];
+    const RayStage stage{s_stage_o, s_stage_d, s_meta};
+    if (threadIdx.x == 0) s_meta[0] = 0u;
+    __syncthreads();
     const uint32_t stride = gridDim.x * blockDim.x;
-    const uint32_t lane = lane_id();
     const bool last = A.max_path_length <= 1;
-    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < A.npix; base += stride) {
-        const uint32_t p = base + lane;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < A.npix; base += stride) {
+        const uint32_t p = base + threadIdx.x;
         const bool valid = p < A.npix;
         uint32_t x = 0, y = 0, gid = TRAVERSAL_DONE;
         ShadingPoint sp;
@@ -799,11 +849,10 @@ __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(2))
                 st_stream(P.state_b + pid, make_float4(rad.x, rad.y, rad.z, __uint_as_float(rng.state)));
                 if (A.record_signature) P.sig[pid] = sig_step(0x9E3779B9u * (s + 1u), gid);
             }
-            const uint32_t k = wave_append(push, count_out);
-            if (push) {
-                st_stream(no + k, make_float4(next_o.x, next_o.y, next_o.z, __uint_as_float(pid)));
-                st_stream(nd + k, make_float4(next_d.x, next_d.y, next_d.z, 0.f));
-            }
+            stage_push(stage, push, make_float4(next_o.x, next_o.y, next_o.z, __uint_as_float(pid)),
+                       make_float4(next_d.x, next_d.y, next_d.z, 0.f));
+            // the block stages up to SHADE_BLOCK rays per sample: flush every SHADE_ROUNDS samples
+            if ((sl + 1) % SHADE_ROUNDS == 0 || sl + 1 == A.chunk) stage_flush(stage, no, nd, count_out);
         }
     }
 }

@@ -9,6 +9,11 @@ WRITE_SIZE counters) and summaries of rocprofv3's CSV output.
         per-kernel mean of COUNTER per dispatch from DIR's *_counter_collection.csv
     python -m merian_amd.profiling traffic FETCH_DIR WRITE_DIR COPY_FETCH_DIR COPY_WRITE_DIR KERNEL
         calibrated HBM bytes per dispatch of KERNEL (JSON)
+    python -m merian_amd.profiling valu DIR
+        per-kernel VALU busy, lane utilisation and issue time from a pass with GRBM_GUI_ACTIVE,
+        SQ_INSTS_VALU, SQ_ACTIVE_INST_VALU and SQ_THREAD_CYCLES_VALU
+    python -m merian_amd.profiling l2 DIR
+        per-kernel L2 hit rate from a TCC_HIT_sum / TCC_MISS_sum pass
 """
 from __future__ import annotations
 
@@ -95,6 +100,57 @@ def traffic(fetch_dir: str, write_dir: str, copy_fetch_dir: str, copy_write_dir:
             "traffic_bytes_per_dispatch": round(f_kib * 1024.0 * cf + w_kib * 1024.0 * cw)}
 
 
+def agent(d: str) -> dict:
+    """The GPU agent's row of rocprofv3's agent info (CU, SIMD and XCC counts, engine clock)."""
+    with open(_one(os.path.join(d, "**", "*agent_info.csv"))) as f:
+        for r in csv.DictReader(f):
+            if r.get("Agent_Type", "").upper().find("GPU") >= 0 or r.get("Name", "").startswith("gfx"):
+                return r
+    raise ValueError(f"no GPU agent in {d}")
+
+
+def _sums(d: str) -> tuple[dict, dict]:
+    """Per kernel: counter sums over its dispatches, and (dispatches, summed duration in ms)."""
+    acc = defaultdict(lambda: defaultdict(float))
+    disp = defaultdict(dict)
+    with open(_one(os.path.join(d, "**", "*counter_collection.csv"))) as f:
+        for r in csv.DictReader(f):
+            k = short_name(r["Kernel_Name"])
+            acc[k][r["Counter_Name"]] += float(r["Counter_Value"])
+            disp[k][r["Dispatch_Id"]] = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6
+    return acc, {k: (len(v), sum(v.values())) for k, v in disp.items()}
+
+
+def valu(d: str) -> dict:
+    """VALU figures per kernel. busy = active VALU cycles per CU over the GPU's busy cycles
+    (GRBM_GUI_ACTIVE summed over the XCDs, so divided by their number); lane utilisation =
+    thread-cycles / (64 x active cycles); issue_ms = wave64 VALU instructions x 4 cycles over
+    all SIMDs at the peak engine clock."""
+    a = agent(d)
+    cus, simds, xcc = int(a["Cu_Count"]), int(a["Simd_Count"]), int(a["Num_Xcc"])
+    clk_hz = float(a["Max_Engine_Clk_Fcompute"]) * 1e6
+    acc, disp = _sums(d)
+    out = {}
+    for k, c in acc.items():
+        n, ms = disp[k]
+        if c.get("SQ_ACTIVE_INST_VALU", 0) <= 0 or c.get("GRBM_GUI_ACTIVE", 0) <= 0:
+            continue
+        gui = c["GRBM_GUI_ACTIVE"] / xcc
+        out[k] = {"dispatches": n, "avg_ms": round(ms / n, 4),
+                  "valu_insts_per_dispatch": round(c.get("SQ_INSTS_VALU", 0) / n),
+                  "valu_busy_pct": round(100 * c["SQ_ACTIVE_INST_VALU"] / cus / gui, 1),
+                  "valu_lane_util_pct": round(100 * c.get("SQ_THREAD_CYCLES_VALU", 0) / (64 * c["SQ_ACTIVE_INST_VALU"]), 1),
+                  "valu_issue_ms": round(c.get("SQ_INSTS_VALU", 0) / n * 4 / simds / clk_hz * 1e3, 4)}
+    return out
+
+
+def l2(d: str) -> dict:
+    """L2 hit rate per kernel (TCC_HIT / (TCC_HIT + TCC_MISS))."""
+    acc, disp = _sums(d)
+    return {k: {"dispatches": disp[k][0], "l2_hit_pct": round(100 * c["TCC_HIT_sum"] / (c["TCC_HIT_sum"] + c["TCC_MISS_sum"]), 1)}
+            for k, c in acc.items() if c.get("TCC_HIT_sum", 0) + c.get("TCC_MISS_sum", 0) > 0}
+
+
 def copy_workload(mib: int, reps: int) -> None:
     import torch
     n = mib << 20
@@ -123,12 +179,22 @@ def main(argv=None) -> int:
     for a in ("fetch_dir", "write_dir", "copy_fetch_dir", "copy_write_dir", "kernel"):
         t.add_argument(a)
     t.add_argument("--copy-mib", type=int, default=1024)
+    v = sub.add_parser("valu")
+    v.add_argument("dir")
+    l = sub.add_parser("l2")
+    l.add_argument("dir")
     args = ap.parse_args(argv)
     if args.cmd == "copy":
         copy_workload(args.mib, args.reps)
     elif args.cmd == "traffic":
         print(json.dumps(traffic(args.fetch_dir, args.write_dir, args.copy_fetch_dir, args.copy_write_dir,
                                  args.kernel, args.copy_mib << 20), indent=1))
+    elif args.cmd in ("valu", "l2"):
+        rows = valu(args.dir) if args.cmd == "valu" else l2(args.dir)
+        keys = list(next(iter(rows.values())).keys()) if rows else []
+        print("| kernel | " + " | ".join(keys) + " |\n|---|" + "---:|" * len(keys))
+        for k, r in sorted(rows.items(), key=lambda kv: -kv[1]["dispatches"] * kv[1].get("avg_ms", 1)):
+            print(f"| {k} | " + " | ".join(str(r[x]) for x in keys) + " |")
     elif args.cmd == "stats":
         print("| kernel | calls | total ms | avg ms | % |\n|---|---:|---:|---:|---:|")
         for r in kernel_stats(args.dir):
