@@ -502,18 +502,6 @@ __global__ void k_pt_init(DeviceScene S, PtArgs A, const mrt_packed_hit* __restr
     }
 }
 
-// Appends one entry per pushing lane with one atomic per wave; returns the lane's slot.
-__device__ __forceinline__ uint32_t wave_append(bool push, uint32_t* counter) {
-    const uint64_t ballot = __ballot(push);
-    if (!ballot) return 0u;
-    const uint32_t lane = __lane_id();
-    const uint32_t leader = __ffsll((unsigned long long)ballot) - 1;
-    uint32_t slot = 0;
-    if (lane == leader) slot = atomicAdd(counter, uint32_t(__popcll(ballot)));
-    slot = __shfl(slot, leader);
-    return slot + uint32_t(__popcll(ballot & ((1ull << lane) - 1ull)));
-}
-
 // Block-level append of continuation rays: lanes stage their ray in LDS (one LDS atomic per
 // wave), then the block reserves its range of the queue with one global atomic and copies the
 // staged rays out contiguously. One global atomic per SHADE_TILE rays instead of one per wave
@@ -537,6 +525,26 @@ __device__ __forceinline__ void stage_push(const RayStage& st, bool push, float4
         st.o[slot] = o;
         st.d[slot] = d;
     }
+}
+
+// Block-wide append without staging: slots within the block from an LDS counter, the block's
+// range from one global atomic. Every thread of the block calls this (it synchronises the
+// block twice); `scratch` is two LDS words.
+__device__ __forceinline__ uint32_t block_append(bool push, uint32_t* __restrict__ counter, uint32_t* scratch) {
+    const uint64_t b = __ballot(push);
+    const uint32_t lane = lane_id();
+    const uint32_t leader = b ? uint32_t(__ffsll((unsigned long long)b) - 1) : 0u;
+    uint32_t off = 0;
+    if (b && lane == leader) off = atomicAdd(scratch, uint32_t(__popcll(b)));
+    off = __shfl(off, int(leader)) + uint32_t(__popcll(b & ((1ull << lane) - 1ull)));
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t n = scratch[0];
+        scratch[1] = n ? atomicAdd(counter, n) : 0u;
+        scratch[0] = 0u;
+    }
+    __syncthreads();
+    return scratch[1] + off;
 }
 
 // Every thread of the block calls this; at most SHADE_TILE rays may be staged between calls.
@@ -653,8 +661,12 @@ __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(MOD
     __shared__ uint32_t s_total[SHADE_CLASSES + 1];
     __shared__ float4 s_stage_o[SHADE_TILE], s_stage_d[SHADE_TILE];
     __shared__ uint32_t s_meta[3];
+    __shared__ uint32_t s_app[2]; // block_append scratch (records, shadow rays)
     const RayStage stage{s_stage_o, s_stage_d, s_meta};
-    if (threadIdx.x == 0) s_meta[0] = 0u;
+    if (threadIdx.x == 0) {
+        s_meta[0] = 0u;
+        s_app[0] = 0u;
+    }
     __syncthreads();
     const uint32_t n = *count_in;
     const bool last = bounce + 1 >= A.max_path_length;
@@ -775,7 +787,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(MOD
             stage_push(stage, push, make_float4(next_o.x, next_o.y, next_o.z, __uint_as_float(pid)),
                        make_float4(next_d.x, next_d.y, next_d.z, 0.f));
             if (guided) {
-                const uint32_t r = wave_append(record, P.grec_count);
+                const uint32_t r = block_append(record, P.grec_count, s_app);
                 if (record && r < P.grec_cap) {
                     float4* g = P.grec + 5 * size_t(r);
                     g[0] = make_float4(rec_pos.x, rec_pos.y, rec_pos.z, __uint_as_float(pid));
@@ -786,7 +798,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(MOD
                 }
             }
             if (nee) {
-                const uint32_t q = wave_append(shadow, shadow_count);
+                const uint32_t q = block_append(shadow, shadow_count, s_app);
                 if (shadow) {
                     st_stream(P.sq_o + q, make_float4(sh_o.x, sh_o.y, sh_o.z, __uint_as_float(pid)));
                     st_stream(P.sq_d + q, make_float4(sh_d.x, sh_d.y, sh_d.z, 0.f));
