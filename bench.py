@@ -265,21 +265,29 @@ def main():
     bytes_per_launch = bytes_per_seg * cont_timed / max(1, launches)
     avg_ms = ms_trace / max(1, launches)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    traffic = None
+    traffic, prof = None, {}
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
             tj = json.load(f)
         # the profile holds for the tuning it was taken with
         prof_t = tj.get("tuning")
-        if prof_t is None or tuning is None or all(abs(float(prof_t[k]) - float(tuning[k])) < 1e-6 for k in prof_t):
-            traffic = tj["traffic_bytes_per_dispatch"] if (prof_t is None) == (tuning is None) else None
+        same = (prof_t is None) == (tuning is None) and (
+            prof_t is None or all(k in tuning and abs(float(prof_t[k]) - float(tuning[k])) < 1e-6 for k in prof_t))
+        if same:
+            traffic = tj["traffic_bytes_per_dispatch"]
+            # measured HBM rate of the same kernel, and its issue figures, from the same profile
+            prof = {"hbm_gbs": round(traffic / (avg_ms * 1e-3) / 1e9, 1),
+                    "hbm_frac": round(traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                    "l2_hit_pct": tj.get("l2_hit_pct"), "valu_busy_pct": tj.get("valu_busy_pct"),
+                    "valu_lane_util_pct": tj.get("valu_lane_util_pct"),
+                    "profile": os.path.relpath(TRAFFIC_FILE, ROOT)}
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "algorithmic_bytes_per_launch": round(bytes_per_launch),
             "kernel": "k_trace_queue", "avg_launch_ms": round(avg_ms, 4), "launches": launches,
             "bytes_per_segment": round(bytes_per_seg, 1), "node_bytes": node_bytes,
             "node_visits_per_segment": round(st_c.node_visits / cont, 2),
-            "triangle_tests_per_segment": round(st_c.triangle_tests / cont, 2)}
+            "triangle_tests_per_segment": round(st_c.triangle_tests / cont, 2), **prof}
 
     line = {"metric": METRIC, "value": round(value, 2), "unit": "Mpath-segments/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1e3 / args.steps, 2),
