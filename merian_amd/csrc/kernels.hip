@@ -17,7 +17,6 @@ namespace mrt {
 
 namespace {
 
-constexpr int TRACE_BLOCK = 128;
 constexpr int SHADE_BLOCK = 256;
 constexpr int SHADE_ROUNDS = 4;                          // paths per thread per tile
 constexpr int SHADE_TILE = SHADE_BLOCK * SHADE_ROUNDS;   // paths a block groups by class at once
@@ -186,7 +185,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DeviceScene S, Src src, S
     __shared__ uint32_t stack[TRACE_LDS_STACK * TRACE_BLOCK];
     const uint32_t n = count_ptr ? *count_ptr : n_fixed;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    const LaneStack stk{stack + threadIdx.x, blockDim.x, ts.ovf + gtid, gridDim.x * blockDim.x};
+    const LaneStack stk{stack + threadIdx.x, ts.ovf + gtid, gridDim.x * blockDim.x};
     TraversalCount cnt;
     if constexpr (W == 8) {
         trace_persistent<TRACE_LDS_STACK, COUNT, Src, Sink, ANY>(DeviceBVH8{S.nodes8, S.tris}, src, sink, n, work,

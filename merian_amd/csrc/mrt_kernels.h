@@ -46,6 +46,8 @@ enum : uint32_t {
 
 // Entries of the per-lane traversal stack kept in LDS; deeper entries go to TraceScratch.ovf.
 constexpr int TRACE_LDS_STACK = 16;
+// Threads per block of the trace kernels (also the LDS stack's column stride).
+constexpr int TRACE_BLOCK = 128;
 
 // Per-context scratch of the trace kernels: the global part of the traversal stacks (`entries`
 // per thread, `threads` columns) and a ray cursor for launches that do not bring their own.

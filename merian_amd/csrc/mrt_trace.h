@@ -9,6 +9,7 @@
 
 #include "bvh.h"
 #include "mrt_device.h"
+#include "mrt_kernels.h"
 
 namespace mrt {
 
@@ -69,15 +70,15 @@ struct TraversalCount {
 // Traversal stack of one lane: the first SL entries in an LDS column (stride = threads per
 // block, so a wave's 64 lanes touch 64 distinct banks), deeper entries in a per-lane column of a
 // global overflow area (stride = threads of the grid). Near-first traversal rarely goes past SL.
+// The LDS column stride is the block size, a compile-time constant (a shift, not a multiply).
 struct LaneStack {
-    uint32_t* lds;
-    uint32_t lds_stride;
+    uint32_t* lds; // this lane's column
     uint32_t* ovf;
     uint32_t ovf_stride;
 };
 template <int SL> __device__ __forceinline__ void stack_push(const LaneStack& s, int& sp, uint32_t e) {
     if (sp < SL) {
-        s.lds[sp * s.lds_stride] = e;
+        s.lds[sp * TRACE_BLOCK] = e;
     } else {
         s.ovf[size_t(sp - SL) * s.ovf_stride] = e;
     }
@@ -85,7 +86,12 @@ template <int SL> __device__ __forceinline__ void stack_push(const LaneStack& s,
 }
 template <int SL> __device__ __forceinline__ uint32_t stack_pop(const LaneStack& s, int& sp) {
     --sp;
-    return sp < SL ? s.lds[sp * s.lds_stride] : s.ovf[size_t(sp - SL) * s.ovf_stride];
+    // explicit address spaces keep the two loads apart (a ds_read and a global load rather
+    // than one flat load through a selected pointer)
+    typedef const __attribute__((address_space(3))) uint32_t* lds_ptr;
+    typedef const __attribute__((address_space(1))) uint32_t* global_ptr;
+    if (sp < SL) return ((lds_ptr)s.lds)[sp * TRACE_BLOCK];
+    return ((global_ptr)s.ovf)[size_t(sp - SL) * s.ovf_stride];
 }
 
 // Per-lane traversal state between steps.
