@@ -182,7 +182,7 @@ def main():
     ap.add_argument("--max-path-length", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--tune-ms", type=float, default=15000.0, help="autotuning budget of --retune")
+    ap.add_argument("--tune-ms", type=float, default=60000.0, help="autotuning budget of --retune")
     ap.add_argument("--retune", action="store_true",
                     help="search the tuning again instead of applying the cached autotune result")
     ap.add_argument("--default-tuning", action="store_true", help="render with the untuned defaults")

@@ -102,7 +102,7 @@ bool valid(const mrt_tuning& t) {
 // Device time of one trial frame (gbuffer + pt) in ms, after one untimed frame.
 mrt_status trial(mrt_ctx* c, mrt_scene* s, const mrt_tune_request& r, float& ms) {
     mrt_pt_params p = r.params;
-    p.spp = std::min(std::max(p.spp, 1), 8);
+    p.spp = std::min(std::max(p.spp, 1), 64); // whole passes of the real frame up to 64 spp
     p.record_signature = 0;
     p.count_traversal = 0;
     const uint64_t npix = uint64_t(r.width) * r.height;
