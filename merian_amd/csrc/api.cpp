@@ -67,7 +67,7 @@ hipStream_t pick_stream(mrt_ctx* c, void* stream) { return stream ? static_cast<
 
 void free_path_buffers(mrt_ctx* c) {
     PathBuffers& p = c->paths;
-    for (void* q : {(void*)p.state_a, (void*)p.state_b, (void*)p.sig, (void*)p.q_o[0], (void*)p.q_o[1], (void*)p.q_d[0],
+    for (void* q : {(void*)p.state, (void*)p.sig, (void*)p.q_o[0], (void*)p.q_o[1], (void*)p.q_d[0],
                     (void*)p.q_d[1], (void*)p.hits, (void*)p.aux, (void*)p.sq_o, (void*)p.sq_d, (void*)p.sq_c,
                     (void*)p.counters, (void*)p.accum, (void*)p.sig_accum})
         dev_free(c, q);
@@ -86,8 +86,7 @@ mrt_status ensure_path_buffers(mrt_ctx* c, uint64_t paths, uint64_t npix, uint64
     free_path_buffers(c);
     mrt_status st = MRT_OK;
     auto A = [&](size_t bytes) { return st == MRT_OK ? dev_alloc(c, bytes, &st) : nullptr; };
-    p.state_a = static_cast<float4*>(A(16 * paths));
-    p.state_b = static_cast<float4*>(A(16 * paths));
+    p.state = static_cast<float4*>(A(32 * paths));
     p.sig = static_cast<uint32_t*>(A(4 * paths));
     for (int k = 0; k < 2; k++) {
         p.q_o[k] = static_cast<float4*>(A(16 * paths));

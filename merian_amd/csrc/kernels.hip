@@ -142,16 +142,16 @@ struct SrcShadow { // NEE shadow queue: (origin, path id), (direction, -); [0, 1
 struct SinkShadow { // unoccluded: add the queued contribution to the path's radiance
     const float4* so;
     const float4* sc;
-    float4* state_b;
+    float4* state; // PathBuffers.state
     __device__ __forceinline__ void store(uint32_t i, const HitRec& h) const {
         if (h.id != TRAVERSAL_DONE) return;
         const uint32_t pid = __float_as_uint(so[i].w);
         const float4 c = ld_stream(sc + i);
-        float4 b = state_b[pid];
+        float4 b = state[2 * size_t(pid) + 1];
         b.x = b.x + c.x;
         b.y = b.y + c.y;
         b.z = b.z + c.z;
-        state_b[pid] = b;
+        state[2 * size_t(pid) + 1] = b;
     }
 };
 
@@ -494,8 +494,8 @@ __global__ void k_pt_init(DeviceScene S, PtArgs A, const mrt_packed_hit* __restr
         st_stream(P.q_o[0] + i, make_float4(r.o.x, r.o.y, r.o.z, __uint_as_float(i)));
         st_stream(P.q_d[0] + i, make_float4(r.d.x, r.d.y, r.d.z, 0.f));
         st_stream(P.hits + i, make_float4(0.f, bu, bv, __uint_as_float(gid)));
-        st_stream(P.state_a + i, make_float4(1.f, 1.f, 1.f, 1.f));
-        st_stream(P.state_b + i, make_float4(0.f, 0.f, 0.f, __uint_as_float(pcg4d16(x, y, A.frame, s))));
+        st_stream(P.state + 2 * size_t(i), make_float4(1.f, 1.f, 1.f, 1.f));
+        st_stream(P.state + 2 * size_t(i) + 1, make_float4(0.f, 0.f, 0.f, __uint_as_float(pcg4d16(x, y, A.frame, s))));
         if (A.record_signature) P.sig[i] = sig_step(0x9E3779B9u * (s + 1u), gid);
         if (A.mode != MRT_MODE_BSDF) P.aux[i] = 0.f;
     }
@@ -691,7 +691,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(MOD
                 pid = __float_as_uint(o4.w);
                 const f3 ro{o4.x, o4.y, o4.z}, rd{d4.x, d4.y, d4.z};
                 const uint32_t gid = __float_as_uint(h4.w);
-                float4 sa = ld_stream(P.state_a + pid), sb = ld_stream(P.state_b + pid);
+                float4 sa = ld_stream(P.state + 2 * size_t(pid)), sb = ld_stream(P.state + 2 * size_t(pid) + 1);
                 f3 thr{sa.x, sa.y, sa.z};
                 float ior = sa.w;
                 f3 rad{sb.x, sb.y, sb.z};
@@ -780,8 +780,8 @@ __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(MOD
                         if (nee && push) P.aux[pid] = pdf;
                     }
                 }
-                st_stream(P.state_a + pid, make_float4(thr.x, thr.y, thr.z, ior));
-                st_stream(P.state_b + pid, make_float4(rad.x, rad.y, rad.z, __uint_as_float(rng.state)));
+                st_stream(P.state + 2 * size_t(pid), make_float4(thr.x, thr.y, thr.z, ior));
+                st_stream(P.state + 2 * size_t(pid) + 1, make_float4(rad.x, rad.y, rad.z, __uint_as_float(rng.state)));
             }
             stage_push(stage, push, make_float4(next_o.x, next_o.y, next_o.z, __uint_as_float(pid)),
                        make_float4(next_d.x, next_d.y, next_d.z, 0.f));
@@ -856,8 +856,8 @@ __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(2))
                 f3 thr = one;
                 float ior = 1.f, pdf = 0.f;
                 if (cont) push = bsdf_continue(sp, wi_local, ng, wi_below, rng, thr, ior, next_o, next_d, pdf);
-                st_stream(P.state_a + pid, make_float4(thr.x, thr.y, thr.z, ior));
-                st_stream(P.state_b + pid, make_float4(rad.x, rad.y, rad.z, __uint_as_float(rng.state)));
+                st_stream(P.state + 2 * size_t(pid), make_float4(thr.x, thr.y, thr.z, ior));
+                st_stream(P.state + 2 * size_t(pid) + 1, make_float4(rad.x, rad.y, rad.z, __uint_as_float(rng.state)));
                 if (A.record_signature) P.sig[pid] = sig_step(0x9E3779B9u * (s + 1u), gid);
             }
             stage_push(stage, push, make_float4(next_o.x, next_o.y, next_o.z, __uint_as_float(pid)),
@@ -879,7 +879,7 @@ __global__ void k_guide_targets(PathBuffers P) {
     for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride) {
         const float4* g = P.grec + 5 * size_t(r);
         const float4 a = g[0], b = g[1], c = g[2], rb = g[3], th = g[4];
-        const float4 rf = P.state_b[__float_as_uint(a.w)];
+        const float4 rf = P.state[2 * size_t(__float_as_uint(a.w)) + 1];
         const f3 L{th.x > 0.f ? fmaxf(rf.x - rb.x, 0.f) / th.x : 0.f, th.y > 0.f ? fmaxf(rf.y - rb.y, 0.f) / th.y : 0.f,
                    th.z > 0.f ? fmaxf(rf.z - rb.z, 0.f) / th.z : 0.f};
         float w = b.w > 0.f ? yuv_luminance(L) / b.w : 0.f;
@@ -903,7 +903,7 @@ __global__ void k_accumulate(PtArgs A, PathBuffers P, int first_chunk, int last_
         float m2 = acc.w;
         for (uint32_t sl = 0; sl < A.chunk; sl++) {
             const uint32_t i = sl * A.npix + p;
-            const float4 b = ld_stream(P.state_b + i);
+            const float4 b = ld_stream(P.state + 2 * size_t(i) + 1);
             const f3 c{b.x, b.y, b.z};
             if (A.record_signature) sa += P.sig[i];
             if (finite3(c)) {
@@ -1142,7 +1142,7 @@ hipError_t launch_pt_frame(const DeviceScene& s, const TraceScratch& ts, PtArgs 
             if (nee && b + 1 < L && err == hipSuccess) {
                 mark(-1);
                 err = launch_trace<false, SrcShadow, SinkShadow, true>(
-                    s, ts, SrcShadow{pb.sq_o, pb.sq_d}, SinkShadow{pb.sq_o, pb.sq_c, pb.state_b}, scnt + b, a.n_paths,
+                    s, ts, SrcShadow{pb.sq_o, pb.sq_d}, SinkShadow{pb.sq_o, pb.sq_c, pb.state}, scnt + b, a.n_paths,
                     swork + b, a.instance_mask, nullptr, st);
                 mark(0);
             }

@@ -81,8 +81,8 @@ struct PtArgs {
 
 // Device buffers of one chunk of paths (structure of arrays).
 struct PathBuffers {
-    float4* state_a;   // throughput.xyz, current ior
-    float4* state_b;   // radiance.xyz, PCG state bits
+    float4* state;     // per path 32 B: [2 pid] = throughput.xyz, current ior; [2 pid + 1] = radiance.xyz,
+                       // PCG state bits (one 32-byte record, one cache line per gather)
     uint32_t* sig;     // hit-sequence hash per path
     float4* q_o[2];    // ray queue: origin.xyz, path index bits
     float4* q_d[2];    // ray queue: direction.xyz

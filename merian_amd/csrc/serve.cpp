@@ -152,8 +152,7 @@ hipError_t alloc_job(Slot& sl, const Job& j, const JobShape& sh) {
     const uint64_t P = l.paths, N = sh.npix;
     b.gbuf = static_cast<mrt_packed_hit*>(A(16 * N));
     b.out = static_cast<float4*>(A(16 * N));
-    b.pb.state_a = static_cast<float4*>(A(16 * P));
-    b.pb.state_b = static_cast<float4*>(A(16 * P));
+    b.pb.state = static_cast<float4*>(A(32 * P));
     b.pb.sig = static_cast<uint32_t*>(A(4 * P));
     for (int k = 0; k < 2; k++) {
         b.pb.q_o[k] = static_cast<float4*>(A(16 * P));

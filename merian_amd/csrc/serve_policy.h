@@ -110,7 +110,7 @@ inline JobLayout job_layout(const JobShape& s) {
     l.threads = uint32_t(std::min<uint64_t>(s.trace_threads, want));
     uint64_t b = 0;
     b += 2 * round256(16 * s.npix);               // gbuffer, irradiance
-    b += 2 * round256(16 * l.paths);              // state_a, state_b
+    b += round256(32 * l.paths);                  // path state
     b += round256(4 * l.paths);                   // signature
     b += 4 * round256(16 * l.paths);              // two ray queues (origin, direction)
     b += round256(16 * l.paths);                  // hits
