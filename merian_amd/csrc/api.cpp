@@ -321,7 +321,7 @@ MRT_API mrt_status mrt_scene_create(mrt_ctx* c, const mrt_scene_desc* d, mrt_sce
     std::vector<float> v0(3 * n_tris), e1(3 * n_tris), e2(3 * n_tris);
     std::vector<uint32_t> flags(n_tris), geom_first(std::max<uint32_t>(1, d->geometry_count));
     std::vector<uint32_t> tri_verts(4 * n_tris);
-    std::vector<uint8_t> tri_class(std::max<uint64_t>(1, n_tris));
+    std::vector<float> tri_shade(16 * std::max<uint64_t>(1, n_tris), 0.f);
     uint64_t gid = 0;
     for (uint32_t g = 0; g < d->geometry_count; g++) {
         const mrt_geometry& G = d->geometries[g];
@@ -336,7 +336,7 @@ MRT_API mrt_status mrt_scene_create(mrt_ctx* c, const mrt_scene_desc* d, mrt_sce
                   : M.metalness >= 0.5f              ? SHADE_CLASS_METAL
                                                      : SHADE_CLASS_DIELECTRIC;
         }
-        std::fill(tri_class.begin() + gid, tri_class.begin() + gid + G.primitive_count, cls);
+        const uint32_t geom_word = g | (uint32_t(cls) << TRI_SHADE_CLASS_SHIFT);
         for (uint32_t p = 0; p < G.primitive_count; p++, gid++) {
             uint32_t vi[3];
             for (int k = 0; k < 3; k++) {
@@ -365,6 +365,15 @@ MRT_API mrt_status mrt_scene_create(mrt_ctx* c, const mrt_scene_desc* d, mrt_sce
             tri_verts[4 * gid + 1] = vi[1];
             tri_verts[4 * gid + 2] = vi[2];
             tri_verts[4 * gid + 3] = g;
+            float* ts = &tri_shade[16 * gid];
+            for (int k = 0; k < 3; k++) {
+                const mrt_vertex& v = d->vertices[vi[k]];
+                std::memcpy(ts + 4 * k, v.position, 12);
+                std::memcpy(ts + 4 * k + 3, &v.encoded_normal, 4);
+                const uint32_t uv = uint32_t(v.uv[0]) | (uint32_t(v.uv[1]) << 16);
+                std::memcpy(ts + 12 + k, &uv, 4);
+            }
+            std::memcpy(ts + 15, &geom_word, 4);
         }
     }
     const BVH4 bvh = build_bvh4(v0.data(), e1.data(), e2.data(), flags.data(), uint32_t(n_tris), 0);
@@ -397,7 +406,7 @@ MRT_API mrt_status mrt_scene_create(mrt_ctx* c, const mrt_scene_desc* d, mrt_sce
     UP(bvh.nodes.data(), bvh.nodes.size() * sizeof(BVH4Node), nodes, const BVH4Node*)
     UP(bvh.tris.data(), bvh.tris.size() * sizeof(TriRecord), tris, const float4*)
     UP(tri_verts.data(), tri_verts.size() * 4, tri_verts, const uint4*)
-    UP(tri_class.data(), tri_class.size(), tri_class, const uint8_t*)
+    UP(tri_shade.data(), tri_shade.size() * 4, tri_shade, const float4*)
     UP(d->vertices, size_t(d->vertex_count) * sizeof(mrt_vertex), verts, const mrt_vertex*)
     UP(d->geometries, size_t(d->geometry_count) * sizeof(mrt_geometry), geoms, const mrt_geometry*)
     UP(geom_first.data(), geom_first.size() * 4, geom_first, const uint32_t*)

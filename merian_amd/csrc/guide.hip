@@ -154,13 +154,11 @@ struct QueueQueries {
             p = f3{0.f, 0.f, 0.f};
             return false;
         }
-        const uint4 tv = S.tri_verts[gid];
-        const mrt_vertex& a = S.verts[tv.x];
-        const mrt_vertex& b = S.verts[tv.y];
-        const mrt_vertex& c = S.verts[tv.z];
-        const f3 p0{a.position[0], a.position[1], a.position[2]};
-        const f3 p1{b.position[0], b.position[1], b.position[2]};
-        const f3 p2{c.position[0], c.position[1], c.position[2]};
+        const float4* t = S.tri_shade + 4 * size_t(gid); // (p0, n0), (p1, n1), (p2, n2), ...
+        const float4 a = t[0], b = t[1], c = t[2];
+        const f3 p0{a.x, a.y, a.z};
+        const f3 p1{b.x, b.y, b.z};
+        const f3 p2{c.x, c.y, c.z};
         p = p0 + h.y * (p1 - p0) + h.z * (p2 - p0);
         return true;
     }

@@ -377,15 +377,27 @@ struct ShadingPoint {
     MaterialSample mat;
 };
 
-__device__ __forceinline__ f3 load_pos(const mrt_vertex& v) { return {v.position[0], v.position[1], v.position[2]}; }
+__device__ __forceinline__ f3 xyz(float4 v) { return {v.x, v.y, v.z}; }
+__device__ __forceinline__ f2 uv_of(uint32_t bits) {
+    return {half_to_float(uint16_t(bits & 0xFFFFu)), half_to_float(uint16_t(bits >> 16))};
+}
 
+// Vertex data of a hit triangle from its 64-byte shading record (DeviceScene.tri_shade: one
+// gather per hit instead of the vertex ids and three vertex records); tangents, which only
+// geometries with HasTangents use, come from the vertex records.
 __device__ __forceinline__ void shading_point(const DeviceScene& S, f3 ro, f3 rd, uint32_t gid, float bu, float bv, float ior,
                               bool with_material, ShadingPoint& sp) {
-    uint4 tv = make_uint4(0, 0, 0, 0);
+    float4 r0{}, r1{}, r2{}, r3{};
+    uint32_t geom = 0;
     bool surface = gid != TRAVERSAL_DONE;
     if (surface) {
-        tv = S.tri_verts[gid];
-        surface = (S.geoms[tv.w].flags & MRT_GEOM_USE_ENV_MAP) == 0;
+        const float4* t = S.tri_shade + 4 * size_t(gid);
+        r0 = t[0];
+        r1 = t[1];
+        r2 = t[2];
+        r3 = t[3];
+        geom = __float_as_uint(r3.w) & TRI_SHADE_GEOM_MASK;
+        surface = (S.geoms[geom].flags & MRT_GEOM_USE_ENV_MAP) == 0;
     }
     if (!surface) {
         sp.pos = ro + rd * ENV_MAP_DISTANCE;
@@ -397,10 +409,9 @@ __device__ __forceinline__ void shading_point(const DeviceScene& S, f3 ro, f3 rd
         return;
     }
     sp.env = false;
-    const mrt_geometry G = S.geoms[tv.w];
-    const mrt_vertex v0 = S.verts[tv.x], v1 = S.verts[tv.y], v2 = S.verts[tv.z];
+    const mrt_geometry G = S.geoms[geom];
     const f3 b{(1.0f - bu) - bv, bu, bv};
-    const f3 p0 = load_pos(v0), p1 = load_pos(v1), p2 = load_pos(v2);
+    const f3 p0 = xyz(r0), p1 = xyz(r1), p2 = xyz(r2);
     const f3 pos = b.x * p0 + b.y * p1 + b.z * p2;
     const f3 e01 = p1 - p0, e02 = p2 - p0;
     f3 face_normal = normalize(cross(e01, e02));
@@ -409,14 +420,14 @@ __device__ __forceinline__ void shading_point(const DeviceScene& S, f3 ro, f3 rd
     if (G.flags & MRT_GEOM_FLAT_SHADING) {
         vn = face_normal;
     } else {
-        vn = normalize(b.x * decode_normal(v0.encoded_normal) + b.y * decode_normal(v1.encoded_normal) +
-                       b.z * decode_normal(v2.encoded_normal));
+        vn = normalize(b.x * decode_normal(__float_as_uint(r0.w)) + b.y * decode_normal(__float_as_uint(r1.w)) +
+                       b.z * decode_normal(__float_as_uint(r2.w)));
     }
-    const f2 uv0{half_to_float(v0.uv[0]), half_to_float(v0.uv[1])};
-    const f2 uv1{half_to_float(v1.uv[0]), half_to_float(v1.uv[1])};
-    const f2 uv2{half_to_float(v2.uv[0]), half_to_float(v2.uv[1])};
+    const f2 uv0 = uv_of(__float_as_uint(r3.x)), uv1 = uv_of(__float_as_uint(r3.y)), uv2 = uv_of(__float_as_uint(r3.z));
     Frame frame;
     if (G.flags & MRT_GEOM_HAS_TANGENTS) {
+        const uint4 tv = S.tri_verts[gid];
+        const mrt_vertex v0 = S.verts[tv.x], v1 = S.verts[tv.y], v2 = S.verts[tv.z];
         const float w0 = (v0.encoded_tangent & 1u) ? -1.f : 1.f;
         const float w1 = (v1.encoded_tangent & 1u) ? -1.f : 1.f;
         const float w2 = (v2.encoded_tangent & 1u) ? -1.f : 1.f;
@@ -580,7 +591,9 @@ __device__ void shade_sort_tile(const DeviceScene& S, const float4* __restrict__
         uint32_t c = SHADE_CLASSES;
         if (j < n) {
             const uint32_t gid = reinterpret_cast<const uint32_t*>(hits + j)[3];
-            c = gid == TRAVERSAL_DONE ? uint32_t(SHADE_CLASS_ENV) : uint32_t(S.tri_class[gid]);
+            // the shading record's last 16 bytes: also brings the line in for the shading pass
+            c = gid == TRAVERSAL_DONE ? uint32_t(SHADE_CLASS_ENV)
+                                      : __float_as_uint(S.tri_shade[4 * size_t(gid) + 3].w) >> TRI_SHADE_CLASS_SHIFT;
         }
         cls[r] = c;
         rank[r] = 0;

@@ -17,7 +17,9 @@ struct DeviceScene {
     uint32_t width;              // BVH node width in use: 4 or 8
     const float4* tris;          // TriRecord stream, leaf order
     const uint4* tri_verts;      // per global triangle id: 3 global vertex indices, geometry index
-    const uint8_t* tri_class;    // per global triangle id: shading class (SHADE_CLASS_*)
+    const float4* tri_shade;     // per global triangle id, 64 B: (p0, n0), (p1, n1), (p2, n2),
+                                 // (uv0, uv1, uv2, geometry | shading class << 24); n = oct normal
+                                 // bits, uv = 2 x half bits
     const mrt_vertex* verts;     // PackedVertexData
     const mrt_geometry* geoms;   // GeometryData
     const uint32_t* geom_first;  // global id of each geometry's first triangle
@@ -34,7 +36,7 @@ struct DeviceScene {
 };
 
 // Shading classes: the shade kernel groups a block's paths by class so each wave runs one
-// material code path. Hit triangles carry theirs (DeviceScene.tri_class); misses are ENV.
+// material code path. Hit triangles carry theirs (DeviceScene.tri_shade); misses are ENV.
 enum : uint32_t {
     SHADE_CLASS_ENV = 0,         // environment (a miss, or a UseEnvMap geometry)
     SHADE_CLASS_DIFFUSE = 1,     // Lambert
@@ -43,6 +45,8 @@ enum : uint32_t {
     SHADE_CLASS_TRANSMISSIVE = 4, // OpenPBR with transmission
     SHADE_CLASSES = 5
 };
+constexpr uint32_t TRI_SHADE_GEOM_MASK = 0xFFFFFFu;
+constexpr uint32_t TRI_SHADE_CLASS_SHIFT = 24;
 
 // Entries of the per-lane traversal stack kept in LDS; deeper entries go to TraceScratch.ovf.
 constexpr int TRACE_LDS_STACK = 16;
