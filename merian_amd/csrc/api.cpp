@@ -376,7 +376,9 @@ MRT_API mrt_status mrt_scene_create(mrt_ctx* c, const mrt_scene_desc* d, mrt_sce
             std::memcpy(ts + 15, &geom_word, 4);
         }
     }
-    const BVH4 bvh = build_bvh4(v0.data(), e1.data(), e2.data(), flags.data(), uint32_t(n_tris), 0);
+    const mrt_tuning defaults = mrt_scene{}.tuning;
+    const BVH4 bvh = build_bvh4(v0.data(), e1.data(), e2.data(), flags.data(), uint32_t(n_tris), 0,
+                                int(defaults.bvh_max_leaf), defaults.bvh_traversal_cost);
     mrt_scene* s = new mrt_scene();
     if (n_tris) {
         for (int k = 0; k < 3; k++) {

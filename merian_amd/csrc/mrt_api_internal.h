@@ -32,7 +32,8 @@ struct mrt_scene {
     // triangle operands kept on the host for BVH rebuilds (tuning)
     std::vector<float> h_v0, h_e1, h_e2;
     std::vector<uint32_t> h_flags;
-    mrt_tuning tuning{0, 16, 100, 4, 1.0f, 4, 0.f, 16, 0};
+    // defaults: what mrt_tune found best on the 1080p 2M-triangle workload, grid and paths aside
+    mrt_tuning tuning{0, 24, 100, 2, 1.0f, 4, 0.f, 16, 0};
 };
 
 namespace mrt {

@@ -60,7 +60,7 @@ struct TraceScratch {
     uint32_t threads;
     uint32_t entries;
     uint32_t* work;
-    uint32_t refill = 16;   // idle lanes before a wave refetches rays (1..64)
+    uint32_t refill = 24;   // idle lanes before a wave refetches rays (1..64)
     uint32_t grid_pct = 100; // persistent grid size, percent of the resident blocks
     uint32_t leaf_lanes = 16; // leaf-only lanes before a wave runs a leaf-test phase (1..64)
 };

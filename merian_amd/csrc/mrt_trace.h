@@ -291,9 +291,6 @@ __device__ __forceinline__ void node_step(const DeviceBVH8 bvh, RayState& r, con
     }
 }
 
-// Default refill threshold: lanes fetch new rays once at least this many of a wave's 64 lanes
-// are idle (one atomic per refill). The autotuner may choose another.
-constexpr int REFILL_LANES = 16;
 
 // Persistent closest-hit loop of one wave: every lane runs one traversal step per iteration and,
 // when its ray is done, hands the hit to `sink` and later takes a fresh ray from `src`, so lanes

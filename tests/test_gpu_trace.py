@@ -12,20 +12,19 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 
-# (BVH width, max leaf size): the default 4-wide tree and the 8-wide tree with quantized
-# child boxes, at leaf sizes the autotuner may choose.
-TREES = [(4, 4), (8, 2), (8, 8)]
+# (BVH width, max leaf size): the 4-wide tree and the 8-wide tree with quantized child boxes,
+# at leaf sizes the autotuner may choose.
+TREES = [(4, 4), (4, 2), (8, 2), (8, 8)]
 
 
 def _tris(ctx, pos, idx, tree, two_sided=True):
     ds = ctx.triangles(pos, idx, two_sided=two_sided)
     width, leaf = tree
-    if (width, leaf) != (4, 4):
-        t = ds.tuning()
-        t.bvh_width, t.bvh_max_leaf = width, leaf
-        ds.set_tuning(t)
-        got = ds.tuning()
-        assert (got.bvh_width, got.bvh_max_leaf) == (width, leaf)
+    t = ds.tuning()
+    t.bvh_width, t.bvh_max_leaf = width, leaf
+    ds.set_tuning(t)
+    got = ds.tuning()
+    assert (got.bvh_width, got.bvh_max_leaf) == (width, leaf)
     return ds
 
 
