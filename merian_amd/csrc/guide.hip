@@ -143,8 +143,9 @@ __device__ __forceinline__ void relu_to_operand(const floatx4 (&acc)[NT][4], hal
 struct QueueQueries {
     DeviceScene S;
     const float4 *qo, *qd, *hits;
-    const uint32_t* count;
-    __device__ __forceinline__ uint32_t n() const { return *count; }
+    const uint32_t* count; // queue size on the device; null: n_fixed
+    uint32_t n_fixed;
+    __device__ __forceinline__ uint32_t n() const { return count ? *count : n_fixed; }
     __device__ __forceinline__ bool load(uint32_t i, f3& p, f3& wo) const {
         const float4 h = hits[i];
         const uint32_t gid = __float_as_uint(h.w);
@@ -571,7 +572,7 @@ hipError_t launch_guide_infer_queue(const GuideNet& net, const DeviceScene& s, c
                                     const float4* hits, const uint32_t* count, uint32_t n_max, uint4* lobes,
                                     hipStream_t st) {
     if (n_max == 0) return hipSuccess;
-    const QueueQueries q{s, qo, qd, hits, count};
+    const QueueQueries q{s, qo, qd, hits, count, n_max};
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_guide_infer<QueueQueries, LobeSink>), dim3(blocks_for(n_max, BQ, 2048)),
                        dim3(GB), 0, st, net, q, LobeSink{lobes});
     return hipGetLastError();

@@ -747,7 +747,8 @@ __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(MOD
                     float alpha = 0.f;
                     GuideLobes L;
                     if (guided && guide_eligible(S.mats[sp.material])) {
-                        unpack_lobes(P.lobes + 4 * size_t(i), L);
+                        // bounce 0: queue slot i = sample * npix + pixel, lobes computed per pixel
+                    unpack_lobes(P.lobes + 4 * size_t(bounce == 0 ? i % A.npix : i), L);
                         if (L.pi_sum > 0.f) alpha = A.guide_fraction;
                     }
                     if (guided && alpha > 0.f) {
@@ -1132,8 +1133,12 @@ hipError_t launch_pt_frame(const DeviceScene& s, const TraceScratch& ts, PtArgs 
             }
             if (guided && b + 1 < L && err == hipSuccess) {
                 mark(-1);
-                err = launch_guide_infer_queue(*guide, s, pb.q_o[qi], pb.q_d[qi], pb.hits, cnt + b, a.n_paths,
-                                               pb.lobes, st);
+                // the samples of a pixel share the primary ray and hit, so bounce 0 queries once
+                // per pixel (the pass's sample-0 entries, queue slots [0, npix))
+                err = b == 0 ? launch_guide_infer_queue(*guide, s, pb.q_o[qi], pb.q_d[qi], pb.hits, nullptr, a.npix,
+                                                        pb.lobes, st)
+                             : launch_guide_infer_queue(*guide, s, pb.q_o[qi], pb.q_d[qi], pb.hits, cnt + b,
+                                                        a.n_paths, pb.lobes, st);
                 mark(1);
             }
             mark(-1);

@@ -108,7 +108,8 @@ __device__ __forceinline__ f3 lobes_sample(const GuideLobes& L, float u0, float 
 }
 
 // Launchers (guide.hip).
-// Lobes of the queued hits [0, *count) of one bounce: 64 bytes per queue entry.
+// Lobes of the queued hits [0, *count) of one bounce (count null: [0, n_max)): 64 bytes per
+// queue entry.
 hipError_t launch_guide_infer_queue(const GuideNet& net, const DeviceScene& s, const float4* qo, const float4* qd,
                                     const float4* hits, const uint32_t* count, uint32_t n_max, uint4* lobes,
                                     hipStream_t st);
