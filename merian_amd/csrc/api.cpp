@@ -60,7 +60,8 @@ void dev_free(mrt_ctx* c, void* p) {
         c->in_use -= it->second;
         c->sizes.erase(it);
     }
-    hipFree(p);
+    const hipError_t e = hipFree(p);
+    if (e != hipSuccess) c->err = std::string("hipFree: ") + hipGetErrorString(e);
 }
 
 hipStream_t pick_stream(mrt_ctx* c, void* stream) { return stream ? static_cast<hipStream_t>(stream) : c->stream; }
@@ -167,7 +168,11 @@ MRT_API mrt_status mrt_create(int32_t device, uint64_t hbm_budget, mrt_ctx** out
         return MRT_ERR_DEVICE;
     }
     size_t free_b = 0, total_b = 0;
-    hipMemGetInfo(&free_b, &total_b);
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return MRT_ERR_DEVICE;
+    }
     c->budget = hbm_budget ? hbm_budget : uint64_t(double(free_b) * 0.9);
     const char* mp = std::getenv("MRT_MAX_PATHS");
     c->max_paths = mp ? std::strtoull(mp, nullptr, 10) : (uint64_t(1) << 24);
@@ -177,14 +182,14 @@ MRT_API mrt_status mrt_create(int32_t device, uint64_t hbm_budget, mrt_ctx** out
 
 MRT_API mrt_status mrt_destroy(mrt_ctx* c) {
     if (!c) return MRT_ERR_INVALID;
-    hipSetDevice(c->device);
-    hipStreamSynchronize(c->stream);
+    // teardown is best effort: every resource is released even if one step fails
+    const bool ok = hipSetDevice(c->device) == hipSuccess && hipStreamSynchronize(c->stream) == hipSuccess;
     free_path_buffers(c);
-    for (auto& kv : c->sizes) hipFree(kv.first);
+    for (auto& kv : c->sizes) (void)hipFree(kv.first);
     c->sizes.clear();
-    hipStreamDestroy(c->stream);
+    const bool destroyed = hipStreamDestroy(c->stream) == hipSuccess;
     delete c;
-    return MRT_OK;
+    return ok && destroyed ? MRT_OK : MRT_ERR_DEVICE;
 }
 
 MRT_API const char* mrt_last_error(mrt_ctx* c) { return c ? c->err.c_str() : "null context"; }
@@ -193,7 +198,8 @@ MRT_API uint64_t mrt_memory_in_use(mrt_ctx* c) { return c ? c->in_use : 0; }
 
 MRT_API mrt_status mrt_alloc_dev(mrt_ctx* c, uint64_t bytes, void** d_ptr) {
     if (!c || !d_ptr) return MRT_ERR_INVALID;
-    hipSetDevice(c->device);
+    const hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) return hip_fail(c, e, "mrt_alloc_dev");
     mrt_status st = MRT_OK;
     *d_ptr = dev_alloc(c, bytes, &st);
     return st;
@@ -202,7 +208,8 @@ MRT_API mrt_status mrt_alloc_dev(mrt_ctx* c, uint64_t bytes, void** d_ptr) {
 MRT_API mrt_status mrt_free_dev(mrt_ctx* c, void* d_ptr) {
     if (!c) return MRT_ERR_INVALID;
     if (d_ptr && !c->sizes.count(d_ptr)) return fail(c, MRT_ERR_INVALID, "mrt_free_dev: not a context allocation");
-    hipStreamSynchronize(c->stream);
+    const hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "mrt_free_dev");
     dev_free(c, d_ptr);
     return MRT_OK;
 }
@@ -311,7 +318,7 @@ static void scene_free(mrt_ctx* c, mrt_scene* s) {
 MRT_API mrt_status mrt_scene_create(mrt_ctx* c, const mrt_scene_desc* d, mrt_scene** out) {
     if (!c || !d || !out) return fail(c, MRT_ERR_INVALID, "mrt_scene_create: null argument");
     *out = nullptr;
-    hipSetDevice(c->device);
+    if (const hipError_t e = hipSetDevice(c->device); e != hipSuccess) return hip_fail(c, e, "mrt_scene_create");
     if (d->geometry_count > 65535u) return fail(c, MRT_ERR_UNSUPPORTED, "more than 65535 geometries (u16 ids)");
     if (d->material_count == 0 && d->geometry_count) return fail(c, MRT_ERR_INVALID, "no materials");
     // flatten triangles in geometry order: global id = geometry prefix + primitive
@@ -479,9 +486,9 @@ MRT_API mrt_status mrt_scene_create_triangles(mrt_ctx* c, const float* positions
 
 MRT_API mrt_status mrt_scene_destroy(mrt_ctx* c, mrt_scene* s) {
     if (!c || !s) return MRT_ERR_INVALID;
-    hipStreamSynchronize(c->stream);
+    const hipError_t e = hipStreamSynchronize(c->stream); // kernels may still read the scene
     scene_free(c, s);
-    return MRT_OK;
+    return e == hipSuccess ? MRT_OK : hip_fail(c, e, "mrt_scene_destroy");
 }
 
 MRT_API mrt_status mrt_scene_info(mrt_scene* s, uint64_t* tris, uint64_t* nodes, uint64_t* bytes) {
@@ -632,7 +639,7 @@ mrt_status render_frame_dev(mrt_ctx* c, mrt_scene* s, const mrt_pt_params* prm, 
     if (npix == 0) return MRT_OK;
     if (!gbuffer || !irradiance) return fail(c, MRT_ERR_INVALID, "mrt_pt_render_dev: null buffer");
     if (npix > 0xFFFFFFFFull) return fail(c, MRT_ERR_UNSUPPORTED, "image too large");
-    hipSetDevice(c->device);
+    if (const hipError_t e = hipSetDevice(c->device); e != hipSuccess) return hip_fail(c, e, "pt frame");
     const uint64_t spp = uint64_t(prm->spp);
     const uint64_t max_paths = s->tuning.max_paths ? s->tuning.max_paths : c->max_paths;
     uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(spp ? spp : 1, max_paths / npix));
@@ -669,18 +676,23 @@ mrt_status render_frame_dev(mrt_ctx* c, mrt_scene* s, const mrt_pt_params* prm, 
     if (prm->count_traversal && stats) {
         tc = static_cast<unsigned long long*>(dev_alloc(c, 16, &st));
         if (st != MRT_OK) return st;
-        hipMemsetAsync(tc, 0, 16, strm);
+        if (const hipError_t e = launch_zero_u32(reinterpret_cast<uint32_t*>(tc), 4, strm); e != hipSuccess) {
+            dev_free(c, tc);
+            return hip_fail(c, e, "traversal counters");
+        }
     }
     const hipError_t e = launch_pt_frame(s->ds, scene_scratch(c, s), a, gbuffer, pb, reinterpret_cast<float4*>(irradiance),
                                          a.record_signature ? signature : nullptr, strm, stats ? &t : nullptr,
                                          stats ? &segs : nullptr, tc, net);
     unsigned long long hc[2] = {0, 0};
+    hipError_t ec = hipSuccess;
     if (tc) {
-        hipMemcpyAsync(hc, tc, 16, hipMemcpyDeviceToHost, strm);
-        hipStreamSynchronize(strm);
+        ec = hipMemcpyAsync(hc, tc, 16, hipMemcpyDeviceToHost, strm);
+        if (ec == hipSuccess) ec = hipStreamSynchronize(strm);
         dev_free(c, tc);
     }
     if (e != hipSuccess) return hip_fail(c, e, "pt frame");
+    if (ec != hipSuccess) return hip_fail(c, ec, "traversal counters");
     if (guide) {
         st = guide_train_frame(c, guide, strm);
         if (st != MRT_OK) return st;
@@ -723,23 +735,26 @@ MRT_API mrt_status mrt_render_frame(mrt_ctx* c, mrt_scene* s, const mrt_pt_param
     void* dout = dev_alloc(c, npix * 16, &st);
     void* dsig = st == MRT_OK ? dev_alloc(c, npix * 4, &st) : nullptr;
     if (st == MRT_OK) {
-        hipEvent_t e0, e1;
-        hipEventCreate(&e0);
-        hipEventCreate(&e1);
-        hipEventRecord(e0, c->stream);
-        st = mrt_gbuffer_dev(c, s, cam, w, h, prm->instance_mask, static_cast<mrt_packed_hit*>(dg), nullptr);
-        hipEventRecord(e1, c->stream);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        hipError_t e = hipEventCreate(&e0);
+        if (e == hipSuccess) e = hipEventCreate(&e1);
+        if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
+        if (e != hipSuccess) st = hip_fail(c, e, "mrt_render_frame events");
+        if (st == MRT_OK)
+            st = mrt_gbuffer_dev(c, s, cam, w, h, prm->instance_mask, static_cast<mrt_packed_hit*>(dg), nullptr);
+        if (st == MRT_OK && (e = hipEventRecord(e1, c->stream)) != hipSuccess) st = hip_fail(c, e, "mrt_render_frame events");
         if (st == MRT_OK)
             st = mrt_pt_render_dev(c, s, prm, cam, frame, w, h, static_cast<mrt_packed_hit*>(dg),
                                    static_cast<float*>(dout), signature ? static_cast<uint32_t*>(dsig) : nullptr, stats,
                                    nullptr);
         if (st == MRT_OK && stats) {
-            hipEventSynchronize(e1);
-            hipEventElapsedTime(&stats->ms_gbuffer, e0, e1);
+            e = hipEventSynchronize(e1);
+            if (e == hipSuccess) e = hipEventElapsedTime(&stats->ms_gbuffer, e0, e1);
+            if (e != hipSuccess) st = hip_fail(c, e, "mrt_render_frame gbuffer time");
         }
-        hipEventDestroy(e0);
-        hipEventDestroy(e1);
-        hipError_t e = hipSuccess;
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        e = hipSuccess;
         if (st == MRT_OK) e = hipMemcpyAsync(irradiance, dout, npix * 16, hipMemcpyDeviceToHost, c->stream);
         if (st == MRT_OK && e == hipSuccess && signature)
             e = hipMemcpyAsync(signature, dsig, npix * 4, hipMemcpyDeviceToHost, c->stream);

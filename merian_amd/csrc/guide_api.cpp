@@ -86,11 +86,13 @@ template <class T> T* galloc(mrt_ctx* c, mrt_guide* g, size_t count, mrt_status*
     return static_cast<T*>(p);
 }
 
-void guide_free(mrt_ctx* c, mrt_guide* g) {
-    if (!g) return;
+// Releases every buffer of the guide; false if freeing the pinned counter failed.
+bool guide_free(mrt_ctx* c, mrt_guide* g) {
+    if (!g) return true;
     for (void* p : g->allocs) dev_free(c, p);
-    if (g->h_count) hipHostFree(g->h_count);
+    const bool ok = !g->h_count || hipHostFree(g->h_count) == hipSuccess;
     delete g;
+    return ok;
 }
 
 // Training buffers for batches of up to `cap` records.
@@ -217,7 +219,7 @@ MRT_API mrt_status mrt_guide_create(mrt_ctx* c, mrt_scene* s, const mrt_guide_co
     *out = nullptr;
     const mrt_guide_config cfg = with_defaults(config);
     if (check_config(cfg) != MRT_OK) return fail(c, MRT_ERR_INVALID, "mrt_guide_create: configuration out of range");
-    hipSetDevice(c->device);
+    if (const hipError_t e = hipSetDevice(c->device); e != hipSuccess) return hip_fail(c, e, "mrt_guide_create");
     mrt_guide* g = new mrt_guide();
     g->cfg = cfg;
     layout(cfg, g->net.res);
@@ -285,9 +287,10 @@ MRT_API mrt_status mrt_guide_create(mrt_ctx* c, mrt_scene* s, const mrt_guide_co
 
 MRT_API mrt_status mrt_guide_destroy(mrt_ctx* c, mrt_guide* g) {
     if (!c || !g) return MRT_ERR_INVALID;
-    hipStreamSynchronize(c->stream);
-    guide_free(c, g);
-    return MRT_OK;
+    const hipError_t e = hipStreamSynchronize(c->stream); // kernels may still use the guide's buffers
+    const bool freed = guide_free(c, g);
+    if (e != hipSuccess) return hip_fail(c, e, "mrt_guide_destroy");
+    return freed ? MRT_OK : fail(c, MRT_ERR_DEVICE, "mrt_guide_destroy: hipHostFree failed");
 }
 
 MRT_API mrt_status mrt_guide_normalisation(mrt_guide* g, float offset[3], float scale[3]) {

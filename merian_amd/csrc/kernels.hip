@@ -968,9 +968,11 @@ template <bool COUNT, class Src, class Sink, bool ANY, int W> int trace_resident
     // computed once per process (thread-safe static initialisation)
     static const int blocks = [] {
         int per_cu = 0, dev = 0, cus = 0;
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<COUNT, Src, Sink, ANY, W>, TRACE_BLOCK, 0);
-        hipGetDevice(&dev);
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<COUNT, Src, Sink, ANY, W>, TRACE_BLOCK, 0) !=
+                hipSuccess ||
+            hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return 256; // query failed: one block per CU of an MI355X still drains every queue
         return per_cu * cus > 0 ? per_cu * cus : 1;
     }();
     return blocks;
@@ -1018,8 +1020,9 @@ void trace_prepare() {
 
 uint32_t trace_max_threads() {
     int dev = 0, cus = 0;
-    hipGetDevice(&dev);
-    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        cus = 256;
     // at most 32 waves of 64 lanes per CU
     return uint32_t(cus > 0 ? cus : 256) * 32u * 64u;
 }
@@ -1085,12 +1088,14 @@ hipError_t launch_pt_frame(const DeviceScene& s, const TraceScratch& ts, PtArgs 
     // timing: one event before and after every launch, read once the frame has completed
     std::vector<hipEvent_t> ev;
     std::vector<int> kind; // per event pair: 0 trace, 1 shade
+    hipError_t terr = hipSuccess; // first timing failure: timing stops, the frame still renders
     auto mark = [&](int k) {
-        if (!timing) return;
+        if (!timing || terr != hipSuccess) return;
         hipEvent_t e;
-        hipEventCreate(&e);
-        hipEventRecord(e, st);
+        terr = hipEventCreate(&e);
+        if (terr != hipSuccess) return;
         ev.push_back(e);
+        terr = hipEventRecord(e, st);
         if (k >= 0) kind.push_back(k);
     };
     mark(-1);
@@ -1172,36 +1177,46 @@ hipError_t launch_pt_frame(const DeviceScene& s, const TraceScratch& ts, PtArgs 
     }
     mark(-1);
     if (timing) {
-        hipEventSynchronize(ev.back());
         float ms_trace = 0.f, ms_shade = 0.f;
+        if (terr == hipSuccess) terr = hipEventSynchronize(ev.back());
         // events: [frame start] then per launch (start, end) pairs, then [frame end]
-        for (size_t k = 0; k < kind.size(); k++) {
+        for (size_t k = 0; k < kind.size() && terr == hipSuccess; k++) {
             float ms = 0.f;
-            hipEventElapsedTime(&ms, ev[1 + 2 * k], ev[2 + 2 * k]);
+            terr = hipEventElapsedTime(&ms, ev[1 + 2 * k], ev[2 + 2 * k]);
             (kind[k] == 0 ? ms_trace : ms_shade) += ms;
         }
-        hipEventElapsedTime(&timing->ms_total, ev.front(), ev.back());
+        if (terr == hipSuccess) terr = hipEventElapsedTime(&timing->ms_total, ev.front(), ev.back());
         timing->ms_trace = ms_trace;
         timing->ms_shade = ms_shade;
         timing->trace_launches = 0;
         for (int k : kind) timing->trace_launches += (k == 0);
-        for (auto& e : ev) hipEventDestroy(e);
+        for (auto& e : ev) (void)hipEventDestroy(e); // release; nothing to recover on failure
     }
     if (err != hipSuccess) return err;
+    if (terr != hipSuccess) return terr;
     if (continuation_segments) {
         std::vector<uint32_t> h(nz);
-        hipMemcpyAsync(h.data(), pb.counters, sizeof(uint32_t) * nz, hipMemcpyDeviceToHost, st);
-        hipStreamSynchronize(st);
+        err = hipMemcpyAsync(h.data(), pb.counters, sizeof(uint32_t) * nz, hipMemcpyDeviceToHost, st);
+        if (err == hipSuccess) err = hipStreamSynchronize(st);
+        if (err != hipSuccess) return err;
         uint64_t segs = 0;
         for (uint32_t c = 0; c < n_chunks; c++)
             for (int b = 1; b < L; b++) segs += h[c * per_chunk + b];
         *continuation_segments = segs;
         if (timing && nee) {
             std::vector<uint32_t> hs(nz);
-            hipMemcpy(hs.data(), pb.counters + 2 * nz, sizeof(uint32_t) * nz, hipMemcpyDeviceToHost);
+            err = hipMemcpy(hs.data(), pb.counters + 2 * nz, sizeof(uint32_t) * nz, hipMemcpyDeviceToHost);
+            if (err != hipSuccess) return err;
             for (uint32_t v : hs) timing->shadow_rays += v;
         }
     }
+    return hipGetLastError();
+}
+
+// Clears n words with a fill kernel (no memset node, so callers may capture it in a graph).
+hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(n, 256, MAX_BLOCKS)), dim3(256), 0, st, p, n);
     return hipGetLastError();
 }
 

@@ -65,6 +65,9 @@ struct TraceScratch {
     uint32_t leaf_lanes = 16; // leaf-only lanes before a wave runs a leaf-test phase (1..64)
 };
 
+// Clears n words with a fill kernel (graph-capturable, unlike a memset).
+hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t st);
+
 // Most trace-kernel threads resident on the current device (the persistent grid's upper bound).
 uint32_t trace_max_threads();
 // Runs the one-time occupancy queries of the frame's trace kernels (call before capturing).

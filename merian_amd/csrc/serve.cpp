@@ -180,9 +180,15 @@ hipError_t alloc_job(Slot& sl, const Job& j, const JobShape& sh) {
     return e;
 }
 
-void free_job(Slot& sl) {
-    for (void* p : sl.buf.allocs) hipFreeAsync(p, sl.stream);
+// Returns every buffer of the slot's job to the pool; the first failure is reported.
+hipError_t free_job(Slot& sl) {
+    hipError_t first = hipSuccess;
+    for (void* p : sl.buf.allocs) {
+        const hipError_t e = hipFreeAsync(p, sl.stream);
+        if (first == hipSuccess) first = e;
+    }
     sl.buf = JobBuffers{};
+    return first;
 }
 
 // Enqueues one job's frame on the slot's stream (captured in a graph when enabled).
@@ -215,7 +221,7 @@ hipError_t enqueue_frame(mrt_server* s, Slot& sl, const Job& j) {
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(sl.stream, &g);
     if (le != hipSuccess) {
-        if (g) hipGraphDestroy(g);
+        if (g) (void)hipGraphDestroy(g); // the launch error is the one reported
         return le;
     }
     if (e != hipSuccess) return e;
@@ -225,14 +231,20 @@ hipError_t enqueue_frame(mrt_server* s, Slot& sl, const Job& j) {
         hipGraphExecUpdateResult res;
         updated = hipGraphExecUpdate(sl.exec, g, &err_node, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
         if (!updated) {
-            hipGetLastError(); // clear the failed update's error
-            hipGraphExecDestroy(sl.exec);
+            (void)hipGetLastError(); // clear the failed update's error
+            e = hipGraphExecDestroy(sl.exec);
             sl.exec = nullptr;
+            if (e != hipSuccess) {
+                (void)hipGraphDestroy(g);
+                return e;
+            }
         }
     }
     if (!updated) e = hipGraphInstantiate(&sl.exec, g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
+    const hipError_t ge = hipGraphDestroy(g);
+    if (e == hipSuccess) e = ge;
     if (e != hipSuccess) {
+        if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
         sl.exec = nullptr;
         return e;
     }
@@ -247,10 +259,11 @@ hipError_t start_job(mrt_server* s, Slot& sl, Job& j) {
     const JobShape sh = shape_of(s, j.d);
     const uint64_t nz = j.layout.n_chunks * uint64_t(sh.L + 1);
     if (nz > sl.h_counters_cap) {
-        if (sl.h_counters) hipHostFree(sl.h_counters);
+        hipError_t e = sl.h_counters ? hipHostFree(sl.h_counters) : hipSuccess;
         sl.h_counters = nullptr;
         sl.h_counters_cap = 0;
-        hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&sl.h_counters), nz * 4, hipHostMallocDefault);
+        if (e != hipSuccess) return e;
+        e = hipHostMalloc(reinterpret_cast<void**>(&sl.h_counters), nz * 4, hipHostMallocDefault);
         if (e != hipSuccess) return e;
         sl.h_counters_cap = nz;
     }
@@ -286,7 +299,11 @@ void finish_job(mrt_server* s, Slot& sl, hipError_t status) {
     } else {
         result = MRT_ERR_DEVICE;
     }
-    free_job(sl);
+    const hipError_t fe = free_job(sl);
+    if (fe != hipSuccess && result == MRT_OK) {
+        result = MRT_ERR_DEVICE;
+        status = fe;
+    }
     std::lock_guard<std::mutex> lk(s->m);
     j.t_done = now_ms(s->t0);
     if (result != MRT_OK) {
@@ -317,7 +334,8 @@ void finish_job(mrt_server* s, Slot& sl, hipError_t status) {
 }
 
 void scheduler(mrt_server* s) {
-    hipSetDevice(s->device);
+    // a thread that cannot select the device fails every job it admits with that error
+    const hipError_t dev = hipSetDevice(s->device);
     std::unique_lock<std::mutex> lk(s->m);
     while (true) {
         // admit while the head of the queue fits
@@ -332,9 +350,9 @@ void scheduler(mrt_server* s) {
             s->peak_bytes = std::max(s->peak_bytes, s->adm.in_use());
             s->peak_in_flight = std::max(s->peak_in_flight, s->adm.busy());
             lk.unlock();
-            const hipError_t e = start_job(s, sl, j);
+            const hipError_t e = dev != hipSuccess ? dev : start_job(s, sl, j);
             if (e != hipSuccess) {
-                hipStreamSynchronize(sl.stream);
+                (void)hipStreamSynchronize(sl.stream); // drain what was queued; the job fails with e
                 finish_job(s, sl, e);
             }
             lk.lock();
@@ -384,7 +402,7 @@ MRT_API mrt_status mrt_server_create(mrt_ctx* c, const mrt_server_config* cfg, m
     const uint64_t left = c->budget > c->in_use ? c->budget - c->in_use : 0;
     if (k.hbm_budget == 0) k.hbm_budget = left / 2;
     if (k.hbm_budget > left) return fail(c, MRT_ERR_BUDGET, "server budget exceeds what the context has left");
-    hipSetDevice(c->device);
+    if (const hipError_t e = hipSetDevice(c->device); e != hipSuccess) return hip_fail(c, e, "mrt_server_create");
     std::unique_ptr<mrt_server> s(new mrt_server());
     s->ctx = c;
     s->device = c->device;
@@ -397,16 +415,17 @@ MRT_API mrt_status mrt_server_create(mrt_ctx* c, const mrt_server_config* cfg, m
     hipMemPool_t pool = nullptr;
     if (hipDeviceGetDefaultMemPool(&pool, c->device) == hipSuccess && pool) {
         uint64_t keep = ~0ull;
-        hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        if (const hipError_t e = hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep); e != hipSuccess)
+            return hip_fail(c, e, "server memory pool");
     }
     s->slots.resize(k.max_slots);
     for (Slot& sl : s->slots) {
         hipError_t e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
         if (e != hipSuccess) {
-            for (Slot& t : s->slots) {
-                if (t.done) hipEventDestroy(t.done);
-                if (t.stream) hipStreamDestroy(t.stream);
+            for (Slot& t : s->slots) { // best effort: the creation error is the one reported
+                if (t.done) (void)hipEventDestroy(t.done);
+                if (t.stream) (void)hipStreamDestroy(t.stream);
             }
             return hip_fail(c, e, "server streams");
         }
@@ -437,17 +456,18 @@ MRT_API mrt_status mrt_server_destroy(mrt_server* s) {
     s->cv_work.notify_all();
     s->cv_done.notify_all();
     if (s->th.joinable()) s->th.join();
-    hipSetDevice(s->device);
+    // teardown is best effort: every slot is released even if one step fails
+    bool ok = hipSetDevice(s->device) == hipSuccess;
     for (Slot& sl : s->slots) {
-        hipStreamSynchronize(sl.stream);
-        if (sl.exec) hipGraphExecDestroy(sl.exec);
-        if (sl.h_counters) hipHostFree(sl.h_counters);
-        hipEventDestroy(sl.done);
-        hipStreamDestroy(sl.stream);
+        ok &= hipStreamSynchronize(sl.stream) == hipSuccess;
+        if (sl.exec) ok &= hipGraphExecDestroy(sl.exec) == hipSuccess;
+        if (sl.h_counters) ok &= hipHostFree(sl.h_counters) == hipSuccess;
+        ok &= hipEventDestroy(sl.done) == hipSuccess;
+        ok &= hipStreamDestroy(sl.stream) == hipSuccess;
     }
     s->ctx->in_use -= s->reserved;
     delete s;
-    return MRT_OK;
+    return ok ? MRT_OK : MRT_ERR_DEVICE;
 }
 
 MRT_API const char* mrt_server_last_error(mrt_server* s) {

@@ -48,7 +48,8 @@ mrt_status rebuild_bvh(mrt_ctx* c, mrt_scene* s, uint32_t width, uint32_t max_le
         tri_bytes = b4.tris.size() * sizeof(TriRecord);
         depth = b4.depth;
     }
-    hipStreamSynchronize(c->stream);
+    // kernels in flight may still read the old tree
+    if (const hipError_t se = hipStreamSynchronize(c->stream); se != hipSuccess) return hip_fail(c, se, "bvh rebuild");
     mrt_status st = MRT_OK;
     void* nodes = dev_alloc(c, node_bytes, &st);
     if (st != MRT_OK) return st;
@@ -116,23 +117,28 @@ mrt_status trial(mrt_ctx* c, mrt_scene* s, const mrt_tune_request& r, float& ms)
     }
     float best = 1e30f;
     for (int k = 0; k < 3 && st == MRT_OK; k++) {
-        hipEvent_t e0, e1;
-        hipEventCreate(&e0);
-        hipEventCreate(&e1);
-        hipEventRecord(e0, c->stream);
-        st = mrt_gbuffer_dev(c, s, &r.camera, r.width, r.height, p.instance_mask, static_cast<mrt_packed_hit*>(gb),
-                             nullptr);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        hipError_t e = hipEventCreate(&e0);
+        if (e == hipSuccess) e = hipEventCreate(&e1);
+        if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
+        if (e != hipSuccess) st = hip_fail(c, e, "tune trial events");
+        if (st == MRT_OK)
+            st = mrt_gbuffer_dev(c, s, &r.camera, r.width, r.height, p.instance_mask,
+                                 static_cast<mrt_packed_hit*>(gb), nullptr);
         if (st == MRT_OK)
             st = render_frame_dev(c, s, &p, &r.camera, uint32_t(k), r.width, r.height,
                                   static_cast<mrt_packed_hit*>(gb), static_cast<float*>(out), nullptr, nullptr,
                                   nullptr, nullptr);
-        hipEventRecord(e1, c->stream);
-        hipEventSynchronize(e1);
         float t = 0.f;
-        hipEventElapsedTime(&t, e0, e1);
-        if (k > 0) best = std::min(best, t); // the first frame may (re)allocate buffers
-        hipEventDestroy(e0);
-        hipEventDestroy(e1);
+        if (st == MRT_OK) {
+            e = hipEventRecord(e1, c->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+            if (e != hipSuccess) st = hip_fail(c, e, "tune trial timing");
+        }
+        if (st == MRT_OK && k > 0) best = std::min(best, t); // the first frame may (re)allocate buffers
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
     }
     dev_free(c, gb);
     dev_free(c, out);
@@ -153,7 +159,7 @@ MRT_API mrt_status mrt_scene_get_tuning(mrt_scene* s, mrt_tuning* out) {
 MRT_API mrt_status mrt_scene_set_tuning(mrt_ctx* c, mrt_scene* s, const mrt_tuning* t) {
     if (!c || !s || !t) return MRT_ERR_INVALID;
     if (!valid(*t)) return fail(c, MRT_ERR_INVALID, "tuning values out of range");
-    hipSetDevice(c->device);
+    if (const hipError_t e = hipSetDevice(c->device); e != hipSuccess) return hip_fail(c, e, "mrt_scene_set_tuning");
     const uint32_t width = t->bvh_width ? t->bvh_width : 4u;
     if (t->bvh_max_leaf != s->tuning.bvh_max_leaf || t->bvh_traversal_cost != s->tuning.bvh_traversal_cost ||
         width != s->tuning.bvh_width) {
@@ -172,7 +178,7 @@ MRT_API mrt_status mrt_tune(mrt_ctx* c, mrt_scene* s, const mrt_tune_request* r,
     if (r->width == 0 || r->height == 0) return fail(c, MRT_ERR_INVALID, "mrt_tune: empty image");
     const mrt_status pc = check_params(c, &r->params);
     if (pc != MRT_OK) return pc;
-    hipSetDevice(c->device);
+    if (const hipError_t e = hipSetDevice(c->device); e != hipSuccess) return hip_fail(c, e, "mrt_tune");
     const auto t0 = std::chrono::steady_clock::now();
     const double budget = r->time_budget_ms > 0.f ? double(r->time_budget_ms) : 20000.0;
     auto elapsed = [&] { return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); };
