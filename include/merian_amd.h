@@ -195,7 +195,7 @@ typedef struct mrt_packed_hit {
  * Render parameters: the "Render (Path-traced)" properties
  * (src/merian-graph/nodes/render_pt/render_pt.cpp:147-187) plus build-side modes.
  * ---------------------------------------------------------------------------------- */
-enum { MRT_MODE_BSDF = 0, MRT_MODE_NEE = 1, MRT_MODE_GUIDED = 2 };
+enum { MRT_MODE_BSDF = 0, MRT_MODE_NEE = 1, MRT_MODE_GUIDED = 2, MRT_MODE_MCPG = 3 };
 
 typedef struct mrt_pt_params {
     int32_t spp;                 /* "samples per pixel" */
@@ -506,6 +506,77 @@ MRT_API mrt_status mrt_guided_render_dev(mrt_ctx* ctx, mrt_scene* scene, mrt_gui
                                          uint32_t frame, uint32_t width, uint32_t height,
                                          const mrt_packed_hit* d_gbuffer, float* d_irradiance,
                                          int32_t train, mrt_pt_stats* stats, void* stream);
+
+/* ------------------------------------------------------------------------------------
+ * Markov-chain path guiding: the surface transport of the "Render (Path-traced, MCPG)" node
+ * (src/merian-graph/nodes/render_pt_mcpg/render_pt_mcpg.cpp:204-258,
+ * render_pt_mcpg.slang:39-244). Guiding state persists across frames in two device hash
+ * grids (utils/hash-grid.slang):
+ *   - the MC grid: per cell a 32-byte Markov-chain state (mc.slang:15-128: weighted mean of
+ *     the hit points a vertex's good samples reached, mean cosine, count) from which a von
+ *     Mises-Fisher lobe is derived; a scattering vertex resamples one of `mc_samples`
+ *     stochastic lookups (RIS) and samples its lobe with probability p_guide, else the BSDF,
+ *     weighing by the mixture of all looked-up lobes and the BSDF (stochastic MIS);
+ *   - the light cache: per cell a running mean of incident radiance (half3 + count,
+ *     hashed-irradiance-cache.slang) that stands in for the incident radiance of a
+ *     non-emitting hit when a chain is updated.
+ * The samples of a pixel run one after another on one RNG stream per pixel and frame
+ * (pcg3d16(pixel, frame)), as the node's per-pixel loop does, so a frame runs spp passes of
+ * one path per pixel. Paths of one pass update the grids concurrently, as the reference's
+ * threads do; with one path per pass (a 1x1 image) the sequence is the oracle's exactly.
+ * Scenes are static here: motion vectors are zero and prev_pos == pos. The volume pass and
+ * the debug outputs of the node are not part of this path. mrt_pt_params.mode is ignored.
+ * ---------------------------------------------------------------------------------- */
+typedef struct mrt_mcpg mrt_mcpg;
+
+enum { MRT_MCPG_MIS = 0, MRT_MCPG_MIS_ROUGHNESS = 1, MRT_MCPG_MAX_SAMPLES = 8 };
+
+typedef struct mrt_mcpg_grid_config {
+    uint32_t buffer_size;            /* slots ("adaptive grid buf size" / "LC buffer size") */
+    uint32_t probe_count;            /* slots probed before evicting (1..32) */
+    uint32_t locality_bits;          /* 2^n-wide cell tiles in Morton order (0..5; 0 = scatter) */
+    uint32_t stochastic_interpolation; /* light cache: jitter the cell per lookup (the MC grid
+                                          always jitters cell and level) */
+    float tan_alpha_half;            /* "grid tan(alpha/2)": cell size per unit camera distance */
+    float level_bias;                /* "grid level bias" */
+    float distribution_dimension;    /* "grid distribution dimension" */
+    uint32_t _reserved;
+} mrt_mcpg_grid_config;
+
+typedef struct mrt_mcpg_config {
+    mrt_mcpg_grid_config mc;         /* the "mc" child properties */
+    mrt_mcpg_grid_config lc;         /* the "lc" child properties */
+    int32_t mc_samples;              /* "MC samples" (0..MRT_MCPG_MAX_SAMPLES) */
+    float p_guiding;                 /* "guiding prob" */
+    int32_t sampling_type;           /* "directional sampling type": MRT_MCPG_MIS(_ROUGHNESS) */
+    float guiding_alpha_threshold;   /* "roughness threshold" squared */
+    float dir_guide_prior;           /* "ML prior" */
+    int32_t missing_light_heuristic; /* "missing light heuristic" */
+    int32_t use_light_cache_tail;    /* "surf: use LC" */
+    float lc_min_pdf;                /* "LC min pdf" */
+} mrt_mcpg_config;
+
+/* The node's defaults (render_pt_mcpg.hpp:125-152, hash_grid.hpp:42-44). */
+MRT_API void mrt_mcpg_default_config(mrt_mcpg_config* out);
+/* Allocates both grids (zeroed, as the node's reset on its first frame). */
+MRT_API mrt_status mrt_mcpg_create(mrt_ctx* ctx, const mrt_mcpg_config* config, mrt_mcpg** out);
+MRT_API mrt_status mrt_mcpg_destroy(mrt_ctx* ctx, mrt_mcpg* mcpg);
+/* Clears both grids and forgets the previous camera (RenderMCPG::process at iteration 0). */
+MRT_API mrt_status mrt_mcpg_reset(mrt_ctx* ctx, mrt_mcpg* mcpg);
+/* One MCPG frame over a gbuffer (as mrt_pt_render_dev): d_irradiance = per pixel mean
+ * radiance.rgb, mean luminance^2. `frame` seeds the RNG and stamps grid slots; the
+ * previous camera is the one of the last frame rendered with this state (the current one
+ * after a reset). */
+MRT_API mrt_status mrt_mcpg_render_dev(mrt_ctx* ctx, mrt_scene* scene, mrt_mcpg* mcpg,
+                                       const mrt_pt_params* params, const mrt_camera* cam,
+                                       uint32_t frame, uint32_t width, uint32_t height,
+                                       const mrt_packed_hit* d_gbuffer, float* d_irradiance,
+                                       mrt_pt_stats* stats, void* stream);
+/* Host copies of the grids (any pointer may be NULL): keys uint32[2*size] (key, stamp),
+ * chain states 32 bytes per slot (w_tgt.xyz, sum_w, w_cos, mv half3, N u16, T), light-cache
+ * entries 8 bytes per slot (irr half3, N u16). */
+MRT_API mrt_status mrt_mcpg_read_grids(mrt_ctx* ctx, mrt_mcpg* mcpg, uint32_t* mc_keys, void* mc_states,
+                                       uint32_t* lc_keys, void* lc_entries);
 
 /* ------------------------------------------------------------------------------------
  * Graph descriptions (host only, no device work): merian's graph JSON (schema v4, v1-v3

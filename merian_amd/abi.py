@@ -46,6 +46,7 @@ JITTER_R2 = 1
 MODE_BSDF = 0
 MODE_NEE = 1
 MODE_GUIDED = 2
+MODE_MCPG = 3
 
 # numpy layouts of the POD arrays handed across the ABI
 VERTEX_DTYPE = np.dtype([("position", "<f4", (3,)), ("encoded_normal", "<u4"),
@@ -167,9 +168,29 @@ class GuideConfig(C.Structure):
 
 GUIDE_LEVELS, GUIDE_WIDTH, GUIDE_LAYERS, GUIDE_OUTPUTS, GUIDE_LOBES = 16, 64, 5, 40, 8
 
+
+class McpgGridConfig(C.Structure):
+    _fields_ = [("buffer_size", C.c_uint32), ("probe_count", C.c_uint32), ("locality_bits", C.c_uint32),
+                ("stochastic_interpolation", C.c_uint32), ("tan_alpha_half", C.c_float), ("level_bias", C.c_float),
+                ("distribution_dimension", C.c_float), ("_reserved", C.c_uint32)]
+
+
+class McpgConfig(C.Structure):
+    _fields_ = [("mc", McpgGridConfig), ("lc", McpgGridConfig), ("mc_samples", C.c_int32), ("p_guiding", C.c_float),
+                ("sampling_type", C.c_int32), ("guiding_alpha_threshold", C.c_float),
+                ("dir_guide_prior", C.c_float), ("missing_light_heuristic", C.c_int32),
+                ("use_light_cache_tail", C.c_int32), ("lc_min_pdf", C.c_float)]
+
+
+MCPG_MIS, MCPG_MIS_ROUGHNESS, MCPG_MAX_SAMPLES = 0, 1, 8
+MCPG_STATE_DTYPE = np.dtype([("w_tgt", "<f4", 3), ("sum_w", "<f4"), ("w_cos", "<f4"), ("mv", "<u2", 3),
+                             ("N", "<u2"), ("T", "<f4")])
+LC_ENTRY_DTYPE = np.dtype([("irr", "<u2", 3), ("N", "<u2")])
+
 assert C.sizeof(SceneDesc) == 128 and C.sizeof(Camera) == 64 and C.sizeof(PtParams) == 32
 assert C.sizeof(ServerConfig) == 24 and C.sizeof(JobDesc) == 128 and C.sizeof(JobStatus) == 40
 assert C.sizeof(ServerStats) == 104 and C.sizeof(GuideConfig) == 64
+assert C.sizeof(McpgGridConfig) == 32 and C.sizeof(McpgConfig) == 96 and MCPG_STATE_DTYPE.itemsize == 32
 
 
 def ptr(a: np.ndarray) -> C.c_void_p:
@@ -237,6 +258,14 @@ EXPORTS = [
     ("mrt_guided_render_dev", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(PtParams),
                                           C.POINTER(Camera), C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
                                           C.c_void_p, C.c_int32, C.POINTER(PtStats), C.c_void_p]),
+    ("mrt_mcpg_default_config", None, [C.POINTER(McpgConfig)]),
+    ("mrt_mcpg_create", C.c_int32, [C.c_void_p, C.POINTER(McpgConfig), C.POINTER(C.c_void_p)]),
+    ("mrt_mcpg_destroy", C.c_int32, [C.c_void_p, C.c_void_p]),
+    ("mrt_mcpg_reset", C.c_int32, [C.c_void_p, C.c_void_p]),
+    ("mrt_mcpg_render_dev", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(PtParams),
+                                        C.POINTER(Camera), C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
+                                        C.c_void_p, C.POINTER(PtStats), C.c_void_p]),
+    ("mrt_mcpg_read_grids", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("mrt_graph_apply_cli", C.c_int32, [C.c_char_p, C.c_int32, C.POINTER(C.c_char_p), C.c_char_p,
                                         C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     ("mrt_graph_resolve_includes", C.c_int32, [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p),

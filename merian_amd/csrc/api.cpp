@@ -632,7 +632,8 @@ mrt_status check_params(mrt_ctx* c, const mrt_pt_params* p) {
 namespace mrt {
 mrt_status render_frame_dev(mrt_ctx* c, mrt_scene* s, const mrt_pt_params* prm, const mrt_camera* cam, uint32_t frame,
                             uint32_t w, uint32_t h, const mrt_packed_hit* gbuffer, float* irradiance,
-                            uint32_t* signature, mrt_pt_stats* stats, void* stream, mrt_guide* guide) {
+                            uint32_t* signature, mrt_pt_stats* stats, void* stream, mrt_guide* guide,
+                            mrt_mcpg* mcpg) {
     if (!c || !s || !cam) return fail(c, MRT_ERR_INVALID, "pt frame: null argument");
     mrt_status st = MRT_OK;
     const uint64_t npix = uint64_t(w) * h;
@@ -644,6 +645,7 @@ mrt_status render_frame_dev(mrt_ctx* c, mrt_scene* s, const mrt_pt_params* prm, 
     const uint64_t max_paths = s->tuning.max_paths ? s->tuning.max_paths : c->max_paths;
     uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(spp ? spp : 1, max_paths / npix));
     if (npix * chunk > 0xFFFFFFFFull) chunk = 0xFFFFFFFFull / npix;
+    if (mcpg) chunk = 1; // the samples of a pixel run one after another
     const uint64_t n_chunks = spp ? (spp + chunk - 1) / chunk : 1;
     st = ensure_path_buffers(c, npix * chunk, npix, n_chunks * uint64_t(prm->max_path_length + 1));
     if (st != MRT_OK) return st;
@@ -663,7 +665,12 @@ mrt_status render_frame_dev(mrt_ctx* c, mrt_scene* s, const mrt_pt_params* prm, 
     a.emission_on_primary = prm->emission_on_primary;
     a.record_signature = prm->record_signature && signature ? 1 : 0;
     a.instance_mask = prm->instance_mask & 0xFFu;
-    a.mode = prm->mode;
+    a.mode = mcpg ? int32_t(MRT_MODE_MCPG) : prm->mode;
+    McpgArgs margs{};
+    if (mcpg) {
+        st = mcpg_prepare_frame(c, mcpg, npix, *cam, margs);
+        if (st != MRT_OK) return st;
+    }
     const GuideNet* net = nullptr;
     if (guide) {
         st = guide_prepare_frame(c, guide, npix * chunk, pb, a.guide_fraction, &net);
@@ -683,7 +690,7 @@ mrt_status render_frame_dev(mrt_ctx* c, mrt_scene* s, const mrt_pt_params* prm, 
     }
     const hipError_t e = launch_pt_frame(s->ds, scene_scratch(c, s), a, gbuffer, pb, reinterpret_cast<float4*>(irradiance),
                                          a.record_signature ? signature : nullptr, strm, stats ? &t : nullptr,
-                                         stats ? &segs : nullptr, tc, net);
+                                         stats ? &segs : nullptr, tc, net, mcpg ? &margs : nullptr);
     unsigned long long hc[2] = {0, 0};
     hipError_t ec = hipSuccess;
     if (tc) {
@@ -693,6 +700,7 @@ mrt_status render_frame_dev(mrt_ctx* c, mrt_scene* s, const mrt_pt_params* prm, 
     }
     if (e != hipSuccess) return hip_fail(c, e, "pt frame");
     if (ec != hipSuccess) return hip_fail(c, ec, "traversal counters");
+    if (mcpg) mcpg_end_frame(mcpg, *cam);
     if (guide) {
         st = guide_train_frame(c, guide, strm);
         if (st != MRT_OK) return st;

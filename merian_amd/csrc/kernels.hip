@@ -8,6 +8,7 @@
 // appends the continuation ray with one wave-aggregated atomic per wavefront.
 #include "mrt_guide.h"
 #include "mrt_kernels.h"
+#include "mrt_mcpg.h"
 #include "mrt_trace.h"
 
 #include <algorithm>
@@ -882,6 +883,236 @@ __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(2))
     }
 }
 
+// -----------------------------------------------------------------------------------------
+// Markov-chain path guiding (render_pt_mcpg.slang:39-244)
+
+// Seeds pass A.s0 of an MCPG frame, one path per pixel: the samples of a pixel run one after
+// another on one RNG stream (render_pt_mcpg.slang:224-244), so the first pass seeds it with
+// pcg3d16(pixel, frame) and later passes continue from where the previous sample's path left
+// it. Path state as k_pt_init.
+__global__ void k_mcpg_init(DeviceScene S, PtArgs A, const mrt_packed_hit* __restrict__ gbuffer, PathBuffers P,
+                            uint32_t* count0) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *count0 = A.npix;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < A.npix; p += stride) {
+        const uint32_t x = p % A.width, y = p / A.width;
+        const RayF r = camera_ray(A.cam, x, y, A.width, A.height);
+        float bu, bv;
+        const uint32_t gid = gbuffer_hit(S, gbuffer, p, bu, bv);
+        const uint32_t rng = A.s0 == 0 ? pcg3d16(x, y, A.frame) : __float_as_uint(P.state[2 * size_t(p) + 1].w);
+        st_stream(P.q_o[0] + p, make_float4(r.o.x, r.o.y, r.o.z, __uint_as_float(p)));
+        st_stream(P.q_d[0] + p, make_float4(r.d.x, r.d.y, r.d.z, 0.f));
+        st_stream(P.hits + p, make_float4(0.f, bu, bv, __uint_as_float(gid)));
+        st_stream(P.state + 2 * size_t(p), make_float4(1.f, 1.f, 1.f, 1.f));
+        st_stream(P.state + 2 * size_t(p) + 1, make_float4(0.f, 0.f, 0.f, __uint_as_float(rng)));
+    }
+}
+
+// What a scatter vertex's chain update needs once its continuation ray has been traced: the
+// lobe's chain state (the RIS-selected one, or a fresh state when the BSDF was sampled) and
+// the slot it came from, the RIS score sum, the vertex position and normal, the light-cache
+// cell its own lookup found, the BSDF value and the sampling pdf.
+struct McpgCarry {
+    McState sel;
+    uint32_t sel_slot;
+    float score_sum;
+    f3 pos;
+    float wo_p;
+    f3 normal;
+    GridPos lc;
+    f3 bsdf;
+};
+__device__ __forceinline__ void carry_store(float4* __restrict__ c, uint32_t pid, const McpgCarry& v) {
+    float4* p = c + 5 * size_t(pid);
+    p[0] = make_float4(v.sel.w_tgt.x, v.sel.w_tgt.y, v.sel.w_tgt.z, v.sel.sum_w);
+    p[1] = make_float4(v.sel.w_cos, __uint_as_float(v.sel.mv_z_n), __uint_as_float(v.sel_slot), v.score_sum);
+    p[2] = make_float4(v.pos.x, v.pos.y, v.pos.z, v.wo_p);
+    p[3] = make_float4(v.normal.x, v.normal.y, v.normal.z, __uint_as_float(v.lc.slot));
+    p[4] = make_float4(v.bsdf.x, v.bsdf.y, v.bsdf.z, __uint_as_float(v.lc.key));
+}
+__device__ __forceinline__ McpgCarry carry_load(const float4* __restrict__ c, uint32_t pid) {
+    const float4* p = c + 5 * size_t(pid);
+    const float4 a = p[0], b = p[1], d = p[2], e = p[3], f = p[4];
+    McpgCarry v;
+    // mv.xy and T are not carried: the update overwrites them before a store
+    v.sel = McState{{a.x, a.y, a.z}, a.w, b.x, 0u, __float_as_uint(b.y), 0.f};
+    v.sel_slot = __float_as_uint(b.z);
+    v.score_sum = b.w;
+    v.pos = {d.x, d.y, d.z};
+    v.wo_p = d.w;
+    v.normal = {e.x, e.y, e.z};
+    v.lc = GridPos{__float_as_uint(e.w), __float_as_uint(f.w)};
+    v.bsdf = {f.x, f.y, f.z};
+    return v;
+}
+
+// The end of a scatter vertex's loop iteration (render_pt_mcpg.slang:161-178): the light
+// cache learns the incident radiance over the sampling pdf (floored at lc_min_pdf); with
+// probability mc_f * mc_samples / score_sum the chain takes the sample and is stored at the
+// vertex, else the missing-light heuristic may kill the cell the lobe came from.
+__device__ __forceinline__ void mcpg_complete(const McpgArgs& M, McpgCarry& c, f3 lc_incident, f3 next_pos, f3 wo,
+                                              f3 cam, f3 prev_cam, uint32_t frame, PCG& rng) {
+    const f3 lb = lc_incident * c.bsdf;
+    const float mc_f = yuv_luminance(lb / c.wo_p);
+    if (!isfinite(mc_f)) return;
+    lc_update(M.lc, c.pos, c.normal, lb / fmaxf(c.wo_p, M.lc_min_pdf), cam, prev_cam, frame, rng, c.lc);
+    if (rng.next_float() * c.score_sum < mc_f * float(M.samples)) {
+        c.sel.add_sample(c.pos, mc_f, next_pos, 0u, 0u, M.time); // static scene: zero motion
+        mc_store(M.mc, c.sel, c.pos, c.normal, cam, frame, rng);
+    } else if (M.missing_light && c.sel_slot != GRID_INVALID_SLOT && c.sel.light_missing(mc_f, wo, c.pos, M.prior)) {
+        mc_kill(M.mc, c.sel_slot);
+    }
+}
+
+// Shades vertex `bounce` of every queued MCPG path (trace_path, render_pt_mcpg.slang:39-182).
+// A continuation hit first completes the vertex it was scattered from: incident radiance
+// (the hit's emission, else the light cache there), throughput and radiance, then that
+// vertex's cache and chain updates. A vertex that scatters picks a lobe by RIS over
+// M.samples chain lookups, samples it with probability p_guide (else the BSDF), weighs the
+// direction by the stochastic-MIS pdf and carries its update inputs to the next bounce.
+__global__ __launch_bounds__(SHADE_BLOCK) void k_shade_mcpg(DeviceScene S, PtArgs A, McpgArgs M, int bounce,
+                                                            PathBuffers P, const float4* __restrict__ qo,
+                                                            const float4* __restrict__ qd,
+                                                            const uint32_t* __restrict__ count_in,
+                                                            float4* __restrict__ no, float4* __restrict__ nd,
+                                                            uint32_t* __restrict__ count_out) {
+    __shared__ float4 s_stage_o[SHADE_TILE], s_stage_d[SHADE_TILE];
+    __shared__ uint32_t s_meta[3];
+    const RayStage stage{s_stage_o, s_stage_d, s_meta};
+    if (threadIdx.x == 0) s_meta[0] = 0u;
+    __syncthreads();
+    const uint32_t n = *count_in;
+    const bool last = bounce + 1 >= A.max_path_length;
+    const f3 cam{M.cam[0], M.cam[1], M.cam[2]}, prev_cam{M.prev_cam[0], M.prev_cam[1], M.prev_cam[2]};
+    for (uint32_t tile = blockIdx.x * SHADE_TILE; tile < n; tile += gridDim.x * SHADE_TILE) {
+        for (int round = 0; round < SHADE_ROUNDS; round++) {
+            const uint32_t i = tile + uint32_t(round) * SHADE_BLOCK + threadIdx.x;
+            bool push = false;
+            f3 next_o{0, 0, 0}, next_d{0, 0, 0};
+            uint32_t pid = 0;
+            if (i < n) {
+                const float4 o4 = ld_stream(qo + i), d4 = ld_stream(qd + i), h4 = ld_stream(P.hits + i);
+                pid = __float_as_uint(o4.w);
+                const f3 ro{o4.x, o4.y, o4.z}, rd{d4.x, d4.y, d4.z};
+                const uint32_t gid = __float_as_uint(h4.w);
+                const float4 sa = ld_stream(P.state + 2 * size_t(pid)), sb = ld_stream(P.state + 2 * size_t(pid) + 1);
+                f3 thr{sa.x, sa.y, sa.z};
+                float ior = sa.w;
+                f3 rad{sb.x, sb.y, sb.z};
+                PCG rng{__float_as_uint(sb.w)};
+                ShadingPoint sp;
+                shading_point(S, ro, rd, gid, h4.y, h4.z, ior, bounce == 0 || !last, sp);
+                GridPos lc_here = invalid_grid_pos();
+                if (bounce == 0) {
+                    rad = A.emission_on_primary ? thr * sp.emission : f3{0.f, 0.f, 0.f};
+                } else {
+                    McpgCarry c = carry_load(M.carry, pid);
+                    const f3 incident = sp.emission;
+                    // evaluated whether or not its value is used, as the reference's select()
+                    const f3 cached = lc_get(M.lc, sp.pos, sp.frame.r2, cam, prev_cam, rng, lc_here);
+                    const bool direct = incident.x > 0.f || incident.y > 0.f || incident.z > 0.f ||
+                                        (!M.lc_tail && A.max_path_length == 2);
+                    const f3 lc_incident = direct ? incident : cached;
+                    thr = thr * (c.bsdf / c.wo_p);
+                    rad = rad + thr * ((M.lc_tail && bounce == A.max_path_length - 1) ? lc_incident : incident);
+                    mcpg_complete(M, c, lc_incident, sp.pos, rd, cam, prev_cam, A.frame, rng);
+                }
+                if (!last && sp.has_material &&
+                    !(thr.x < THROUGHPUT_THRESHOLD && thr.y < THROUGHPUT_THRESHOLD && thr.z < THROUGHPUT_THRESHOLD)) {
+                    const f3 cur = sp.pos, normal = sp.frame.r2;
+                    const f3 wi_local = sp.frame.to_local(-rd);
+                    // resample one lobe from the chains (RIS); slot 0 holds the selected one
+                    McState sel = mc_zero();
+                    uint32_t sel_slot = GRID_INVALID_SLOT;
+                    float score_sum = 0.f;
+                    float scores[MCPG_MAX_SAMPLES];
+                    Vmf vmfs[MCPG_MAX_SAMPLES];
+#pragma unroll
+                    for (int k = 0; k < MCPG_MAX_SAMPLES; k++) {
+                        scores[k] = 0.f;
+                        vmfs[k] = Vmf{0.f, {0.f, 0.f, 0.f}};
+                    }
+#pragma unroll
+                    for (int k = 0; k < MCPG_MAX_SAMPLES; k++) {
+                        if (k < M.samples) {
+                            GridPos gp;
+                            const McState s = mc_load(M.mc, cur, normal, cam, M.time, rng, gp);
+                            score_sum += s.sum_w;
+                            if (rng.next_float() < s.sum_w / score_sum) {
+                                sel = s;
+                                sel_slot = gp.slot;
+                                vmfs[k] = vmfs[0];
+                                scores[k] = scores[0];
+                                scores[0] = s.sum_w;
+                                vmfs[0] = mc_vmf(s, cur, M.prior);
+                            } else {
+                                scores[k] = s.sum_w;
+                                vmfs[k] = mc_vmf(s, cur, M.prior);
+                            }
+                        }
+                    }
+                    const float bsdf_alpha = sp.mat.alpha();
+                    const bool can_guide = score_sum > 0.f && bsdf_alpha >= M.alpha_threshold;
+                    const float p_guide = M.roughness ? M.p_guiding * bsdf_alpha : M.p_guiding;
+                    const bool guide = can_guide && rng.next_float() < p_guide;
+                    f3 wo_world{0, 0, 0}, wo_local{0, 0, 0};
+                    bool ok = true;
+                    if (guide) {
+                        wo_world = vmfs[0].sample(rng);
+                        wo_local = sp.frame.to_local(wo_world);
+                    } else {
+                        const Sample s = sp.mat.sample_eval(wi_local, rng); // BSDF.sample: same draws
+                        ok = s.valid;
+                        if (ok) {
+                            wo_local = s.wo;
+                            wo_world = sp.frame.to_world(s.wo);
+                            sel = mc_zero();
+                            sel_slot = GRID_INVALID_SLOT;
+                            (void)grid_target_level(M.mc, cur, cam, rng); // the reference's unused level draw
+                        }
+                    }
+                    if (ok) {
+                        float mix = 0.f;
+                        if (can_guide) {
+#pragma unroll
+                            for (int k = 0; k < MCPG_MAX_SAMPLES; k++)
+                                if (k < M.samples) mix += scores[k] * vmfs[k].pdf(wo_world);
+                            mix /= score_sum;
+                        }
+                        const float bsdf_w = can_guide ? (1.0f - p_guide) : 1.0f;
+                        const float wo_p = bsdf_w * sp.mat.pdf(wi_local, wo_local) + (can_guide ? p_guide : 0.f) * mix;
+                        const f3 bsdf = sp.mat.eval(wi_local, wo_local);
+                        const bool transmission = wi_local.z * wo_local.z < 0.f;
+                        bool has_next = false;
+                        if (bsdf.x > 0.f || bsdf.y > 0.f || bsdf.z > 0.f) { // ShadingPoint.make_ray
+                            const f3 ng = decode_normal(sp.enc_face_normal);
+                            const float wo_ng = dot(wo_world, ng);
+                            if (((dot(-rd, ng) < 0.f) != (wo_ng < 0.f)) == transmission) {
+                                has_next = true;
+                                next_o = offset_ray_origin(sp.pos, wo_ng < 0.f ? -ng : ng);
+                                next_d = wo_world;
+                            }
+                        }
+                        McpgCarry c{sel, sel_slot, score_sum, cur, wo_p, normal, lc_here, bsdf};
+                        if (has_next) {
+                            if (transmission) ior /= sp.mat.eta;
+                            carry_store(M.carry, pid, c);
+                            push = true;
+                        } else {
+                            mcpg_complete(M, c, f3{0.f, 0.f, 0.f}, cur, wo_world, cam, prev_cam, A.frame, rng);
+                        }
+                    }
+                }
+                st_stream(P.state + 2 * size_t(pid), make_float4(thr.x, thr.y, thr.z, ior));
+                st_stream(P.state + 2 * size_t(pid) + 1, make_float4(rad.x, rad.y, rad.z, __uint_as_float(rng.state)));
+            }
+            stage_push(stage, push, make_float4(next_o.x, next_o.y, next_o.z, __uint_as_float(pid)),
+                       make_float4(next_d.x, next_d.y, next_d.z, 0.f));
+        }
+        stage_flush(stage, no, nd, count_out);
+    }
+}
+
 // Training records of the guide from the first chunk's sample-0 paths, once their radiance
 // is final: incident radiance along the sampled direction = (final radiance - radiance before
 // the vertex's continuation) / throughput after it; weight = its luminance / sampling pdf.
@@ -1063,10 +1294,14 @@ hipError_t launch_gbuffer(const DeviceScene& s, const TraceScratch& ts, const mr
 
 hipError_t launch_pt_frame(const DeviceScene& s, const TraceScratch& ts, PtArgs a, const mrt_packed_hit* gbuffer,
                            PathBuffers& pb, float4* out, uint32_t* sig_out, hipStream_t st, FrameTiming* timing,
-                           uint64_t* continuation_segments, unsigned long long* trav_counters, const GuideNet* guide) {
+                           uint64_t* continuation_segments, unsigned long long* trav_counters, const GuideNet* guide,
+                           const McpgArgs* mcpg) {
     const int L = a.max_path_length;
     const bool guided = a.mode == MRT_MODE_GUIDED;
     if (guided && (!guide || !pb.lobes || !pb.grec || !pb.gtrain || !pb.grec_count)) return hipErrorInvalidValue;
+    const bool mc = a.mode == MRT_MODE_MCPG;
+    if (mc && (!mcpg || !mcpg->carry || mcpg->samples < 0 || mcpg->samples > MCPG_MAX_SAMPLES))
+        return hipErrorInvalidValue;
     if (guided) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, pb.grec_count, 1u);
     const uint32_t spp = uint32_t(a.spp > 0 ? a.spp : 0);
     if (spp == 0 || a.npix == 0) {
@@ -1081,7 +1316,9 @@ hipError_t launch_pt_frame(const DeviceScene& s, const TraceScratch& ts, PtArgs 
         }
         return hipGetLastError();
     }
-    const uint32_t chunk = uint32_t(pb.capacity / a.npix) < spp ? uint32_t(pb.capacity / a.npix) : spp;
+    // MCPG: the samples of a pixel follow each other on one RNG stream, one pass per sample
+    const uint32_t chunk = mc ? 1u : (uint32_t(pb.capacity / a.npix) < spp ? uint32_t(pb.capacity / a.npix) : spp);
+    if (mc && pb.capacity < a.npix) return hipErrorInvalidValue;
     if (chunk == 0) return hipErrorInvalidValue;
     const uint32_t n_chunks = (spp + chunk - 1) / chunk;
     const uint32_t per_chunk = uint32_t(L + 1);
@@ -1115,7 +1352,10 @@ hipError_t launch_pt_frame(const DeviceScene& s, const TraceScratch& ts, PtArgs 
         // BSDF mode shades the primary vertices per pixel (k_shade_primary); the other modes
         // seed the path queue and shade bounce 0 like every other bounce
         const bool fused_primary = a.mode == MRT_MODE_BSDF;
-        if (fused_primary) {
+        if (mc) {
+            hipLaunchKernelGGL(k_mcpg_init, dim3(grid_for(a.npix, 256, MAX_BLOCKS)), dim3(256), 0, st, s, a, gbuffer, pb,
+                               cnt);
+        } else if (fused_primary) {
             mark(-1);
             hipLaunchKernelGGL(k_shade_primary, dim3(grid_for(a.npix, SHADE_BLOCK, MAX_BLOCKS)), dim3(SHADE_BLOCK), 0,
                                st, s, a, gbuffer, pb, pb.q_o[1], pb.q_d[1], cnt + 1);
@@ -1148,7 +1388,10 @@ hipError_t launch_pt_frame(const DeviceScene& s, const TraceScratch& ts, PtArgs 
             }
             mark(-1);
             const dim3 sg(grid_for(a.n_paths, SHADE_TILE, MAX_BLOCKS));
-            if (nee) {
+            if (mc) {
+                hipLaunchKernelGGL(k_shade_mcpg, sg, dim3(SHADE_BLOCK), 0, st, s, a, *mcpg, b, pb, pb.q_o[qi],
+                                   pb.q_d[qi], cnt + b, pb.q_o[qi ^ 1], pb.q_d[qi ^ 1], cnt + b + 1);
+            } else if (nee) {
                 hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<MRT_MODE_NEE>), sg, dim3(SHADE_BLOCK), 0, st, s, a, b, pb,
                                    pb.q_o[qi], pb.q_d[qi], cnt + b, pb.q_o[qi ^ 1], pb.q_d[qi ^ 1], cnt + b + 1,
                                    scnt + b);

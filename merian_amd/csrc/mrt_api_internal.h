@@ -50,10 +50,16 @@ TraceScratch scene_scratch(const mrt_ctx* c, const mrt_scene* s);
 // Rebuilds and uploads the scene's BVH with the given node width (4 or 8), leaf size and SAH
 // traversal cost.
 mrt_status rebuild_bvh(mrt_ctx* c, mrt_scene* s, uint32_t width, uint32_t max_leaf, float traversal_cost);
-// One pt frame on the device (mrt_pt_render_dev and, with a guide, mrt_guided_render_dev).
+// One pt frame on the device (mrt_pt_render_dev; with a guide, mrt_guided_render_dev; with
+// MCPG state, mrt_mcpg_render_dev).
 mrt_status render_frame_dev(mrt_ctx* c, mrt_scene* s, const mrt_pt_params* prm, const mrt_camera* cam, uint32_t frame,
                             uint32_t w, uint32_t h, const mrt_packed_hit* gbuffer, float* irradiance,
-                            uint32_t* signature, mrt_pt_stats* stats, void* stream, mrt_guide* guide);
+                            uint32_t* signature, mrt_pt_stats* stats, void* stream, mrt_guide* guide,
+                            mrt_mcpg* mcpg = nullptr);
+// MCPG hooks of render_frame_dev (mcpg_api.cpp): the carry buffer for `paths` paths and the
+// frame's kernel arguments; after the frame, its camera becomes the previous one.
+mrt_status mcpg_prepare_frame(mrt_ctx* c, mrt_mcpg* m, uint64_t paths, const mrt_camera& cam, McpgArgs& a);
+void mcpg_end_frame(mrt_mcpg* m, const mrt_camera& cam);
 // Guide hooks of render_frame_dev (guide_api.cpp): buffers for `paths` paths in flight, and the
 // training of the frame's records after it was enqueued.
 mrt_status guide_prepare_frame(mrt_ctx* c, mrt_guide* g, uint64_t paths, PathBuffers& pb, float& fraction,

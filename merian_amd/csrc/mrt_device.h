@@ -69,6 +69,17 @@ MRT_HD uint32_t pcg4d16(uint32_t x, uint32_t y, uint32_t z, uint32_t w) {
     x += y * w;
     return x;
 }
+// pcg3d16 (utils/hash.slang:83-90): the MCPG renderer's per-pixel RNG seed
+MRT_HD uint32_t pcg3d16(uint32_t x, uint32_t y, uint32_t z) {
+    x = x * 1664525u + 1013904223u;
+    y = y * 1664525u + 1013904223u;
+    z = z * 1664525u + 1013904223u;
+    x += y * z;
+    y += z * x;
+    z += x * y;
+    x += y * z;
+    return x;
+}
 MRT_HD float unit_float(uint32_t u) { return float(u >> 8u) * 5.9604644775390625e-08f; }
 struct PCG {
     uint32_t state;
@@ -254,6 +265,7 @@ MRT_HD float vndf_pdf_wo(f3 wi, f3 h, f2 a) { return (ggx_ndf(h, a) * g1_over_wi
 
 struct Lambert { // brdf-lambert-diffuse.slang:9-61
     f3 albedo;
+    MRT_HD float alpha() const { return 1.0f; } // brdf-lambert-diffuse.slang:58-60
     MRT_HD f3 eval(f3 wi, f3 wo) const {
         if (fminf(wi.z, wo.z) < BSDF_MIN_COS_THETA) return {0, 0, 0};
         return (INV_PI * albedo) * wo.z;
@@ -271,6 +283,7 @@ struct Lambert { // brdf-lambert-diffuse.slang:9-61
 
 struct GGX { // brdf-ggx.slang:331-412
     f2 a;
+    MRT_HD float alpha() const { return fminf(a.x, a.y); } // brdf-ggx.slang:409-411
     MRT_HD f3 eval(f3 wi, f3 wo) const {
         if (fminf(wi.z, wo.z) < BSDF_MIN_COS_THETA) return {0, 0, 0};
         const f3 h = normalize(wo + wi);
@@ -320,6 +333,7 @@ struct GGXBTDF { // brdf-ggx.slang:416-503
 struct RoughDielectric { // bsdf-rough-dielectric.slang:13-120
     float alpha, eta;
     f3 tint;
+    MRT_HD float lobe_alpha() const { return alpha; } // the `alpha` field is the property
     MRT_HD f3 eval(f3 wi, f3 wo) const {
         if (wi.z < BSDF_MIN_COS_THETA) return {0, 0, 0};
         if (wo.z > BSDF_MIN_COS_THETA) {
@@ -379,6 +393,7 @@ struct RoughDielectric { // bsdf-rough-dielectric.slang:13-120
 struct Sheen { // brdf-sheen.slang:9-90 (Estevez & Kulla 2017)
     f3 color;
     float alpha;
+    MRT_HD float lobe_alpha() const { return alpha; } // max(alpha, 1e-6) from the constructor
     MRT_HD float D(float ndoth) const {
         const float inv_alpha = 1.f / alpha;
         const float cos2h = ndoth * ndoth;
@@ -425,10 +440,17 @@ struct Sheen { // brdf-sheen.slang:9-90 (Estevez & Kulla 2017)
 // x * (1 - w) + y * w == x for w == 0 whenever y is finite, which holds for every lobe here
 // (alpha >= 0.0316^2 keeps the GGX terms finite), so the other operand is never evaluated.
 
+// Lobe width of a BSDF (the `alpha` property of bsdf.slang:26): the MCPG renderer guides only
+// where it is at least its threshold. Combinators blend their operands' squared widths.
+template <class B> MRT_HD float lobe_alpha_of(const B& b) { return b.alpha(); }
+MRT_HD float lobe_alpha_of(const RoughDielectric& b) { return b.lobe_alpha(); }
+MRT_HD float lobe_alpha_of(const Sheen& b) { return b.lobe_alpha(); }
+
 template <class Layer, class Base> struct FresnelMix { // bsdf-fresnel-mix.slang:7-63
     Layer layer;
     Base base;
     float f0;
+    MRT_HD float alpha() const { return sqrtf(lerp(pow2(lobe_alpha_of(base)), pow2(lobe_alpha_of(layer)), f0)); }
     MRT_HD f3 eval(f3 wi, f3 wo) const {
         const f3 h = normalize(wi + wo);
         const float F = fresnel_schlick(dot(wi, h), f0);
@@ -451,6 +473,7 @@ template <class B0, class B1> struct Mix { // bsdf-mix.slang:6-65
     B0 b0;
     B1 b1;
     float weight;
+    MRT_HD float alpha() const { return sqrtf(lerp(pow2(lobe_alpha_of(b0)), pow2(lobe_alpha_of(b1)), weight)); }
     MRT_HD f3 eval(f3 wi, f3 wo) const {
         if (weight == 0.0f) return b0.eval(wi, wo);
         if (weight == 1.0f) return b1.eval(wi, wo);
@@ -482,6 +505,7 @@ template <class B0, class B1> struct Mix { // bsdf-mix.slang:6-65
 template <class Inner> struct Conductor { // bsdf-conductor-fresnel.slang:6-50
     Inner inner;
     f3 f0;
+    MRT_HD float alpha() const { return lobe_alpha_of(inner); }
     MRT_HD f3 eval(f3 wi, f3 wo) const {
         const f3 h = normalize(wi + wo);
         return fresnel_schlick(dot(wi, h), f0) * inner.eval(wi, wo);
@@ -524,6 +548,7 @@ struct MaterialSample {
     }
     MRT_HD f3 eval(f3 wi, f3 wo) const { return diffuse ? lambert.eval(wi, wo) : pbr.eval(wi, wo); }
     MRT_HD float pdf(f3 wi, f3 wo) const { return diffuse ? lambert.pdf(wi, wo) : pbr.pdf(wi, wo); }
+    MRT_HD float alpha() const { return diffuse ? lambert.alpha() : pbr.alpha(); }
 };
 
 MRT_HD f3 material_emission(const mrt_material& m, bool front_facing) {

@@ -112,6 +112,40 @@ struct PathBuffers {
 
 struct GuideNet;
 
+// One hash grid of the MCPG renderer (utils/hash-grid.slang:44-100) in device memory.
+struct HashGridDev {
+    uint2* keys;               // per slot: (cell key, frame stamp | lock bit); zero after a reset
+    void* data;                // per slot payload (chain state 32 B, light-cache entry 8 B)
+    uint32_t size;             // slots
+    uint32_t probes;           // PROBE_COUNT
+    uint32_t locality;         // LOCALITY_BITS
+    uint32_t stochastic;       // STOCHASTIC_INTERPOLATION (jittered cell)
+    uint32_t level_stochastic; // LEVEL_STOCHASTIC_INTERPOLATION (jittered level)
+    uint32_t lock_free;        // LOCK_FREE writers
+    float tan_alpha_half;      // grid_tan_alpha_half
+    float level_bias;          // grid_level_bias
+    float dist_dim;            // grid_distribution_dimension
+};
+
+// Frame arguments of the MCPG kernels (render_pt_mcpg.slang:12-30 constants, the two grids,
+// the cameras) and the per-path record that carries a scatter vertex's chain update to the
+// next bounce: 5 float4 per path (selected state, score sum, position, pdf, normal,
+// light-cache cell, BSDF value).
+struct McpgArgs {
+    HashGridDev mc, lc;
+    float4* carry;
+    int32_t samples;         // mc_samples (RIS candidates, <= 8)
+    float p_guiding;
+    int32_t roughness;       // GUIDING_DIRECTIONAL_SAMPLING_MIS_ROUGHNESS
+    float alpha_threshold;   // guiding_alpha_threshold
+    float prior;             // dir_guide_prior
+    int32_t missing_light;   // missing_light_heuristic
+    int32_t lc_tail;         // use_light_cache_tail
+    float lc_min_pdf;
+    float cam[3], prev_cam[3];
+    float time;
+};
+
 // Launch entry points (kernels.hip). All asynchronous on `stream`.
 // counters (optional, device): node visits and triangle tests are added to counters[0..1]
 hipError_t launch_trace_rays(const DeviceScene& s, const TraceScratch& ts, const mrt_ray* rays, uint64_t n,
@@ -130,9 +164,10 @@ struct FrameTiming {
 // counters_host (optional) receives every chunk's queue sizes after the frame completes.
 // guide (MRT_MODE_GUIDED): the network whose lobes are inferred for every bounce's queue; the
 // first chunk's sample-0 paths write training records (pb.grec / pb.gtrain).
+// mcpg (MRT_MODE_MCPG): grids and carry buffer; the frame runs one pass per sample.
 hipError_t launch_pt_frame(const DeviceScene& s, const TraceScratch& ts, PtArgs a, const mrt_packed_hit* gbuffer,
                            PathBuffers& pb, float4* out, uint32_t* sig_out, hipStream_t st, FrameTiming* timing,
                            uint64_t* continuation_segments, unsigned long long* trav_counters = nullptr,
-                           const GuideNet* guide = nullptr);
+                           const GuideNet* guide = nullptr, const McpgArgs* mcpg = nullptr);
 
 } // namespace mrt

@@ -318,3 +318,57 @@ class Guide:
                                                        frame, w, h, C.c_void_p(d_gbuffer), C.c_void_p(d_out),
                                                        int(train), C.byref(st) if st is not None else None, None))
         return st
+
+
+def mcpg_config(**kw) -> abi.McpgConfig:
+    """The MCPG node's defaults (mrt_mcpg_default_config) with overrides: top-level fields by
+    name, grid fields as mc_<field> / lc_<field> (e.g. mc_buffer_size=4096)."""
+    c = abi.McpgConfig()
+    abi.load_library().mrt_mcpg_default_config(C.byref(c))
+    for k, v in kw.items():
+        if k.startswith("mc_") and hasattr(c.mc, k[3:]):
+            setattr(c.mc, k[3:], v)
+        elif k.startswith("lc_") and hasattr(c.lc, k[3:]):
+            setattr(c.lc, k[3:], v)
+        elif hasattr(c, k):
+            setattr(c, k, v)
+        else:
+            raise KeyError(k)
+    return c
+
+
+class Mcpg:
+    """Markov-chain path guiding state over mrt_mcpg_* (the two hash grids on the device)."""
+
+    def __init__(self, ctx: Context, cfg: abi.McpgConfig | None = None):
+        self.ctx, self.lib = ctx, ctx.lib
+        self.cfg = cfg if cfg is not None else mcpg_config()
+        h = C.c_void_p()
+        ctx._check(self.lib.mrt_mcpg_create(ctx.h, C.byref(self.cfg), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.mrt_mcpg_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def reset(self):
+        self.ctx._check(self.lib.mrt_mcpg_reset(self.ctx.h, self.h))
+
+    def render_dev(self, scene: DeviceScene, params: abi.PtParams, cam: abi.Camera, frame: int, w: int, h: int,
+                   d_gbuffer: int, d_out: int, stats: bool = False):
+        st = abi.PtStats() if stats else None
+        self.ctx._check(self.lib.mrt_mcpg_render_dev(self.ctx.h, scene.h, self.h, C.byref(params), C.byref(cam),
+                                                     frame, w, h, C.c_void_p(d_gbuffer), C.c_void_p(d_out),
+                                                     C.byref(st) if st is not None else None, None))
+        return st
+
+    def grids(self):
+        """(mc keys uint32[n, 2], mc states MCPG_STATE_DTYPE[n], lc keys uint32[m, 2], lc entries LC_ENTRY_DTYPE[m])."""
+        n, m = self.cfg.mc.buffer_size, self.cfg.lc.buffer_size
+        mk = np.zeros((n, 2), np.uint32)
+        ms = np.zeros(n, abi.MCPG_STATE_DTYPE)
+        lk = np.zeros((m, 2), np.uint32)
+        le = np.zeros(m, abi.LC_ENTRY_DTYPE)
+        self.ctx._check(self.lib.mrt_mcpg_read_grids(self.ctx.h, self.h, _p(mk), _p(ms), _p(lk), _p(le)))
+        return mk, ms, lk, le

@@ -485,6 +485,7 @@ struct MaterialSample {
     }
     f3 eval(f3 wi, f3 wo) const { return model == MRT_MAT_DIFFUSE ? diffuse.eval(wi, wo) : pbr.eval(wi, wo); }
     float pdf(f3 wi, f3 wo) const { return model == MRT_MAT_DIFFUSE ? diffuse.pdf(wi, wo) : pbr.pdf(wi, wo); }
+    float alpha() const { return model == MRT_MAT_DIFFUSE ? diffuse.alpha() : pbr.alpha(); }
 };
 
 constexpr float MIN_ROUGHNESS = 0.0316f; // material.slang:23
@@ -681,6 +682,17 @@ inline float yuv_luminance(f3 c) { return dot(c, f3{0.299f, 0.587f, 0.114f}); }
 inline uint32_t sig_step(uint32_t h, uint32_t id) { return (h ^ (id + 0x7F4A7C15u)) * 0x01000193u; }
 
 // trace_path (render_pt.slang:24-90)
+// pcg3d16 (utils/hash.slang:83-90): the MCPG renderer's per-pixel seed
+inline uint32_t pcg3d16_(uint32_t x, uint32_t y, uint32_t z) {
+    const uint32_t a = 1664525u, c = 1013904223u;
+    uint32_t v[3] = {x * a + c, y * a + c, z * a + c};
+    v[0] += v[1] * v[2];
+    v[1] += v[2] * v[0];
+    v[2] += v[0] * v[1];
+    v[0] += v[1] * v[2];
+    return v[0];
+}
+
 inline f3 trace_path(const Scene& sc, const mrt_pt_params& prm, Ray ray, ShadingPoint sp, PCG& rng,
                      uint32_t& sig, uint64_t& segs) {
     float current_ior = 1.0f;
@@ -1016,3 +1028,5 @@ __attribute__((visibility("default"))) void orc_bsdf(const orc_bsdf_cfg* c, int 
 }
 
 } // extern "C"
+
+#include "oracle_mcpg.h"

@@ -70,6 +70,14 @@ def lib() -> C.CDLL:
             "orc_guide_sh": (None, [vp, vp]),
             "orc_guide_mixture_pdf": (C.c_double, [vp, vp, vp, vp, vp]),
             "orc_vmf_pdf": (C.c_double, [C.c_double, C.c_double]),
+            "orc_mcpg_create": (vp, [C.POINTER(abi.McpgConfig)]),
+            "orc_mcpg_destroy": (None, [vp]),
+            "orc_mcpg_grids": (None, [vp, vp, vp, vp, vp]),
+            "orc_mcpg_render": (u64, [vp, vp, C.POINTER(abi.PtParams), C.POINTER(abi.Camera), u32, u32, u32, vp,
+                                      vp]),
+            "orc_mcpg_probe": (None, [i32, vp, u64, vp, u32]),
+            "orc_mcpg_cells": (None, [C.POINTER(abi.McpgGridConfig), vp, vp, vp, u64, vp]),
+            "orc_mcpg_add_sample": (None, [vp, vp, f32, vp, f32, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -274,3 +282,79 @@ def guide_mixture(out40: np.ndarray, d) -> tuple[float, np.ndarray, np.ndarray, 
 
 def vmf_pdf(kappa: float, cos_mu: float) -> float:
     return lib().orc_vmf_pdf(kappa, cos_mu)
+
+
+# ---- Markov-chain path guiding (oracle_mcpg.h) ------------------------------------------
+# orc_mcpg_cfg has the byte layout of mrt_mcpg_config (abi.McpgConfig).
+
+class McpgOracle:
+    """The MCPG renderer's state and frames on the CPU, one pixel and sample after another."""
+
+    def __init__(self, cfg: abi.McpgConfig):
+        self.cfg = cfg
+        self.h = lib().orc_mcpg_create(C.byref(cfg))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_mcpg_destroy(self.h)
+            self.h = None
+
+    def render(self, scene: OracleScene, params: abi.PtParams, cam, frame: int, w: int, h: int, gbuffer):
+        out = np.zeros((h, w, 4), np.float32)
+        gb = np.ascontiguousarray(gbuffer)
+        segs = lib().orc_mcpg_render(scene.h, self.h, C.byref(params), C.byref(cam), frame, w, h, _p(gb), _p(out))
+        return out, int(segs)
+
+    def grids(self):
+        n, m = self.cfg.mc.buffer_size, self.cfg.lc.buffer_size
+        mk = np.zeros((n, 2), np.uint32)
+        ms = np.zeros(n, abi.MCPG_STATE_DTYPE)
+        lk = np.zeros((m, 2), np.uint32)
+        le = np.zeros(m, abi.LC_ENTRY_DTYPE)
+        lib().orc_mcpg_grids(self.h, _p(mk), _p(ms), _p(lk), _p(le))
+        return mk, ms, lk, le
+
+
+def vmf_pdf_tokuyoshi(kappa, mu, w) -> np.ndarray:
+    """The oracle's vMF pdf (float32) for rows kappa[n], mu[n, 3], w[n, 3]."""
+    a = np.zeros((len(w), 7), np.float32)
+    a[:, 0], a[:, 1:4], a[:, 4:7] = kappa, mu, w
+    out = np.zeros(len(w), np.float32)
+    lib().orc_mcpg_probe(0, _p(a), len(w), _p(out), 0)
+    return out
+
+
+def vmf_sample_tokuyoshi(kappa, mu, seed: int = 1) -> np.ndarray:
+    a = np.zeros((len(kappa), 4), np.float32)
+    a[:, 0], a[:, 1:4] = kappa, mu
+    out = np.zeros((len(kappa), 3), np.float32)
+    lib().orc_mcpg_probe(1, _p(a), len(kappa), _p(out), seed)
+    return out
+
+
+def mcpg_scalar(op: str, x) -> np.ndarray:
+    """log1p / expm1 / x_over_expm1 as the oracle evaluates them (float32)."""
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.zeros_like(x)
+    lib().orc_mcpg_probe({"log1p": 2, "expm1": 3, "x_over_expm1": 4}[op], _p(x), len(x), _p(out), 0)
+    return out
+
+
+def mcpg_cells(grid: abi.McpgGridConfig, pos, nrm, levels) -> np.ndarray:
+    """(slot, key) of get_hash_grid_pos for each position / normal / level (exact cells)."""
+    P = np.ascontiguousarray(pos, np.float32)
+    N = np.ascontiguousarray(nrm, np.float32)
+    L = np.ascontiguousarray(levels, np.uint32)
+    out = np.zeros((len(P), 2), np.uint32)
+    lib().orc_mcpg_cells(C.byref(grid), _p(P), _p(N), _p(L), len(P), _p(out))
+    return out
+
+
+def mcpg_add_sample(state: np.ndarray, pos, w: float, tgt, prior: float = 0.2):
+    """Applies MCState.add_sample to a copy of a 32-byte state; returns (state, (kappa, mu))."""
+    s = np.array(state, dtype=abi.MCPG_STATE_DTYPE).copy()
+    p = np.ascontiguousarray(pos, np.float32)
+    t = np.ascontiguousarray(tgt, np.float32)
+    kv = np.zeros(4, np.float32)
+    lib().orc_mcpg_add_sample(_p(s), _p(p), w, _p(t), prior, _p(kv))
+    return s, kv

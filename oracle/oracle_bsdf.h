@@ -166,6 +166,7 @@ inline float vndf_pdf_wo(f3 wi, f3 h, f2 a) { return (ggx_ndf(h, a) * g1_over_wi
 // brdf-lambert-diffuse.slang:9-61
 struct Lambert {
     f3 albedo;
+    float alpha() const { return 1.0f; }
     f3 eval(f3 wi, f3 wo) const {
         if (std::fmin(wi.z, wo.z) < BSDF_MIN_COS_THETA) return {0, 0, 0};
         return (INV_PI * albedo) * wo.z;
@@ -184,6 +185,7 @@ struct Lambert {
 // brdf-ggx.slang:331-412
 struct GGX {
     f2 a;
+    float alpha() const { return std::fmin(a.x, a.y); }
     f3 eval(f3 wi, f3 wo) const {
         if (std::fmin(wi.z, wo.z) < BSDF_MIN_COS_THETA) return {0, 0, 0};
         const f3 h = normalize(wo + wi);
@@ -251,6 +253,7 @@ struct GGXBTDF {
 struct RoughDielectric {
     float alpha, eta;
     f3 tint;
+    float width() const { return alpha; }
     f3 eval(f3 wi, f3 wo) const {
         if (wi.z < BSDF_MIN_COS_THETA) return {0, 0, 0};
         if (wo.z > BSDF_MIN_COS_THETA) {
@@ -311,6 +314,7 @@ struct RoughDielectric {
 struct Sheen {
     f3 color;
     float alpha;
+    float width() const { return alpha; }
     static Sheen make(f3 c, float a) { return Sheen{c, std::fmax(a, 1e-6f)}; }
     float D(float ndoth) const {
         const float inv_alpha = 1.f / alpha;
@@ -354,11 +358,17 @@ struct Sheen {
     }
 };
 
+// The `alpha` property (bsdf.slang:26): lobe width; combinators interpolate squared widths.
+template <class B> inline float width_of(const B& b) { return b.alpha(); }
+inline float width_of(const RoughDielectric& b) { return b.width(); }
+inline float width_of(const Sheen& b) { return b.width(); }
+
 // bsdf-fresnel-mix.slang:7-63
 template <class Layer, class Base> struct FresnelMix {
     Layer layer;
     Base base;
     float f0;
+    float alpha() const { return std::sqrt(lerp(pow2(width_of(base)), pow2(width_of(layer)), f0)); }
     f3 eval(f3 wi, f3 wo) const {
         const f3 h = normalize(wi + wo);
         const float F = fresnel_schlick(dot(wi, h), f0);
@@ -382,6 +392,7 @@ template <class B0, class B1> struct Mix {
     B0 b0;
     B1 b1;
     float weight;
+    float alpha() const { return std::sqrt(lerp(pow2(width_of(b0)), pow2(width_of(b1)), weight)); }
     f3 eval(f3 wi, f3 wo) const { return lerp(b0.eval(wi, wo), b1.eval(wi, wo), weight); }
     float pdf(f3 wi, f3 wo) const { return lerp(b0.pdf(wi, wo), b1.pdf(wi, wo), weight); }
     Sample sample_eval(f3 wi, PCG& rng) const {
@@ -404,6 +415,7 @@ template <class B0, class B1> struct Mix {
 template <class Inner> struct Conductor {
     Inner inner;
     f3 f0;
+    float alpha() const { return width_of(inner); }
     f3 eval(f3 wi, f3 wo) const {
         const f3 h = normalize(wi + wo);
         return fresnel_schlick(dot(wi, h), f0) * inner.eval(wi, wo);

@@ -1,0 +1,216 @@
+"""Markov-chain path guiding on the GPU (mrt_mcpg_*) against the CPU oracle and the plain pt.
+
+- One path in flight (a 1x1 image): the GPU's sequence of grid reads and writes is the oracle's,
+  so radiance and both grids (keys, stamps, chain states, light-cache entries) must be
+  bit-identical after every frame; small grids force collisions and evictions, moving cameras
+  exercise the previous-camera lookups.
+- Whole images: paths of a pass update the grids concurrently (as the reference's threads do),
+  so images are checked statistically: under a constant environment a plane's expected
+  radiance is exactly its albedo, and under a sun the image mean agrees with a converged plain-pt
+  reference within the frame-to-frame spread. The relMSE against the plain pt is reported
+  (printed here; bench.py measures it at equal time).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from merian_amd import abi, scenes
+from merian_amd.runtime import Mcpg, MrtError, mcpg_config, pt_params
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _cameras(sc, n):
+    """n 1x1 cameras looking at points spread over the scene from the scene's eye."""
+    c = sc.camera
+    eye, tgt = np.array(c.eye, np.float64), np.array(c.target, np.float64)
+    fwd = tgt - eye
+    rng = np.random.default_rng(3)
+    out = []
+    for k in range(n):
+        d = fwd + np.cross(fwd, c.up) * rng.uniform(-0.35, 0.35) + np.array(c.up) * rng.uniform(-0.25, 0.2) * np.linalg.norm(fwd)
+        out.append(scenes.CameraSpec(c.eye, tuple(eye + d), c.up, c.vfov, 1, 1, jitter=abi.JITTER_NONE))
+    return out
+
+
+@pytest.fixture(scope="module")
+def room(ctx):
+    sc = scenes.full_scene(seed=5, instances=8, env_size=(64, 32), res=(64, 36))
+    ds = ctx.scene(sc)
+    yield sc, ds, O.OracleScene(sc)
+    ds.destroy()
+
+
+def _frame(ctx, ds, m, p, cam, frame, w, h):
+    import torch
+    gb = torch.empty(w * h * 16, dtype=torch.uint8, device="cuda")
+    out = torch.empty(w * h * 4, dtype=torch.float32, device="cuda")
+    ctx.gbuffer_dev(ds, cam, w, h, gb.data_ptr())
+    st = m.render_dev(ds, p, cam, frame, w, h, gb.data_ptr(), out.data_ptr(), stats=True)
+    torch.cuda.synchronize()
+    gbh = gb.cpu().numpy().view(abi.PACKED_HIT_DTYPE).copy()
+    return out.cpu().numpy().reshape(h, w, 4), gbh, st
+
+
+CONFIGS = [
+    # small grids: collisions, evictions, Morton tiles (4096 >> 9 = 8 tiles); a high guiding
+    # probability so that chains continue (N >= 2) and the missing-light heuristic kills cells
+    dict(mc_buffer_size=4096, lc_buffer_size=1024, p_guiding=0.9, sampling_type=abi.MCPG_MIS),
+    # scattered cells, 3 probes (no probe alignment), jittered light-cache cells, cache tail,
+    # no missing-light heuristic
+    dict(mc_buffer_size=1531, mc_probe_count=3, mc_locality_bits=0, lc_buffer_size=509, lc_probe_count=3,
+         lc_locality_bits=0, lc_stochastic_interpolation=1, use_light_cache_tail=1, sampling_type=abi.MCPG_MIS,
+         missing_light_heuristic=0, mc_samples=3, p_guiding=0.9),
+]
+
+
+@pytest.mark.parametrize("cfg_i", range(len(CONFIGS)))
+def test_single_path_sequence_matches_oracle(ctx, room, cfg_i):
+    sc, ds, osc = room
+    cfg = mcpg_config(**CONFIGS[cfg_i])
+    m = Mcpg(ctx, cfg)
+    om = O.McpgOracle(cfg)
+    try:
+        p = pt_params(spp=8, max_path_length=6, record_signature=False)
+        cams = _cameras(sc, 12)
+        total_segs = 0
+        for f in range(48):
+            spec = cams[f // 4]  # the camera changes every fourth frame
+            cam = ctx.camera(spec, f)
+            img, gb, st = _frame(ctx, ds, m, p, cam, f, 1, 1)
+            ref, segs = om.render(osc, p, cam, f, 1, 1, gb)
+            assert img.view(np.uint32).tolist() == ref.view(np.uint32).tolist(), (f, img, ref)
+            assert int(st.continuation_segments) == segs, (f, st.continuation_segments, segs)
+            total_segs += segs
+        g, o = m.grids(), om.grids()
+        for name, a, b in zip(("mc keys", "mc states", "lc keys", "lc entries"), g, o):
+            assert a.tobytes() == b.tobytes(), (name, int((a != b).sum()))
+        mk, ms, lk, le = g
+        # the run exercised the grids: many cells, continued chains, cache means, kills
+        assert (ms["sum_w"] > 0).sum() > 50 and ms["N"].max() >= 2, ms["N"].max()
+        assert (le["N"] > 1).sum() > 10
+        if CONFIGS[cfg_i].get("missing_light_heuristic", 1):
+            assert ((ms["N"] > 0) & (ms["sum_w"] == 0)).any()
+        assert total_segs > 300
+    finally:
+        m.close()
+
+
+def test_config_validation_and_reset(ctx, room):
+    sc, ds, _ = room
+    for bad in (dict(mc_samples=9), dict(mc_probe_count=0), dict(lc_locality_bits=6), dict(p_guiding=1.5),
+                dict(mc_tan_alpha_half=0.0), dict(sampling_type=2), dict(lc_distribution_dimension=0.0)):
+        with pytest.raises(MrtError):
+            Mcpg(ctx, mcpg_config(mc_buffer_size=1024, lc_buffer_size=1024, **bad))
+    m = Mcpg(ctx, mcpg_config(mc_buffer_size=2048, lc_buffer_size=1024))
+    try:
+        cam = ctx.camera(scenes.CameraSpec(sc.camera.eye, sc.camera.target, sc.camera.up, sc.camera.vfov, 8, 8), 0)
+        _frame(ctx, ds, m, pt_params(spp=2, max_path_length=4), cam, 0, 8, 8)
+        assert (m.grids()[1]["sum_w"] > 0).any()
+        m.reset()
+        assert all(not np.ascontiguousarray(a).view(np.uint8).any() for a in m.grids())
+        # spp = 0 writes zeros
+        img, _, _ = _frame(ctx, ds, m, pt_params(spp=0, max_path_length=4), cam, 1, 8, 8)
+        assert not img.any()
+    finally:
+        m.close()
+
+
+W_, H_ = 192, 108
+
+
+def _frames(ctx, ds, cam, params, frames, m=None, f0=1000):
+    """Per-frame images (float64) of `frames` frames on a fixed camera."""
+    import torch
+    gb = torch.empty(W_ * H_ * 16, dtype=torch.uint8, device="cuda")
+    out = torch.empty(W_ * H_ * 4, dtype=torch.float32, device="cuda")
+    ctx.gbuffer_dev(ds, cam, W_, H_, gb.data_ptr())
+    imgs = []
+    for f in range(frames):
+        if m is None:
+            ctx.pt_render_dev(ds, params, cam, f0 + f, W_, H_, gb.data_ptr(), out.data_ptr())
+        else:
+            m.render_dev(ds, params, cam, f0 + f, W_, H_, gb.data_ptr(), out.data_ptr())
+        torch.cuda.synchronize()
+        imgs.append(out.cpu().numpy().reshape(H_, W_, 4).astype(np.float64))
+    return np.array(imgs)
+
+
+def _lum(img):
+    return img[..., 0] * 0.299 + img[..., 1] * 0.587 + img[..., 2] * 0.114
+
+
+def _plane(env):
+    """A 100 x 100 Lambert plane (albedo 0.5) seen from above under an environment."""
+    b = scenes.SceneBuilder("plane")
+    g = b.add_material(scenes.diffuse_material((0.5, 0.5, 0.5)))
+    P, N, T = scenes.quad((-50, 0, -50), (-50, 0, 50), (50, 0, 50), (50, 0, -50))
+    b.add_mesh(P, N, T, g)
+    cam = scenes.CameraSpec((0.0, 3.0, 6.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), math.radians(50.0), W_, H_,
+                            jitter=abi.JITTER_NONE)
+    return b.build(camera=cam, **env)
+
+
+def test_mcpg_furnace_expectation_is_exact(ctx):
+    # under a constant unit environment every plane pixel has expected radiance = albedo = 0.5
+    # for any path length >= 2 (the plane never sees itself); the plain pt estimates it with no
+    # variance, MCPG's guided directions are reweighted by the mixture pdf and must average to it
+    sc = _plane(scenes.constant_env())
+    ds = ctx.scene(sc)
+    m = Mcpg(ctx, mcpg_config(mc_buffer_size=1 << 20, lc_buffer_size=1 << 18))
+    try:
+        cam = ctx.camera(sc.camera, 0)
+        p = pt_params(spp=4, max_path_length=4, record_signature=False)
+        plain = _frames(ctx, ds, cam, p, 1)[0]
+        on_plane = np.abs(plain[..., 0] - 0.5) < 1e-6
+        assert on_plane.mean() > 0.5 and np.all(plain[~on_plane][..., :3] == 1.0)
+        _frames(ctx, ds, cam, p, 8, m=m)  # learning
+        imgs = _frames(ctx, ds, cam, p, 32, m=m, f0=2000)
+        per = _lum(imgs)[:, on_plane].mean(1)
+        sem = per.std(ddof=1) / np.sqrt(len(per))
+        assert abs(per.mean() - 0.5) < 4 * sem + 2e-5, (per.mean(), sem)
+        assert per.std() > 0  # the guide was used (else every sample would be exactly 0.5)
+        assert np.all(imgs[:, ~on_plane][..., :3] == 1.0)
+    finally:
+        m.close()
+        ds.destroy()
+
+
+@pytest.fixture(scope="module")
+def sky(ctx):
+    sc = scenes.full_scene(seed=5, instances=16, env_size=(256, 128))
+    ds = ctx.scene(sc)
+    yield sc, ds
+    ds.destroy()
+
+
+def test_mcpg_unbiased_under_a_sun_and_relmse_reported(ctx, sky):
+    sc, ds = sky
+    spec = scenes.CameraSpec(sc.camera.eye, sc.camera.target, sc.camera.up, sc.camera.vfov, W_, H_,
+                             jitter=abi.JITTER_NONE)
+    cam = ctx.camera(spec, 0)
+    L = 6
+    ref = _frames(ctx, ds, cam, pt_params(spp=1024, max_path_length=L, record_signature=False), 8).mean(0)
+    # the MCPG example graph's grid settings (examples/renderers/mcpg.json)
+    m = Mcpg(ctx, mcpg_config(mc_buffer_size=1 << 22, lc_buffer_size=1 << 20, mc_tan_alpha_half=0.003,
+                              mc_distribution_dimension=1.5, lc_tan_alpha_half=0.012))
+    try:
+        p = pt_params(spp=4, max_path_length=L, record_signature=False)
+        _frames(ctx, ds, cam, p, 16, m=m)  # learning
+        imgs = _frames(ctx, ds, cam, p, 48, m=m, f0=2000)
+        per = _lum(imgs).mean((1, 2))
+        sem = per.std(ddof=1) / np.sqrt(len(per))
+        # the sun makes the estimator heavy-tailed: the frame-to-frame spread bounds the error
+        assert abs(per.mean() - _lum(ref).mean()) < 5 * sem, (per.mean(), _lum(ref).mean(), sem)
+        plain = _frames(ctx, ds, cam, p, 8, f0=5000).mean(0)
+
+        def rel_mse(x):
+            return float(np.mean((x[..., :3] - ref[..., :3]) ** 2 / (ref[..., :3] ** 2 + 1e-2)))
+
+        e_m, e_p = rel_mse(imgs[:8].mean(0)), rel_mse(plain)
+        print(f"equal spp (8 frames x 4 spp): mcpg relMSE {e_m:.4g}, plain {e_p:.4g}")
+        assert np.isfinite(e_m)
+    finally:
+        m.close()
