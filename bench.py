@@ -69,11 +69,13 @@ def cpu_baseline(scene, spec, frame, target_s: float, threads: int):
 def guided_leg(ctx, scene, L: int) -> dict:
     """Equal-time relMSE of guided vs plain pt on a 480x270 view of the 1080p scene (same camera,
     lower resolution), against a 16384-spp plain render of that view. The guided side trains
-    online during its frames; both sides get the same wall time (timed around whole frames)."""
+    online during its frames; both sides get the same wall time (timed around whole frames).
+    The Markov-chain guiding (MCPG, the non-learned comparator) is measured the same way, with
+    the MCPG example graph's grid settings, learning online from an empty state."""
     import numpy as np
     import torch
     from merian_amd import abi, scenes
-    from merian_amd.runtime import Guide, guide_config, pt_params
+    from merian_amd.runtime import Guide, Mcpg, guide_config, mcpg_config, pt_params
     w, h = 480, 270
     spec = scenes.CameraSpec(scene.camera.eye, scene.camera.target, scene.camera.up, scene.camera.vfov, w, h)
     ds = ctx.scene(scene)
@@ -83,16 +85,25 @@ def guided_leg(ctx, scene, L: int) -> dict:
     acc = torch.zeros(w * h * 4, dtype=torch.float64, device="cuda")
     ctx.gbuffer_dev(ds, cam, w, h, gb.data_ptr())
 
-    def run(frames, params, guide=None, train=True, base=0):
-        acc.zero_()
+    # frames and the accumulation run in order on one stream (the library's own context
+    # stream does not synchronise with torch's)
+    strm = torch.cuda.Stream()
+    sh = strm.cuda_stream
+
+    def run(frames, params, guide=None, train=True, base=0, mcpg=None):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for f in range(frames):
-            if guide is None:
-                ctx.pt_render_dev(ds, params, cam, base + f, w, h, gb.data_ptr(), out.data_ptr())
-            else:
-                guide.render_dev(ds, params, cam, base + f, w, h, gb.data_ptr(), out.data_ptr(), train=train)
-            acc.add_(out)
+        with torch.cuda.stream(strm):
+            acc.zero_()
+            for f in range(frames):
+                if mcpg is not None:
+                    mcpg.render_dev(ds, params, cam, base + f, w, h, gb.data_ptr(), out.data_ptr(), stream=sh)
+                elif guide is None:
+                    ctx.pt_render_dev(ds, params, cam, base + f, w, h, gb.data_ptr(), out.data_ptr(), stream=sh)
+                else:
+                    guide.render_dev(ds, params, cam, base + f, w, h, gb.data_ptr(), out.data_ptr(), train=train,
+                                     stream=sh)
+                acc.add_(out)
         torch.cuda.synchronize()
         return (acc / frames).cpu().numpy().reshape(h, w, 4), time.perf_counter() - t0
 
@@ -106,11 +117,19 @@ def guided_leg(ctx, scene, L: int) -> dict:
     frames_p = max(1, int(round(frames_p * t_g / t_p)))
     pimg, t_p = run(frames_p, plain)
     g.close()
+    m = Mcpg(ctx, mcpg_config(mc_tan_alpha_half=0.003, mc_distribution_dimension=1.5, lc_tan_alpha_half=0.012,
+                              lc_buffer_size=8000009, lc_probe_count=2))
+    mimg, t_m = run(8, plain, mcpg=m, base=5000)
+    m.close()
+    frames_pm = max(1, int(round(8 * t_m / (t_p / frames_p))))
+    pmimg, t_pm = run(frames_pm, plain, base=20000)
     ds.destroy()
     rel = lambda x: float(np.mean((x[..., :3] - ref[..., :3]) ** 2 / (ref[..., :3] ** 2 + 1e-2)))
     res = {"workload": f"{w}x{h} view of the 1080p scene, L={L}, 64 spp per frame; reference 16384 spp plain",
            "relmse_guided": rel(gimg), "relmse_plain": rel(pimg), "seconds_guided": round(t_g, 3),
            "seconds_plain": round(t_p, 3), "frames_guided": 8, "frames_plain": frames_p,
+           "relmse_mcpg": rel(mimg), "seconds_mcpg": round(t_m, 3), "frames_mcpg": 8,
+           "relmse_plain_at_mcpg_time": rel(pmimg), "frames_plain_at_mcpg_time": frames_pm,
            "reference_seconds": round(t_ref, 2)}
     log(f"guided leg: {res}")
     return res

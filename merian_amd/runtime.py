@@ -126,12 +126,16 @@ class Context:
         self._check(self.lib.mrt_gbuffer_dev(self.h, scene.h, C.byref(cam), w, h, mask, C.c_void_p(d_out), None))
 
     def pt_render_dev(self, scene: DeviceScene, params: abi.PtParams, cam: abi.Camera, frame: int, w: int, h: int,
-                      d_gbuffer: int, d_out: int, d_sig: int | None = None, stats: bool = False):
+                      d_gbuffer: int, d_out: int, d_sig: int | None = None, stats: bool = False,
+                      stream: int | None = None):
+        """Enqueues a frame on `stream` (a HIP stream handle; None = the context's own stream,
+        which does not synchronise with torch's)."""
         st = abi.PtStats() if stats else None
         self._check(self.lib.mrt_pt_render_dev(self.h, scene.h, C.byref(params), C.byref(cam), frame, w, h,
                                                C.c_void_p(d_gbuffer), C.c_void_p(d_out),
                                                C.c_void_p(d_sig) if d_sig else None,
-                                               C.byref(st) if st is not None else None, None))
+                                               C.byref(st) if st is not None else None,
+                                               C.c_void_p(stream) if stream else None))
         return st
 
     def render(self, scene: DeviceScene, params: abi.PtParams, cam: abi.Camera, frame: int, w: int, h: int,
@@ -312,11 +316,12 @@ class Guide:
         return loss.value
 
     def render_dev(self, scene: DeviceScene, params: abi.PtParams, cam: abi.Camera, frame: int, w: int, h: int,
-                   d_gbuffer: int, d_out: int, train: bool = True, stats: bool = False):
+                   d_gbuffer: int, d_out: int, train: bool = True, stats: bool = False, stream: int | None = None):
         st = abi.PtStats() if stats else None
         self.ctx._check(self.lib.mrt_guided_render_dev(self.ctx.h, scene.h, self.h, C.byref(params), C.byref(cam),
                                                        frame, w, h, C.c_void_p(d_gbuffer), C.c_void_p(d_out),
-                                                       int(train), C.byref(st) if st is not None else None, None))
+                                                       int(train), C.byref(st) if st is not None else None,
+                                                       C.c_void_p(stream) if stream else None))
         return st
 
 
@@ -356,11 +361,12 @@ class Mcpg:
         self.ctx._check(self.lib.mrt_mcpg_reset(self.ctx.h, self.h))
 
     def render_dev(self, scene: DeviceScene, params: abi.PtParams, cam: abi.Camera, frame: int, w: int, h: int,
-                   d_gbuffer: int, d_out: int, stats: bool = False):
+                   d_gbuffer: int, d_out: int, stats: bool = False, stream: int | None = None):
         st = abi.PtStats() if stats else None
         self.ctx._check(self.lib.mrt_mcpg_render_dev(self.ctx.h, scene.h, self.h, C.byref(params), C.byref(cam),
                                                      frame, w, h, C.c_void_p(d_gbuffer), C.c_void_p(d_out),
-                                                     C.byref(st) if st is not None else None, None))
+                                                     C.byref(st) if st is not None else None,
+                                                     C.c_void_p(stream) if stream else None))
         return st
 
     def grids(self):
