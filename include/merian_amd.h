@@ -470,7 +470,8 @@ typedef struct mrt_guide_config {
     uint32_t max_records;           /* training records collected per frame (0 = 2^20) */
     float learning_rate;            /* Adam step size (0 = 5e-3) */
     float guide_fraction;           /* probability of sampling the guide (0 = 0.5) */
-    uint32_t _pad;
+    uint32_t guided_bounces;        /* vertices 0 .. guided_bounces-1 may sample the guide; later
+                                       ones sample the BSDF only and skip inference (0 = all) */
 } mrt_guide_config;
 
 /* Per-level grid resolutions and table size the guide uses (host-side computation). */

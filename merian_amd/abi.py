@@ -163,7 +163,7 @@ class GuideConfig(C.Structure):
     _fields_ = [("bbox_min", C.c_float * 3), ("bbox_max", C.c_float * 3), ("seed", C.c_uint32),
                 ("log2_table", C.c_uint32), ("base_resolution", C.c_uint32), ("max_resolution", C.c_uint32),
                 ("train_batch", C.c_uint32), ("train_steps", C.c_uint32), ("max_records", C.c_uint32),
-                ("learning_rate", C.c_float), ("guide_fraction", C.c_float), ("_pad", C.c_uint32)]
+                ("learning_rate", C.c_float), ("guide_fraction", C.c_float), ("guided_bounces", C.c_uint32)]
 
 
 GUIDE_LEVELS, GUIDE_WIDTH, GUIDE_LAYERS, GUIDE_OUTPUTS, GUIDE_LOBES = 16, 64, 5, 40, 8

@@ -673,7 +673,7 @@ mrt_status render_frame_dev(mrt_ctx* c, mrt_scene* s, const mrt_pt_params* prm, 
     }
     const GuideNet* net = nullptr;
     if (guide) {
-        st = guide_prepare_frame(c, guide, npix * chunk, pb, a.guide_fraction, &net);
+        st = guide_prepare_frame(c, guide, npix * chunk, pb, a.guide_fraction, a.guided_bounces, &net);
         if (st != MRT_OK) return st;
     }
     FrameTiming t;

@@ -46,6 +46,7 @@ mrt_guide_config with_defaults(const mrt_guide_config* in) {
     if (c.max_records == 0) c.max_records = 1u << 20;
     if (c.learning_rate == 0.f) c.learning_rate = 5e-3f;
     if (c.guide_fraction == 0.f) c.guide_fraction = 0.5f;
+    if (c.guided_bounces == 0) c.guided_bounces = 64;
     return c;
 }
 
@@ -155,7 +156,7 @@ mrt_status train_step(mrt_ctx* c, mrt_guide* g, const float4* rec, uint32_t n, h
 namespace mrt {
 
 mrt_status guide_prepare_frame(mrt_ctx* c, mrt_guide* g, uint64_t paths, PathBuffers& pb, float& fraction,
-                               const GuideNet** net) {
+                               uint32_t& guided_bounces, const GuideNet** net) {
     if (paths > g->lobes_cap) {
         if (g->lobes) {
             dev_free(c, g->lobes);
@@ -176,6 +177,7 @@ mrt_status guide_prepare_frame(mrt_ctx* c, mrt_guide* g, uint64_t paths, PathBuf
     pb.grec_count = g->grec_count;
     pb.grec_cap = g->cfg.max_records;
     fraction = g->cfg.guide_fraction;
+    guided_bounces = g->cfg.guided_bounces;
     *net = &g->net;
     return MRT_OK;
 }

@@ -747,7 +747,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(MOD
                     }
                     float alpha = 0.f;
                     GuideLobes L;
-                    if (guided && guide_eligible(S.mats[sp.material])) {
+                    if (guided && uint32_t(bounce) < A.guided_bounces && guide_eligible(S.mats[sp.material])) {
                         // bounce 0: queue slot i = sample * npix + pixel, lobes computed per pixel
                     unpack_lobes(P.lobes + 4 * size_t(bounce == 0 ? i % A.npix : i), L);
                         if (L.pi_sum > 0.f) alpha = A.guide_fraction;
@@ -1376,7 +1376,7 @@ hipError_t launch_pt_frame(const DeviceScene& s, const TraceScratch& ts, PtArgs 
                                                           a.instance_mask, nullptr, st);
                 mark(0);
             }
-            if (guided && b + 1 < L && err == hipSuccess) {
+            if (guided && b + 1 < L && uint32_t(b) < a.guided_bounces && err == hipSuccess) {
                 mark(-1);
                 // the samples of a pixel share the primary ray and hit, so bounce 0 queries once
                 // per pixel (the pass's sample-0 entries, queue slots [0, npix))

@@ -63,6 +63,6 @@ void mcpg_end_frame(mrt_mcpg* m, const mrt_camera& cam);
 // Guide hooks of render_frame_dev (guide_api.cpp): buffers for `paths` paths in flight, and the
 // training of the frame's records after it was enqueued.
 mrt_status guide_prepare_frame(mrt_ctx* c, mrt_guide* g, uint64_t paths, PathBuffers& pb, float& fraction,
-                               const GuideNet** net);
+                               uint32_t& guided_bounces, const GuideNet** net);
 mrt_status guide_train_frame(mrt_ctx* c, mrt_guide* g, hipStream_t st);
 } // namespace mrt

@@ -81,6 +81,7 @@ struct PtArgs {
     uint32_t instance_mask;
     int32_t mode;     // MRT_MODE_*
     float guide_fraction; // MRT_MODE_GUIDED: probability of sampling the guide at an eligible vertex
+    uint32_t guided_bounces; // MRT_MODE_GUIDED: vertices [0, guided_bounces) use the guide
     uint32_t s0;      // first sample index of this chunk
     uint32_t chunk;   // samples in this chunk
     uint32_t n_paths; // npix * chunk

@@ -239,11 +239,12 @@ class Server:
 
 def guide_config(seed: int = 1, log2_table: int = 0, base_resolution: int = 0, max_resolution: int = 0,
                  train_batch: int = 0, train_steps: int = 0, max_records: int = 0, learning_rate: float = 0.0,
-                 guide_fraction: float = 0.0, bbox=None) -> abi.GuideConfig:
+                 guide_fraction: float = 0.0, bbox=None, guided_bounces: int = 0) -> abi.GuideConfig:
     c = abi.GuideConfig()
     c.seed, c.log2_table, c.base_resolution, c.max_resolution = seed, log2_table, base_resolution, max_resolution
     c.train_batch, c.train_steps, c.max_records = train_batch, train_steps, max_records
     c.learning_rate, c.guide_fraction = learning_rate, guide_fraction
+    c.guided_bounces = guided_bounces
     if bbox is not None:
         for k in range(3):
             c.bbox_min[k], c.bbox_max[k] = bbox[0][k], bbox[1][k]
