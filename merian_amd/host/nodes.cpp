@@ -1,5 +1,6 @@
 // Node types of the graph host: "PBRT Scene", "GBuffer (Raytraced)", "Render (Path-traced)",
-// "Accumulate" and "Image Write" (names and connectors of merian's node_registry.cpp:63-96).
+// "Render (Path-traced, MCPG)", "Accumulate" and "Image Write" (names and connectors of
+// merian's node_registry.cpp:63-96).
 #include <cmath>
 #include <cstring>
 #include <sys/stat.h>
@@ -257,6 +258,125 @@ class RenderPTNode : public Node {
     DeviceBuffer buf_;
 };
 
+// "Render (Path-traced, MCPG)" (render_pt_mcpg.cpp:204-258,480-650): scene + gbuffer in,
+// irradiance out; the Markov-chain and light-cache grids persist across iterations and are
+// cleared on iteration 0. The surface transport only: the volume pass, distance chains and
+// debug outputs are accepted as properties and not run.
+class RenderMCPGNode : public Node {
+  public:
+    std::vector<std::string> inputs() const override { return {"scene", "gbuffer"}; }
+    std::vector<std::string> outputs() const override { return {"irradiance"}; }
+    Value properties(const Value& in) override {
+        Props p(in);
+        p.get_bool("raygen executor", false);
+        const bool reference_mode = p.get_bool("reference mode", false);
+        emission_on_primary_ = p.get_bool("emission on primary", true);
+        if (p.get_bool("demodulate albedo", false)) throw node_error("\"demodulate albedo\" is not supported");
+        spp_ = int32_t(p.get_int("samples per pixel", 1));
+        max_path_length_ = int32_t(p.get_int("max path length", 5));
+        mrt_mcpg_config k;
+        mrt_mcpg_default_config(&k);
+        const float p_guiding = float(p.get_float("guiding prob", 0.5));
+        k.p_guiding = reference_mode ? 0.0f : p_guiding;
+        const Value* st = in.is_object() ? in.find("directional sampling type") : nullptr;
+        if (st && st->is_string()) {
+            if (st->as_string() == "MIS") k.sampling_type = MRT_MCPG_MIS;
+            else if (st->as_string() == "MIS+Roughness") k.sampling_type = MRT_MCPG_MIS_ROUGHNESS;
+            else throw node_error("unknown directional sampling type '" + st->as_string() + "'");
+        } else if (st && st->is_number()) {
+            k.sampling_type = int32_t(st->as_int());
+        }
+        const float rough = float(p.get_float("roughness threshold", std::sqrt(0.05)));
+        k.guiding_alpha_threshold = rough * rough;
+        for (const char* v : {"volume samples per pixel", "distance MC samples"}) p.get_int(v, 1);
+        for (const char* v : {"volume guiding prob", "distance guiding prob", "distance MC base width",
+                              "distance MC max width", "distance MC level spread", "volume forward project min z"})
+            p.get_float(v, 0.0);
+        for (const char* v : {"volume: use LC", "volume forward project"}) p.get_bool(v, true);
+        const auto grid = [&](const Value& child, mrt_mcpg_grid_config& g) {
+            if (const Value* v = child.find("grid tan(alpha/2)"); v && v->is_number()) g.tan_alpha_half = float(v->as_double());
+            if (const Value* v = child.find("grid level bias"); v && v->is_number()) g.level_bias = float(v->as_double());
+            if (const Value* v = child.find("grid distribution dimension"); v && v->is_number())
+                g.distribution_dimension = float(v->as_double());
+        };
+        Value mc = p.get_object("mc", Value::object());
+        if (const Value* v = mc.find("ML prior"); v && v->is_number()) k.dir_guide_prior = float(v->as_double());
+        if (const Value* v = mc.find("MC samples"); v && v->is_number()) k.mc_samples = int32_t(v->as_int());
+        if (const Value* v = mc.find("missing light heuristic"); v && v->is_bool()) k.missing_light_heuristic = v->as_bool();
+        if (const Value* v = mc.find("adaptive grid buf size"); v && v->is_number()) k.mc.buffer_size = uint32_t(v->as_int());
+        if (const Value* v = mc.find("MC probe count"); v && v->is_number()) k.mc.probe_count = uint32_t(v->as_int());
+        if (const Value* v = mc.find("locality bits"); v && v->is_number()) k.mc.locality_bits = uint32_t(v->as_int());
+        grid(mc, k.mc);
+        Value lc = p.get_object("lc", Value::object());
+        if (const Value* v = lc.find("surf: use LC"); v && v->is_bool()) k.use_light_cache_tail = v->as_bool();
+        if (const Value* v = lc.find("LC buffer size"); v && v->is_number()) k.lc.buffer_size = uint32_t(v->as_int());
+        if (const Value* v = lc.find("LC probe count"); v && v->is_number()) k.lc.probe_count = uint32_t(v->as_int());
+        if (const Value* v = lc.find("LC stochastic interpolation"); v && v->is_bool())
+            k.lc.stochastic_interpolation = v->as_bool();
+        if (const Value* v = lc.find("LC locality bits"); v && v->is_number()) k.lc.locality_bits = uint32_t(v->as_int());
+        if (const Value* v = lc.find("LC min pdf"); v && v->is_number()) k.lc_min_pdf = float(v->as_double());
+        grid(lc, k.lc);
+        mask_ = mask_bits(p);
+        p.get_int("debug output selector", 0);
+        const std::string fmt = p.get_string("irradiance format", "R32G32B32A32Sfloat");
+        if (fmt != "R32G32B32A32Sfloat")
+            throw node_error("irradiance format '" + fmt + "' is not supported (R32G32B32A32Sfloat only)");
+        if (std::memcmp(&k, &cfg_, sizeof(k)) != 0) {
+            cfg_ = k;
+            recreate_ = true; // grid sizes or constants changed: the node rebuilds its state
+        }
+        return p.result();
+    }
+    void process(RunInfo& info, const std::map<std::string, const Resource*>& in,
+                 std::map<std::string, Resource>& out) override {
+        auto gi = in.find("gbuffer");
+        if (gi == in.end() || !gi->second || !std::holds_alternative<GBufferResource>(*gi->second))
+            throw node_error("input \"gbuffer\" is not connected");
+        auto si = in.find("scene");
+        if (si == in.end() || !si->second || !std::holds_alternative<SceneResource>(*si->second))
+            throw node_error("input \"scene\" is not connected");
+        const GBufferResource& g = std::get<GBufferResource>(*gi->second);
+        const SceneResource& s = std::get<SceneResource>(*si->second);
+        if (recreate_ || !state_) {
+            if (state_) mrt_mcpg_destroy(info.ctx, state_);
+            state_ = nullptr;
+            check(info.ctx, mrt_mcpg_create(info.ctx, &cfg_, &state_), "mcpg state");
+            recreate_ = false;
+        } else if (info.iteration == 0) {
+            check(info.ctx, mrt_mcpg_reset(info.ctx, state_), "mcpg reset");
+        }
+        mrt_pt_params p{};
+        p.spp = spp_;
+        p.max_path_length = max_path_length_;
+        p.emission_on_primary = emission_on_primary_;
+        p.instance_mask = mask_;
+        ImageResource img;
+        img.width = g.width;
+        img.height = g.height;
+        buf_.ensure(info.ctx, uint64_t(g.width) * g.height * 16);
+        img.d = static_cast<float*>(buf_.d);
+        check(info.ctx,
+              mrt_mcpg_render_dev(info.ctx, s.scene, state_, &p, &g.camera, uint32_t(info.iteration), g.width,
+                                  g.height, g.d, img.d, nullptr, nullptr),
+              "mcpg");
+        out["irradiance"] = img;
+    }
+    void release(mrt_ctx* c) override {
+        buf_.release(c);
+        if (state_) mrt_mcpg_destroy(c, state_);
+        state_ = nullptr;
+    }
+
+  private:
+    int32_t spp_ = 1, max_path_length_ = 5;
+    bool emission_on_primary_ = true;
+    uint32_t mask_ = 0xFF;
+    mrt_mcpg_config cfg_{};
+    bool recreate_ = true;
+    mrt_mcpg* state_ = nullptr;
+    DeviceBuffer buf_;
+};
+
 // "Accumulate": running mean of its source over iterations, restarted when the source's
 // extent changes; "max history" caps the count (a number, or "inf"). The reference's SVGF-style
 // reprojection, firefly filter and adaptivity settings are accepted and kept but not applied
@@ -393,6 +513,8 @@ void register_nodes(NodeRegistry& r) {
                          [] { return std::unique_ptr<Node>(new GBufferNode()); });
     r.register_node_type("Render (Path-traced)", "Wavefront path tracer on gfx950.",
                          [] { return std::unique_ptr<Node>(new RenderPTNode()); });
+    r.register_node_type("Render (Path-traced, MCPG)", "Markov-chain path guiding on gfx950 (surface transport).",
+                         [] { return std::unique_ptr<Node>(new RenderMCPGNode()); });
     r.register_node_type("Accumulate", "Running mean of the source over iterations.",
                          [] { return std::unique_ptr<Node>(new AccumulateNode()); });
     r.register_node_type("Image Write", "Stores a graph output as PFM image.",

@@ -71,3 +71,51 @@ def test_runner_accumulates_frames(ctx, tmp_path):
     frames = _direct(ctx, p, info, 4, 2, 5)
     mean = np.mean([f[..., :3] for f in frames], axis=0)
     assert np.allclose(got, mean, rtol=1e-5, atol=1e-6)
+
+
+def test_runner_mcpg_variant_matches_direct_mcpg(ctx, tmp_path):
+    # "MC samples" 0: no chain lookups, so the frame does not depend on other paths' grid writes
+    # and the node's output must equal a direct mrt_mcpg frame with the same properties bit for bit
+    from merian_amd.runtime import Mcpg, mcpg_config
+    sc = scenes.plain_small(seed=11, subdiv=1, grid=3, res=64)
+    p = tmp_path / "scene.pbrt"
+    scenes.write_pbrt(sc, str(p))
+    info = pbrt_io.inspect(p)
+    out = tmp_path / "mcpg"
+    rc = pbrt_io.run_graph([str(GRAPH), str(p), "--renderer", "mcpg", "--mc-samples", "0", "--spp", "3",
+                            "--out", str(out), "--max-iterations=1"])
+    assert rc == 0
+    got = pbrt_io.read_pfm(str(out) + ".pfm")
+    h = C.c_void_p()
+    ctx._check(ctx.lib.mrt_pbrt_scene_create(ctx.h, str(p).encode(), C.byref(h)))
+    ds = DeviceScene(ctx, h, None)
+    cam_i = info["camera"]
+    w, hgt = cam_i["width"], cam_i["height"]
+    eye = np.array(cam_i["eye"], np.float32)
+    target = (eye + np.array(cam_i["forward"], np.float32)).astype(np.float32)
+    cam = abi.Camera()
+    f3 = C.c_float * 3
+    ctx._check(ctx.lib.mrt_camera_look_at(f3(*eye), f3(*target), f3(*cam_i["up"]), cam_i["vfov"], w / hgt, 0.01,
+                                          1000.0, abi.JITTER_R2, 0, C.byref(cam)))
+    m = Mcpg(ctx, mcpg_config(mc_buffer_size=65521, mc_probe_count=2, mc_tan_alpha_half=0.003,
+                              mc_distribution_dimension=1.5, lc_buffer_size=16381, lc_probe_count=2,
+                              lc_tan_alpha_half=0.012, mc_samples=0))
+    try:
+        import torch
+        gb = torch.empty(w * hgt * 16, dtype=torch.uint8, device="cuda")
+        o = torch.empty(w * hgt * 4, dtype=torch.float32, device="cuda")
+        ctx.gbuffer_dev(ds, cam, w, hgt, gb.data_ptr())
+        m.render_dev(ds, pt_params(spp=3, max_path_length=5), cam, 0, w, hgt, gb.data_ptr(), o.data_ptr())
+        torch.cuda.synchronize()
+        ref = o.cpu().numpy().reshape(hgt, w, 4)[..., :3]
+    finally:
+        m.close()
+        ds.destroy()
+    assert got.tobytes() == np.ascontiguousarray(ref).tobytes()
+    # the default variant settings (5 chain lookups) render a finite image of the same mean
+    out2 = tmp_path / "mcpg5"
+    rc = pbrt_io.run_graph([str(GRAPH), str(p), "--renderer", "mcpg", "--spp", "3", "--out", str(out2),
+                            "--max-iterations=1"])
+    assert rc == 0
+    got2 = pbrt_io.read_pfm(str(out2) + ".pfm")
+    assert np.isfinite(got2).all() and abs(got2.mean() - ref.mean()) < 0.1 * ref.mean()

@@ -96,7 +96,7 @@ Shape "sphere" "float radius" 1
 def test_runner_help_lists_overrides():
     r = subprocess.run([str(RUNNER), str(GRAPHS / "headless_pt.json"), "--help"], capture_output=True, text=True)
     assert r.returncode == 0
-    for s in ("--spp <value>", "--max-path-length <value>", "--renderer <pt|file.json>", "(required)"):
+    for s in ("--spp <value>", "--max-path-length <value>", "--renderer <mcpg|pt|file.json>", "(required)"):
         assert s in r.stdout, r.stdout
 
 
