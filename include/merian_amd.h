@@ -618,8 +618,8 @@ MRT_API int32_t mrt_graph_run_main(int32_t argc, const char* const* argv);
  * argument is this library's node registry (merian_amd/host), not the Vulkan one.
  * ---------------------------------------------------------------------------------- */
 MRT_API uint32_t merian_node_plugin_abi_version(void);
-/* Registers "PBRT Scene", "GBuffer (Raytraced)", "Render (Path-traced)", "Accumulate" and
- * "Image Write" into a node registry of this library's graph host (mrt::graph::NodeRegistry;
+/* Registers "PBRT Scene", "GBuffer (Raytraced)", "Render (Path-traced)", "Render (Path-traced,
+ * MCPG)", "Accumulate" and "Image Write" into a node registry of this library's graph host (mrt::graph::NodeRegistry;
  * the reference passes merian::NodeRegistry& with the same calling convention). */
 typedef struct mrt_node_registry mrt_node_registry;
 MRT_API void merian_register_nodes(mrt_node_registry* registry);
