@@ -164,9 +164,45 @@ def copy_workload(mib: int, reps: int) -> None:
     print(json.dumps({"copy_bytes": n, "reps": reps}), flush=True)
 
 
+def mcpg_workload(width: int, height: int, spp: int, frames: int, max_path_length: int) -> None:
+    """MCPG frames on a view of the 1080p scene (the MCPG example graph's grid settings), then
+    one plain frame of the same view: a workload to profile the MCPG kernels."""
+    import torch
+
+    from . import scenes
+    from .runtime import Context, Mcpg, mcpg_config, pt_params
+    ctx = Context()
+    scene = scenes.full_scene()
+    spec = scenes.CameraSpec(scene.camera.eye, scene.camera.target, scene.camera.up, scene.camera.vfov, width, height)
+    ds = ctx.scene(scene)
+    cam = ctx.camera(spec, 0)
+    gb = torch.empty(width * height * 16, dtype=torch.uint8, device="cuda")
+    out = torch.empty(width * height * 4, dtype=torch.float32, device="cuda")
+    ctx.gbuffer_dev(ds, cam, width, height, gb.data_ptr())
+    m = Mcpg(ctx, mcpg_config(mc_tan_alpha_half=0.003, mc_distribution_dimension=1.5, lc_tan_alpha_half=0.012))
+    p = pt_params(spp=spp, max_path_length=max_path_length, record_signature=False)
+    for f in range(frames):
+        st = m.render_dev(ds, p, cam, f, width, height, gb.data_ptr(), out.data_ptr(), stats=True)
+        torch.cuda.synchronize()
+        print(json.dumps({"frame": f, "mcpg_ms": round(st.ms_total, 3), "trace_ms": round(st.ms_trace, 3),
+                          "shade_ms": round(st.ms_shade, 3), "segments": st.continuation_segments}), flush=True)
+    st = ctx.pt_render_dev(ds, p, cam, frames, width, height, gb.data_ptr(), out.data_ptr(), stats=True)
+    torch.cuda.synchronize()
+    print(json.dumps({"plain_ms": round(st.ms_total, 3), "segments": st.continuation_segments}), flush=True)
+    m.close()
+    ds.destroy()
+    ctx.close()
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(prog="python -m merian_amd.profiling")
     sub = ap.add_subparsers(dest="cmd", required=True)
+    g = sub.add_parser("mcpg")
+    g.add_argument("--width", type=int, default=1920)
+    g.add_argument("--height", type=int, default=1080)
+    g.add_argument("--spp", type=int, default=4)
+    g.add_argument("--frames", type=int, default=3)
+    g.add_argument("--max-path-length", type=int, default=8)
     c = sub.add_parser("copy")
     c.add_argument("--mib", type=int, default=1024)
     c.add_argument("--reps", type=int, default=4)
@@ -186,6 +222,8 @@ def main(argv=None) -> int:
     args = ap.parse_args(argv)
     if args.cmd == "copy":
         copy_workload(args.mib, args.reps)
+    elif args.cmd == "mcpg":
+        mcpg_workload(args.width, args.height, args.spp, args.frames, args.max_path_length)
     elif args.cmd == "traffic":
         print(json.dumps(traffic(args.fetch_dir, args.write_dir, args.copy_fetch_dir, args.copy_write_dir,
                                  args.kernel, args.copy_mib << 20), indent=1))
