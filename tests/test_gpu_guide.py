@@ -210,3 +210,27 @@ def test_guided_is_unbiased_and_wins_at_equal_time(ctx, sky):
         assert e_g < e_p, (e_g, e_p)
     finally:
         g.close()
+
+
+def test_guided_bounces_limit_stays_unbiased(ctx, sky):
+    # only vertex 0 samples the guide (later vertices sample the BSDF and skip inference);
+    # the estimator stays unbiased and the frame is faster than guiding every vertex
+    sc, ds = sky
+    spec = scenes.CameraSpec(sc.camera.eye, sc.camera.target, sc.camera.up, sc.camera.vfov, W_, H_)
+    L = 6
+    ref = _frames(ctx, ds, spec, 0, pt_params(spp=1024, max_path_length=L, record_signature=False), 8)
+    g = Guide(ctx, ds, guide_config(seed=6, guided_bounces=1))
+    try:
+        gp = pt_params(spp=16, max_path_length=L, mode=abi.MODE_GUIDED, record_signature=False)
+        _frames(ctx, ds, spec, 0, gp, 8, guide=g, train=True)
+        gp.spp = 256
+        img = _frames(ctx, ds, spec, 0, gp, 4, guide=g, train=False)
+        lum_g, lum_r = _lum(img), _lum(ref)
+        var_g = np.maximum(img[..., 3] - lum_g * lum_g, 0) / (256 * 4)
+        var_r = np.maximum(ref[..., 3] - lum_r * lum_r, 0) / (1024 * 8)
+        b = lambda a: a.reshape(H_ // 6, 6, W_ // 6, 6).mean((1, 3))
+        z = (b(lum_g) - b(lum_r)) / np.sqrt((b(var_g) + b(var_r)) / 36 + 1e-12)
+        assert np.abs(z).max() < 6.0, np.abs(z).max()
+        assert abs(z.mean()) < 0.6, z.mean()
+    finally:
+        g.close()
