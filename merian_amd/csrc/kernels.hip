@@ -19,7 +19,7 @@ namespace mrt {
 namespace {
 
 constexpr int SHADE_BLOCK = 256;
-constexpr int SHADE_ROUNDS = 4;                          // paths per thread per tile
+constexpr int SHADE_ROUNDS = 2;                          // paths per thread per tile
 constexpr int SHADE_TILE = SHADE_BLOCK * SHADE_ROUNDS;   // paths a block groups by class at once
 constexpr float SCENE_RAY_TMAX = 1e7f;     // scene/hit.slang:8
 constexpr float ENV_MAP_DISTANCE = 1e7f;   // scene/environment-map.slang:11
