@@ -533,7 +533,7 @@ typedef struct mrt_mcpg mrt_mcpg;
 enum { MRT_MCPG_MIS = 0, MRT_MCPG_MIS_ROUGHNESS = 1, MRT_MCPG_MAX_SAMPLES = 8 };
 
 typedef struct mrt_mcpg_grid_config {
-    uint32_t buffer_size;            /* slots ("adaptive grid buf size" / "LC buffer size") */
+    uint32_t buffer_size;            /* slots ("adaptive grid buf size" / "LC buffer size"), <= 2^28 */
     uint32_t probe_count;            /* slots probed before evicting (1..32) */
     uint32_t locality_bits;          /* 2^n-wide cell tiles in Morton order (0..5; 0 = scatter) */
     uint32_t stochastic_interpolation; /* light cache: jitter the cell per lookup (the MC grid

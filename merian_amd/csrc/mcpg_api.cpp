@@ -24,9 +24,13 @@ using namespace mrt;
 
 namespace {
 
+// Slots per grid: at most 2^28 (the node's own limits are 1e8 for the light cache), so every
+// grid's word count fits the 32-bit fill kernel.
+constexpr uint32_t MAX_GRID_SLOTS = 1u << 28;
+
 bool grid_ok(const mrt_mcpg_grid_config& g) {
-    return g.buffer_size >= 1u && g.probe_count >= 1u && g.probe_count <= 32u && g.locality_bits <= 5u &&
-           g.stochastic_interpolation <= 1u && g.tan_alpha_half > 0.f && std::isfinite(g.tan_alpha_half) &&
+    return g.buffer_size >= 1u && g.buffer_size <= MAX_GRID_SLOTS && g.probe_count >= 1u && g.probe_count <= 32u &&
+           g.locality_bits <= 5u && g.stochastic_interpolation <= 1u && g.tan_alpha_half > 0.f && std::isfinite(g.tan_alpha_half) &&
            std::isfinite(g.level_bias) && g.distribution_dimension > 0.f && std::isfinite(g.distribution_dimension);
 }
 
@@ -38,10 +42,12 @@ bool config_ok(const mrt_mcpg_config& c) {
            std::isfinite(c.dir_guide_prior) && c.lc_min_pdf >= 0.f && std::isfinite(c.lc_min_pdf);
 }
 
-// Zero-fills both grids with a fill kernel on the context stream and waits for it.
+// Zero-fills both grids with a fill kernel on the context stream and waits for it. Frames may
+// have run on a caller's stream, so the whole device is drained first.
 mrt_status clear_grids(mrt_ctx* c, mrt_mcpg* m) {
     const uint64_t mc = m->cfg.mc.buffer_size, lc = m->cfg.lc.buffer_size;
-    hipError_t e = launch_zero_u32(reinterpret_cast<uint32_t*>(m->mc_keys), uint32_t(2 * mc), c->stream);
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = launch_zero_u32(reinterpret_cast<uint32_t*>(m->mc_keys), uint32_t(2 * mc), c->stream);
     if (e == hipSuccess) e = launch_zero_u32(reinterpret_cast<uint32_t*>(m->mc_data), uint32_t(8 * mc), c->stream);
     if (e == hipSuccess) e = launch_zero_u32(reinterpret_cast<uint32_t*>(m->lc_keys), uint32_t(2 * lc), c->stream);
     if (e == hipSuccess) e = launch_zero_u32(reinterpret_cast<uint32_t*>(m->lc_data), uint32_t(2 * lc), c->stream);
@@ -164,7 +170,7 @@ MRT_API mrt_status mrt_mcpg_create(mrt_ctx* c, const mrt_mcpg_config* config, mr
 
 MRT_API mrt_status mrt_mcpg_destroy(mrt_ctx* c, mrt_mcpg* m) {
     if (!c || !m) return MRT_ERR_INVALID;
-    const hipError_t e = hipStreamSynchronize(c->stream); // kernels may still use the grids
+    const hipError_t e = hipDeviceSynchronize(); // kernels on any stream may still use the grids
     free_grids(c, m);
     delete m;
     return e == hipSuccess ? MRT_OK : hip_fail(c, e, "mrt_mcpg_destroy");

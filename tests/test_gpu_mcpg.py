@@ -101,9 +101,10 @@ def test_single_path_sequence_matches_oracle(ctx, room, cfg_i):
 def test_config_validation_and_reset(ctx, room):
     sc, ds, _ = room
     for bad in (dict(mc_samples=9), dict(mc_probe_count=0), dict(lc_locality_bits=6), dict(p_guiding=1.5),
-                dict(mc_tan_alpha_half=0.0), dict(sampling_type=2), dict(lc_distribution_dimension=0.0)):
+                dict(mc_tan_alpha_half=0.0), dict(sampling_type=2), dict(lc_distribution_dimension=0.0),
+                dict(mc_buffer_size=(1 << 28) + 1)):
         with pytest.raises(MrtError):
-            Mcpg(ctx, mcpg_config(mc_buffer_size=1024, lc_buffer_size=1024, **bad))
+            Mcpg(ctx, mcpg_config(**(dict(mc_buffer_size=1024, lc_buffer_size=1024) | bad)))
     m = Mcpg(ctx, mcpg_config(mc_buffer_size=2048, lc_buffer_size=1024))
     try:
         cam = ctx.camera(scenes.CameraSpec(sc.camera.eye, sc.camera.target, sc.camera.up, sc.camera.vfov, 8, 8), 0)
