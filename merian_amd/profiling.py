@@ -144,6 +144,58 @@ def valu(d: str) -> dict:
     return out
 
 
+def mfma(d: str) -> dict:
+    """MFMA figures per kernel from a pass with SQ_INSTS_VALU_MFMA_MOPS_F16 (F16 MFMA work in
+    units of 512 flops), SQ_VALU_MFMA_BUSY_CYCLES and GRBM_GUI_ACTIVE: achieved fp16 TFLOP/s over
+    the kernel's own duration, its fraction of the 2.5 PFLOP/s dense fp16 peak, and MFMA busy =
+    busy cycles per SIMD over the GPU's busy cycles."""
+    a = agent(d)
+    simds, xcc = int(a["Simd_Count"]), int(a["Num_Xcc"])
+    acc, disp = _sums(d)
+    out = {}
+    for k, c in acc.items():
+        n, ms = disp[k]
+        mops = c.get("SQ_INSTS_VALU_MFMA_MOPS_F16", 0.0)
+        if mops <= 0:
+            continue
+        tflops = mops * 512.0 / (ms * 1e-3) / 1e12
+        gui = c.get("GRBM_GUI_ACTIVE", 0.0) / xcc
+        busy = c.get("SQ_VALU_MFMA_BUSY_CYCLES", 0.0)
+        out[k] = {"dispatches": n, "avg_ms": round(ms / n, 4), "fp16_gflop_per_dispatch": round(mops * 512 / n / 1e9, 3),
+                  "fp16_tflops": round(tflops, 1), "frac_of_2500": round(tflops / 2500.0, 4),
+                  "mfma_busy_pct": round(100 * busy / simds / gui, 1) if gui > 0 else None}
+    return out
+
+
+def guided_workload(width: int, height: int, spp: int, train_frames: int, frames: int, guided_bounces: int) -> None:
+    """Learned-guide frames on a view of the 1080p scene: online-training frames (forward,
+    backward, weight gradients, Adam), then frames with the guide frozen (inference only)."""
+    import torch
+
+    from . import abi as _abi
+    from . import scenes
+    from .runtime import Context, Guide, guide_config, pt_params
+    ctx = Context()
+    scene = scenes.full_scene()
+    spec = scenes.CameraSpec(scene.camera.eye, scene.camera.target, scene.camera.up, scene.camera.vfov, width, height)
+    ds = ctx.scene(scene)
+    cam = ctx.camera(spec, 0)
+    gb = torch.empty(width * height * 16, dtype=torch.uint8, device="cuda")
+    out = torch.empty(width * height * 4, dtype=torch.float32, device="cuda")
+    ctx.gbuffer_dev(ds, cam, width, height, gb.data_ptr())
+    g = Guide(ctx, ds, guide_config(seed=1, train_steps=16, guided_bounces=guided_bounces))
+    p = pt_params(spp=spp, max_path_length=8, mode=_abi.MODE_GUIDED, record_signature=False)
+    for f in range(train_frames + frames):
+        train = f < train_frames
+        st = g.render_dev(ds, p, cam, f, width, height, gb.data_ptr(), out.data_ptr(), train=train, stats=True)
+        torch.cuda.synchronize()
+        print(json.dumps({"frame": f, "train": train, "ms": round(st.ms_total, 3),
+                          "segments": st.continuation_segments}), flush=True)
+    g.close()
+    ds.destroy()
+    ctx.close()
+
+
 def l2(d: str) -> dict:
     """L2 hit rate per kernel (TCC_HIT / (TCC_HIT + TCC_MISS))."""
     acc, disp = _sums(d)
@@ -197,6 +249,15 @@ def mcpg_workload(width: int, height: int, spp: int, frames: int, max_path_lengt
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(prog="python -m merian_amd.profiling")
     sub = ap.add_subparsers(dest="cmd", required=True)
+    gd = sub.add_parser("guided")
+    gd.add_argument("--width", type=int, default=480)
+    gd.add_argument("--height", type=int, default=270)
+    gd.add_argument("--spp", type=int, default=64)
+    gd.add_argument("--train-frames", type=int, default=4)
+    gd.add_argument("--frames", type=int, default=4)
+    gd.add_argument("--guided-bounces", type=int, default=3)
+    mf = sub.add_parser("mfma")
+    mf.add_argument("dir")
     g = sub.add_parser("mcpg")
     g.add_argument("--width", type=int, default=1920)
     g.add_argument("--height", type=int, default=1080)
@@ -224,11 +285,13 @@ def main(argv=None) -> int:
         copy_workload(args.mib, args.reps)
     elif args.cmd == "mcpg":
         mcpg_workload(args.width, args.height, args.spp, args.frames, args.max_path_length)
+    elif args.cmd == "guided":
+        guided_workload(args.width, args.height, args.spp, args.train_frames, args.frames, args.guided_bounces)
     elif args.cmd == "traffic":
         print(json.dumps(traffic(args.fetch_dir, args.write_dir, args.copy_fetch_dir, args.copy_write_dir,
                                  args.kernel, args.copy_mib << 20), indent=1))
-    elif args.cmd in ("valu", "l2"):
-        rows = valu(args.dir) if args.cmd == "valu" else l2(args.dir)
+    elif args.cmd in ("valu", "l2", "mfma"):
+        rows = valu(args.dir) if args.cmd == "valu" else (l2(args.dir) if args.cmd == "l2" else mfma(args.dir))
         keys = list(next(iter(rows.values())).keys()) if rows else []
         print("| kernel | " + " | ".join(keys) + " |\n|---|" + "---:|" * len(keys))
         for k, r in sorted(rows.items(), key=lambda kv: -kv[1]["dispatches"] * kv[1].get("avg_ms", 1)):
