@@ -142,6 +142,8 @@ def guided_leg(ctx, scene, L: int) -> dict:
            "frames_guided": RENDER, "frames_plain": frames_p,
            "relmse_mcpg": rel(mimg), "seconds_mcpg": round(t_m, 3), "frames_mcpg": 8,
            "relmse_plain_at_mcpg_time": rel(pmimg), "frames_plain_at_mcpg_time": frames_pm,
+           "mcpg_note": "MCPG guides rough glass: the GGX BTDF evaluation it shares with the plain pt is non-zero "
+                        "where its pdf is 0, so guided refractions there carry unbounded weights (fireflies)",
            "reference_seconds": round(t_ref, 2)}
     log(f"guided leg: {res}")
     return res
