@@ -38,17 +38,21 @@ def test_ctypes_layouts_match_the_header(tmp_path):
     src = tmp_path / "layout_probe.c"
     exe = tmp_path / "layout_probe"
     src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "include/merian_amd.h"\n'
-                   "int main(void){printf(\"%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n\","
+                   "int main(void){printf(\"%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n\","
                    "sizeof(mrt_vertex),sizeof(mrt_geometry),sizeof(mrt_material),sizeof(mrt_env),"
                    "sizeof(mrt_scene_desc),sizeof(mrt_camera),sizeof(mrt_ray),sizeof(mrt_hit),"
                    "sizeof(mrt_packed_hit),sizeof(mrt_pt_params),sizeof(mrt_pt_stats),"
-                   "offsetof(mrt_pt_stats,node_visits));return 0;}\n")
+                   "offsetof(mrt_pt_stats,node_visits),sizeof(mrt_guide_config),"
+                   "offsetof(mrt_guide_config,guided_bounces),sizeof(mrt_mcpg_grid_config),"
+                   "sizeof(mrt_mcpg_config),offsetof(mrt_mcpg_config,lc_min_pdf));return 0;}\n")
     subprocess.run(["gcc", "-std=c11", "-I", str(ROOT), "-o", str(exe), str(src)], check=True)
     sizes = list(map(int, subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()))
     ours = [abi.VERTEX_DTYPE.itemsize, abi.GEOMETRY_DTYPE.itemsize, abi.MATERIAL_DTYPE.itemsize,
             C.sizeof(abi.Env), C.sizeof(abi.SceneDesc), C.sizeof(abi.Camera), abi.RAY_DTYPE.itemsize,
             abi.HIT_DTYPE.itemsize, abi.PACKED_HIT_DTYPE.itemsize, C.sizeof(abi.PtParams),
-            C.sizeof(abi.PtStats), abi.PtStats.node_visits.offset]
+            C.sizeof(abi.PtStats), abi.PtStats.node_visits.offset, C.sizeof(abi.GuideConfig),
+            abi.GuideConfig.guided_bounces.offset, C.sizeof(abi.McpgGridConfig), C.sizeof(abi.McpgConfig),
+            abi.McpgConfig.lc_min_pdf.offset]
     assert sizes == ours
 
 
