@@ -561,21 +561,23 @@ typedef struct mrt_mcpg_config {
 MRT_API void mrt_mcpg_default_config(mrt_mcpg_config* out);
 /* Allocates both grids (zeroed, as the node's reset on its first frame). */
 MRT_API mrt_status mrt_mcpg_create(mrt_ctx* ctx, const mrt_mcpg_config* config, mrt_mcpg** out);
+/* Waits for the device (frames on any stream may still use the grids), then frees them. */
 MRT_API mrt_status mrt_mcpg_destroy(mrt_ctx* ctx, mrt_mcpg* mcpg);
 /* Clears both grids and forgets the previous camera (RenderMCPG::process at iteration 0). */
 MRT_API mrt_status mrt_mcpg_reset(mrt_ctx* ctx, mrt_mcpg* mcpg);
 /* One MCPG frame over a gbuffer (as mrt_pt_render_dev): d_irradiance = per pixel mean
  * radiance.rgb, mean luminance^2. `frame` seeds the RNG and stamps grid slots; the
  * previous camera is the one of the last frame rendered with this state (the current one
- * after a reset). */
+ * after a reset). Frames of one mrt_mcpg are stream-ordered by the caller: two frames of the
+ * same state in flight on different streams race on its grids and carry buffer. */
 MRT_API mrt_status mrt_mcpg_render_dev(mrt_ctx* ctx, mrt_scene* scene, mrt_mcpg* mcpg,
                                        const mrt_pt_params* params, const mrt_camera* cam,
                                        uint32_t frame, uint32_t width, uint32_t height,
                                        const mrt_packed_hit* d_gbuffer, float* d_irradiance,
                                        mrt_pt_stats* stats, void* stream);
-/* Host copies of the grids (any pointer may be NULL): keys uint32[2*size] (key, stamp),
- * chain states 32 bytes per slot (w_tgt.xyz, sum_w, w_cos, mv half3, N u16, T), light-cache
- * entries 8 bytes per slot (irr half3, N u16). */
+/* Host copies of the grids after the device has drained (any pointer may be NULL): keys
+ * uint32[2*size] (key, stamp), chain states 32 bytes per slot (w_tgt.xyz, sum_w, w_cos,
+ * mv half3, N u16, T), light-cache entries 8 bytes per slot (irr half3, N u16). */
 MRT_API mrt_status mrt_mcpg_read_grids(mrt_ctx* ctx, mrt_mcpg* mcpg, uint32_t* mc_keys, void* mc_states,
                                        uint32_t* lc_keys, void* lc_entries);
 

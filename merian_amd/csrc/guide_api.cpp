@@ -151,6 +151,14 @@ mrt_status train_step(mrt_ctx* c, mrt_guide* g, const float4* rec, uint32_t n, h
     return e == hipSuccess ? MRT_OK : hip_fail(c, e, "guide adam");
 }
 
+// Guided frames may run (and train) on a caller's stream: the calls that read, write or free
+// the guide's buffers on the context stream drain the device first.
+mrt_status drain(mrt_ctx* c, const char* what) {
+    hipError_t e = hipSetDevice(c->device);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    return e == hipSuccess ? MRT_OK : hip_fail(c, e, what);
+}
+
 } // namespace
 
 namespace mrt {
@@ -289,9 +297,9 @@ MRT_API mrt_status mrt_guide_create(mrt_ctx* c, mrt_scene* s, const mrt_guide_co
 
 MRT_API mrt_status mrt_guide_destroy(mrt_ctx* c, mrt_guide* g) {
     if (!c || !g) return MRT_ERR_INVALID;
-    const hipError_t e = hipStreamSynchronize(c->stream); // kernels may still use the guide's buffers
+    const mrt_status st = drain(c, "mrt_guide_destroy"); // kernels may still use the guide's buffers
     const bool freed = guide_free(c, g);
-    if (e != hipSuccess) return hip_fail(c, e, "mrt_guide_destroy");
+    if (st != MRT_OK) return st;
     return freed ? MRT_OK : fail(c, MRT_ERR_DEVICE, "mrt_guide_destroy: hipHostFree failed");
 }
 
@@ -306,6 +314,7 @@ MRT_API mrt_status mrt_guide_normalisation(mrt_guide* g, float offset[3], float 
 
 MRT_API mrt_status mrt_guide_get_params(mrt_ctx* c, mrt_guide* g, float* weights, float* grid) {
     if (!c || !g) return MRT_ERR_INVALID;
+    if (const mrt_status st = drain(c, "mrt_guide_get_params"); st != MRT_OK) return st;
     hipError_t e = hipSuccess;
     if (weights)
         e = hipMemcpyAsync(weights, g->P.w, size_t(G_LAYERS) * G_WIDTH * G_WIDTH * 4, hipMemcpyDeviceToHost, c->stream);
@@ -316,6 +325,7 @@ MRT_API mrt_status mrt_guide_get_params(mrt_ctx* c, mrt_guide* g, float* weights
 
 MRT_API mrt_status mrt_guide_set_params(mrt_ctx* c, mrt_guide* g, const float* weights, const float* grid) {
     if (!c || !g) return MRT_ERR_INVALID;
+    if (const mrt_status st = drain(c, "mrt_guide_set_params"); st != MRT_OK) return st;
     const size_t nw = size_t(G_LAYERS) * G_WIDTH * G_WIDTH;
     std::vector<float> w;
     if (weights) {
@@ -344,6 +354,7 @@ MRT_API mrt_status mrt_guide_forward(mrt_ctx* c, mrt_guide* g, const float* posi
     if (!c || !g || (n && (!positions || !directions || !outputs)))
         return fail(c, MRT_ERR_INVALID, "mrt_guide_forward: null argument");
     if (n == 0) return MRT_OK;
+    if (const mrt_status st = drain(c, "mrt_guide_forward"); st != MRT_OK) return st;
     std::vector<float4> hp(n), hd(n);
     for (uint32_t i = 0; i < n; i++) {
         hp[i] = make_float4(positions[3 * i], positions[3 * i + 1], positions[3 * i + 2], 0.f);
@@ -374,7 +385,8 @@ MRT_API mrt_status mrt_guide_gradient(mrt_ctx* c, mrt_guide* g, const float* rec
                                       float* d_grid, float* loss) {
     if (!c || !g || (n && !records)) return fail(c, MRT_ERR_INVALID, "mrt_guide_gradient: null argument");
     if (n == 0) return fail(c, MRT_ERR_INVALID, "mrt_guide_gradient: no records");
-    mrt_status st = ensure_train(c, g, n);
+    mrt_status st = drain(c, "mrt_guide_gradient");
+    if (st == MRT_OK) st = ensure_train(c, g, n);
     if (st == MRT_OK) st = upload_records(c, g, records, n);
     if (st != MRT_OK) return st;
     GuideTrainBuffers tb = g->T;
@@ -400,7 +412,8 @@ MRT_API mrt_status mrt_guide_train(mrt_ctx* c, mrt_guide* g, const float* record
     if (!c || !g || (n && !records)) return fail(c, MRT_ERR_INVALID, "mrt_guide_train: null argument");
     if (n == 0 || steps == 0) return MRT_OK;
     const uint32_t batch = std::min(n, g->cfg.train_batch);
-    mrt_status st = ensure_train(c, g, batch);
+    mrt_status st = drain(c, "mrt_guide_train");
+    if (st == MRT_OK) st = ensure_train(c, g, batch);
     if (st == MRT_OK) st = upload_records(c, g, records, n);
     if (st != MRT_OK) return st;
     uint32_t last = batch;

@@ -170,7 +170,8 @@ MRT_API mrt_status mrt_mcpg_create(mrt_ctx* c, const mrt_mcpg_config* config, mr
 
 MRT_API mrt_status mrt_mcpg_destroy(mrt_ctx* c, mrt_mcpg* m) {
     if (!c || !m) return MRT_ERR_INVALID;
-    const hipError_t e = hipDeviceSynchronize(); // kernels on any stream may still use the grids
+    hipError_t e = hipSetDevice(c->device);
+    if (e == hipSuccess) e = hipDeviceSynchronize(); // kernels on any stream may still use the grids
     free_grids(c, m);
     delete m;
     return e == hipSuccess ? MRT_OK : hip_fail(c, e, "mrt_mcpg_destroy");
@@ -198,7 +199,9 @@ MRT_API mrt_status mrt_mcpg_render_dev(mrt_ctx* c, mrt_scene* s, mrt_mcpg* m, co
 MRT_API mrt_status mrt_mcpg_read_grids(mrt_ctx* c, mrt_mcpg* m, uint32_t* mc_keys, void* mc_states, uint32_t* lc_keys,
                                        void* lc_entries) {
     if (!c || !m) return MRT_ERR_INVALID;
-    if (const hipError_t e = hipStreamSynchronize(c->stream); e != hipSuccess) return hip_fail(c, e, "mrt_mcpg_read_grids");
+    // frames may have run on a caller's stream: drain the device before the copies
+    if (const hipError_t e = hipSetDevice(c->device); e != hipSuccess) return hip_fail(c, e, "mrt_mcpg_read_grids");
+    if (const hipError_t e = hipDeviceSynchronize(); e != hipSuccess) return hip_fail(c, e, "mrt_mcpg_read_grids");
     const uint64_t mc = m->cfg.mc.buffer_size, lc = m->cfg.lc.buffer_size;
     hipError_t e = hipSuccess;
     if (mc_keys) e = hipMemcpy(mc_keys, m->mc_keys, 8 * mc, hipMemcpyDeviceToHost);
