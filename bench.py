@@ -7,6 +7,11 @@ materials (Lambert / GGX conductor / dielectric 1.5), 2048x1024 lat-long sky wit
 5e4:1 dynamic range. A step = one frame: GBuffer pass + pt (BSDF-only estimator, the
 reference's). Inputs (scene, BVH, buffers) are resident in HBM before timing starts.
 
+Further legs in the same line: `guided` (equal-time relMSE of the learned guide and MCPG against
+plain pt), `serving` (Poisson job mix, p50/p99), `ray_query` (1M random triangles, 16M
+incoherent + 4096^2 coherent closest-hit rays, Mrays/s) and `cpu_baseline` (the CPU oracle on
+bounded samples of the frame and of the ray queries).
+
 Prints one JSON line. --gpus N > 1 exits: this build runs on one GPU.
 """
 from __future__ import annotations
@@ -64,6 +69,27 @@ def cpu_baseline(scene, spec, frame, target_s: float, threads: int):
             "kind": "port",
             "sample": f"{cw}x{ch} centre crop of the same 1920x1080 frame, 64 spp, L=8 "
                       f"({segs + npx} segments, {t:.1f} s incl. the crop's share of the CPU gbuffer pass)"}
+
+
+def cpu_ray_query(threads: int) -> dict:
+    """The CPU oracle's BVH traversal on bounded samples of the ray-query workload: the same 1M
+    triangles, 1,048,576 incoherent rays of the same distribution and every 16th of the 4096^2
+    coherent rays."""
+    import numpy as np
+    from merian_amd import scenes
+    from oracle import oracle as O
+    pos, idx = scenes.ray_query(seed=1, n_tri=1_000_000)
+    osc = O.OracleScene(triangles=(pos, idx), use_bvh=True)
+    spec = scenes.ray_query_camera(4096)
+    sets = (("incoherent", scenes.incoherent_rays(1 << 20, seed=12)),
+            ("coherent", np.ascontiguousarray(O.camera_rays(O.camera_look_at(spec, 0), 4096, 4096)[::16])))
+    out = {"unit": "Mrays/s", "cores": threads, "kind": "port"}
+    for name, rays in sets:
+        t0 = time.perf_counter()
+        osc.trace(rays, threads=threads)
+        out[name] = round(len(rays) / (time.perf_counter() - t0) / 1e6, 3)
+    out["sample"] = "1,048,576 incoherent rays and every 16th of the 4096x4096 coherent rays over the same 1M triangles"
+    return out
 
 
 def guided_leg(ctx, scene, L: int) -> dict:
@@ -206,6 +232,49 @@ def serving_leg(ctx) -> dict:
     return res
 
 
+def ray_query_leg(ctx, reps: int = 3) -> dict:
+    """Ray-query microbenchmark: 1M random triangles in the unit cube (centroids uniform, sizes
+    log-uniform in [1e-3, 1e-1]), 16M incoherent rays (origins uniform in the cube, directions
+    uniform on the sphere) and 4096^2 coherent pinhole rays; closest hit id, t and barycentrics.
+    Rays and hits are resident in HBM; the time is the closest-hit launch alone (HIP events on
+    the launching stream)."""
+    import numpy as np
+    import torch
+    from merian_amd import scenes
+    pos, idx = scenes.ray_query(seed=1, n_tri=1_000_000)
+    ds = ctx.triangles(pos, idx)
+    t = ds.tuning()
+    t.bvh_width, t.bvh_max_leaf = 4, 2
+    ds.set_tuning(t)
+    n = 1 << 24
+    d_inc = torch.from_numpy(scenes.incoherent_rays(n, seed=11).view(np.uint8)).cuda()
+    d_coh = torch.empty(n * 32, dtype=torch.uint8, device="cuda")
+    ctx.camera_rays_dev(ctx.camera(scenes.ray_query_camera(4096), 0), 4096, 4096, d_coh.data_ptr())
+    d_hits = torch.empty(n * 16, dtype=torch.uint8, device="cuda")
+    ctx.synchronize()
+    stream = torch.cuda.Stream()
+    res = {"workload": "1,000,000 random triangles in the unit cube, BVH4 leaf 2; 16,777,216 incoherent rays "
+                       "and 4096x4096 coherent pinhole rays"}
+    for name, d_r in (("incoherent", d_inc), ("coherent", d_coh)):
+        nodes, tris = ctx.trace_count_dev(ds, d_r.data_ptr(), n, d_hits.data_ptr())
+        with torch.cuda.stream(stream):
+            ctx.trace_dev(ds, d_r.data_ptr(), n, d_hits.data_ptr(), stream=stream.cuda_stream)  # warm-up
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+            ev[0].record(stream)
+            for _ in range(reps):
+                ctx.trace_dev(ds, d_r.data_ptr(), n, d_hits.data_ptr(), stream=stream.cuda_stream)
+            ev[1].record(stream)
+        stream.synchronize()
+        ms = ev[0].elapsed_time(ev[1]) / reps
+        bytes_per_ray = (nodes * NODE_BYTES[4] + tris * TRI_BYTES) / n + RAY_IO_BYTES
+        res[name] = {"mrays_per_s": round(n / ms / 1e3, 1), "ms_per_launch": round(ms, 3),
+                     "node_visits_per_ray": round(nodes / n, 2), "triangle_tests_per_ray": round(tris / n, 2),
+                     "algorithmic_gbs": round(bytes_per_ray * n / ms / 1e6, 1)}
+    ds.destroy()
+    log(f"ray-query leg: {res}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -221,6 +290,7 @@ def main():
     ap.add_argument("--default-tuning", action="store_true", help="render with the untuned defaults")
     ap.add_argument("--no-guided", action="store_true", help="skip the equal-time guided vs plain leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-mix leg")
+    ap.add_argument("--no-ray-query", action="store_true", help="skip the ray-query microbenchmark leg")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; --gpus N > 1 is not supported", file=sys.stderr)
@@ -337,12 +407,19 @@ def main():
         line["guided"] = guided_leg(ctx, scene, args.max_path_length)
     if not args.no_serving:
         line["serving"] = serving_leg(ctx)
+    if not args.no_ray_query:
+        line["ray_query"] = ray_query_leg(ctx)
     ctx.close()
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         log(f"cpu baseline on {threads} threads ...")
         line["cpu_baseline"] = cpu_baseline(scene, spec, args.warmup, args.cpu_seconds, threads)
         line["gpu_over_cpu"] = round(value / line["cpu_baseline"]["value"], 1)
+        if "ray_query" in line:
+            rq = cpu_ray_query(threads)
+            line["cpu_baseline"]["ray_query"] = rq
+            for k in ("incoherent", "coherent"):
+                line["ray_query"][k]["gpu_over_cpu"] = round(line["ray_query"][k]["mrays_per_s"] / rq[k], 1)
     print(json.dumps(line), flush=True)
 
 

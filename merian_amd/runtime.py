@@ -119,6 +119,12 @@ class Context:
         self._check(self.lib.mrt_trace_closest_dev(self.h, scene.h, C.c_void_p(d_rays), n, C.c_void_p(d_hits),
                                                    C.c_void_p(stream) if stream else None))
 
+    def trace_count_dev(self, scene: DeviceScene, d_rays: int, n: int, d_hits: int) -> tuple[int, int]:
+        """As trace_dev with the counting kernel (synchronous): (node visits, triangle tests)."""
+        counts = (C.c_uint64 * 2)()
+        self._check(self.lib.mrt_trace_count_dev(self.h, scene.h, C.c_void_p(d_rays), n, C.c_void_p(d_hits), counts))
+        return int(counts[0]), int(counts[1])
+
     def camera_rays_dev(self, cam: abi.Camera, w: int, h: int, d_rays: int):
         self._check(self.lib.mrt_camera_rays_dev(self.h, C.byref(cam), w, h, C.c_void_p(d_rays), None))
 

@@ -118,3 +118,59 @@ def test_coherent_camera_rays_vs_oracle(ctx, tree):
     got = ctx.trace(ds, rays)
     ds.destroy()
     _assert_hits_equal(got, ref)
+
+
+def _sample_hits(d_hits, sel):
+    """Host copy of the hits at indices `sel` of a device hit array (torch uint8, 16 B per hit)."""
+    import torch
+    rows = d_hits.view(-1, 16)[torch.as_tensor(sel, device=d_hits.device)]
+    return rows.cpu().numpy().reshape(-1).view(abi.HIT_DTYPE)
+
+
+@pytest.mark.parametrize("tree", [(4, 2), (8, 2)])
+def test_ray_query_workload_full_size(ctx, tree):
+    # the ray-query microbenchmark at its full size: 1M random triangles, 16M incoherent rays and
+    # 4096^2 coherent camera rays. A seeded sample of 65,536 rays of each set is bit-exact
+    # against the oracle's traversal; over every ray, ids are in range, t lies in the ray's
+    # interval and two launches give identical results.
+    import torch
+    n_tri = 1_000_000
+    pos, idx = scenes.ray_query(seed=1, n_tri=n_tri)
+    osc = O.OracleScene(triangles=(pos, idx), use_bvh=True)
+    ds = _tris(ctx, pos, idx, tree)
+    try:
+        rays = scenes.incoherent_rays(1 << 24, seed=11)
+        d_rays = torch.from_numpy(rays.view(np.uint8)).cuda()
+        cam = ctx.camera(scenes.ray_query_camera(4096), 0)
+        d_cam = torch.empty(4096 * 4096 * 32, dtype=torch.uint8, device="cuda")
+        ctx.camera_rays_dev(cam, 4096, 4096, d_cam.data_ptr())
+        ctx.synchronize()
+        rng = np.random.default_rng(5)
+        for name, d_r, n, host_rays in (("incoherent", d_rays, 1 << 24, rays), ("coherent", d_cam, 4096 * 4096, None)):
+            d_hits = torch.empty(n * 16, dtype=torch.uint8, device="cuda")
+            d_again = torch.empty_like(d_hits)
+            ctx.trace_dev(ds, d_r.data_ptr(), n, d_hits.data_ptr())
+            ctx.trace_dev(ds, d_r.data_ptr(), n, d_again.data_ptr())
+            ctx.synchronize()
+            assert torch.equal(d_hits, d_again), name
+            sel = np.sort(rng.choice(n, 65536, replace=False))
+            if host_rays is None:
+                host_rays = O.camera_rays(O.camera_look_at(scenes.ray_query_camera(4096), 0), 4096, 4096)
+                got_rays = d_r.view(-1, 32)[torch.as_tensor(sel, device="cuda")].cpu().numpy().reshape(-1)
+                assert got_rays.tobytes() == np.ascontiguousarray(host_rays[sel]).view(np.uint8).tobytes()
+            ref = osc.trace(np.ascontiguousarray(host_rays[sel]))
+            got = _sample_hits(d_hits, sel)
+            _assert_hits_equal(got, ref)
+            # every ray: ids in range, t inside [tmin, tmax] of its ray
+            tri = d_hits.view(torch.int32).view(-1, 4)[:, 0]
+            t = d_hits.view(torch.float32).view(-1, 4)[:, 1]
+            hit = tri != -1
+            assert bool(((tri[hit] >= 0) & (tri[hit] < n_tri)).all()), name
+            tmin = d_r.view(torch.float32).view(-1, 8)[:, 3]
+            tmax = d_r.view(torch.float32).view(-1, 8)[:, 7]
+            assert bool(((t[hit] >= tmin[hit]) & (t[hit] <= tmax[hit])).all()), name
+            frac = float(hit.float().mean())
+            assert abs(frac - float((ref["triangle"] != 0xFFFFFFFF).mean())) < 0.01, (name, frac)
+            del d_hits, d_again
+    finally:
+        ds.destroy()
