@@ -232,20 +232,67 @@ def serving_leg(ctx) -> dict:
     return res
 
 
-def ray_query_leg(ctx, reps: int = 3) -> dict:
+# Traversal settings of the ray-query leg, per ray set: the fastest of a grid search over BVH
+# width {4, 8}, leaf size {1, 2, 4}, SAH node cost {0.5, 1, 2} (BVH4 leaf 2 only), refill lanes
+# {8, 16, 24, 32, 48}, leaf-phase lanes {4, 8, 16, 32} and grid {50, 75, 100} % on this workload
+# (re-run with --retune-rays). No setting changes a hit.
+RAY_QUERY_TUNING = {
+    "incoherent": dict(bvh_width=8, bvh_max_leaf=2, bvh_traversal_cost=1.0, refill_lanes=16, leaf_lanes=4,
+                       grid_percent=100),
+    "coherent": dict(bvh_width=4, bvh_max_leaf=2, bvh_traversal_cost=1.0, refill_lanes=48, leaf_lanes=8,
+                     grid_percent=100),
+}
+
+
+def _apply(ds, settings: dict):
+    t = ds.tuning()
+    for k, v in settings.items():
+        setattr(t, k, v)
+    ds.set_tuning(t)
+
+
+def _time_rays(ctx, ds, d_r, n, d_hits, stream, reps):
+    """Average ms of `reps` closest-hit launches after one warm-up (HIP events on the launching stream)."""
+    import torch
+    with torch.cuda.stream(stream):
+        ctx.trace_dev(ds, d_r.data_ptr(), n, d_hits.data_ptr(), stream=stream.cuda_stream)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ev[0].record(stream)
+        for _ in range(reps):
+            ctx.trace_dev(ds, d_r.data_ptr(), n, d_hits.data_ptr(), stream=stream.cuda_stream)
+        ev[1].record(stream)
+    stream.synchronize()
+    return ev[0].elapsed_time(ev[1]) / reps
+
+
+def _search_ray_tuning(ctx, ds, d_r, n, d_hits, stream) -> dict:
+    import itertools
+    best = None
+    trees = [(4, 2, 1.0), (4, 1, 1.0), (4, 4, 1.0), (8, 2, 1.0), (8, 4, 1.0), (4, 2, 2.0), (4, 2, 0.5)]
+    for width, leaf, cost in trees:
+        _apply(ds, dict(bvh_width=width, bvh_max_leaf=leaf, bvh_traversal_cost=cost))
+        for refill, ll, grid in itertools.product([8, 16, 24, 32, 48], [4, 8, 16, 32], [50, 75, 100]):
+            cand = dict(bvh_width=width, bvh_max_leaf=leaf, bvh_traversal_cost=cost, refill_lanes=refill,
+                        leaf_lanes=ll, grid_percent=grid)
+            _apply(ds, cand)
+            ms = _time_rays(ctx, ds, d_r, n, d_hits, stream, 2)
+            log(f"  {cand}: {ms:.2f} ms")
+            if best is None or ms < best[0]:
+                best = (ms, cand)
+    return best[1]
+
+
+def ray_query_leg(ctx, reps: int = 3, retune: bool = False) -> dict:
     """Ray-query microbenchmark: 1M random triangles in the unit cube (centroids uniform, sizes
     log-uniform in [1e-3, 1e-1]), 16M incoherent rays (origins uniform in the cube, directions
     uniform on the sphere) and 4096^2 coherent pinhole rays; closest hit id, t and barycentrics.
     Rays and hits are resident in HBM; the time is the closest-hit launch alone (HIP events on
-    the launching stream)."""
+    the launching stream), with each ray set's traversal settings (RAY_QUERY_TUNING)."""
     import numpy as np
     import torch
     from merian_amd import scenes
     pos, idx = scenes.ray_query(seed=1, n_tri=1_000_000)
     ds = ctx.triangles(pos, idx)
-    t = ds.tuning()
-    t.bvh_width, t.bvh_max_leaf = 4, 2
-    ds.set_tuning(t)
     n = 1 << 24
     d_inc = torch.from_numpy(scenes.incoherent_rays(n, seed=11).view(np.uint8)).cuda()
     d_coh = torch.empty(n * 32, dtype=torch.uint8, device="cuda")
@@ -253,23 +300,17 @@ def ray_query_leg(ctx, reps: int = 3) -> dict:
     d_hits = torch.empty(n * 16, dtype=torch.uint8, device="cuda")
     ctx.synchronize()
     stream = torch.cuda.Stream()
-    res = {"workload": "1,000,000 random triangles in the unit cube, BVH4 leaf 2; 16,777,216 incoherent rays "
+    res = {"workload": "1,000,000 random triangles in the unit cube; 16,777,216 incoherent rays "
                        "and 4096x4096 coherent pinhole rays"}
     for name, d_r in (("incoherent", d_inc), ("coherent", d_coh)):
+        settings = _search_ray_tuning(ctx, ds, d_r, n, d_hits, stream) if retune else RAY_QUERY_TUNING[name]
+        _apply(ds, settings)
         nodes, tris = ctx.trace_count_dev(ds, d_r.data_ptr(), n, d_hits.data_ptr())
-        with torch.cuda.stream(stream):
-            ctx.trace_dev(ds, d_r.data_ptr(), n, d_hits.data_ptr(), stream=stream.cuda_stream)  # warm-up
-            ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-            ev[0].record(stream)
-            for _ in range(reps):
-                ctx.trace_dev(ds, d_r.data_ptr(), n, d_hits.data_ptr(), stream=stream.cuda_stream)
-            ev[1].record(stream)
-        stream.synchronize()
-        ms = ev[0].elapsed_time(ev[1]) / reps
-        bytes_per_ray = (nodes * NODE_BYTES[4] + tris * TRI_BYTES) / n + RAY_IO_BYTES
+        ms = _time_rays(ctx, ds, d_r, n, d_hits, stream, reps)
+        bytes_per_ray = (nodes * NODE_BYTES[settings["bvh_width"]] + tris * TRI_BYTES) / n + RAY_IO_BYTES
         res[name] = {"mrays_per_s": round(n / ms / 1e3, 1), "ms_per_launch": round(ms, 3),
                      "node_visits_per_ray": round(nodes / n, 2), "triangle_tests_per_ray": round(tris / n, 2),
-                     "algorithmic_gbs": round(bytes_per_ray * n / ms / 1e6, 1)}
+                     "algorithmic_gbs": round(bytes_per_ray * n / ms / 1e6, 1), "tuning": settings}
     ds.destroy()
     log(f"ray-query leg: {res}")
     return res
@@ -291,6 +332,8 @@ def main():
     ap.add_argument("--no-guided", action="store_true", help="skip the equal-time guided vs plain leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-mix leg")
     ap.add_argument("--no-ray-query", action="store_true", help="skip the ray-query microbenchmark leg")
+    ap.add_argument("--retune-rays", action="store_true",
+                    help="grid-search the ray-query leg's traversal settings instead of using RAY_QUERY_TUNING")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; --gpus N > 1 is not supported", file=sys.stderr)
@@ -408,7 +451,7 @@ def main():
     if not args.no_serving:
         line["serving"] = serving_leg(ctx)
     if not args.no_ray_query:
-        line["ray_query"] = ray_query_leg(ctx)
+        line["ray_query"] = ray_query_leg(ctx, retune=args.retune_rays)
     ctx.close()
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)

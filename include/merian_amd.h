@@ -269,8 +269,9 @@ MRT_API mrt_status mrt_trace_closest(mrt_ctx* ctx, mrt_scene* scene, const mrt_r
                                      uint64_t n, mrt_hit* hits);
 MRT_API mrt_status mrt_trace_closest_dev(mrt_ctx* ctx, mrt_scene* scene, const mrt_ray* d_rays,
                                          uint64_t n, mrt_hit* d_hits, void* stream);
-/* As mrt_trace_closest_dev, with the counting kernel: counts[0] = BVH4 node visits,
- * counts[1] = ray-triangle tests (host array). Synchronous; for the bandwidth roofline. */
+/* As mrt_trace_closest_dev, with the counting kernel: counts[0] = node visits of the scene's
+ * tree (4- or 8-wide), counts[1] = ray-triangle tests (host array). Synchronous; for the
+ * bandwidth roofline. */
 MRT_API mrt_status mrt_trace_count_dev(mrt_ctx* ctx, mrt_scene* scene, const mrt_ray* d_rays, uint64_t n,
                                        mrt_hit* d_hits, uint64_t counts[2]);
 /* Pinhole rays of a w x h image on the device (camera.slang:45-56), for coherent
