@@ -9,8 +9,9 @@ reference's). Inputs (scene, BVH, buffers) are resident in HBM before timing sta
 
 Further legs in the same line: `guided` (equal-time relMSE of the learned guide and MCPG against
 plain pt), `serving` (Poisson job mix, p50/p99), `ray_query` (1M random triangles, 16M
-incoherent + 4096^2 coherent closest-hit rays, Mrays/s) and `cpu_baseline` (the CPU oracle on
-bounded samples of the frame and of the ray queries).
+incoherent + 4096^2 coherent closest-hit rays, Mrays/s), `small` (the small plain-pt
+configuration: 81,922 triangles, 512^2, 4 spp, L = 4) and `cpu_baseline` (the CPU oracle on
+bounded samples of the frame and of the ray queries, and on the whole small frame).
 
 Prints one JSON line. --gpus N > 1 exits: this build runs on one GPU.
 """
@@ -69,6 +70,64 @@ def cpu_baseline(scene, spec, frame, target_s: float, threads: int):
             "kind": "port",
             "sample": f"{cw}x{ch} centre crop of the same 1920x1080 frame, 64 spp, L=8 "
                       f"({segs + npx} segments, {t:.1f} s incl. the crop's share of the CPU gbuffer pass)"}
+
+
+def cpu_small(threads: int, min_s: float = 3.0) -> dict:
+    """The CPU oracle on whole small plain-pt frames (512^2, 4 spp, L = 4), gbuffer included,
+    frames 0, 1, ... until at least `min_s` seconds have passed."""
+    from merian_amd import scenes
+    from oracle import oracle as O
+    sc = scenes.plain_small(seed=0)
+    spec = sc.camera
+    osc = O.OracleScene(sc)
+    p = O.pt_params(4, 4, record_signature=False)
+    segs, f, t = 0, 0, 0.0
+    while t < min_s:
+        cam = O.camera_look_at(spec, f)
+        t0 = time.perf_counter()
+        gb = osc.gbuffer(cam, spec.width, spec.height, threads=threads)
+        _, _, s_ = osc.render(p, cam, f, spec.width, spec.height, gb, threads=threads)
+        t += time.perf_counter() - t0
+        segs += s_ + spec.width * spec.height
+        f += 1
+    return {"value": round(segs / t / 1e6, 3), "unit": "Mpath-segments/s", "cores": threads, "kind": "port",
+            "sample": f"{f} whole frames ({segs} segments, {t:.2f} s)"}
+
+
+def small_leg(ctx, frames: int = 64) -> dict:
+    """The small plain-pt configuration: 8x8 icospheres on a ground plane (81,922 triangles),
+    constant white environment, 512x512, 4 spp, max path length 4; a step = GBuffer + pt frame."""
+    import torch
+    from merian_amd import scenes
+    from merian_amd.runtime import pt_params
+    sc = scenes.plain_small(seed=0)
+    spec = sc.camera
+    W, H = spec.width, spec.height
+    ds = ctx.scene(sc)
+    gb = torch.empty(W * H * 16, dtype=torch.uint8, device="cuda")
+    out = torch.empty(W * H * 4, dtype=torch.float32, device="cuda")
+    params = pt_params(spp=4, max_path_length=4, record_signature=False)
+
+    def frame(f):
+        cam = ctx.camera(spec, f)
+        ctx.gbuffer_dev(ds, cam, W, H, gb.data_ptr())
+        return ctx.pt_render_dev(ds, params, cam, f, W, H, gb.data_ptr(), out.data_ptr(), stats=True)
+
+    frame(0)
+    torch.cuda.synchronize()
+    segs = 0
+    t0 = time.perf_counter()
+    for f in range(1, frames + 1):
+        st = frame(f)
+        segs += st.primary_segments + st.continuation_segments
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ds.destroy()
+    res = {"workload": f"{sc.triangle_count} triangles, 512x512, 4 spp, max path length 4, constant environment",
+           "mseg_per_s": round(segs / dt / 1e6, 1), "ms_per_frame": round(dt * 1e3 / frames, 3),
+           "segments_per_frame": segs // frames, "frames": frames}
+    log(f"small leg: {res}")
+    return res
 
 
 def cpu_ray_query(threads: int) -> dict:
@@ -332,6 +391,7 @@ def main():
     ap.add_argument("--no-guided", action="store_true", help="skip the equal-time guided vs plain leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-mix leg")
     ap.add_argument("--no-ray-query", action="store_true", help="skip the ray-query microbenchmark leg")
+    ap.add_argument("--no-small", action="store_true", help="skip the small plain-pt leg (512^2, 4 spp, L = 4)")
     ap.add_argument("--retune-rays", action="store_true",
                     help="grid-search the ray-query leg's traversal settings instead of using RAY_QUERY_TUNING")
     args = ap.parse_args()
@@ -452,12 +512,18 @@ def main():
         line["serving"] = serving_leg(ctx)
     if not args.no_ray_query:
         line["ray_query"] = ray_query_leg(ctx, retune=args.retune_rays)
+    if not args.no_small:
+        line["small"] = small_leg(ctx)
     ctx.close()
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         log(f"cpu baseline on {threads} threads ...")
         line["cpu_baseline"] = cpu_baseline(scene, spec, args.warmup, args.cpu_seconds, threads)
         line["gpu_over_cpu"] = round(value / line["cpu_baseline"]["value"], 1)
+        if "small" in line:
+            cs = cpu_small(threads)
+            line["cpu_baseline"]["small"] = cs
+            line["small"]["gpu_over_cpu"] = round(line["small"]["mseg_per_s"] / cs["value"], 1)
         if "ray_query" in line:
             rq = cpu_ray_query(threads)
             line["cpu_baseline"]["ray_query"] = rq
