@@ -126,3 +126,59 @@ def test_converged_image_relmse_against_16384_spp(ctx):
         assert float(np.mean((few[..., :3] - ref[..., :3]) ** 2 / (ref[..., :3] ** 2 + 1e-2))) > rel
     finally:
         ds.destroy()
+
+
+def test_zero_spp_writes_zeros(ctx, small):
+    # render_pt.slang:113-129: no sample, no division, irradiance and second moment stay 0
+    sc, ds, osc = small
+    cam = ctx.camera(sc.camera, 0)
+    img, sig, gb, st = ctx.render(ds, pt_params(spp=0, max_path_length=4), cam, 0, 32, 24)
+    assert not img.any() and not sig.any()
+    assert int(st.continuation_segments) == 0
+    ref, _, segs = osc.render(O.pt_params(0, 4), cam, 0, 32, 24, gb)
+    assert not ref.any() and segs == 0
+
+
+@pytest.mark.parametrize("W,H", [(1, 1), (1, 37), (53, 1), (7, 13)])
+def test_ragged_image_sizes(ctx, small, W, H):
+    sc, ds, osc = small
+    cam = ctx.camera(sc.camera, 4)
+    img, sig, gb, st = ctx.render(ds, pt_params(spp=3, max_path_length=5), cam, 4, W, H)
+    assert img.shape == (H, W, 4)
+    ref_gb = osc.gbuffer(O.camera_look_at(sc.camera, 4), W, H)
+    assert gb.tobytes() == ref_gb.tobytes()
+    ref, ref_sig, _ = osc.render(O.pt_params(3, 5), cam, 4, W, H, gb)
+    _compare(img, sig, ref, ref_sig, min_match=1.0 if W * H < 1000 else 0.999)
+
+
+def test_frame_split_into_passes(ctx, small):
+    # paths in flight below spp x pixels: the frame runs in passes of whole samples (2, 2, 1
+    # here) and must equal the single-pass frame bit for bit, and the oracle
+    sc, ds, osc = small
+    W, H = 48, 40
+    cam = ctx.camera(sc.camera, 6)
+    p = pt_params(spp=5, max_path_length=5)
+    one, sig1, gb, st1 = ctx.render(ds, p, cam, 6, W, H)
+    t = ds.tuning()
+    saved = t.max_paths
+    t.max_paths = 2 * W * H + 7
+    ds.set_tuning(t)
+    try:
+        split, sig2, gb2, st2 = ctx.render(ds, p, cam, 6, W, H)
+    finally:
+        t = ds.tuning()
+        t.max_paths = saved
+        ds.set_tuning(t)
+    assert gb2.tobytes() == gb.tobytes()
+    assert split.tobytes() == one.tobytes() and sig2.tobytes() == sig1.tobytes()
+    assert int(st2.continuation_segments) == int(st1.continuation_segments)
+    ref, ref_sig, _ = osc.render(O.pt_params(5, 5), cam, 6, W, H, gb)
+    _compare(split, sig2, ref, ref_sig)
+
+
+def test_zero_sized_image_is_a_no_op(ctx, small):
+    sc, ds, _ = small
+    cam = ctx.camera(sc.camera, 0)
+    for W, H in ((0, 16), (16, 0)):
+        img, sig, gb, st = ctx.render(ds, pt_params(spp=2, max_path_length=3), cam, 0, W, H)
+        assert img.size == 0 and int(st.continuation_segments) == 0
