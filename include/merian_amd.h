@@ -328,7 +328,7 @@ typedef struct mrt_tuning {
 typedef struct mrt_tune_request {
     mrt_camera camera;        /* the view to tune on */
     uint32_t width, height;
-    mrt_pt_params params;     /* spp is capped at 8 for the trial frames */
+    mrt_pt_params params;     /* spp is capped at 64 for the trial frames */
     float time_budget_ms;     /* wall time the search may take (0 = 20000) */
     uint32_t _pad;
 } mrt_tune_request;

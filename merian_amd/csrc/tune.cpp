@@ -233,9 +233,11 @@ MRT_API mrt_status mrt_tune(mrt_ctx* c, mrt_scene* s, const mrt_tune_request* r,
             cand.bvh_traversal_cost = b.cost;
             if ((st = try_one(cand)) != MRT_OK) return st;
         }
-        // paths in flight (whole samples of the image per pass)
+        // paths in flight (whole samples of the image per pass): up to 2^28 (about 48 GB of path
+        // state), so that a 1080p frame of 64 spp can run as one pass; a candidate beyond the
+        // context's HBM budget is skipped
         for (uint64_t mp : {uint64_t(1) << 21, uint64_t(1) << 22, uint64_t(1) << 23, uint64_t(1) << 24,
-                            uint64_t(1) << 25}) {
+                            uint64_t(1) << 25, uint64_t(1) << 26, uint64_t(1) << 27, uint64_t(1) << 28}) {
             if (mp == best.max_paths || mp < npix) continue;
             mrt_tuning cand = best;
             cand.max_paths = mp;
