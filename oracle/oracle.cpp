@@ -575,20 +575,24 @@ inline ShadingPoint env_shading_point(const Scene& sc, const Ray& ray) {
     return sp;
 }
 
-inline ShadingPoint surface_shading_point(const Scene& sc, const Ray& ray, const Hit& hit, uint32_t bounce,
-                                          float ior, bool with_material) {
-    const mrt_geometry& G = sc.geoms[hit.instance_id + hit.geometry_index];
-    const mrt_vertex& v0 = sc.vertices[G.vertex_offset + sc.vertex_index(G, hit.primitive_id, 0)];
-    const mrt_vertex& v1 = sc.vertices[G.vertex_offset + sc.vertex_index(G, hit.primitive_id, 1)];
-    const mrt_vertex& v2 = sc.vertices[G.vertex_offset + sc.vertex_index(G, hit.primitive_id, 2)];
-    const f3 b{(1.0f - hit.bary.x) - hit.bary.y, hit.bary.x, hit.bary.y};
+// The geometric part of Scene.get_surface (scene.slang:322-382): face normal (FlipFacing),
+// vertex normal (FlatShading), and the shading frame from per-vertex tangents or the Lengyel
+// UV-derived fallback (scene.slang:357-371).
+struct SurfaceGeom {
+    f3 pos, face_normal, vertex_normal;
+    f2 uv0, uv1, uv2;
+    Frame frame;
+};
+inline SurfaceGeom surface_geometry(const mrt_vertex& v0, const mrt_vertex& v1, const mrt_vertex& v2, uint32_t flags,
+                                    f3 b) {
     const f3 p0 = ld3(v0.position), p1 = ld3(v1.position), p2 = ld3(v2.position);
-    const f3 pos = b.x * p0 + b.y * p1 + b.z * p2;
+    SurfaceGeom g;
+    g.pos = b.x * p0 + b.y * p1 + b.z * p2;
     const f3 e01 = p1 - p0, e02 = p2 - p0;
     f3 face_normal = normalize(cross(e01, e02));
-    if (G.flags & MRT_GEOM_FLIP_FACING) face_normal = -face_normal;
+    if (flags & MRT_GEOM_FLIP_FACING) face_normal = -face_normal;
     f3 vertex_normal;
-    if (G.flags & MRT_GEOM_FLAT_SHADING) {
+    if (flags & MRT_GEOM_FLAT_SHADING) {
         vertex_normal = face_normal;
     } else {
         vertex_normal = normalize(b.x * decode_normal(v0.encoded_normal) + b.y * decode_normal(v1.encoded_normal) +
@@ -598,7 +602,7 @@ inline ShadingPoint surface_shading_point(const Scene& sc, const Ray& ray, const
     const f2 uv1{f16_to_f32(v1.uv[0]), f16_to_f32(v1.uv[1])};
     const f2 uv2{f16_to_f32(v2.uv[0]), f16_to_f32(v2.uv[1])};
     Frame frame;
-    if (G.flags & MRT_GEOM_HAS_TANGENTS) {
+    if (flags & MRT_GEOM_HAS_TANGENTS) {
         const auto dec_t = [](uint32_t enc, float& w) {
             w = (enc & 1u) ? -1.f : 1.f;
             return decode_normal(enc & ~1u);
@@ -614,13 +618,33 @@ inline ShadingPoint surface_shading_point(const Scene& sc, const Ray& ray, const
         const f2 d2{uv2.x - uv0.x, uv2.y - uv0.y};
         const float det = d1.x * d2.y - d2.x * d1.y;
         if (std::fabs(det) > 1e-8f) {
-            const float ws = (G.flags & MRT_GEOM_FLIP_FACING) ? 1.f : -1.f;
+            const float ws = (flags & MRT_GEOM_FLIP_FACING) ? 1.f : -1.f;
             const f3 t = (d2.y * e01 - d1.y * e02) / det;
             frame = frame_from_nt(vertex_normal, t, signf(det) * ws);
         } else {
             frame = frame_from_normal(vertex_normal);
         }
     }
+    g.face_normal = face_normal;
+    g.vertex_normal = vertex_normal;
+    g.uv0 = uv0;
+    g.uv1 = uv1;
+    g.uv2 = uv2;
+    g.frame = frame;
+    return g;
+}
+
+inline ShadingPoint surface_shading_point(const Scene& sc, const Ray& ray, const Hit& hit, uint32_t bounce,
+                                          float ior, bool with_material) {
+    const mrt_geometry& G = sc.geoms[hit.instance_id + hit.geometry_index];
+    const mrt_vertex& v0 = sc.vertices[G.vertex_offset + sc.vertex_index(G, hit.primitive_id, 0)];
+    const mrt_vertex& v1 = sc.vertices[G.vertex_offset + sc.vertex_index(G, hit.primitive_id, 1)];
+    const mrt_vertex& v2 = sc.vertices[G.vertex_offset + sc.vertex_index(G, hit.primitive_id, 2)];
+    const f3 b{(1.0f - hit.bary.x) - hit.bary.y, hit.bary.x, hit.bary.y};
+    const SurfaceGeom g = surface_geometry(v0, v1, v2, G.flags, b);
+    const f3 pos = g.pos, face_normal = g.face_normal;
+    const f2 uv0 = g.uv0, uv1 = g.uv1, uv2 = g.uv2;
+    Frame frame = g.frame;
     f3 fn = face_normal;
     bool front = true;
     if (G.flags & MRT_GEOM_TWO_SIDED) {
@@ -775,6 +799,40 @@ __attribute__((visibility("default"))) void orc_math(int op, const float* x, con
         default: r = o_pow(a, b); break;
         }
         out[i] = r;
+    }
+}
+// Frames (frame.slang:7-13,90-96,123-132): op 0 make_frame(a); op 1 OrthonormalFrame(a, b, hs);
+// op 2 OrthonormalFrame(a, b, hs).apply_normal_map(c). out = rows T, B, N (9 floats).
+__attribute__((visibility("default"))) void orc_frame(int32_t op, const float* a, const float* b, float hs,
+                                                      const float* c, float* out) {
+    Frame f = op == 0 ? frame_from_normal(ld3(a)) : frame_from_nt(ld3(a), ld3(b), hs);
+    if (op == 2) f = apply_normal_map(f, ld3(c));
+    const f3 rows[3] = {f.r0, f.r1, f.r2};
+    for (int r = 0; r < 3; r++) {
+        out[3 * r] = rows[r].x;
+        out[3 * r + 1] = rows[r].y;
+        out[3 * r + 2] = rows[r].z;
+    }
+}
+// EqualAreaOctMap::dir_to_uv (environment-map.slang:44-63) over n directions (float[3n] -> float[2n]).
+__attribute__((visibility("default"))) void orc_equal_area_uv(const float* d, uint64_t n, float* uv) {
+    for (uint64_t i = 0; i < n; i++) {
+        const f2 r = equal_area_dir_to_uv(ld3(d + 3 * i));
+        uv[2 * i] = r.x;
+        uv[2 * i + 1] = r.y;
+    }
+}
+// Geometric part of Scene.get_surface (scene.slang:322-382) for one triangle and barycentrics
+// (b1, b2): out = frame rows T, B, N, face normal, vertex normal (15 floats).
+__attribute__((visibility("default"))) void orc_surface_frame(const mrt_vertex* v, uint32_t flags, const float* bary,
+                                                              float* out) {
+    const f3 b{(1.0f - bary[0]) - bary[1], bary[0], bary[1]};
+    const SurfaceGeom g = surface_geometry(v[0], v[1], v[2], flags, b);
+    const f3 rows[5] = {g.frame.r0, g.frame.r1, g.frame.r2, g.face_normal, g.vertex_normal};
+    for (int r = 0; r < 5; r++) {
+        out[3 * r] = rows[r].x;
+        out[3 * r + 1] = rows[r].y;
+        out[3 * r + 2] = rows[r].z;
     }
 }
 __attribute__((visibility("default"))) uint16_t orc_f32_to_f16(float f) { return f32_to_f16(f); }

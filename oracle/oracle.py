@@ -53,6 +53,9 @@ def lib() -> C.CDLL:
             "orc_decode_normal": (None, [u32, vp]),
             "orc_offset_ray_origin": (None, [vp, vp, vp]),
             "orc_f32_to_f16": (C.c_uint16, [f32]),
+            "orc_frame": (None, [i32, vp, vp, f32, vp, vp]),
+            "orc_equal_area_uv": (None, [vp, u64, vp]),
+            "orc_surface_frame": (None, [vp, u32, vp, vp]),
             "orc_f16_to_f32": (f32, [C.c_uint16]),
             "orc_camera_look_at": (None, [vp, vp, vp, f32, f32, f32, f32, i32, u32, C.POINTER(abi.Camera)]),
             "orc_camera_rays": (None, [C.POINTER(abi.Camera), u32, u32, vp]),
@@ -155,6 +158,36 @@ def math(op: str, x, y=None) -> np.ndarray:
     out = np.zeros_like(x)
     lib().orc_math(MATH_OPS[op], _p(x), _p(yy), x.size, _p(out))
     return out
+
+
+def frame(op: str, a, b=None, hs: float = 1.0, c=None) -> np.ndarray:
+    """Shading frame rows (T, B, N) as a 3x3 array: op "normal" = make_frame(a), "nt" =
+    OrthonormalFrame(a, b, hs), "normal_map" = OrthonormalFrame(a, b, hs).apply_normal_map(c)."""
+    codes = {"normal": 0, "nt": 1, "normal_map": 2}
+    f = lambda v: np.ascontiguousarray(np.zeros(3) if v is None else v, dtype=np.float32)
+    a, b, c = f(a), f(b), f(c)
+    out = np.zeros(9, np.float32)
+    lib().orc_frame(codes[op], _p(a), _p(b), float(hs), _p(c), _p(out))
+    return out.reshape(3, 3)
+
+
+def equal_area_uv(dirs) -> np.ndarray:
+    """EqualAreaOctMap.dir_to_uv over float32 directions (n, 3) -> (n, 2)."""
+    d = np.ascontiguousarray(dirs, dtype=np.float32).reshape(-1, 3)
+    uv = np.zeros((len(d), 2), np.float32)
+    lib().orc_equal_area_uv(_p(d), len(d), _p(uv))
+    return uv
+
+
+def surface_frame(vertices: np.ndarray, flags: int, bary) -> dict:
+    """Geometric part of Scene.get_surface for one triangle (3 abi.VERTEX_DTYPE records)."""
+    v = np.ascontiguousarray(vertices)
+    assert v.nbytes == 72
+    b = np.ascontiguousarray(bary, dtype=np.float32)
+    out = np.zeros(15, np.float32)
+    lib().orc_surface_frame(_p(v), int(flags), _p(b), _p(out))
+    o = out.reshape(5, 3)
+    return {"T": o[0], "B": o[1], "N": o[2], "face_normal": o[3], "vertex_normal": o[4]}
 
 
 def encode_normal(n) -> int:
