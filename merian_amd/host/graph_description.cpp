@@ -1,324 +1,572 @@
 // Graph description engine of the graph host (graph_description.h).
+//
+// The file has four parts:
+//   1. files: locating JSON files against a list of search directories and reading them;
+//   2. the description schema: names, connection strings and the readers of each version;
+//   3. fragment merge and include expansion;
+//   4. command-line overrides. The "cli" block of a config declares named overrides. Each is
+//      read into a Declaration; applying one runs an Action (merge fragment files, then assign
+//      values at JSON pointers). An OverrideRun applies one command line to a config: variant
+//      defaults first, then the "--name value" pairs in argument order (a pair whose name is
+//      not declared yet waits until a merged fragment declares it), then the positional rest.
 #include "graph_description.h"
 
 #include <algorithm>
 #include <cctype>
+#include <cstdlib>
 #include <fstream>
+#include <iterator>
 #include <optional>
-#include <sstream>
 #include <stdexcept>
 #include <sys/stat.h>
 
 namespace mrt::graph {
 
-using json::Array;
-using json::Object;
 using json::Value;
 
 namespace {
 
-bool file_exists(const std::string& p) {
+// Error messages of one engine call, and whether there were any.
+struct Log {
+    std::vector<std::string>* sink = nullptr;
+    bool ok = true;
+    void error(const std::string& message) {
+        ok = false;
+        if (sink) sink->push_back(message);
+    }
+};
+
+// ------------------------------------------------------------------------------------------
+// 1. files
+
+bool is_regular_file(const std::string& path) {
     struct stat st;
-    return ::stat(p.c_str(), &st) == 0 && S_ISREG(st.st_mode);
+    return ::stat(path.c_str(), &st) == 0 && S_ISREG(st.st_mode);
 }
 
-std::string parent_dir(const std::string& p) {
-    const size_t k = p.find_last_of('/');
-    return k == std::string::npos ? std::string(".") : (k == 0 ? std::string("/") : p.substr(0, k));
-}
-
-std::optional<std::string> resolve_file(const std::string& file, const std::vector<std::string>& dirs) {
-    if (file_exists(file)) return file;
-    if (!file.empty() && file[0] == '/') return std::nullopt;
-    for (const std::string& d : dirs) {
-        const std::string cand = d.empty() ? file : d + "/" + file;
-        if (file_exists(cand)) return cand;
+// A name is tried as given, then (if relative) inside each search directory in order.
+std::optional<std::string> locate(const std::string& name, const std::vector<std::string>& dirs) {
+    if (is_regular_file(name)) return name;
+    if (name.empty() || name.front() == '/') return std::nullopt;
+    for (const std::string& dir : dirs) {
+        if (dir.empty()) continue;
+        std::string candidate = dir + "/" + name;
+        if (is_regular_file(candidate)) return candidate;
     }
     return std::nullopt;
 }
 
-bool read_json_file(const std::string& path, Value& out) {
-    std::ifstream in(path);
-    if (!in) return false;
-    std::stringstream ss;
-    ss << in.rdbuf();
-    return json::try_parse(ss.str(), out);
+std::string directory_of(const std::string& path) {
+    const size_t slash = path.rfind('/');
+    if (slash == std::string::npos) return ".";
+    return slash == 0 ? std::string("/") : path.substr(0, slash);
 }
 
-void note(std::vector<std::string>* errors, const std::string& msg) {
-    if (errors) errors->push_back(msg);
+std::optional<Value> load_json(const std::string& path) {
+    std::ifstream in(path, std::ios::binary);
+    if (!in) return std::nullopt;
+    const std::string text((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+    Value v;
+    if (!json::try_parse(text, v)) return std::nullopt;
+    return v;
 }
 
-void validate_identifier(const std::string& id, const char* what) {
-    if (id.find('.') != std::string::npos || id.find("->") != std::string::npos || id.find('/') != std::string::npos)
-        throw std::invalid_argument(std::string(what) + " '" + id + "' contains a reserved character ('.', '->' or '/')");
-}
-
-// "output->dst.input": output has no '-', dst has no '.'
-bool split_arrow(const std::string& s, std::string& out, std::string& dst, std::string& in) {
-    const size_t a = s.find("->");
-    if (a == std::string::npos || a == 0) return false;
-    out = s.substr(0, a);
-    if (out.find('-') != std::string::npos) return false;
-    const std::string rest = s.substr(a + 2);
-    const size_t d = rest.find('.');
-    if (d == std::string::npos || d == 0 || d + 1 >= rest.size()) return false;
-    dst = rest.substr(0, d);
-    in = rest.substr(d + 1);
-    return true;
-}
-
-bool split_dot(const std::string& s, std::string& dst, std::string& in) {
-    const size_t d = s.find('.');
-    if (d == std::string::npos || d == 0 || d + 1 >= s.size()) return false;
-    dst = s.substr(0, d);
-    in = s.substr(d + 1);
-    return true;
+// "include" and "merge" accept one file name or a list of them.
+std::vector<std::string> file_names(const Value& v) {
+    std::vector<std::string> names;
+    if (v.is_string()) names.push_back(v.as_string());
+    if (v.is_array())
+        for (const Value& item : v.as_array())
+            if (item.is_string()) names.push_back(item.as_string());
+    return names;
 }
 
 // ------------------------------------------------------------------------------------------
-// command-line overrides
+// 2. schema
 
-struct Patch {
-    std::vector<std::pair<std::string, Value>> set; // JSON pointer -> value
-    std::vector<std::string> merges;                // fragment files, merged before the sets
-};
-
-Patch parse_patch(const Value& spec) {
-    Patch p;
-    const Value set = spec.value("set", Value::object());
-    if (set.is_object())
-        for (const auto& [ptr, v] : set.as_object()) p.set.emplace_back(ptr, v);
-    if (const Value* m = spec.find("merge")) {
-        if (m->is_array()) {
-            for (const Value& f : m->as_array()) p.merges.push_back(f.as_string());
-        } else {
-            p.merges.push_back(m->as_string());
-        }
-    }
-    return p;
+// Node ids and connector names are joined with '.', '->' and '/' in connection strings and JSON
+// pointers, so none of them may appear inside a name.
+void require_plain_name(const std::string& name, const char* what) {
+    for (const char* reserved : {".", "->", "/"})
+        if (name.find(reserved) != std::string::npos)
+            throw std::invalid_argument(std::string(what) + " \"" + name + "\" may not contain '.', '->' or '/'");
 }
 
-enum class Kind { Value_, Flag, Variant };
+struct Endpoint {
+    std::string node, input;
+};
 
-struct Override {
-    std::string name;
-    Kind kind = Kind::Value_;
+// "node.input", split at the first '.'
+std::optional<Endpoint> parse_endpoint(const std::string& text) {
+    const size_t dot = text.find('.');
+    if (dot == std::string::npos || dot == 0 || dot + 1 == text.size()) return std::nullopt;
+    return Endpoint{text.substr(0, dot), text.substr(dot + 1)};
+}
+
+// "output->node.input"; an output name has no '-'
+std::optional<std::pair<std::string, Endpoint>> parse_link(const std::string& text) {
+    const size_t arrow = text.find("->");
+    if (arrow == std::string::npos || arrow == 0) return std::nullopt;
+    std::string output = text.substr(0, arrow);
+    if (output.find('-') != std::string::npos) return std::nullopt;
+    std::optional<Endpoint> to = parse_endpoint(text.substr(arrow + 2));
+    if (!to) return std::nullopt;
+    return std::make_pair(std::move(output), std::move(*to));
+}
+
+const std::string& string_field(const Value& obj, const char* key, const std::string& where) {
+    const Value* f = obj.find(key);
+    if (!f || !f->is_string())
+        throw std::runtime_error(where + ": \"" + key + "\" must be a string");
+    return f->as_string();
+}
+
+void read_sections(const Value& v, GraphDescription& d) {
+    if (const Value* gp = v.find("graph_properties")) d.graph_properties = *gp;
+    if (const Value* pp = v.find("profiler")) d.profiler_properties = *pp;
+}
+
+// Schema v1 and v2: nodes keyed by id with a "disable" flag, connections listed separately.
+void read_v1(const Value& v, GraphDescription& d) {
+    read_sections(v, d);
+    if (const Value* nodes = v.find("nodes"); nodes && nodes->is_object()) {
+        for (const auto& [id, n] : nodes->as_object()) {
+            d.add_node(string_field(n, "type", "node \"" + id + "\""), id, n.value("properties", Value()));
+            d.nodes[id].enabled = !n.value("disable", false).as_bool();
+        }
+    }
+    if (const Value* links = v.find("connections"); links && links->is_array()) {
+        for (const Value& c : links->as_array()) {
+            const std::string where = "connection";
+            d.add_connection(string_field(c, "src", where), string_field(c, "dst", where),
+                             string_field(c, "src_output", where), string_field(c, "dst_input", where));
+        }
+    }
+}
+
+// Schema v4: nodes keyed by id, each listing its outgoing connections.
+void read_v4(const Value& v, GraphDescription& d) {
+    read_sections(v, d);
+    if (const Value* c = v.find("cli")) d.cli = *c;
+    const Value* nodes = v.find("nodes");
+    if (!nodes || !nodes->is_object()) return;
+    for (const auto& [id, n] : nodes->as_object()) {
+        const Value* type = n.find("type");
+        if (!type || !type->is_string())
+            throw std::runtime_error("node \"" + id + "\" has no \"type\" (a cli pointer into an id that names no " +
+                                     "node creates an entry like this)");
+        d.add_node(type->as_string(), id, n.value("properties", Value()));
+        NodeEntry& e = d.nodes[id];
+        e.enabled = n.value("enabled", true).as_bool();
+        if (const Value* lo = n.find("linearization order")) e.linearization_order = uint32_t(lo->as_int());
+        if (const Value* md = n.find("metadata")) e.metadata = *md;
+        const Value* outs = n.find("outputs");
+        if (!outs) continue;
+        if (outs->is_array()) {
+            for (const Value& link : outs->as_array()) {
+                const std::string text = link.is_string() ? link.as_string() : json::dump(link);
+                const auto parsed = parse_link(text);
+                if (!parsed)
+                    throw std::runtime_error("node \"" + id + "\": output \"" + text +
+                                             "\" is not of the form output->node.input");
+                d.add_connection(id, parsed->second.node, parsed->first, parsed->second.input);
+            }
+        } else if (outs->is_object()) {
+            for (const auto& [output, targets] : outs->as_object()) {
+                if (!targets.is_array()) throw std::runtime_error("node \"" + id + "\": targets of \"" + output +
+                                                                  "\" must be a list");
+                for (const Value& target : targets.as_array()) {
+                    const std::string text = target.is_string() ? target.as_string() : json::dump(target);
+                    const auto to = parse_endpoint(text);
+                    if (!to)
+                        throw std::runtime_error("node \"" + id + "\": target \"" + text + "\" of output \"" + output +
+                                                 "\" is not of the form node.input");
+                    d.add_connection(id, to->node, output, to->input);
+                }
+            }
+        }
+    }
+}
+
+// Schema v3 lists nodes as an array of objects with an "id"; v4 keys them by it.
+Value v3_as_v4(const Value& v) {
+    Value out = v;
+    const Value* nodes = v.find("nodes");
+    if (!nodes || !nodes->is_array()) return out;
+    Value keyed = Value::object();
+    for (const Value& n : nodes->as_array()) {
+        Value entry = n;
+        const std::string id = string_field(n, "id", "v3 node");
+        entry.erase("id");
+        keyed[id] = std::move(entry);
+    }
+    out["nodes"] = std::move(keyed);
+    return out;
+}
+
+// ------------------------------------------------------------------------------------------
+// 3. includes
+
+// Depth-first expansion: the included files (each expanded first, relative to its own
+// directory) are merged in order, then the including config on top. `open` holds the canonical
+// paths of the files being expanded; meeting one of them again is a cycle.
+void expand_includes(Value& config, const std::vector<std::string>& dirs, std::set<std::string>& open, Log& log) {
+    const Value* include = config.find("include");
+    if (!include) return;
+    Value merged = Value::object();
+    for (const std::string& name : file_names(*include)) {
+        const std::optional<std::string> path = locate(name, dirs);
+        if (!path) {
+            log.error("include \"" + name + "\": file not found in the search path");
+            continue;
+        }
+        char* real = ::realpath(path->c_str(), nullptr);
+        const std::string key = real ? std::string(real) : *path;
+        std::free(real);
+        if (open.count(key)) {
+            log.error("include \"" + *path + "\" includes itself (cycle)");
+            continue;
+        }
+        std::optional<Value> part = load_json(*path);
+        if (!part) {
+            log.error("include \"" + *path + "\": not a JSON file");
+            continue;
+        }
+        open.insert(key);
+        std::vector<std::string> inner{directory_of(*path)};
+        inner.insert(inner.end(), dirs.begin(), dirs.end());
+        expand_includes(*part, inner, open, log);
+        open.erase(key);
+        GraphDescription::merge_into(merged, *part);
+    }
+    GraphDescription::merge_into(merged, config);
+    config = std::move(merged);
+}
+
+// ------------------------------------------------------------------------------------------
+// 4. command-line overrides
+
+// What applying an override does: merge the fragment files in order, then assign values at
+// JSON pointers (so a set always wins over a fragment of the same action).
+struct Action {
+    std::vector<std::string> fragments;
+    std::vector<std::pair<std::string, Value>> assignments;
+
+    // {"merge": file or [files], "set": {pointer: value, ...}}
+    static Action from_spec(const Value& spec) {
+        Action a;
+        if (!spec.is_object()) return a;
+        if (const Value* m = spec.find("merge")) a.fragments = file_names(*m);
+        if (const Value* s = spec.find("set"); s && s->is_object())
+            for (const auto& [pointer, value] : s->as_object()) a.assignments.emplace_back(pointer, value);
+        return a;
+    }
+    size_t size() const { return fragments.size() + assignments.size(); }
+};
+
+// One entry of the "cli" block.
+struct Declaration {
+    enum class Form { Value, Flag, Variant };
+    Form form = Form::Value;
+    std::string name;         // given as --name; "--" receives the positional arguments
     bool required = false;
-    std::string pointer;
-    Patch extra, on, off;
-    std::map<std::string, Patch> variants;
-    std::optional<std::string> default_variant, selected;
+    std::string pointer;      // Value: where the argument is stored
+    Action with;              // Value: applied with the argument; Flag: applied for "on"
+    Action off;               // Flag: applied for "off"
+    std::map<std::string, Action> variants;
+    std::optional<std::string> fallback; // Variant: applied when the command line does not choose
+    std::optional<std::string> selected; // Variant: a persisted choice, not applied again
 };
 
-Kind classify(const Value& spec) {
-    if (const Value* t = spec.find("type")) {
-        const std::string& s = t->as_string();
-        if (s == "variant") return Kind::Variant;
-        if (s == "flag") return Kind::Flag;
-        return Kind::Value_;
+Declaration::Form form_of(const Value& spec) {
+    if (const Value* t = spec.find("type"); t && t->is_string()) {
+        if (t->as_string() == "variant") return Declaration::Form::Variant;
+        if (t->as_string() == "flag") return Declaration::Form::Flag;
+        return Declaration::Form::Value;
     }
-    if (spec.contains("variants")) return Kind::Variant;
-    return spec.contains("pointer") ? Kind::Value_ : Kind::Flag;
+    if (spec.contains("variants")) return Declaration::Form::Variant;
+    return spec.contains("pointer") ? Declaration::Form::Value : Declaration::Form::Flag;
 }
 
-std::vector<Override> parse_overrides(const Value& config) {
-    std::vector<Override> out;
-    const Value cli = config.value("cli", Value::object());
-    if (!cli.is_object()) return out;
-    for (const auto& [name, spec] : cli.as_object()) {
-        if (spec.is_null()) continue; // removed by a merged fragment
-        Override o;
-        o.name = name;
-        o.kind = classify(spec);
-        o.required = spec.value("required", false).as_bool();
-        switch (o.kind) {
-        case Kind::Value_:
-            o.pointer = spec.find("pointer")->as_string();
-            o.extra = parse_patch(spec);
-            break;
-        case Kind::Flag:
-            o.on = parse_patch(spec);
-            if (const Value* off = spec.find("off")) o.off = parse_patch(*off);
-            break;
-        case Kind::Variant:
-            if (const Value* vs = spec.find("variants"))
-                for (const auto& [k, p] : vs->as_object()) o.variants.emplace(k, parse_patch(p));
-            if (const Value* d = spec.find("default")) o.default_variant = d->as_string();
-            if (const Value* s = spec.find("selected")) o.selected = s->as_string();
-            break;
+// The declarations of a config. Entries set to null (removed by a merged fragment) are skipped;
+// malformed ones are reported and skipped.
+std::map<std::string, Declaration> declarations_of(const Value& config, Log* log) {
+    std::map<std::string, Declaration> out;
+    const Value* cli = config.find("cli");
+    if (!cli || !cli->is_object()) return out;
+    for (const auto& [name, spec] : cli->as_object()) {
+        if (spec.is_null()) continue;
+        if (!spec.is_object()) {
+            if (log) log->error("cli entry \"" + name + "\" is not an object");
+            continue;
         }
-        out.push_back(std::move(o));
+        Declaration d;
+        d.name = name;
+        d.form = form_of(spec);
+        d.required = spec.value("required", false).as_bool();
+        if (d.form == Declaration::Form::Value) {
+            const Value* p = spec.find("pointer");
+            if (!p || !p->is_string()) {
+                if (log) log->error("cli entry \"" + name + "\" has no \"pointer\"");
+                continue;
+            }
+            d.pointer = p->as_string();
+            d.with = Action::from_spec(spec);
+        } else if (d.form == Declaration::Form::Flag) {
+            d.with = Action::from_spec(spec);
+            if (const Value* off = spec.find("off")) d.off = Action::from_spec(*off);
+        } else {
+            if (const Value* vs = spec.find("variants"); vs && vs->is_object())
+                for (const auto& [variant, action] : vs->as_object()) d.variants.emplace(variant, Action::from_spec(action));
+            if (const Value* f = spec.find("default"); f && f->is_string()) d.fallback = f->as_string();
+            if (const Value* s = spec.find("selected"); s && s->is_string()) d.selected = s->as_string();
+        }
+        out.emplace(name, std::move(d));
     }
     return out;
 }
 
-std::optional<bool> parse_bool(std::string t) {
-    for (char& c : t) c = char(std::tolower(static_cast<unsigned char>(c)));
-    if (t.empty() || t == "on" || t == "true" || t == "1" || t == "yes") return true;
-    if (t == "off" || t == "false" || t == "0" || t == "no") return false;
-    return std::nullopt;
-}
-
-// A command-line value keeps the type of what it replaces: text where the target is a
-// string, JSON where it parses, text otherwise.
-Value typed_value(const Value& config, const std::string& ptr, const std::string& text) {
-    const Value* cur = json::pointer_find(config, ptr);
-    if (cur && cur->is_string()) return Value(text);
-    Value parsed;
-    return json::try_parse(text, parsed) ? parsed : Value(text);
-}
-
-std::string variant_names(const Override& o, const char* sep) {
-    std::string s;
-    for (const auto& [k, _] : o.variants) s += (s.empty() ? "" : sep) + k;
+std::string lowercase(std::string s) {
+    for (char& c : s) c = char(std::tolower(static_cast<unsigned char>(c)));
     return s;
 }
 
-std::optional<Patch> resolve(const Override& o, const std::string& value, const Value& config,
-                             std::vector<std::string>* errors) {
-    switch (o.kind) {
-    case Kind::Value_: {
-        Patch p = o.extra;
-        p.set.insert(p.set.begin(), {o.pointer, typed_value(config, o.pointer, value)});
-        return p;
-    }
-    case Kind::Flag:
-        if (const auto b = parse_bool(value)) return *b ? o.on : o.off;
-        note(errors, "--" + o.name + " expects on|off, got '" + value + "'");
-        return std::nullopt;
-    case Kind::Variant: {
-        auto it = o.variants.find(value);
-        if (it != o.variants.end()) return it->second;
-        const bool is_json = value.size() >= 5 && value.compare(value.size() - 5, 5, ".json") == 0;
-        if (is_json || value.find('/') != std::string::npos) {
-            Patch p;
-            p.merges.push_back(value); // a custom fragment file
-            return p;
-        }
-        note(errors, "--" + o.name + ": unknown '" + value + "', expected one of " + variant_names(o, ", ") +
-                         " or a .json fragment");
-        return std::nullopt;
-    }
-    }
+// State named by a flag's argument.
+std::optional<bool> switch_state(const std::string& text) {
+    const std::string t = lowercase(text);
+    for (const char* on : {"", "on", "true", "yes", "1"})
+        if (t == on) return true;
+    for (const char* off : {"off", "false", "no", "0"})
+        if (t == off) return false;
     return std::nullopt;
 }
 
-bool apply_patch(Value& config, const Patch& p, const std::vector<std::string>& dirs, std::vector<std::string>* errors) {
-    for (const std::string& f : p.merges) {
-        const auto path = resolve_file(f, dirs);
-        if (!path) {
-            note(errors, "merge file '" + f + "' not found");
-            return false;
-        }
-        Value frag;
-        if (!read_json_file(*path, frag)) {
-            note(errors, "merge file '" + *path + "' is not valid JSON");
-            return false;
-        }
-        GraphDescription::merge_into(config, frag);
-    }
-    for (const auto& [ptr, v] : p.set) json::pointer_ref(config, ptr) = v;
-    return true;
+// A command-line value becomes JSON (a number stays a number), unless what it replaces is a
+// string: file names such as "123" stay strings. Text that is not JSON stays text.
+Value argument_value(const Value* current, const std::string& text) {
+    if (current && current->is_string()) return Value(text);
+    Value parsed;
+    if (json::try_parse(text, parsed)) return parsed;
+    return Value(text);
 }
 
-// Applies variant defaults that are neither selected already nor suppressed by an explicit
-// argument, re-reading the overrides after each (a merged fragment may declare more).
-bool settle_variants(Value& config, const std::set<std::string>& suppressed, const std::vector<std::string>& dirs,
-                     std::vector<std::string>* errors) {
-    bool ok = true;
-    std::set<std::string> settled;
-    while (true) {
-        const std::vector<Override> ovs = parse_overrides(config);
-        const Override* pending = nullptr;
-        for (const Override& o : ovs) {
-            if (o.kind == Kind::Variant && (o.default_variant || o.selected) && !settled.count(o.name) &&
-                !suppressed.count(o.name)) {
-                pending = &o;
-                break;
+bool names_fragment_file(const std::string& value) {
+    const std::string ext = ".json";
+    const bool json_ext = value.size() >= ext.size() && value.compare(value.size() - ext.size(), ext.size(), ext) == 0;
+    return json_ext || value.find('/') != std::string::npos;
+}
+
+std::string joined_names(const std::map<std::string, Action>& variants, const char* separator) {
+    std::string s;
+    for (const auto& kv : variants) {
+        if (!s.empty()) s += separator;
+        s += kv.first;
+    }
+    return s;
+}
+
+bool is_option(const std::string& arg) { return arg.size() > 2 && arg[0] == '-' && arg[1] == '-'; }
+
+// One application of a command line to a config.
+class OverrideRun {
+  public:
+    OverrideRun(Value& config, const std::vector<std::string>& args, const std::vector<std::string>& dirs, Log& log)
+        : cfg_(config), args_(args), dirs_(dirs), log_(log), used_(args.size(), false) {
+        for (const std::string& a : args_)
+            if (is_option(a)) named_.insert(a.substr(2)); // an explicit choice suppresses the default
+    }
+
+    void run() {
+        apply_defaults();
+        if (!take_options()) return;
+        take_positionals();
+        check_required();
+        prune_removed();
+    }
+
+    // Applies the defaults of the variants that neither have a selection nor are named on the
+    // command line, re-reading the declarations after each (a fragment may declare more).
+    void apply_defaults() {
+        while (true) {
+            const auto decls = declarations_of(cfg_, nullptr);
+            const Declaration* next = nullptr;
+            for (const auto& kv : decls) {
+                const Declaration& d = kv.second;
+                if (d.form == Declaration::Form::Variant && d.fallback && !d.selected && !named_.count(d.name) &&
+                    !failed_defaults_.count(d.name)) {
+                    next = &d;
+                    break;
+                }
             }
+            if (!next) return;
+            const std::string name = next->name, choice = *next->fallback;
+            const auto v = next->variants.find(choice);
+            if (v == next->variants.end()) {
+                log_.error("--" + name + ": its default \"" + choice + "\" names no declared variant");
+                failed_defaults_.insert(name);
+                continue;
+            }
+            const Action action = v->second;
+            perform(action);
+            cfg_["cli"][name]["selected"] = choice;
         }
-        if (!pending) return ok;
-        settled.insert(pending->name);
-        if (pending->selected) continue;
-        auto v = pending->variants.find(*pending->default_variant);
-        if (v == pending->variants.end()) {
-            note(errors, "--" + pending->name + ": default '" + *pending->default_variant + "' is not declared");
-            ok = false;
-            continue;
-        }
-        const std::string name = pending->name, def = *pending->default_variant;
-        ok = apply_patch(config, v->second, dirs, errors) && ok;
-        config["cli"][name]["selected"] = def;
     }
-}
 
-std::vector<std::string> file_list(const Value& v) {
-    std::vector<std::string> out;
-    if (v.is_array()) {
-        for (const Value& f : v.as_array()) out.push_back(f.as_string());
-    } else {
-        out.push_back(v.as_string());
+  private:
+    Value& cfg_;
+    const std::vector<std::string>& args_;
+    const std::vector<std::string>& dirs_;
+    Log& log_;
+    std::vector<bool> used_;
+    std::set<std::string> named_;           // every --name on the command line
+    std::set<std::string> given_;           // overrides that received a value
+    std::set<std::string> failed_defaults_; // defaults reported as undeclared
+
+    bool perform(const Action& a) {
+        for (const std::string& name : a.fragments) {
+            const std::optional<std::string> path = locate(name, dirs_);
+            if (!path) {
+                log_.error("fragment \"" + name + "\": file not found in the search path");
+                return false;
+            }
+            const std::optional<Value> fragment = load_json(*path);
+            if (!fragment) {
+                log_.error("fragment \"" + *path + "\": not a JSON file");
+                return false;
+            }
+            GraphDescription::merge_into(cfg_, *fragment);
+        }
+        for (const auto& [pointer, value] : a.assignments) json::pointer_ref(cfg_, pointer) = value;
+        return true;
     }
-    return out;
-}
 
-bool includes_rec(Value& config, const std::vector<std::string>& dirs, std::set<std::string>& active,
-                  std::vector<std::string>* errors) {
-    const Value* inc = config.find("include");
-    if (!inc) return true;
-    bool ok = true;
-    Value base = Value::object();
-    for (const std::string& f : file_list(*inc)) {
-        const auto path = resolve_file(f, dirs);
-        if (!path) {
-            note(errors, "include file '" + f + "' not found");
-            ok = false;
-            continue;
+    std::optional<Action> action_for(const Declaration& d, const std::string& value) {
+        switch (d.form) {
+        case Declaration::Form::Value: {
+            Action a = d.with;
+            a.assignments.insert(a.assignments.begin(),
+                                 {d.pointer, argument_value(json::pointer_find(cfg_, d.pointer), value)});
+            return a;
         }
-        char* rp = ::realpath(path->c_str(), nullptr);
-        const std::string key = rp ? std::string(rp) : *path;
-        std::free(rp);
-        if (!active.insert(key).second) {
-            note(errors, "include cycle at '" + *path + "'");
-            ok = false;
-            continue;
+        case Declaration::Form::Flag: {
+            const std::optional<bool> on = switch_state(value);
+            if (!on) {
+                log_.error("--" + d.name + " takes on or off, not \"" + value + "\"");
+                return std::nullopt;
+            }
+            return *on ? d.with : d.off;
         }
-        Value included;
-        if (!read_json_file(*path, included)) {
-            note(errors, "include file '" + *path + "' is not valid JSON");
-            ok = false;
+        case Declaration::Form::Variant: {
+            const auto v = d.variants.find(value);
+            if (v != d.variants.end()) return v->second;
+            if (names_fragment_file(value)) return Action{{value}, {}};
+            log_.error("--" + d.name + " " + value + ": neither a declared variant (" +
+                       joined_names(d.variants, ", ") + ") nor a .json fragment file");
+            return std::nullopt;
+        }
+        }
+        return std::nullopt;
+    }
+
+    // The first unused "--name" whose name is declared now (or is the built-in "merge").
+    std::optional<size_t> next_option(const std::map<std::string, Declaration>& decls) const {
+        for (size_t i = 0; i < args_.size(); i++) {
+            if (used_[i] || !is_option(args_[i])) continue;
+            const std::string name = args_[i].substr(2);
+            if (name == "merge" || decls.count(name)) return i;
+        }
+        return std::nullopt;
+    }
+
+    // Applies "--name value" pairs until none is applicable. Returns false if a name lacks its
+    // value (the run stops there).
+    bool take_options() {
+        while (true) {
+            const auto decls = declarations_of(cfg_, &log_);
+            const std::optional<size_t> at = next_option(decls);
+            if (!at) return true;
+            const size_t i = *at;
+            if (i + 1 >= args_.size() || used_[i + 1]) {
+                log_.error("missing value after " + args_[i]);
+                return false;
+            }
+            used_[i] = used_[i + 1] = true;
+            const std::string name = args_[i].substr(2), value = args_[i + 1];
+            given_.insert(name);
+            if (name == "merge") {
+                perform(Action{{value}, {}});
+            } else {
+                const Declaration& d = decls.at(name);
+                const std::optional<Action> a = action_for(d, value);
+                if (a && perform(*a) && d.form == Declaration::Form::Variant) cfg_["cli"][name]["selected"] = value;
+            }
+            apply_defaults();
+        }
+    }
+
+    // Tokens no option used are positional: one is stored as it is, several are quoted and
+    // joined with spaces, into the "--" override.
+    void take_positionals() {
+        std::vector<std::string> rest;
+        for (size_t i = 0; i < args_.size(); i++)
+            if (!used_[i]) rest.push_back(args_[i]);
+        if (rest.empty()) return;
+        std::string text;
+        if (rest.size() == 1) {
+            text = rest.front();
         } else {
-            std::vector<std::string> nested{parent_dir(*path)};
-            nested.insert(nested.end(), dirs.begin(), dirs.end());
-            ok = includes_rec(included, nested, active, errors) && ok;
-            GraphDescription::merge_into(base, included);
+            for (const std::string& a : rest) text += (text.empty() ? "" : " ") + quote_arg(a);
         }
-        active.erase(key);
+        given_.insert("--");
+        const auto decls = declarations_of(cfg_, nullptr);
+        const auto it = decls.find("--");
+        if (it == decls.end()) {
+            log_.error("positional arguments (" + text + ") but the graph declares no \"--\" entry to receive them");
+            return;
+        }
+        if (const std::optional<Action> a = action_for(it->second, text)) perform(*a);
     }
-    GraphDescription::merge_into(base, config);
-    config = std::move(base);
-    return ok;
-}
+
+    void check_required() {
+        for (const auto& [name, d] : declarations_of(cfg_, nullptr)) {
+            if (!d.required || given_.count(name) || d.fallback || d.selected) continue;
+            if (d.form == Declaration::Form::Value) {
+                // a value the config already holds (a stored session) satisfies the requirement
+                const Value* v = json::pointer_find(cfg_, d.pointer);
+                if (v && (v->is_string() ? !v->as_string().empty() : !v->is_null())) continue;
+            }
+            log_.error(name == "--" ? std::string("the graph requires positional arguments")
+                                    : "the graph requires --" + name);
+        }
+    }
+
+    void prune_removed() {
+        Value* cli = const_cast<Value*>(cfg_.find("cli"));
+        if (!cli || !cli->is_object()) return;
+        json::Object& entries = cli->as_object();
+        for (auto it = entries.begin(); it != entries.end();) it = it->second.is_null() ? entries.erase(it) : ++it;
+    }
+};
 
 } // namespace
 
 std::string quote_arg(const std::string& value) {
     if (!value.empty() && value.find_first_of(" \t\n\v\f\r\"") == std::string::npos) return value;
-    std::string q = "\"";
+    std::string out = "\"";
     for (const char c : value) {
-        if (c == '"') q += '"';
-        q += c;
+        out.push_back(c);
+        if (c == '"') out.push_back('"');
     }
-    return q + "\"";
+    out.push_back('"');
+    return out;
 }
 
 // ------------------------------------------------------------------------------------------
 // description
 
 void GraphDescription::add_node(const std::string& type, const std::string& id, const Value& properties) {
-    validate_identifier(id, "node identifier");
-    if (nodes.count(id)) throw std::runtime_error("node identifier '" + id + "' is not unique");
+    require_plain_name(id, "node id");
+    if (nodes.count(id)) throw std::runtime_error("node id \"" + id + "\" appears twice");
     NodeEntry e;
     e.type = type;
     e.properties = properties.is_null() ? Value::object() : properties;
@@ -327,108 +575,34 @@ void GraphDescription::add_node(const std::string& type, const std::string& id, 
 
 void GraphDescription::add_connection(const std::string& src, const std::string& dst, const std::string& output,
                                       const std::string& input) {
-    auto it = nodes.find(src);
-    if (it == nodes.end()) throw std::runtime_error("connection from unknown node '" + src + "'");
-    validate_identifier(output, "output name");
-    validate_identifier(input, "input name");
+    const auto it = nodes.find(src);
+    if (it == nodes.end()) throw std::runtime_error("connection from \"" + src + "\", which is not a node");
+    require_plain_name(output, "output");
+    require_plain_name(input, "input");
     it->second.outputs[output][dst].insert(input);
-}
-
-void GraphDescription::parse_v1(const Value& v, GraphDescription& d) {
-    if (const Value* gp = v.find("graph_properties")) d.graph_properties = *gp;
-    if (const Value* pp = v.find("profiler")) d.profiler_properties = *pp;
-    if (const Value* ns = v.find("nodes")) {
-        for (const auto& [id, n] : ns->as_object()) {
-            d.add_node(n.find("type")->as_string(), id, n.value("properties", Value()));
-            d.nodes[id].enabled = !n.value("disable", false).as_bool();
-        }
-    }
-    if (const Value* cs = v.find("connections")) {
-        for (const Value& c : cs->as_array())
-            d.add_connection(c.find("src")->as_string(), c.find("dst")->as_string(), c.find("src_output")->as_string(),
-                             c.find("dst_input")->as_string());
-    }
-}
-
-void GraphDescription::parse_v4(const Value& v, GraphDescription& d) {
-    if (const Value* gp = v.find("graph_properties")) d.graph_properties = *gp;
-    if (const Value* pp = v.find("profiler")) d.profiler_properties = *pp;
-    if (const Value* c = v.find("cli")) d.cli = *c;
-    const Value* ns = v.find("nodes");
-    if (!ns) return;
-    for (const auto& [id, n] : ns->as_object()) {
-        const Value* t = n.find("type");
-        if (!t)
-            throw std::runtime_error("node '" + id +
-                                     "' has no type (a cli pointer to a nonexistent node id creates such an entry)");
-        d.add_node(t->as_string(), id, n.value("properties", Value()));
-        NodeEntry& e = d.nodes[id];
-        e.enabled = n.value("enabled", true).as_bool();
-        if (const Value* lo = n.find("linearization order")) e.linearization_order = uint32_t(lo->as_int());
-        if (const Value* md = n.find("metadata")) e.metadata = *md;
-    }
-    for (const auto& [id, n] : ns->as_object()) {
-        const Value* outs = n.find("outputs");
-        if (!outs) continue;
-        if (outs->is_object()) {
-            for (const auto& [out, targets] : outs->as_object())
-                for (const Value& t : targets.as_array()) {
-                    std::string dst, in;
-                    if (!split_dot(t.as_string(), dst, in))
-                        throw std::runtime_error("invalid connection target '" + t.as_string() + "' for output '" + out +
-                                                 "' in node '" + id + "': expected 'dst.input'");
-                    d.add_connection(id, dst, out, in);
-                }
-        } else if (outs->is_array()) {
-            for (const Value& c : outs->as_array()) {
-                std::string out, dst, in;
-                if (!split_arrow(c.as_string(), out, dst, in))
-                    throw std::runtime_error("invalid connection '" + c.as_string() + "' in node '" + id +
-                                             "': expected 'output->dst.input'");
-                d.add_connection(id, dst, out, in);
-            }
-        }
-    }
 }
 
 GraphDescription GraphDescription::from_json(const Value& v) {
     GraphDescription d;
-    const Value* ver = v.find("version");
-    if (!ver) {
-        parse_v1(v, d);
-        return d;
-    }
-    const int64_t version = ver->as_int();
-    if (version == 2) {
-        parse_v1(v, d);
-    } else if (version == 3) {
-        Value v4 = v;
-        if (const Value* ns = v.find("nodes")) {
-            Value keyed = Value::object();
-            for (const Value& n : ns->as_array()) {
-                Value e = n;
-                const std::string id = e.find("id")->as_string();
-                e.erase("id");
-                keyed[id] = e;
-            }
-            v4["nodes"] = keyed;
-        }
-        parse_v4(v4, d);
-    } else if (version == 4) {
-        parse_v4(v, d);
-    } else {
-        throw std::runtime_error("schema version " + std::to_string(version) + " unsupported");
+    const Value* version = v.find("version");
+    const int64_t ver = version ? version->as_int() : 1;
+    switch (ver) {
+    case 1:
+    case 2: read_v1(v, d); break;
+    case 3: read_v4(v3_as_v4(v), d); break;
+    case 4: read_v4(v, d); break;
+    default: throw std::runtime_error("graph schema version " + std::to_string(ver) + " is not supported (1-4)");
     }
     return d;
 }
 
 Value GraphDescription::to_json() const {
-    Value j = Value::object();
-    j["version"] = 4;
-    if (!graph_properties.empty()) j["graph_properties"] = graph_properties;
-    if (!profiler_properties.empty()) j["profiler"] = profiler_properties;
-    if (!cli.empty()) j["cli"] = cli;
-    Value ns = Value::object();
+    Value out = Value::object();
+    out["version"] = 4;
+    if (!graph_properties.empty()) out["graph_properties"] = graph_properties;
+    if (!profiler_properties.empty()) out["profiler"] = profiler_properties;
+    if (!cli.empty()) out["cli"] = cli;
+    Value all = Value::object();
     for (const auto& [id, e] : nodes) {
         Value n = Value::object();
         n["type"] = e.type;
@@ -436,162 +610,81 @@ Value GraphDescription::to_json() const {
         if (e.linearization_order) n["linearization order"] = e.linearization_order;
         if (!e.properties.empty()) n["properties"] = e.properties;
         if (!e.metadata.empty()) n["metadata"] = e.metadata;
-        if (!e.outputs.empty()) {
-            Value outs = Value::array();
-            for (const auto& [out, dsts] : e.outputs)
-                for (const auto& [dst, ins] : dsts)
-                    for (const std::string& in : ins) outs.push_back(out + "->" + dst + "." + in);
-            n["outputs"] = outs;
-        }
-        ns[id] = n;
+        Value links = Value::array();
+        for (const auto& [output, targets] : e.outputs)
+            for (const auto& [dst, inputs] : targets)
+                for (const std::string& input : inputs) links.push_back(output + "->" + dst + "." + input);
+        if (!links.empty()) n["outputs"] = std::move(links);
+        all[id] = std::move(n);
     }
-    if (!ns.empty()) j["nodes"] = ns;
-    return j;
+    if (!all.empty()) out["nodes"] = std::move(all);
+    return out;
 }
 
 void GraphDescription::merge_into(Value& base, const Value& over) {
-    if (base.is_object() && over.is_object()) {
-        static const std::string append = "$+$";
-        for (const auto& [k, v] : over.as_object()) {
-            if (v.is_array() && k.compare(0, append.size(), append) == 0) {
-                Value& target = base[k.substr(append.size())];
-                if (!target.is_array()) target = Value::array();
-                for (const Value& e : v.as_array()) {
-                    const Array& ta = target.as_array();
-                    if (std::find(ta.begin(), ta.end(), e) == ta.end()) target.push_back(e);
-                }
-            } else {
-                merge_into(base[k], v);
-            }
-        }
-    } else {
+    if (!base.is_object() || !over.is_object()) {
         base = over;
+        return;
+    }
+    static const std::string append_prefix = "$+$";
+    for (const auto& [key, value] : over.as_object()) {
+        if (!value.is_array() || key.compare(0, append_prefix.size(), append_prefix) != 0) {
+            merge_into(base[key], value);
+            continue;
+        }
+        Value& list = base[key.substr(append_prefix.size())];
+        if (!list.is_array()) list = Value::array();
+        for (const Value& item : value.as_array()) {
+            const json::Array& present = list.as_array();
+            if (std::find(present.begin(), present.end(), item) == present.end()) list.push_back(item);
+        }
     }
 }
 
 bool GraphDescription::resolve_includes(Value& config, const std::vector<std::string>& dirs,
                                         std::vector<std::string>* errors) {
-    std::set<std::string> active;
-    return includes_rec(config, dirs, active, errors);
+    Log log{errors};
+    std::set<std::string> open;
+    expand_includes(config, dirs, open, log);
+    return log.ok;
 }
 
 bool GraphDescription::apply_cli(Value& config, const std::vector<std::string>& args,
                                  const std::vector<std::string>& dirs, std::vector<std::string>* errors) {
-    // an explicit --name suppresses that variant's default, even if its value then fails
-    std::set<std::string> raw;
-    for (const std::string& a : args)
-        if (a.size() > 2 && a.compare(0, 2, "--") == 0) raw.insert(a.substr(2));
-    bool ok = settle_variants(config, raw, dirs, errors);
-
-    // one recognised "--name value" at a time, re-reading the overrides after each, so that
-    // overrides declared by merged fragments are recognised in the same invocation
-    std::vector<bool> used(args.size(), false);
-    std::set<std::string> given;
-    while (true) {
-        const std::vector<Override> ovs = parse_overrides(config);
-        bool progressed = false;
-        for (size_t i = 0; i < args.size(); i++) {
-            const std::string& a = args[i];
-            if (used[i] || a.size() <= 2 || a.compare(0, 2, "--") != 0) continue;
-            const std::string name = a.substr(2);
-            const Override* o = nullptr;
-            for (const Override& c : ovs)
-                if (c.name == name) o = &c;
-            if (name != "merge" && !o) continue;
-            if (i + 1 >= args.size() || used[i + 1]) {
-                note(errors, "missing value for override '" + a + "'");
-                return false;
-            }
-            used[i] = used[i + 1] = true;
-            given.insert(name);
-            const std::string value = args[i + 1];
-            if (name == "merge") {
-                Patch p;
-                p.merges.push_back(value);
-                ok = apply_patch(config, p, dirs, errors) && ok;
-            } else {
-                const bool variant = o->kind == Kind::Variant;
-                const std::optional<Patch> p = resolve(*o, value, config, errors);
-                const bool done = p && apply_patch(config, *p, dirs, errors);
-                ok = done && ok;
-                if (done && variant) config["cli"][name]["selected"] = value;
-            }
-            ok = settle_variants(config, raw, dirs, errors) && ok;
-            progressed = true;
-            break;
-        }
-        if (!progressed) break;
-    }
-
-    // the remaining tokens are positional and feed the "--" override
-    std::vector<std::string> pos;
-    for (size_t i = 0; i < args.size(); i++)
-        if (!used[i]) pos.push_back(args[i]);
-    std::string joined;
-    if (pos.size() == 1) {
-        joined = pos[0];
-    } else {
-        for (const std::string& a : pos) joined += (joined.empty() ? "" : " ") + quote_arg(a);
-    }
-    const std::vector<Override> ovs = parse_overrides(config);
-    if (!pos.empty()) {
-        given.insert("--");
-        const Override* o = nullptr;
-        for (const Override& c : ovs)
-            if (c.name == "--") o = &c;
-        if (!o) {
-            note(errors, "positional arguments given but '--' is not declared in the cli block");
-            ok = false;
-        } else {
-            const std::optional<Patch> p = resolve(*o, joined, config, errors);
-            ok = p && apply_patch(config, *p, dirs, errors) && ok;
-        }
-    }
-    for (const Override& o : ovs) {
-        if (!o.required || given.count(o.name) || o.default_variant || o.selected) continue;
-        if (o.kind == Kind::Value_) {
-            // a value already materialised in the config (a stored session) satisfies it
-            const Value* v = json::pointer_find(config, o.pointer);
-            if (v && (v->is_string() ? !v->as_string().empty() : !v->is_null())) continue;
-        }
-        note(errors, o.name == "--" ? std::string("missing required positional args (see --help)")
-                                    : "missing required --" + o.name);
-        ok = false;
-    }
-    if (Value* cli = const_cast<Value*>(config.find("cli")); cli && cli->is_object()) {
-        Object& co = cli->as_object();
-        for (auto it = co.begin(); it != co.end();) it = it->second.is_null() ? co.erase(it) : std::next(it);
-    }
-    return ok;
+    Log log{errors};
+    OverrideRun(config, args, dirs, log).run();
+    return log.ok;
 }
 
 std::string GraphDescription::cli_help(const Value& config, const std::vector<std::string>& dirs) {
     Value settled = config;
-    settle_variants(settled, {}, dirs, nullptr);
-    const std::vector<Override> ovs = parse_overrides(settled);
-    if (ovs.empty()) return "this graph declares no cli overrides\n";
-    std::string out = "cli overrides:\n";
-    for (const Override& o : ovs) {
-        std::string usage, target;
-        switch (o.kind) {
-        case Kind::Value_:
-            usage = o.name == "--" ? "args... (after graph.json)" : "--" + o.name + " <value>";
-            target = o.pointer;
-            if (size_t n = o.extra.set.size() + o.extra.merges.size()) target += " (+" + std::to_string(n) + " more)";
+    Log quiet;
+    OverrideRun(settled, {}, dirs, quiet).apply_defaults();
+    const auto decls = declarations_of(settled, nullptr);
+    if (decls.empty()) return "this graph declares no options\n";
+    std::string text = "options of this graph:\n";
+    for (const auto& [name, d] : decls) {
+        std::string usage, effect;
+        switch (d.form) {
+        case Declaration::Form::Value:
+            usage = name == "--" ? "ARGS... (after the graph file)" : "--" + name + " VALUE";
+            effect = "sets " + d.pointer;
+            if (d.with.size()) effect += ", and applies " + std::to_string(d.with.size()) + " more change(s)";
             break;
-        case Kind::Flag:
-            usage = "--" + o.name + " <on|off>";
-            target = "on/off";
+        case Declaration::Form::Flag:
+            usage = "--" + name + " on|off";
+            effect = "switch";
             break;
-        case Kind::Variant:
-            usage = "--" + o.name + " <" + variant_names(o, "|") + "|file.json>";
-            target = o.selected ? "selected: " + *o.selected : (o.default_variant ? "default: " + *o.default_variant : "");
+        case Declaration::Form::Variant:
+            usage = "--" + name + " " + joined_names(d.variants, "|") + "|FILE.json";
+            effect = "variant";
+            if (d.selected) effect += ", now " + *d.selected;
+            else if (d.fallback) effect += ", by default " + *d.fallback;
             break;
         }
-        usage.resize(std::max<size_t>(usage.size(), 28), ' ');
-        out += "  " + usage + " -> " + target + (o.required ? " (required)" : "") + "\n";
+        text += "  " + usage + (d.required ? "   (required)" : "") + "\n      " + effect + "\n";
     }
-    return out;
+    return text;
 }
 
 } // namespace mrt::graph

@@ -1,12 +1,12 @@
 // Graph descriptions as merian stores them (schema v4; v1-v3 read), the fragment merge, include
 // resolution and the command-line override engine of the graph runner.
 //
-// Semantics follow merian's GraphDescription (src/merian-graph/graph/graph_description.cpp):
-// from_json/to_json (:119-238, v3 array -> v4 map :246-256), parse of connections (:296-323),
-// merge_into (:706-729, "$+$key" appends), resolve_includes (:659-705), apply_cli (:731-857)
-// with value / flag / variant overrides (:443-557), patches that merge before they set (:593-610),
-// variant defaults and persisted selections (:612-643). Errors are collected as messages
-// instead of logged.
+// The behaviour is merian's GraphDescription (src/merian-graph/graph/graph_description.cpp):
+// from_json/to_json (:119-238; the v3 node array becomes the v4 map, :246-256), connection
+// strings (:296-323), merge_into with "$+$key" appends (:706-729), resolve_includes (:659-705)
+// and apply_cli (:731-857): value, flag and variant overrides, fragment merges before pointer
+// sets, variant defaults and persisted selections, positional arguments for the "--" entry.
+// Errors are collected as messages instead of logged.
 #pragma once
 
 #include <map>
@@ -52,21 +52,19 @@ class GraphDescription {
     static bool resolve_includes(json::Value& config, const std::vector<std::string>& search_dirs,
                                  std::vector<std::string>* errors = nullptr);
 
-    // Applies the config's "cli" overrides to the arguments (see the file comment). Returns
-    // false on any error; config may be partially modified then.
+    // Applies the config's "cli" overrides to the arguments. Returns false on any error;
+    // config may be partially modified then.
     static bool apply_cli(json::Value& config, const std::vector<std::string>& args,
                           const std::vector<std::string>& search_dirs = {},
                           std::vector<std::string>* errors = nullptr);
 
+    // Usage text of the overrides a config declares (after its default variants are applied).
     static std::string cli_help(const json::Value& config, const std::vector<std::string>& search_dirs = {});
-
-  private:
-    static void parse_v1(const json::Value& v, GraphDescription& d);
-    static void parse_v4(const json::Value& v, GraphDescription& d);
 };
 
-// Quotes one argument for a space-joined positional string (merian's quote_arg,
-// include/merian/utils/string.hpp:114-126).
+// Quotes one argument for a space-joined positional string, as merian's quote_arg
+// (include/merian/utils/string.hpp:114-126): arguments without blanks or quotes stay as they
+// are; others are wrapped in double quotes with inner quotes doubled.
 std::string quote_arg(const std::string& value);
 
 } // namespace mrt::graph

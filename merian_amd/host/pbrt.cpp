@@ -264,7 +264,7 @@ struct Loader {
             std::memcpy(m.base_color, c, 12);
             m.specular_weight = 0.f;
             m.specular_alpha[0] = m.specular_alpha[1] = 1.f;
-            if (t != "diffuse") warn("mat_" + t, "material '" + t + "' approximated as diffuse");
+            if (t != "diffuse") warn("mat_" + t, "pbrt material \"" + t + "\": loaded as a Lambert surface with its reflectance");
         } else if (t == "coateddiffuse") {
             float c[3] = {0.5f, 0.5f, 0.5f};
             prgb(p, "reflectance", c);
@@ -287,7 +287,7 @@ struct Loader {
             }
             std::memcpy(m.base_color, c, 12);
             roughness(p, pre, m.specular_alpha);
-            if (t == "coatedconductor") warn("mat_cc", "coatedconductor interface layer is ignored");
+            if (t == "coatedconductor") warn("mat_cc", "coatedconductor: only the conductor base is loaded, its dielectric coat is dropped");
         } else if (t == "dielectric" || t == "thindielectric") {
             m.specular_ior = pf(p, "eta", 1.5f);
             m.features = MRT_PBR_TRANSMISSION;
@@ -295,7 +295,7 @@ struct Loader {
             for (int a = 0; a < 3; a++) m.transmission_color[a] = 1.f;
             roughness(p, "", m.specular_alpha);
             two_sided = true;
-            if (t == "thindielectric") warn("mat_thin", "thindielectric approximated as solid dielectric");
+            if (t == "thindielectric") warn("mat_thin", "thindielectric: loaded as a solid (refracting) dielectric");
         } else if (t == "mix") {
             auto it = p.find("materials");
             if (it != p.end() && it->second.strs.size() == 2) {

@@ -223,5 +223,6 @@ def test_string_targets_stay_strings():
 def test_cli_help():
     text = G.cli_help({"cli": {"spp": {"pointer": "/nodes/render/properties/samples per pixel"},
                                "scene": {"pointer": "/x", "required": True}, **VARIANT["cli"]}})
-    assert "--spp <value>" in text and "(required)" in text and "--quality <high|low|file.json>" in text
-    assert G.cli_help({}) == "this graph declares no cli overrides\n"
+    assert "--spp VALUE" in text and "(required)" in text and "--quality high|low|FILE.json" in text
+    assert "sets /nodes/render/properties/samples per pixel" in text and "variant, now high" in text
+    assert G.cli_help({}) == "this graph declares no options\n"

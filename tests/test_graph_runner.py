@@ -96,12 +96,12 @@ Shape "sphere" "float radius" 1
 def test_runner_help_lists_overrides():
     r = subprocess.run([str(RUNNER), str(GRAPHS / "headless_pt.json"), "--help"], capture_output=True, text=True)
     assert r.returncode == 0
-    for s in ("--spp <value>", "--max-path-length <value>", "--renderer <mcpg|pt|file.json>", "(required)"):
+    for s in ("--spp VALUE", "--max-path-length VALUE", "--renderer mcpg|pt|FILE.json", "(required)"):
         assert s in r.stdout, r.stdout
 
 
 def test_runner_rejects_missing_required_scene():
     r = subprocess.run([str(RUNNER), str(GRAPHS / "headless_pt.json"), "--spp", "4"], capture_output=True, text=True)
-    assert r.returncode == 1 and "missing required positional args" in r.stderr
+    assert r.returncode == 1 and "requires positional arguments" in r.stderr
     r = subprocess.run([str(RUNNER), "-x"], capture_output=True, text=True)
     assert r.returncode == 1 and "unknown option" in r.stderr
