@@ -13,6 +13,12 @@
  *   - One host thread per context (the reference graph is single-threaded,
  *     src/merian-graph/graph/graph.cpp:68-253). Work is asynchronous on the context's
  *     stream; host-buffer entry points synchronise before returning.
+ *   - The context's device scratch (traversal stacks, path buffers) is shared by its
+ *     trace, gbuffer and pt launches. When a _dev call passes a stream other than the
+ *     previous such call's, the library orders the new launch after the previous one with
+ *     an event, so calls on different streams serialise rather than race. Inside a stream
+ *     capture (hipStreamBeginCapture) that ordering is the caller's: capture every call of
+ *     one context on one stream.
  *   - mrt_last_error(ctx) describes the most recent failure on that context.
  *
  * Layouts mirror the reference's packed device structs where a layout is fixed there
@@ -201,7 +207,8 @@ typedef struct mrt_pt_params {
     int32_t spp;                 /* "samples per pixel" */
     int32_t max_path_length;     /* "max path length" (segments incl. the primary hit) */
     int32_t emission_on_primary; /* "emission on primary" */
-    int32_t demodulate_albedo;   /* "demodulate albedo" (must be 0: unsupported) */
+    int32_t demodulate_albedo;   /* "demodulate albedo": every sample is divided by the pixel's GBuffer
+                                    albedo floored at 1e-2 (render_pt.slang:104-116) */
     uint32_t instance_mask;      /* bits "0".."7" */
     int32_t mode;                /* MRT_MODE_*: BSDF = the reference estimator */
     int32_t record_signature;    /* write a per-pixel hash of the hit ids (parity aid) */
@@ -287,6 +294,13 @@ MRT_API mrt_status mrt_camera_rays_dev(mrt_ctx* ctx, const mrt_camera* cam, uint
 MRT_API mrt_status mrt_gbuffer_dev(mrt_ctx* ctx, mrt_scene* scene, const mrt_camera* cam,
                                    uint32_t width, uint32_t height, uint32_t instance_mask,
                                    mrt_packed_hit* d_gbuffer, void* stream);
+
+/* The GBuffer's albedo texture for a hit texture (gbuffer_rt.slang:76-77,92; stored as
+ * float4(albedo, 1), gbuffer.slang:160-162): per pixel the BSDF's directional albedo towards the
+ * camera for a surface hit, the environment's emission for a miss. d_albedo: float4 per pixel. */
+MRT_API mrt_status mrt_gbuffer_albedo_dev(mrt_ctx* ctx, mrt_scene* scene, const mrt_camera* cam, uint32_t width,
+                                          uint32_t height, const mrt_packed_hit* d_gbuffer, float* d_albedo,
+                                          void* stream);
 
 /* One frame of pt (render_pt.slang:92-130): reads the gbuffer, writes irradiance
  * float4(mean rgb, mean yuv-luminance^2) per pixel, row-major. d_signature (optional,

@@ -227,6 +227,8 @@ EXPORTS = [
                                         C.c_void_p, C.c_void_p]),
     ("mrt_gbuffer_dev", C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(Camera), C.c_uint32,
                                     C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
+    ("mrt_gbuffer_albedo_dev", C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(Camera), C.c_uint32, C.c_uint32,
+                                           C.c_void_p, C.c_void_p, C.c_void_p]),
     ("mrt_pt_render_dev", C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(PtParams),
                                       C.POINTER(Camera), C.c_uint32, C.c_uint32, C.c_uint32,
                                       C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(PtStats),

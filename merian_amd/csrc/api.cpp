@@ -70,7 +70,7 @@ void free_path_buffers(mrt_ctx* c) {
     PathBuffers& p = c->paths;
     for (void* q : {(void*)p.state, (void*)p.sig, (void*)p.q_o[0], (void*)p.q_o[1], (void*)p.q_d[0],
                     (void*)p.q_d[1], (void*)p.hits, (void*)p.aux, (void*)p.sq_o, (void*)p.sq_d, (void*)p.sq_c,
-                    (void*)p.counters, (void*)p.accum, (void*)p.sig_accum})
+                    (void*)p.counters, (void*)p.accum, (void*)p.demod, (void*)p.sig_accum})
         dev_free(c, q);
     p = PathBuffers{};
     c->paths_npix = 0;
@@ -101,6 +101,7 @@ mrt_status ensure_path_buffers(mrt_ctx* c, uint64_t paths, uint64_t npix, uint64
     // per chunk and bounce: queue sizes, trace cursors, shadow-queue sizes, shadow cursors
     p.counters = static_cast<uint32_t*>(A(4 * 4 * counters));
     p.accum = static_cast<float4*>(A(16 * npix));
+    p.demod = static_cast<float4*>(A(16 * npix));
     p.sig_accum = static_cast<uint32_t*>(A(4 * npix));
     if (st != MRT_OK) {
         const std::string keep = c->err;
@@ -145,6 +146,26 @@ TraceScratch scene_scratch(const mrt_ctx* c, const mrt_scene* s) {
     return t;
 }
 
+static bool capturing(hipStream_t st) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+}
+
+hipError_t scratch_begin(mrt_ctx* c, hipStream_t st) {
+    if (!c->scratch_event || c->scratch_stream == st || capturing(st)) return hipSuccess;
+    return hipStreamWaitEvent(st, c->scratch_event, 0);
+}
+
+hipError_t scratch_end(mrt_ctx* c, hipStream_t st) {
+    if (capturing(st)) return hipSuccess;
+    if (!c->scratch_event) {
+        const hipError_t e = hipEventCreateWithFlags(&c->scratch_event, hipEventDisableTiming);
+        if (e != hipSuccess) return e;
+    }
+    c->scratch_stream = st;
+    return hipEventRecord(c->scratch_event, st);
+}
+
 } // namespace mrt
 
 extern "C" {
@@ -187,6 +208,7 @@ MRT_API mrt_status mrt_destroy(mrt_ctx* c) {
     free_path_buffers(c);
     for (auto& kv : c->sizes) (void)hipFree(kv.first);
     c->sizes.clear();
+    if (c->scratch_event) (void)hipEventDestroy(c->scratch_event);
     const bool destroyed = hipStreamDestroy(c->stream) == hipSuccess;
     delete c;
     return ok && destroyed ? MRT_OK : MRT_ERR_DEVICE;
@@ -551,7 +573,10 @@ MRT_API mrt_status mrt_trace_closest_dev(mrt_ctx* c, mrt_scene* s, const mrt_ray
     if (!c || !s || (n && (!rays || !hits))) return fail(c, MRT_ERR_INVALID, "mrt_trace_closest_dev: bad argument");
     mrt_status st = ensure_trace_scratch(c, s);
     if (st != MRT_OK) return st;
-    const hipError_t e = launch_trace_rays(s->ds, scene_scratch(c, s), rays, n, hits, pick_stream(c, stream));
+    const hipStream_t strm = pick_stream(c, stream);
+    hipError_t e = scratch_begin(c, strm);
+    if (e == hipSuccess) e = launch_trace_rays(s->ds, scene_scratch(c, s), rays, n, hits, strm);
+    if (e == hipSuccess) e = scratch_end(c, strm);
     return e == hipSuccess ? MRT_OK : hip_fail(c, e, "trace kernel");
 }
 
@@ -562,9 +587,11 @@ MRT_API mrt_status mrt_trace_count_dev(mrt_ctx* c, mrt_scene* s, const mrt_ray* 
     if (st != MRT_OK) return st;
     void* dc = dev_alloc(c, 16, &st);
     if (st != MRT_OK) return st;
-    hipError_t e = hipMemsetAsync(dc, 0, 16, c->stream);
+    hipError_t e = scratch_begin(c, c->stream);
+    if (e == hipSuccess) e = launch_zero_u32(static_cast<uint32_t*>(dc), 4, c->stream);
     if (e == hipSuccess)
         e = launch_trace_rays(s->ds, scene_scratch(c, s), rays, n, hits, c->stream, static_cast<unsigned long long*>(dc));
+    if (e == hipSuccess) e = scratch_end(c, c->stream);
     unsigned long long h[2] = {0, 0};
     if (e == hipSuccess) e = hipMemcpyAsync(h, dc, 16, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -587,10 +614,12 @@ MRT_API mrt_status mrt_trace_closest(mrt_ctx* c, mrt_scene* s, const mrt_ray* ra
         dev_free(c, dr);
         return st;
     }
-    hipError_t e = hipMemcpyAsync(dr, rays, n * sizeof(mrt_ray), hipMemcpyHostToDevice, c->stream);
+    hipError_t e = scratch_begin(c, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dr, rays, n * sizeof(mrt_ray), hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess)
         e = launch_trace_rays(s->ds, scene_scratch(c, s), static_cast<mrt_ray*>(dr), n, static_cast<mrt_hit*>(dh),
                               c->stream);
+    if (e == hipSuccess) e = scratch_end(c, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(hits, dh, n * sizeof(mrt_hit), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     dev_free(c, dr);
@@ -610,8 +639,20 @@ MRT_API mrt_status mrt_gbuffer_dev(mrt_ctx* c, mrt_scene* s, const mrt_camera* c
     if (!c || !s || !cam || (uint64_t(w) * h && !out)) return fail(c, MRT_ERR_INVALID, "mrt_gbuffer_dev: bad argument");
     const mrt_status st = ensure_trace_scratch(c, s);
     if (st != MRT_OK) return st;
-    const hipError_t e = launch_gbuffer(s->ds, scene_scratch(c, s), *cam, w, h, mask & 0xFFu, out, pick_stream(c, stream));
+    const hipStream_t strm = pick_stream(c, stream);
+    hipError_t e = scratch_begin(c, strm);
+    if (e == hipSuccess) e = launch_gbuffer(s->ds, scene_scratch(c, s), *cam, w, h, mask & 0xFFu, out, strm);
+    if (e == hipSuccess) e = scratch_end(c, strm);
     return e == hipSuccess ? MRT_OK : hip_fail(c, e, "gbuffer kernel");
+}
+
+MRT_API mrt_status mrt_gbuffer_albedo_dev(mrt_ctx* c, mrt_scene* s, const mrt_camera* cam, uint32_t w, uint32_t h,
+                                          const mrt_packed_hit* gbuffer, float* albedo, void* stream) {
+    if (!c || !s || !cam || (uint64_t(w) * h && (!gbuffer || !albedo)))
+        return fail(c, MRT_ERR_INVALID, "mrt_gbuffer_albedo_dev: bad argument");
+    const hipError_t e = launch_gbuffer_albedo(s->ds, *cam, w, h, gbuffer, reinterpret_cast<float4*>(albedo), 0,
+                                               pick_stream(c, stream));
+    return e == hipSuccess ? MRT_OK : hip_fail(c, e, "gbuffer albedo kernel");
 }
 
 } // extern "C"
@@ -622,7 +663,6 @@ mrt_status check_params(mrt_ctx* c, const mrt_pt_params* p) {
     if (p->spp < 0) return fail(c, MRT_ERR_INVALID, "samples per pixel < 0");
     if (p->max_path_length < 1 || p->max_path_length > 64)
         return fail(c, MRT_ERR_INVALID, "max path length outside [1, 64]");
-    if (p->demodulate_albedo) return fail(c, MRT_ERR_UNSUPPORTED, "demodulate albedo is not supported");
     if (p->mode != MRT_MODE_BSDF && p->mode != MRT_MODE_NEE)
         return fail(c, MRT_ERR_UNSUPPORTED, "render mode not supported by this entry point");
     return MRT_OK;
@@ -663,6 +703,7 @@ mrt_status render_frame_dev(mrt_ctx* c, mrt_scene* s, const mrt_pt_params* prm, 
     a.spp = prm->spp;
     a.max_path_length = prm->max_path_length;
     a.emission_on_primary = prm->emission_on_primary;
+    a.demodulate = prm->demodulate_albedo ? 1 : 0;
     a.record_signature = prm->record_signature && signature ? 1 : 0;
     a.instance_mask = prm->instance_mask & 0xFFu;
     a.mode = mcpg ? int32_t(MRT_MODE_MCPG) : prm->mode;
@@ -688,9 +729,11 @@ mrt_status render_frame_dev(mrt_ctx* c, mrt_scene* s, const mrt_pt_params* prm, 
             return hip_fail(c, e, "traversal counters");
         }
     }
-    const hipError_t e = launch_pt_frame(s->ds, scene_scratch(c, s), a, gbuffer, pb, reinterpret_cast<float4*>(irradiance),
-                                         a.record_signature ? signature : nullptr, strm, stats ? &t : nullptr,
-                                         stats ? &segs : nullptr, tc, net, mcpg ? &margs : nullptr);
+    hipError_t e = scratch_begin(c, strm);
+    if (e == hipSuccess)
+        e = launch_pt_frame(s->ds, scene_scratch(c, s), a, gbuffer, pb, reinterpret_cast<float4*>(irradiance),
+                            a.record_signature ? signature : nullptr, strm, stats ? &t : nullptr, stats ? &segs : nullptr,
+                            tc, net, mcpg ? &margs : nullptr);
     unsigned long long hc[2] = {0, 0};
     hipError_t ec = hipSuccess;
     if (tc) {
@@ -702,9 +745,10 @@ mrt_status render_frame_dev(mrt_ctx* c, mrt_scene* s, const mrt_pt_params* prm, 
     if (ec != hipSuccess) return hip_fail(c, ec, "traversal counters");
     if (mcpg) mcpg_end_frame(mcpg, *cam);
     if (guide) {
-        st = guide_train_frame(c, guide, strm);
+        st = guide_train_frame(c, guide, strm); // reads the frame's records from the path buffers
         if (st != MRT_OK) return st;
     }
+    if (const hipError_t ee = scratch_end(c, strm); ee != hipSuccess) return hip_fail(c, ee, "pt frame");
     if (stats) {
         stats->primary_segments = npix;
         stats->continuation_segments = segs;

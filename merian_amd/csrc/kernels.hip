@@ -883,6 +883,26 @@ __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(2))
     }
 }
 
+// GBuffer albedo per pixel: the BSDF's directional albedo towards the camera in the material's
+// shading frame for a surface, the environment's emission for a miss (gbuffer_rt.slang:76-77,92);
+// floored at 1e-2 per channel for demodulation (gbuffer.slang:135-140).
+__global__ void k_gbuffer_albedo(DeviceScene S, mrt_camera cam, uint32_t w, uint32_t h,
+                                 const mrt_packed_hit* __restrict__ gbuffer, float4* __restrict__ out, int floor) {
+    const uint32_t n = w * h;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
+        const uint32_t x = p % w, y = p / w;
+        const RayF r = camera_ray(cam, x, y, w, h);
+        float bu, bv;
+        const uint32_t gid = gbuffer_hit(S, gbuffer, p, bu, bv);
+        ShadingPoint sp;
+        shading_point(S, r.o, r.d, gid, bu, bv, 1.f, true, sp);
+        const f3 a = sp.has_material ? sp.mat.get_albedo(sp.frame.to_local(-r.d)) : sp.emission;
+        out[p] = floor ? make_float4(fmaxf(a.x, 1e-2f), fmaxf(a.y, 1e-2f), fmaxf(a.z, 1e-2f), 1.f)
+                       : make_float4(a.x, a.y, a.z, 1.f);
+    }
+}
+
 // -----------------------------------------------------------------------------------------
 // Markov-chain path guiding (render_pt_mcpg.slang:39-244)
 
@@ -1150,10 +1170,12 @@ __global__ void k_accumulate(PtArgs A, PathBuffers P, int first_chunk, int last_
         uint32_t sa = first_chunk ? 0u : P.sig_accum[p];
         f3 irr{acc.x, acc.y, acc.z};
         float m2 = acc.w;
+        // render_pt.slang:106-108,115: every sample divided by the floored GBuffer albedo
+        const float4 dm = A.demodulate ? P.demod[p] : make_float4(1.f, 1.f, 1.f, 1.f);
         for (uint32_t sl = 0; sl < A.chunk; sl++) {
             const uint32_t i = sl * A.npix + p;
             const float4 b = ld_stream(P.state + 2 * size_t(i) + 1);
-            const f3 c{b.x, b.y, b.z};
+            const f3 c = A.demodulate ? f3{b.x / dm.x, b.y / dm.y, b.z / dm.z} : f3{b.x, b.y, b.z};
             if (A.record_signature) sa += P.sig[i];
             if (finite3(c)) {
                 irr = irr + c;
@@ -1296,6 +1318,16 @@ hipError_t launch_gbuffer(const DeviceScene& s, const TraceScratch& ts, const mr
                                uint32_t(n), ts.work, mask, nullptr, st);
 }
 
+hipError_t launch_gbuffer_albedo(const DeviceScene& s, const mrt_camera& cam, uint32_t w, uint32_t h,
+                                 const mrt_packed_hit* gbuffer, float4* out, int floor, hipStream_t st) {
+    const uint64_t n = uint64_t(w) * h;
+    if (n == 0) return hipSuccess;
+    if (n > 0xFFFFFFFFull) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_gbuffer_albedo, dim3(grid_for(n, 256, MAX_BLOCKS)), dim3(256), 0, st, s, cam, w, h, gbuffer,
+                       out, floor);
+    return hipGetLastError();
+}
+
 hipError_t launch_pt_frame(const DeviceScene& s, const TraceScratch& ts, PtArgs a, const mrt_packed_hit* gbuffer,
                            PathBuffers& pb, float4* out, uint32_t* sig_out, hipStream_t st, FrameTiming* timing,
                            uint64_t* continuation_segments, unsigned long long* trav_counters, const GuideNet* guide,
@@ -1340,6 +1372,11 @@ hipError_t launch_pt_frame(const DeviceScene& s, const TraceScratch& ts, PtArgs 
         if (k >= 0) kind.push_back(k);
     };
     mark(-1);
+    if (a.demodulate) {
+        if (!pb.demod) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_gbuffer_albedo, dim3(grid_for(a.npix, 256, MAX_BLOCKS)), dim3(256), 0, st, s, a.cam,
+                           a.width, a.height, gbuffer, pb.demod, 1);
+    }
     // [chunk][bounce] arrays: queue sizes, trace cursors, shadow-queue sizes, shadow cursors
     const uint32_t nz = n_chunks * per_chunk;
     hipLaunchKernelGGL(k_zero_u32, dim3((4 * nz + 255) / 256), dim3(256), 0, st, pb.counters, 4 * nz);

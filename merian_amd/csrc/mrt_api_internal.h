@@ -21,6 +21,9 @@ struct mrt_ctx {
     mrt::PathBuffers paths{};
     uint64_t paths_npix = 0, paths_counters = 0;
     mrt::TraceScratch trace{};
+    // last launch that used the context's shared scratch (trace stacks and cursor, path buffers)
+    hipEvent_t scratch_event = nullptr;
+    hipStream_t scratch_stream = nullptr;
 };
 
 struct mrt_scene {
@@ -47,6 +50,14 @@ mrt_status check_params(mrt_ctx* c, const mrt_pt_params* p);
 mrt_status ensure_trace_scratch(mrt_ctx* c, const mrt_scene* s);
 // The context's traversal scratch with the scene's launch settings.
 TraceScratch scene_scratch(const mrt_ctx* c, const mrt_scene* s);
+// Every launch that uses the context's scratch (traversal stacks and work cursor, path buffers)
+// is bracketed by these two calls: a launch on a stream other than the previous user's first
+// waits for that user's work, then records itself as the last user. Launches on one stream are
+// ordered by the stream already. A stream being captured into a graph is skipped (an event
+// recorded outside a capture cannot be waited on inside it): captured launches are ordered by
+// the caller.
+hipError_t scratch_begin(mrt_ctx* c, hipStream_t st);
+hipError_t scratch_end(mrt_ctx* c, hipStream_t st);
 // Rebuilds and uploads the scene's BVH with the given node width (4 or 8), leaf size and SAH
 // traversal cost.
 mrt_status rebuild_bvh(mrt_ctx* c, mrt_scene* s, uint32_t width, uint32_t max_leaf, float traversal_cost);

@@ -263,6 +263,52 @@ MRT_HD float vndf_pdf_wo(float widotn, float ndoth2, float a2) {
 }
 MRT_HD float vndf_pdf_wo(f3 wi, f3 h, f2 a) { return (ggx_ndf(h, a) * g1_over_widotn(wi, a)) / 4.0f; }
 
+// Directional albedo (the GBuffer's demodulation albedo, gbuffer_rt.slang:76-77, reached through
+// each BSDF's get_albedo).
+//
+// Pre-integrated GGX reflectance for the height-correlated Smith G2: the rational fit of Ray
+// Tracing Gems ch. 32 eq. 4 (brdf-ggx.slang:283-317), F0 * scale + bias, each a ratio of
+// polynomials in c = |cos theta| and alpha. The bias is faded out for F0 near zero.
+MRT_HD f3 ggx_albedo_fit(f3 F0, float alpha, float cos_theta) {
+    const float c = fabsf(cos_theta);
+    const float c2 = c * c, c3 = c2 * c;
+    const float a3 = (alpha * alpha) * alpha;
+    const float bias_num = (0.995367f + -1.38839f * c) + (-0.24751f + 1.97442f * c) * alpha;
+    const float bias_den = ((1.0f + 2.68132f * c) + 52.366f * c3) + ((16.0932f + -3.98452f * c) + 59.3013f * c3) * alpha +
+                           ((-5.18731f + 255.259f * c) + 2544.07f * c3) * a3;
+    const float scale_num = (-0.0564526f + 3.82901f * c) + (16.91f + -11.0303f * c) * alpha;
+    const float scale_den = ((1.0f + 4.11118f * c2) + -1.37886f * c3) + ((19.3254f + -28.9947f * c2) + 16.9514f * c3) * alpha +
+                            ((0.545386f + 96.0994f * c2) + -79.4492f * c3) * a3;
+    const float luma = ((F0.x + F0.y) + F0.z) * (1.0f / 3.0f);
+    const float bias = fmaxf(0.0f, (bias_num / bias_den) * clampf(luma * 50.0f, 0.0f, 1.0f));
+    const float scale = fmaxf(0.0f, scale_num / scale_den);
+    return {F0.x * scale + bias, F0.y * scale + bias, F0.z * scale + bias};
+}
+
+// Hemispherical reflectance of the Estevez & Kulla sheen lobe (brdf-sheen.slang:92-118): per
+// alpha row (0, 0.1, .., 1) a fit t(x) = e + x^d + a / (1 + b x^c) of the transmitted fraction in
+// x = 1 + 20 cos theta, interpolated linearly between the two rows around alpha.
+MRT_HD float sheen_albedo_row(int row, float x) {
+    // (a, b, c, d, e) per row
+    constexpr float k[11][5] = {
+        {1.01108f, 75.2286f, -9.4671f, 0.000973445f, -1.01325f}, {8.44447f, 0.0135054f, 1.11637f, 0.479139f, -9.30873f},
+        {5.67124f, 0.0194668f, 1.12489f, 0.453551f, -6.53104f},  {4.50086f, 0.0237758f, 1.09376f, 0.423665f, -5.36521f},
+        {3.77781f, 0.0273286f, 1.0666f, 0.400267f, -4.65601f},   {3.35338f, 0.0307392f, 0.989189f, 0.371074f, -4.24089f},
+        {3.39605f, 0.0308828f, 0.845455f, 0.333813f, -4.28662f}, {3.62583f, 0.0303531f, 0.79754f, 0.325588f, -4.51725f},
+        {4.11524f, 0.0172797f, 0.701166f, 0.275242f, -5.04394f}, {4.4012f, 0.0159469f, 0.685892f, 0.270221f, -5.32997f},
+        {5.8912f, 0.0171461f, 0.66366f, 0.288735f, -6.78867f}};
+    const float* r = k[row];
+    return (r[4] + mpow(x, r[3])) + r[0] / (1.0f + r[1] * mpow(x, r[2]));
+}
+MRT_HD float sheen_directional_albedo(float cos_theta, float alpha) {
+    const float x = clampf(cos_theta, 0.0f, 1.0f) * 20.0f + 1.0f;
+    const int lo = int(clampf(floorf(alpha * 10.0f), 0.0f, 10.0f));
+    const int hi = lo + 1 < 10 ? lo + 1 : 10;
+    const float w_lo = float(hi) - alpha * 10.0f;
+    const float t = sheen_albedo_row(lo, x) * w_lo + sheen_albedo_row(hi, x) * (1.0f - w_lo);
+    return 1.0f - clampf(t, 0.0f, 1.0f);
+}
+
 struct Lambert { // brdf-lambert-diffuse.slang:9-61
     f3 albedo;
     MRT_HD float alpha() const { return 1.0f; } // brdf-lambert-diffuse.slang:58-60
@@ -279,6 +325,7 @@ struct Lambert { // brdf-lambert-diffuse.slang:9-61
         if (fminf(wi.z, wo.z) < BSDF_MIN_COS_THETA) return none_sample();
         return make_sample(wo, INV_PI * wo.z, albedo);
     }
+    MRT_HD f3 get_albedo(f3) const { return albedo; }
 };
 
 struct GGX { // brdf-ggx.slang:331-412
@@ -303,6 +350,9 @@ struct GGX { // brdf-ggx.slang:331-412
         const float w = g2_over_widotn(wo, wi, a) / g1_over_widotn(wi, a);
         return make_sample(wo, p, {w, w, w});
     }
+    // the fit is isotropic: sqrt(ax * ay) keeps the NDF peak (brdf-ggx.slang:399-406)
+    MRT_HD f3 preintegrated(f3 F0, f3 wi) const { return ggx_albedo_fit(F0, sqrtf(a.x * a.y), wi.z); }
+    MRT_HD f3 get_albedo(f3 wi) const { return preintegrated(f3{1.0f, 1.0f, 1.0f}, wi); }
 };
 
 struct GGXBTDF { // brdf-ggx.slang:416-503
@@ -388,6 +438,13 @@ struct RoughDielectric { // bsdf-rough-dielectric.slang:13-120
         const float w = pow2(eta) * g2_over_widotn(fabsf(wo.z), wi.z, a2) / g1_over_widotn(wi.z, a2);
         return make_sample(wo, p, tint * w);
     }
+    // bsdf-rough-dielectric.slang:111-115: reflected part plus the transmitted eta^2 * tint
+    MRT_HD f3 get_albedo(f3 wi) const {
+        const float f0 = pow2((1.0f - eta) / (1.0f + eta));
+        const f3 spec = GGX{{alpha, alpha}}.preintegrated(f3{f0, f0, f0}, wi);
+        const f3 t = pow2(eta) * tint;
+        return {spec.x + (1.0f - spec.x) * t.x, spec.y + (1.0f - spec.y) * t.y, spec.z + (1.0f - spec.z) * t.z};
+    }
 };
 
 struct Sheen { // brdf-sheen.slang:9-90 (Estevez & Kulla 2017)
@@ -433,6 +490,7 @@ struct Sheen { // brdf-sheen.slang:9-90 (Estevez & Kulla 2017)
         const float p = pdf(wi, wo);
         return make_sample(wo, p, eval(wi, wo) / p);
     }
+    MRT_HD f3 get_albedo(f3 wi) const { return color * sheen_directional_albedo(wi.z, alpha); }
 };
 
 // ---------------------------------------------------------------------------------------
@@ -467,6 +525,12 @@ template <class Layer, class Base> struct FresnelMix { // bsdf-fresnel-mix.slang
         const float p = pdf(wi, s.wo);
         return make_sample(s.wo, p, eval(wi, s.wo) / p);
     }
+    // bsdf-fresnel-mix.slang:50-53
+    MRT_HD f3 get_albedo(f3 wi) const {
+        const f3 spec = layer.preintegrated(f3{f0, f0, f0}, wi);
+        const f3 b = base.get_albedo(wi);
+        return {spec.x + (1.0f - spec.x) * b.x, spec.y + (1.0f - spec.y) * b.y, spec.z + (1.0f - spec.z) * b.z};
+    }
 };
 
 template <class B0, class B1> struct Mix { // bsdf-mix.slang:6-65
@@ -500,6 +564,7 @@ template <class B0, class B1> struct Mix { // bsdf-mix.slang:6-65
         const f3 mix_eval = lerp(s.weight * s.pdf, b1.eval(wi, s.wo), weight);
         return make_sample(s.wo, mix_pdf, mix_eval / mix_pdf);
     }
+    MRT_HD f3 get_albedo(f3 wi) const { return lerp(b0.get_albedo(wi), b1.get_albedo(wi), weight); } // bsdf-mix.slang:51-54
 };
 
 template <class Inner> struct Conductor { // bsdf-conductor-fresnel.slang:6-50
@@ -518,6 +583,7 @@ template <class Inner> struct Conductor { // bsdf-conductor-fresnel.slang:6-50
         s.weight = s.weight * fresnel_schlick(dot(wi, h), f0);
         return s;
     }
+    MRT_HD f3 get_albedo(f3 wi) const { return inner.preintegrated(f0, wi); } // bsdf-conductor-fresnel.slang:39-41
 };
 
 // OpenPBR tree (openpbr-material.slang:19-25)
@@ -549,6 +615,7 @@ struct MaterialSample {
     MRT_HD f3 eval(f3 wi, f3 wo) const { return diffuse ? lambert.eval(wi, wo) : pbr.eval(wi, wo); }
     MRT_HD float pdf(f3 wi, f3 wo) const { return diffuse ? lambert.pdf(wi, wo) : pbr.pdf(wi, wo); }
     MRT_HD float alpha() const { return diffuse ? lambert.alpha() : pbr.alpha(); }
+    MRT_HD f3 get_albedo(f3 wi) const { return diffuse ? lambert.get_albedo(wi) : pbr.get_albedo(wi); }
 };
 
 MRT_HD f3 material_emission(const mrt_material& m, bool front_facing) {

@@ -82,6 +82,7 @@ struct PtArgs {
     int32_t mode;     // MRT_MODE_*
     float guide_fraction; // MRT_MODE_GUIDED: probability of sampling the guide at an eligible vertex
     uint32_t guided_bounces; // MRT_MODE_GUIDED: vertices [0, guided_bounces) use the guide
+    int32_t demodulate; // divide every sample by the pixel's floored GBuffer albedo
     uint32_t s0;      // first sample index of this chunk
     uint32_t chunk;   // samples in this chunk
     uint32_t n_paths; // npix * chunk
@@ -101,6 +102,7 @@ struct PathBuffers {
     float4* sq_d;      // shadow queue: direction.xyz
     float4* sq_c;      // shadow queue: unoccluded contribution.rgb
     float4* accum;     // per pixel running sum: radiance.xyz, luminance^2
+    float4* demod;     // per pixel demodulation albedo (floored GBuffer albedo), "demodulate albedo" only
     uint32_t* sig_accum;
     // MRT_MODE_GUIDED only
     uint4* lobes;      // packed guide lobes per queue entry (4 x uint4)
@@ -153,6 +155,10 @@ hipError_t launch_trace_rays(const DeviceScene& s, const TraceScratch& ts, const
                              mrt_hit* hits, hipStream_t st, unsigned long long* counters = nullptr);
 hipError_t launch_camera_rays(const mrt_camera& cam, uint32_t w, uint32_t h, mrt_ray* rays, hipStream_t st);
 hipError_t launch_running_mean(float4* acc, const float4* src, uint32_t n, uint32_t count, hipStream_t st);
+// The GBuffer albedo of every pixel of a hit texture (gbuffer_rt.slang:76-77,92): with floor != 0
+// the demodulation albedo max(albedo, 1e-2) (gbuffer.slang:138-140), else float4(albedo, 1).
+hipError_t launch_gbuffer_albedo(const DeviceScene& s, const mrt_camera& cam, uint32_t w, uint32_t h,
+                                 const mrt_packed_hit* gbuffer, float4* out, int floor, hipStream_t st);
 hipError_t launch_gbuffer(const DeviceScene& s, const TraceScratch& ts, const mrt_camera& cam, uint32_t w, uint32_t h,
                           uint32_t mask, mrt_packed_hit* out, hipStream_t st);
 

@@ -75,6 +75,9 @@ struct mrt_server {
     uint64_t reserved = 0;
     uint32_t trace_threads = 0;
     std::chrono::steady_clock::time_point t0;
+    // the server's own stream-ordered pool: job buffers come from it and return to it, it keeps
+    // at most the server's budget of freed memory, and it is trimmed and destroyed with the server
+    hipMemPool_t pool = nullptr;
 
     std::mutex m;
     std::condition_variable cv_work, cv_done;
@@ -104,6 +107,7 @@ JobShape shape_of(const mrt_server* s, const mrt_job_desc& d) {
     sh.spp = uint32_t(d.params.spp > 0 ? d.params.spp : 0);
     sh.L = uint32_t(d.params.max_path_length);
     sh.nee = d.params.mode == MRT_MODE_NEE;
+    sh.demod = d.params.demodulate_albedo != 0;
     sh.max_paths = s->cfg.max_paths_per_job;
     sh.trace_threads = s->trace_threads;
     const uint32_t need = d.scene->ds.stack_need;
@@ -121,10 +125,6 @@ mrt_status validate(const mrt_job_desc* d, std::string* why) {
         *why = "job: samples per pixel < 0 or max path length outside [1, 64]";
         return MRT_ERR_INVALID;
     }
-    if (p.demodulate_albedo) {
-        *why = "job: demodulate albedo is not supported";
-        return MRT_ERR_UNSUPPORTED;
-    }
     if (p.mode != MRT_MODE_BSDF && p.mode != MRT_MODE_NEE) {
         *why = "job: render mode not supported by the server";
         return MRT_ERR_UNSUPPORTED;
@@ -137,7 +137,7 @@ mrt_status validate(const mrt_job_desc* d, std::string* why) {
     return MRT_OK;
 }
 
-hipError_t alloc_job(Slot& sl, const Job& j, const JobShape& sh) {
+hipError_t alloc_job(hipMemPool_t pool, Slot& sl, const Job& j, const JobShape& sh) {
     JobBuffers& b = sl.buf;
     b = JobBuffers{};
     const JobLayout& l = j.layout;
@@ -145,7 +145,7 @@ hipError_t alloc_job(Slot& sl, const Job& j, const JobShape& sh) {
     auto A = [&](uint64_t bytes) -> void* {
         if (e != hipSuccess) return nullptr;
         void* p = nullptr;
-        e = hipMallocAsync(&p, round256(bytes), sl.stream);
+        e = hipMallocFromPoolAsync(&p, round256(bytes), pool, sl.stream);
         if (e == hipSuccess) b.allocs.push_back(p);
         return p;
     };
@@ -167,6 +167,7 @@ hipError_t alloc_job(Slot& sl, const Job& j, const JobShape& sh) {
     }
     b.pb.counters = static_cast<uint32_t*>(A(4 * l.counters));
     b.pb.accum = static_cast<float4*>(A(16 * N));
+    if (sh.demod) b.pb.demod = static_cast<float4*>(A(16 * N));
     b.pb.sig_accum = static_cast<uint32_t*>(A(4 * N));
     b.pb.capacity = P;
     b.ts.ovf = static_cast<uint32_t*>(A(uint64_t(l.threads) * sh.stack_entries * 4));
@@ -203,6 +204,7 @@ hipError_t enqueue_frame(mrt_server* s, Slot& sl, const Job& j) {
     a.spp = d.params.spp;
     a.max_path_length = d.params.max_path_length;
     a.emission_on_primary = d.params.emission_on_primary;
+    a.demodulate = d.params.demodulate_albedo ? 1 : 0;
     a.record_signature = 0;
     a.instance_mask = d.params.instance_mask & 0xFFu;
     a.mode = d.params.mode;
@@ -267,7 +269,7 @@ hipError_t start_job(mrt_server* s, Slot& sl, Job& j) {
         if (e != hipSuccess) return e;
         sl.h_counters_cap = nz;
     }
-    hipError_t e = alloc_job(sl, j, sh);
+    hipError_t e = alloc_job(s->pool, sl, j, sh);
     if (e == hipSuccess) e = enqueue_frame(s, sl, j);
     // queue sizes of the frame -> pinned host memory, then the completion event
     if (e == hipSuccess)
@@ -411,12 +413,20 @@ MRT_API mrt_status mrt_server_create(mrt_ctx* c, const mrt_server_config* cfg, m
     trace_prepare();
     s->t0 = std::chrono::steady_clock::now();
     s->adm = Admission(k.hbm_budget, k.max_slots);
-    // keep freed job buffers in the stream-ordered pool instead of returning them to the driver
-    hipMemPool_t pool = nullptr;
-    if (hipDeviceGetDefaultMemPool(&pool, c->device) == hipSuccess && pool) {
-        uint64_t keep = ~0ull;
-        if (const hipError_t e = hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep); e != hipSuccess)
-            return hip_fail(c, e, "server memory pool");
+    // freed job buffers stay in the server's pool, up to its budget, instead of going back to
+    // the driver on every synchronisation
+    hipMemPoolProps props{};
+    props.allocType = hipMemAllocationTypePinned;
+    props.location.type = hipMemLocationTypeDevice;
+    props.location.id = c->device;
+    if (const hipError_t e = hipMemPoolCreate(&s->pool, &props); e != hipSuccess) {
+        s->pool = nullptr;
+        return hip_fail(c, e, "server memory pool");
+    }
+    uint64_t keep = k.hbm_budget;
+    if (const hipError_t e = hipMemPoolSetAttribute(s->pool, hipMemPoolAttrReleaseThreshold, &keep); e != hipSuccess) {
+        (void)hipMemPoolDestroy(s->pool);
+        return hip_fail(c, e, "server memory pool");
     }
     s->slots.resize(k.max_slots);
     for (Slot& sl : s->slots) {
@@ -427,6 +437,7 @@ MRT_API mrt_status mrt_server_create(mrt_ctx* c, const mrt_server_config* cfg, m
                 if (t.done) (void)hipEventDestroy(t.done);
                 if (t.stream) (void)hipStreamDestroy(t.stream);
             }
+            (void)hipMemPoolDestroy(s->pool);
             return hip_fail(c, e, "server streams");
         }
     }
@@ -464,6 +475,11 @@ MRT_API mrt_status mrt_server_destroy(mrt_server* s) {
         if (sl.h_counters) ok &= hipHostFree(sl.h_counters) == hipSuccess;
         ok &= hipEventDestroy(sl.done) == hipSuccess;
         ok &= hipStreamDestroy(sl.stream) == hipSuccess;
+    }
+    if (s->pool) {
+        // every job buffer was freed on its slot stream, and every slot stream has drained
+        ok &= hipMemPoolTrimTo(s->pool, 0) == hipSuccess;
+        ok &= hipMemPoolDestroy(s->pool) == hipSuccess;
     }
     s->ctx->in_use -= s->reserved;
     delete s;

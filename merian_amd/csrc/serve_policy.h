@@ -83,6 +83,7 @@ struct JobShape {
     uint32_t spp;       // samples per pixel (0 allowed)
     uint32_t L;         // max path length
     bool nee;           // shadow queue needed
+    bool demod;         // "demodulate albedo": per-pixel demodulation albedo
     uint64_t max_paths; // paths in flight per job
     uint32_t trace_threads; // resident trace-kernel threads of the device
     uint32_t stack_entries; // global traversal-stack entries per thread the scene needs
@@ -118,6 +119,7 @@ inline JobLayout job_layout(const JobShape& s) {
     if (s.nee) b += 3 * round256(16 * l.paths);   // shadow queue
     b += round256(4 * l.counters);                // queue counters
     b += round256(16 * s.npix) + round256(4 * s.npix); // accumulation, signature sum
+    if (s.demod) b += round256(16 * s.npix);      // demodulation albedo
     b += round256(uint64_t(l.threads) * std::max<uint32_t>(1, s.stack_entries) * 4); // stack overflow
     b += round256(64);                            // ray cursor
     l.bytes = b;

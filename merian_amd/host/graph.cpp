@@ -81,7 +81,13 @@ void Graph::load(const GraphDescription& d) {
                 }
             }
     }
-    // topological order (Kahn), ties by identifier
+    order_nodes();
+    connected_ = false;
+}
+
+// Topological order (Kahn), ties by identifier.
+void Graph::order_nodes() {
+    order_.clear();
     std::map<std::string, int> indeg;
     std::map<std::string, std::set<std::string>> succ;
     for (const auto& [id, inst] : nodes_) {
@@ -115,18 +121,116 @@ const Resource* Graph::output(const std::string& node, const std::string& name) 
     return o == n->second.outputs.end() ? nullptr : &o->second;
 }
 
+std::vector<EventPattern> parse_event_patterns(const std::string& list, std::vector<std::string>* invalid) {
+    std::vector<EventPattern> out;
+    size_t begin = 0;
+    while (begin <= list.size()) {
+        size_t end = list.find(',', begin);
+        if (end == std::string::npos) end = list.size();
+        const std::string item = list.substr(begin, end - begin);
+        begin = end + 1;
+        if (item.empty()) continue;
+        const size_t a = item.find('/');
+        const size_t b = a == std::string::npos ? a : item.find('/', a + 1);
+        if (b == std::string::npos || item.find('/', b + 1) != std::string::npos || b + 1 == item.size()) {
+            if (invalid) invalid->push_back(item);
+            continue;
+        }
+        out.push_back(EventPattern{item.substr(0, a), item.substr(a + 1, b - a - 1), item.substr(b + 1)});
+    }
+    return out;
+}
+
+void Graph::dispatch(const GraphEvent& e) {
+    // a listener may send further events; iterate over a copy of the list
+    const auto listeners = listeners_;
+    for (const auto& [pattern, listener] : listeners)
+        if (pattern.matches(e)) listener(e);
+}
+
+void Graph::send_event(const std::string& name) { dispatch(GraphEvent{"", "user", name}); }
+
+// Every node's on_connected, in run order, with a fresh set of listeners (the reference
+// re-registers listeners on each connect, graph.cpp:268-290).
+void Graph::connect() {
+    listeners_.clear();
+    for (const std::string& id : order_) {
+        Instance& inst = nodes_.at(id);
+        ConnectInfo ci;
+        ci.ctx = ctx_;
+        ci.identifier = id;
+        ci.listen = [this, id](const std::string& patterns, EventListener listener) {
+            std::vector<std::string> bad;
+            for (const EventPattern& p : parse_event_patterns(patterns, &bad)) listeners_.emplace_back(p, listener);
+            for (const std::string& b : bad)
+                warnings_.push_back("node '" + id + "': invalid event pattern '" + b + "' (expected type/identifier/event)");
+        };
+        try {
+            inst.node->on_connected(ci);
+        } catch (const std::exception& ex) {
+            throw node_error("node '" + id + "' (" + inst.type + "): " + ex.what());
+        }
+    }
+    connected_ = true;
+}
+
+void Graph::remove_node(const std::string& id) {
+    auto it = nodes_.find(id);
+    if (it == nodes_.end()) return;
+    it->second.node->release(ctx_);
+    nodes_.erase(it);
+    for (auto& [_, inst] : nodes_)
+        for (auto s = inst.sources.begin(); s != inst.sources.end();)
+            s = s->second.first == id ? inst.sources.erase(s) : std::next(s);
+    desc_.nodes[id].enabled = false;
+    order_nodes();
+    connected_ = false;
+}
+
+void Graph::set_properties(const std::string& node, const json::Value& properties) {
+    auto it = nodes_.find(node);
+    if (it == nodes_.end()) throw node_error("no node '" + node + "'");
+    json::Value merged = it->second.props;
+    GraphDescription::merge_into(merged, properties);
+    try {
+        it->second.props = it->second.node->properties(merged);
+    } catch (const std::exception& ex) {
+        throw node_error("node '" + node + "' (" + it->second.type + "): " + ex.what());
+    }
+}
+
 void Graph::run() {
+    if (!connected_) connect();
     RunInfo info;
     info.ctx = ctx_;
     info.iteration = iteration_;
     info.log = &warnings_;
-    for (const std::string& id : order_) {
-        Instance& inst = nodes_.at(id);
+    std::map<std::string, NodeStatusFlags> flags;
+    const auto inputs_of = [this](const Instance& inst) {
         std::map<std::string, const Resource*> in;
         for (const auto& [name, srcp] : inst.sources) in[name] = output(srcp.first, srcp.second);
+        return in;
+    };
+    const auto sender = [this, &info](const Instance& inst) {
+        info.send_event = [this, type = inst.type, id = inst.id](const std::string& name) {
+            dispatch(GraphEvent{type, id, name});
+        };
+    };
+    for (const std::string& id : order_) {
+        Instance& inst = nodes_.at(id);
+        sender(inst);
+        try {
+            flags[id] |= inst.node->pre_process(info, inputs_of(inst));
+        } catch (const std::exception& ex) {
+            throw node_error("node '" + id + "' (" + inst.type + "): " + ex.what());
+        }
+    }
+    for (const std::string& id : order_) {
+        Instance& inst = nodes_.at(id);
+        sender(inst);
         const auto t0 = std::chrono::steady_clock::now();
         try {
-            inst.node->process(info, in, inst.outputs);
+            flags[id] |= inst.node->process(info, inputs_of(inst), inst.outputs);
         } catch (const std::exception& ex) {
             throw node_error("node '" + id + "' (" + inst.type + "): " + ex.what());
         }
@@ -135,8 +239,16 @@ void Graph::run() {
             inst.total_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         }
     }
+    info.send_event = nullptr;
     iteration_++;
     stop_ = stop_ || info.stop;
+    // status flags: removed nodes leave the graph, and any of them or a reconnect request
+    // makes the next run call on_connected again. One iteration is in flight at a time here,
+    // so RESET_IN_FLIGHT_DATA has nothing to drop.
+    for (const auto& [id, f] : flags) {
+        if (f & REMOVE_NODE) remove_node(id);
+        if (f & NEEDS_RECONNECT) connected_ = false;
+    }
 }
 
 std::string Graph::report() const {

@@ -106,9 +106,11 @@ class PbrtSceneNode : public Node {
         dirty_ = true;
         return p.result();
     }
-    void process(RunInfo& info, const std::map<std::string, const Resource*>&, std::map<std::string, Resource>& out) override {
+    NodeStatusFlags process(RunInfo& info, const std::map<std::string, const Resource*>&,
+                            std::map<std::string, Resource>& out) override {
         if (dirty_) load(info);
         out["scene"] = res_;
+        return 0;
     }
     void release(mrt_ctx* c) override {
         if (res_.scene) mrt_scene_destroy(c, res_.scene);
@@ -168,8 +170,8 @@ class GBufferNode : public Node {
         height_ = uint32_t(p.get_int("height", 1080));
         return p.result();
     }
-    void process(RunInfo& info, const std::map<std::string, const Resource*>& in,
-                 std::map<std::string, Resource>& out) override {
+    NodeStatusFlags process(RunInfo& info, const std::map<std::string, const Resource*>& in,
+                            std::map<std::string, Resource>& out) override {
         auto it = in.find("scene");
         if (it == in.end() || !it->second || !std::holds_alternative<SceneResource>(*it->second))
             throw node_error("input \"scene\" is not connected");
@@ -193,15 +195,23 @@ class GBufferNode : public Node {
                                  MRT_JITTER_R2, uint32_t(info.iteration), &g.camera),
               "camera");
         buf_.ensure(info.ctx, uint64_t(g.width) * g.height * sizeof(mrt_packed_hit));
+        albedo_.ensure(info.ctx, uint64_t(g.width) * g.height * 16);
         g.d = static_cast<mrt_packed_hit*>(buf_.d);
+        g.albedo = static_cast<float*>(albedo_.d);
         check(info.ctx, mrt_gbuffer_dev(info.ctx, s.scene, &g.camera, g.width, g.height, mask_, g.d, nullptr), "gbuffer");
+        check(info.ctx, mrt_gbuffer_albedo_dev(info.ctx, s.scene, &g.camera, g.width, g.height, g.d, g.albedo, nullptr),
+              "gbuffer albedo");
         out["gbuffer"] = g;
+        return 0;
     }
-    void release(mrt_ctx* c) override { buf_.release(c); }
+    void release(mrt_ctx* c) override {
+        buf_.release(c);
+        albedo_.release(c);
+    }
 
   private:
     uint32_t mask_ = 0xFF, width_ = 1920, height_ = 1080;
-    DeviceBuffer buf_;
+    DeviceBuffer buf_, albedo_;
 };
 
 // "Render (Path-traced)" (render_pt.cpp:26-34,151-181): scene + gbuffer in, irradiance out.
@@ -215,15 +225,15 @@ class RenderPTNode : public Node {
         max_path_length_ = int32_t(p.get_int("max path length", 5));
         emission_on_primary_ = p.get_bool("emission on primary", true);
         p.get_bool("shader execution reordering", false); // accepted; wavefront queues already reorder
-        if (p.get_bool("demodulate albedo", false)) throw node_error("\"demodulate albedo\" is not supported");
+        demodulate_ = p.get_bool("demodulate albedo", false);
         mask_ = mask_bits(p);
         const std::string fmt = p.get_string("irradiance format", "R32G32B32A32Sfloat");
         if (fmt != "R32G32B32A32Sfloat")
             throw node_error("irradiance format '" + fmt + "' is not supported (R32G32B32A32Sfloat only)");
         return p.result();
     }
-    void process(RunInfo& info, const std::map<std::string, const Resource*>& in,
-                 std::map<std::string, Resource>& out) override {
+    NodeStatusFlags process(RunInfo& info, const std::map<std::string, const Resource*>& in,
+                            std::map<std::string, Resource>& out) override {
         auto gi = in.find("gbuffer");
         if (gi == in.end() || !gi->second || !std::holds_alternative<GBufferResource>(*gi->second))
             throw node_error("input \"gbuffer\" is not connected");
@@ -236,8 +246,14 @@ class RenderPTNode : public Node {
         p.spp = spp_;
         p.max_path_length = max_path_length_;
         p.emission_on_primary = emission_on_primary_;
+        p.demodulate_albedo = demodulate_;
         p.instance_mask = mask_;
         p.mode = MRT_MODE_BSDF;
+        // a changed path length invalidates accumulated history (render_pt.cpp:102-105)
+        if (max_path_length_ != emitted_max_path_length_) {
+            emitted_max_path_length_ = max_path_length_;
+            if (info.send_event) info.send_event("bounces_changed");
+        }
         ImageResource img;
         img.width = g.width;
         img.height = g.height;
@@ -248,12 +264,14 @@ class RenderPTNode : public Node {
                                 img.d, nullptr, nullptr, nullptr),
               "pt");
         out["irradiance"] = img;
+        return 0;
     }
     void release(mrt_ctx* c) override { buf_.release(c); }
 
   private:
     int32_t spp_ = 1, max_path_length_ = 5;
-    bool emission_on_primary_ = true;
+    int32_t emitted_max_path_length_ = 5; // the default: a loaded length other than 5 is announced once
+    bool emission_on_primary_ = true, demodulate_ = false;
     uint32_t mask_ = 0xFF;
     DeviceBuffer buf_;
 };
@@ -271,7 +289,7 @@ class RenderMCPGNode : public Node {
         p.get_bool("raygen executor", false);
         const bool reference_mode = p.get_bool("reference mode", false);
         emission_on_primary_ = p.get_bool("emission on primary", true);
-        if (p.get_bool("demodulate albedo", false)) throw node_error("\"demodulate albedo\" is not supported");
+        demodulate_ = p.get_bool("demodulate albedo", false);
         spp_ = int32_t(p.get_int("samples per pixel", 1));
         max_path_length_ = int32_t(p.get_int("max path length", 5));
         mrt_mcpg_config k;
@@ -327,8 +345,8 @@ class RenderMCPGNode : public Node {
         }
         return p.result();
     }
-    void process(RunInfo& info, const std::map<std::string, const Resource*>& in,
-                 std::map<std::string, Resource>& out) override {
+    NodeStatusFlags process(RunInfo& info, const std::map<std::string, const Resource*>& in,
+                            std::map<std::string, Resource>& out) override {
         auto gi = in.find("gbuffer");
         if (gi == in.end() || !gi->second || !std::holds_alternative<GBufferResource>(*gi->second))
             throw node_error("input \"gbuffer\" is not connected");
@@ -349,7 +367,12 @@ class RenderMCPGNode : public Node {
         p.spp = spp_;
         p.max_path_length = max_path_length_;
         p.emission_on_primary = emission_on_primary_;
+        p.demodulate_albedo = demodulate_;
         p.instance_mask = mask_;
+        if (max_path_length_ != emitted_max_path_length_) { // render_pt_mcpg.cpp:212-215
+            emitted_max_path_length_ = max_path_length_;
+            if (info.send_event) info.send_event("bounces_changed");
+        }
         ImageResource img;
         img.width = g.width;
         img.height = g.height;
@@ -360,6 +383,7 @@ class RenderMCPGNode : public Node {
                                   g.height, g.d, img.d, nullptr, nullptr),
               "mcpg");
         out["irradiance"] = img;
+        return 0;
     }
     void release(mrt_ctx* c) override {
         buf_.release(c);
@@ -368,8 +392,8 @@ class RenderMCPGNode : public Node {
     }
 
   private:
-    int32_t spp_ = 1, max_path_length_ = 5;
-    bool emission_on_primary_ = true;
+    int32_t spp_ = 1, max_path_length_ = 5, emitted_max_path_length_ = 5;
+    bool emission_on_primary_ = true, demodulate_ = false;
     uint32_t mask_ = 0xFF;
     mrt_mcpg_config cfg_{};
     bool recreate_ = true;
@@ -377,10 +401,12 @@ class RenderMCPGNode : public Node {
     DeviceBuffer buf_;
 };
 
-// "Accumulate": running mean of its source over iterations, restarted when the source's
-// extent changes; "max history" caps the count (a number, or "inf"). The reference's SVGF-style
-// reprojection, firefly filter and adaptivity settings are accepted and kept but not applied
-// (a static camera needs none of them to converge).
+// "Accumulate": running mean of its source over iterations (accumulate.cpp), restarted on the
+// first iteration, when the source's extent changes, when "clear" is set, and when an event
+// matching "clear event pattern" arrives (accumulate.cpp:60-64,154-158; default "/user/clear").
+// A restart sends the node's own "clear" event. "max history" caps the count (a number, or
+// "inf"). The reference's SVGF-style reprojection, firefly filter and adaptivity settings are
+// accepted and kept but not applied (a static camera needs none of them to converge).
 class AccumulateNode : public Node {
   public:
     std::vector<std::string> inputs() const override { return {"src", "gbuffer", "prev_history", "prev_out"}; }
@@ -390,19 +416,35 @@ class AccumulateNode : public Node {
         const Value* mh = in.find("max history");
         max_history_ = mh && mh->is_number() ? uint64_t(std::max<int64_t>(1, mh->as_int())) : UINT64_MAX;
         if (!in.contains("max history")) p.get_string("max history", "inf");
-        count_ = 0;
+        const std::string pattern = p.get_string("clear event pattern", "/user/clear");
+        if (pattern != pattern_) {
+            pattern_ = pattern;
+            reconnect_ = true; // listeners are registered in on_connected
+        }
+        clear_ = clear_ || p.get_bool("clear", false);
         return p.result();
     }
-    void process(RunInfo& info, const std::map<std::string, const Resource*>& in,
-                 std::map<std::string, Resource>& out) override {
+    NodeStatusFlags on_connected(ConnectInfo& ci) override {
+        ci.listen(pattern_, [this](const GraphEvent&) {
+            clear_ = true;
+            return true;
+        });
+        reconnect_ = false;
+        return 0;
+    }
+    NodeStatusFlags process(RunInfo& info, const std::map<std::string, const Resource*>& in,
+                            std::map<std::string, Resource>& out) override {
         auto si = in.find("src");
         if (si == in.end() || !si->second || !std::holds_alternative<ImageResource>(*si->second))
             throw node_error("input \"src\" is not connected");
         const ImageResource& src = std::get<ImageResource>(*si->second);
-        if (src.width != w_ || src.height != h_) {
+        if (src.width != w_ || src.height != h_ || info.iteration == 0 || clear_) {
             count_ = 0;
             w_ = src.width;
             h_ = src.height;
+            clear_ = false;
+            cleared_++;
+            if (info.send_event) info.send_event("clear");
         }
         buf_.ensure(info.ctx, uint64_t(w_) * h_ * 16);
         const uint32_t n = uint32_t(std::min<uint64_t>(count_, max_history_ - 1));
@@ -412,12 +454,15 @@ class AccumulateNode : public Node {
         ImageResource img{static_cast<float*>(buf_.d), w_, h_};
         out["out"] = img;
         out["history"] = img;
+        return reconnect_ ? NEEDS_RECONNECT : 0;
     }
     void release(mrt_ctx* c) override { buf_.release(c); }
 
   private:
-    uint64_t count_ = 0, max_history_ = UINT64_MAX;
+    uint64_t count_ = 0, max_history_ = UINT64_MAX, cleared_ = 0;
     uint32_t w_ = 0, h_ = 0;
+    std::string pattern_ = "/user/clear";
+    bool clear_ = false, reconnect_ = true;
     DeviceBuffer buf_;
 };
 
@@ -446,8 +491,8 @@ class ImageWriteNode : public Node {
         if (p.get_float("scale", 1.0) != 1.0) throw node_error("scale != 1 is not supported");
         return p.result();
     }
-    void process(RunInfo& info, const std::map<std::string, const Resource*>& in,
-                 std::map<std::string, Resource>&) override {
+    NodeStatusFlags process(RunInfo& info, const std::map<std::string, const Resource*>& in,
+                            std::map<std::string, Resource>&) override {
         auto si = in.find("src");
         if (si == in.end() || !si->second || !std::holds_alternative<ImageResource>(*si->second))
             throw node_error("input \"src\" is not connected");
@@ -464,6 +509,7 @@ class ImageWriteNode : public Node {
         }
         if (exit_at_ == iteration_) info.stop = true;
         iteration_++;
+        return 0;
     }
 
   private:

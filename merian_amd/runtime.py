@@ -131,6 +131,12 @@ class Context:
     def gbuffer_dev(self, scene: DeviceScene, cam: abi.Camera, w: int, h: int, d_out: int, mask: int = 0xFF):
         self._check(self.lib.mrt_gbuffer_dev(self.h, scene.h, C.byref(cam), w, h, mask, C.c_void_p(d_out), None))
 
+    def gbuffer_albedo_dev(self, scene: DeviceScene, cam: abi.Camera, w: int, h: int, d_gbuffer: int, d_albedo: int,
+                           stream: int | None = None):
+        """mrt_gbuffer_albedo_dev: the GBuffer albedo texture (float4 per pixel) of a hit texture."""
+        self._check(self.lib.mrt_gbuffer_albedo_dev(self.h, scene.h, C.byref(cam), w, h, C.c_void_p(d_gbuffer),
+                                                    C.c_void_p(d_albedo), C.c_void_p(stream) if stream else None))
+
     def pt_render_dev(self, scene: DeviceScene, params: abi.PtParams, cam: abi.Camera, frame: int, w: int, h: int,
                       d_gbuffer: int, d_out: int, d_sig: int | None = None, stats: bool = False,
                       stream: int | None = None):
@@ -157,10 +163,11 @@ class Context:
 
 
 def pt_params(spp=4, max_path_length=4, emission_on_primary=True, mask=0xFF, mode=abi.MODE_BSDF,
-              record_signature=True) -> abi.PtParams:
+              record_signature=True, demodulate_albedo=False) -> abi.PtParams:
     p = abi.PtParams()
     p.spp, p.max_path_length, p.emission_on_primary = spp, max_path_length, int(emission_on_primary)
-    p.demodulate_albedo, p.instance_mask, p.mode, p.record_signature = 0, mask, mode, int(record_signature)
+    p.demodulate_albedo, p.instance_mask, p.mode = int(demodulate_albedo), mask, mode
+    p.record_signature = int(record_signature)
     return p
 
 

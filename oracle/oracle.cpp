@@ -486,6 +486,7 @@ struct MaterialSample {
     f3 eval(f3 wi, f3 wo) const { return model == MRT_MAT_DIFFUSE ? diffuse.eval(wi, wo) : pbr.eval(wi, wo); }
     float pdf(f3 wi, f3 wo) const { return model == MRT_MAT_DIFFUSE ? diffuse.pdf(wi, wo) : pbr.pdf(wi, wo); }
     float alpha() const { return model == MRT_MAT_DIFFUSE ? diffuse.alpha() : pbr.alpha(); }
+    f3 get_albedo(f3 wi) const { return model == MRT_MAT_DIFFUSE ? diffuse.get_albedo(wi) : pbr.get_albedo(wi); }
 };
 
 constexpr float MIN_ROUGHNESS = 0.0316f; // material.slang:23
@@ -698,6 +699,18 @@ inline bool make_ray(const ShadingPoint& sp, f3 wi, f3 wo, bool transmission, Ra
     out.tmin = 0.f;
     out.tmax = SCENE_RAY_TMAX;
     return true;
+}
+
+// The GBuffer albedo of a primary shading point (gbuffer_rt.slang:76-77 for a surface: the BSDF's
+// directional albedo towards the camera in the material's shading frame; :92 for a miss: the
+// environment's emission).
+inline f3 gbuffer_albedo(const Ray& ray, const ShadingPoint& sp) {
+    return sp.has_material ? sp.mat.get_albedo(sp.shading_frame.to_local(-ray.d)) : sp.emission;
+}
+// get_demodulation_albedo (gbuffer.slang:135-140): floored at 1e-2 per channel.
+inline f3 demodulation_albedo(const Ray& ray, const ShadingPoint& sp) {
+    const f3 a = gbuffer_albedo(ray, sp);
+    return {std::fmax(a.x, 1e-2f), std::fmax(a.y, 1e-2f), std::fmax(a.z, 1e-2f)};
 }
 
 // yuv_luminance (colors/colorspace_yuv.slang:5-7)
@@ -973,6 +986,22 @@ __attribute__((visibility("default"))) void orc_gbuffer(void* sp, const mrt_came
     });
 }
 
+// The GBuffer's albedo texture (gbuffer.slang:160-162 stores float4(albedo, 1)) for a hit texture.
+__attribute__((visibility("default"))) void orc_gbuffer_albedo(void* sp, const mrt_camera* cam, uint32_t w, uint32_t h,
+                                                               const mrt_packed_hit* gbuffer, float* out, int threads) {
+    const Scene& sc = *static_cast<Scene*>(sp);
+    parallel_for(uint64_t(w) * h, threads, [&](uint64_t i) {
+        const uint32_t x = uint32_t(i % w), y = uint32_t(i / w);
+        const Ray r = camera_ray(*cam, x, y, w, h);
+        const ShadingPoint s = get_shading_point(sc, r, unpack_hit(gbuffer[i]), 0u, 1.0f, true);
+        const f3 a = gbuffer_albedo(r, s);
+        out[4 * i + 0] = a.x;
+        out[4 * i + 1] = a.y;
+        out[4 * i + 2] = a.z;
+        out[4 * i + 3] = 1.f;
+    });
+}
+
 // Render (Path-traced) main (render_pt.slang:92-130) over the pixel rectangle
 // [x0, x0+cw) x [y0, y0+ch) of a w x h image. out: float4 per rectangle pixel.
 // sig (optional): per-pixel sum over samples of a hash of each path's hit id sequence.
@@ -989,6 +1018,7 @@ __attribute__((visibility("default"))) uint64_t orc_pt_render(void* scp, const m
         const Hit primary_hit = unpack_hit(gbuffer[size_t(y) * w + x]);
         const Ray primary_ray = camera_ray(*cam, x, y, w, h);
         const ShadingPoint primary_sp = get_shading_point(sc, primary_ray, primary_hit, 0u, 1.0f, true);
+        const f3 demod = prm->demodulate_albedo ? demodulation_albedo(primary_ray, primary_sp) : f3{1.f, 1.f, 1.f};
         f3 irr{0, 0, 0};
         float m2 = 0.f;
         uint32_t psig = 0;
@@ -998,7 +1028,7 @@ __attribute__((visibility("default"))) uint64_t orc_pt_render(void* scp, const m
         for (int s = 0; s < prm->spp; s++) {
             PCG rng{pcg4d16(x, y, frame, uint32_t(s))};
             uint32_t ps = sig_step(0x9E3779B9u * uint32_t(s + 1), pid);
-            const f3 contrib = trace_path(sc, *prm, primary_ray, primary_sp, rng, ps, segs) / f3{1.f, 1.f, 1.f};
+            const f3 contrib = trace_path(sc, *prm, primary_ray, primary_sp, rng, ps, segs) / demod;
             psig += ps;
             if (finite3(contrib)) {
                 irr = irr + contrib;
@@ -1021,6 +1051,7 @@ __attribute__((visibility("default"))) uint64_t orc_pt_render(void* scp, const m
 }
 
 // ---- BSDF probes for the statistical invariant tests ----------------------------------
+// op: 0 sample_eval, 1 eval, 2 pdf, 3 get_albedo (directional albedo towards wi)
 // kind: 0 Lambert(albedo) 1 GGX(alpha.xy) 2 FresnelMix(GGX(alpha.x), Lambert(albedo), f0)
 //       3 Conductor(GGX(alpha.x), f0rgb=albedo) 4 RoughDielectric(alpha.x, eta, tint=albedo)
 //       5 Sheen(albedo, alpha.x) 6 GGXBTDF(alpha.x, eta, albedo)
@@ -1050,6 +1081,9 @@ extern "C++" template <class B> void bsdf_batch(const B& b, int op, const float*
         } else if (op == 1) {
             const f3 f = b.eval(WI, ld3(wo + 3 * i));
             o_w[3 * i] = f.x; o_w[3 * i + 1] = f.y; o_w[3 * i + 2] = f.z;
+        } else if (op == 3) {
+            const f3 a = b.get_albedo(WI);
+            o_w[3 * i] = a.x; o_w[3 * i + 1] = a.y; o_w[3 * i + 2] = a.z;
         } else {
             o_pdf[i] = b.pdf(WI, ld3(wo + 3 * i));
         }

@@ -64,6 +64,7 @@ def lib() -> C.CDLL:
             "orc_scene_destroy": (None, [vp]),
             "orc_trace_closest": (None, [vp, vp, u64, vp, i32]),
             "orc_gbuffer": (None, [vp, C.POINTER(abi.Camera), u32, u32, u32, vp, i32]),
+            "orc_gbuffer_albedo": (None, [vp, C.POINTER(abi.Camera), u32, u32, vp, vp, i32]),
             "orc_pt_render": (u64, [vp, C.POINTER(abi.PtParams), C.POINTER(abi.Camera), u32, u32, u32, vp,
                                     u32, u32, u32, u32, vp, vp, i32]),
             "orc_bsdf": (None, [C.POINTER(BsdfCfg), i32, vp, vp, u32, u64, vp, vp, vp, vp]),
@@ -135,6 +136,13 @@ class OracleScene:
     def gbuffer(self, cam, w, h, mask=0xFF, threads: int = 0) -> np.ndarray:
         out = np.zeros(w * h, dtype=abi.PACKED_HIT_DTYPE)
         lib().orc_gbuffer(self.h, C.byref(cam), w, h, mask, _p(out), threads)
+        return out
+
+    def gbuffer_albedo(self, cam, w, h, gbuffer, threads: int = 0) -> np.ndarray:
+        """The GBuffer albedo texture, float32[h, w, 4] = (albedo.rgb, 1)."""
+        out = np.zeros((h, w, 4), dtype=np.float32)
+        gb = np.ascontiguousarray(gbuffer)
+        lib().orc_gbuffer_albedo(self.h, C.byref(cam), w, h, _p(gb), _p(out), threads)
         return out
 
     def render(self, params: abi.PtParams, cam, frame, w, h, gbuffer, rect=None, threads: int = 0):
@@ -228,11 +236,11 @@ def bsdf(cfg: BsdfCfg, op: str, wi: np.ndarray, wo: np.ndarray | None = None, se
     o_w = np.zeros((n, 3), np.float32)
     o_v = np.zeros(n, np.int32)
     wo_a = None if wo is None else np.ascontiguousarray(wo, dtype=np.float32)
-    code = {"sample_eval": 0, "eval": 1, "pdf": 2}[op]
+    code = {"sample_eval": 0, "eval": 1, "pdf": 2, "albedo": 3}[op]
     lib().orc_bsdf(C.byref(cfg), code, _p(wi), _p(wo_a), seed, n, _p(o_wo), _p(o_pdf), _p(o_w), _p(o_v))
     if op == "sample_eval":
         return o_wo, o_pdf, o_w, o_v.astype(bool)
-    if op == "eval":
+    if op in ("eval", "albedo"):
         return o_w
     return o_pdf
 
@@ -252,10 +260,11 @@ def bsdf_cfg(kind: int, albedo=(0.8, 0.8, 0.8), alpha=(0.3, 0.3), f0=0.04, eta=1
 
 
 def pt_params(spp=4, max_path_length=4, emission_on_primary=True, mask=0xFF, mode=abi.MODE_BSDF,
-              record_signature=True) -> abi.PtParams:
+              record_signature=True, demodulate_albedo=False) -> abi.PtParams:
     p = abi.PtParams()
     p.spp, p.max_path_length, p.emission_on_primary = spp, max_path_length, int(emission_on_primary)
-    p.demodulate_albedo, p.instance_mask, p.mode, p.record_signature = 0, mask, mode, int(record_signature)
+    p.demodulate_albedo, p.instance_mask, p.mode = int(demodulate_albedo), mask, mode
+    p.record_signature = int(record_signature)
     return p
 
 

@@ -161,6 +161,48 @@ inline float vndf_pdf_wo(float widotn, float ndoth2, float a2) {
 inline float vndf_pdf_wo(f3 wi, f3 h, f2 a) { return (ggx_ndf(h, a) * g1_over_widotn(wi, a)) / 4.0f; }
 
 // ---------------------------------------------------------------------------------------
+// Directional albedo (BSDF::get_albedo; the GBuffer albedo of gbuffer_rt.slang:76-77, floored and
+// divided out by render_pt.slang:104-116 under "demodulate albedo")
+
+// Pre-integrated GGX with height-correlated G2 (brdf-ggx.slang:283-317, Ray Tracing Gems ch. 32
+// eq. 4): F0 * scale + bias. The matrices of the fit are written out as polynomials:
+// bias = [M1 (1,c)].(1,a) / [M2 (1,c,c^3)].(1,a,a^3), scale = [M3 (1,c)].(1,a) / [M4 (1,c^2,c^3)].(1,a,a^3).
+inline f3 ggx_albedo_fit(f3 F0, float alpha, float cos_theta) {
+    const float c = std::fabs(cos_theta);
+    const float c2 = c * c, c3 = c2 * c;
+    const float a3 = (alpha * alpha) * alpha;
+    const float bias_num = (0.995367f + -1.38839f * c) + (-0.24751f + 1.97442f * c) * alpha;
+    const float bias_den = ((1.0f + 2.68132f * c) + 52.366f * c3) + ((16.0932f + -3.98452f * c) + 59.3013f * c3) * alpha +
+                           ((-5.18731f + 255.259f * c) + 2544.07f * c3) * a3;
+    const float scale_num = (-0.0564526f + 3.82901f * c) + (16.91f + -11.0303f * c) * alpha;
+    const float scale_den = ((1.0f + 4.11118f * c2) + -1.37886f * c3) + ((19.3254f + -28.9947f * c2) + 16.9514f * c3) * alpha +
+                            ((0.545386f + 96.0994f * c2) + -79.4492f * c3) * a3;
+    const float luma = ((F0.x + F0.y) + F0.z) * (1.0f / 3.0f);
+    const float bias = std::fmax(0.0f, (bias_num / bias_den) * clampf(luma * 50.0f, 0.0f, 1.0f));
+    const float scale = std::fmax(0.0f, scale_num / scale_den);
+    return {F0.x * scale + bias, F0.y * scale + bias, F0.z * scale + bias};
+}
+
+// Sheen hemispherical reflectance (brdf-sheen.slang:92-118): 1 - the fitted transmitted fraction
+// e + x^d + a / (1 + b x^c), x = 1 + 20 cos theta, linear between the alpha rows 0, 0.1, .., 1.
+inline float sheen_directional_albedo(float cos_theta, float alpha) {
+    static const float k[11][5] = {
+        {1.01108f, 75.2286f, -9.4671f, 0.000973445f, -1.01325f}, {8.44447f, 0.0135054f, 1.11637f, 0.479139f, -9.30873f},
+        {5.67124f, 0.0194668f, 1.12489f, 0.453551f, -6.53104f},  {4.50086f, 0.0237758f, 1.09376f, 0.423665f, -5.36521f},
+        {3.77781f, 0.0273286f, 1.0666f, 0.400267f, -4.65601f},   {3.35338f, 0.0307392f, 0.989189f, 0.371074f, -4.24089f},
+        {3.39605f, 0.0308828f, 0.845455f, 0.333813f, -4.28662f}, {3.62583f, 0.0303531f, 0.79754f, 0.325588f, -4.51725f},
+        {4.11524f, 0.0172797f, 0.701166f, 0.275242f, -5.04394f}, {4.4012f, 0.0159469f, 0.685892f, 0.270221f, -5.32997f},
+        {5.8912f, 0.0171461f, 0.66366f, 0.288735f, -6.78867f}};
+    const float x = clampf(cos_theta, 0.0f, 1.0f) * 20.0f + 1.0f;
+    const int lo = int(clampf(std::floor(alpha * 10.0f), 0.0f, 10.0f));
+    const int hi = std::min(lo + 1, 10);
+    const float w_lo = float(hi) - alpha * 10.0f;
+    auto row = [&](int r) { return (k[r][4] + o_pow(x, k[r][3])) + k[r][0] / (1.0f + k[r][1] * o_pow(x, k[r][2])); };
+    const float t = row(lo) * w_lo + row(hi) * (1.0f - w_lo);
+    return 1.0f - clampf(t, 0.0f, 1.0f);
+}
+
+// ---------------------------------------------------------------------------------------
 // Lobes
 
 // brdf-lambert-diffuse.slang:9-61
@@ -180,6 +222,7 @@ struct Lambert {
         if (std::fmin(wi.z, wo.z) < BSDF_MIN_COS_THETA) return none();
         return some(wo, INV_PI * wo.z, albedo, LOBE_DIFFUSE);
     }
+    f3 get_albedo(f3) const { return albedo; } // brdf-lambert-diffuse.slang:50-52
 };
 
 // brdf-ggx.slang:331-412
@@ -208,6 +251,9 @@ struct GGX {
         const float w = g2_over_widotn(wo, wi, a) / g1_over_widotn(wi, a);
         return some(wo, p, {w, w, w}, LOBE_GLOSSY);
     }
+    // brdf-ggx.slang:399-406 (isotropic fit at sqrt(ax * ay))
+    f3 preintegrated(f3 F0, f3 wi) const { return ggx_albedo_fit(F0, std::sqrt(a.x * a.y), wi.z); }
+    f3 get_albedo(f3 wi) const { return preintegrated(f3{1.0f, 1.0f, 1.0f}, wi); }
 };
 
 // brdf-ggx.slang:416-503
@@ -247,6 +293,7 @@ struct GGXBTDF {
         const float w = pow2(eta) * g2_over_widotn(std::fabs(wo.z), wi.z, a2) / g1_over_widotn(wi.z, a2);
         return some(wo, p, tint * w, LOBE_GLOSSY);
     }
+    f3 get_albedo(f3) const { return pow2(eta) * tint; } // brdf-ggx.slang:496-498
 };
 
 // bsdf-rough-dielectric.slang:13-120
@@ -308,6 +355,13 @@ struct RoughDielectric {
         const float w = pow2(eta) * g2_over_widotn(std::fabs(wo.z), wi.z, a2) / g1_over_widotn(wi.z, a2);
         return some(wo, p, tint * w, LOBE_GLOSSY);
     }
+    // bsdf-rough-dielectric.slang:111-115
+    f3 get_albedo(f3 wi) const {
+        const float f0 = pow2((1.0f - eta) / (1.0f + eta));
+        const f3 spec = GGX{{alpha, alpha}}.preintegrated(f3{f0, f0, f0}, wi);
+        const f3 t = pow2(eta) * tint;
+        return {spec.x + (1.0f - spec.x) * t.x, spec.y + (1.0f - spec.y) * t.y, spec.z + (1.0f - spec.z) * t.z};
+    }
 };
 
 // brdf-sheen.slang:9-90 (Estevez & Kulla 2017)
@@ -356,6 +410,7 @@ struct Sheen {
         const float p = pdf(wi, wo);
         return some(wo, p, eval(wi, wo) / p, LOBE_DIFFUSE);
     }
+    f3 get_albedo(f3 wi) const { return color * sheen_directional_albedo(wi.z, alpha); } // brdf-sheen.slang:83-85
 };
 
 // The `alpha` property (bsdf.slang:26): lobe width; combinators interpolate squared widths.
@@ -385,6 +440,12 @@ template <class Layer, class Base> struct FresnelMix {
         const float p = pdf(wi, s.wo);
         return some(s.wo, p, eval(wi, s.wo) / p, s.lobe);
     }
+    // bsdf-fresnel-mix.slang:50-53
+    f3 get_albedo(f3 wi) const {
+        const f3 spec = layer.preintegrated(f3{f0, f0, f0}, wi);
+        const f3 b = base.get_albedo(wi);
+        return {spec.x + (1.0f - spec.x) * b.x, spec.y + (1.0f - spec.y) * b.y, spec.z + (1.0f - spec.z) * b.z};
+    }
 };
 
 // bsdf-mix.slang:6-65
@@ -409,6 +470,7 @@ template <class B0, class B1> struct Mix {
         const f3 mix_eval = lerp(s.weight * s.pdf, b1.eval(wi, s.wo), weight);
         return some(s.wo, mix_pdf, mix_eval / mix_pdf, s.lobe);
     }
+    f3 get_albedo(f3 wi) const { return lerp(b0.get_albedo(wi), b1.get_albedo(wi), weight); } // bsdf-mix.slang:51-54
 };
 
 // bsdf-conductor-fresnel.slang:6-50
@@ -428,6 +490,7 @@ template <class Inner> struct Conductor {
         s.weight = s.weight * fresnel_schlick(dot(wi, h), f0);
         return s;
     }
+    f3 get_albedo(f3 wi) const { return inner.preintegrated(f0, wi); } // bsdf-conductor-fresnel.slang:39-41
 };
 
 // OpenPBR tree (openpbr-material.slang:19-25)

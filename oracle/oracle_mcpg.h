@@ -566,11 +566,13 @@ __attribute__((visibility("default"))) uint64_t orc_mcpg_render(void* scp, void*
             PCG rng{pcg3d16_(x, y, frame)};
             const Ray primary_ray = camera_ray(*cam, x, y, w, h);
             const ShadingPoint primary_sp = get_shading_point(sc, primary_ray, unpack_hit(gbuffer[i]), 0u, 1.0f, true);
+            const f3 demod =
+                prm->demodulate_albedo ? demodulation_albedo(primary_ray, primary_sp) : f3{1.f, 1.f, 1.f};
             f3 irr{0, 0, 0};
             float m2 = 0.f;
             for (int s = 0; s < prm->spp; s++) {
-                const f3 contrib = mcpg_trace_path(sc, M, *prm, primary_ray, primary_sp, cpos, pcam, frame, rng, segs) /
-                                   f3{1.f, 1.f, 1.f};
+                const f3 contrib =
+                    mcpg_trace_path(sc, M, *prm, primary_ray, primary_sp, cpos, pcam, frame, rng, segs) / demod;
                 if (finite3(contrib)) {
                     irr = irr + contrib;
                     const float l = yuv_luminance(contrib);
