@@ -2,7 +2,7 @@
 """Benchmark: path segments per second of one pt frame on one MI355X.
 
 Workload (the headline metric's configuration): 1920x1080, 64 spp, max path length 8, the seeded
-procedural scene of 1,999,880 triangles (256 displaced tori instances + an open room), 256
+procedural scene of 1,999,896 triangles (256 displaced tori instances in an enclosing room), 256
 materials (Lambert / GGX conductor / dielectric 1.5), 2048x1024 lat-long sky with a sun of
 5e4:1 dynamic range. A step = one frame: GBuffer pass + pt (BSDF-only estimator, the
 reference's). Inputs (scene, BVH, buffers) are resident in HBM before timing starts.
@@ -34,12 +34,58 @@ RAY_IO_BYTES = 32 + 16  # queued ray in, hit out
 # HBM bytes per dispatch of the intersect kernel, measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
 # passes over this workload and calibrated against a plain copy (python -m merian_amd.profiling traffic)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_trace_queue_full_1080p.traffic.json")
+TRAFFIC_WORKLOAD = "full_1080p enclosed room: 1920x1080, 64 spp, max path length 8, 1999896 triangles, BSDF-only pt"
 # autotune result of the workload, applied before timing (re-run the search with --retune)
 TUNING_CACHE = os.path.join(ROOT, "merian_amd", "tuning", "full_1080p.json")
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def roofline(kernel: str, avg_ms: float, launches: int, algorithmic_bytes: float, profile: str,
+             match: dict | None = None, extra: dict | None = None) -> dict:
+    """Bandwidth roofline of one kernel. `achieved` is the HBM-side traffic per dispatch that a
+    rocprofv3 profile of the same workload measured (FETCH_SIZE + WRITE_SIZE, each scaled by what
+    the counter reads for a plain copy of known size; profiles/*.traffic.json) over the kernel's
+    average launch time measured here with HIP events: the bytes the kernel actually moves. The
+    algorithmic bytes (every node, triangle and ray the traversal touches, most of them served by
+    L2) stay beside it as a diagnostic. The profile applies only if it was taken with the settings
+    in `match` (tuning, workload); otherwise `achieved` falls back to the algorithmic rate and says
+    so."""
+    alg_gbs = algorithmic_bytes / (avg_ms * 1e-3) / 1e9
+    r = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s", "kernel": kernel, "avg_launch_ms": round(avg_ms, 4),
+         "launches": launches, "algorithmic_bytes_per_launch": round(algorithmic_bytes),
+         "algorithmic_gbs": round(alg_gbs, 1)}
+    tj = None
+    if os.path.exists(profile):
+        with open(profile) as f:
+            tj = json.load(f)
+        for k, v in (match or {}).items():
+            pv = tj.get(k)
+            if isinstance(v, dict):
+                if not isinstance(pv, dict) or any(kk not in v or abs(float(pv[kk]) - float(v[kk])) > 1e-6 for kk in pv):
+                    tj = None
+                    break
+            elif pv != v:
+                tj = None
+                break
+    if tj is not None:
+        traffic = float(tj["traffic_bytes_per_dispatch"])
+        achieved = traffic / (avg_ms * 1e-3) / 1e9
+        r.update({"achieved": round(achieved, 1), "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": round(traffic),
+                  "basis": "HBM-side bytes per dispatch (FETCH_SIZE + WRITE_SIZE, copy-calibrated) / launch time",
+                  "traffic_over_algorithmic": round(traffic / algorithmic_bytes, 3),
+                  "profile": os.path.relpath(profile, ROOT)})
+        for k in ("l2_hit_pct", "valu_busy_pct", "valu_lane_util_pct", "mall_hit_pct", "dram_read_gbs"):
+            if tj.get(k) is not None:
+                r[k] = tj[k]
+    else:
+        r.update({"achieved": round(alg_gbs, 1), "frac": round(alg_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                  "basis": "algorithmic bytes (no profile of this workload and tuning)"})
+    if extra:
+        r.update(extra)
+    return r
 
 
 def cpu_baseline(scene, spec, frame, target_s: float, threads: int):
@@ -94,10 +140,17 @@ def cpu_small(threads: int, min_s: float = 3.0) -> dict:
             "sample": f"{f} whole frames ({segs} segments, {t:.2f} s)"}
 
 
-def small_leg(ctx, frames: int = 64) -> dict:
+SMALL_TUNING = os.path.join(ROOT, "merian_amd", "tuning", "small_512.json")
+SMALL_TRAFFIC = os.path.join(ROOT, "profiles", "k_trace_queue_small_512.traffic.json")
+SMALL_WORKLOAD = "small: 8x8 icospheres (81,922 triangles), 512x512, 4 spp, max path length 4, constant environment"
+
+
+def small_leg(ctx, frames: int = 64, retune: bool = False) -> dict:
     """The small plain-pt configuration: 8x8 icospheres on a ground plane (81,922 triangles),
     constant white environment, 512x512, 4 spp, max path length 4; a step = GBuffer + pt frame
-    with its statistics read back. The scene is autotuned on the view first (mrt_tune)."""
+    with its statistics read back. Settings: the cached mrt_tune result of this view
+    (merian_amd/tuning/small_512.json; --retune searches again). Roofline of its intersect kernel
+    from its own profile (profiles/k_trace_queue_small_512.traffic.json)."""
     import torch
     from merian_amd import scenes
     from merian_amd.runtime import pt_params
@@ -109,29 +162,47 @@ def small_leg(ctx, frames: int = 64) -> dict:
     out = torch.empty(W * H * 4, dtype=torch.float32, device="cuda")
     params = pt_params(spp=4, max_path_length=4, record_signature=False)
     t0 = time.perf_counter()
-    ds.tune(ctx.camera(spec, 0), W, H, params, budget_ms=20000.0)  # mrt_tune on this view, untimed
+    if retune or not os.path.exists(SMALL_TUNING):
+        ds.tune(ctx.camera(spec, 0), W, H, params, budget_ms=20000.0)  # mrt_tune on this view, untimed
+        source = f"mrt_tune now ({time.perf_counter() - t0:.1f} s)"
+    else:
+        with open(SMALL_TUNING) as f:
+            _apply(ds, json.load(f)["tuning"])
+        source = "cached mrt_tune result (merian_amd/tuning/small_512.json)"
     tuning = {k: v for k, v in ds.tuning().as_dict().items() if k not in ("ms_per_frame", "_reserved")}
-    tune_s = time.perf_counter() - t0
 
-    def frame(f):
+    def frame(f, count=False):
         cam = ctx.camera(spec, f)
         ctx.gbuffer_dev(ds, cam, W, H, gb.data_ptr())
+        params.count_traversal = int(count)
         return ctx.pt_render_dev(ds, params, cam, f, W, H, gb.data_ptr(), out.data_ptr(), stats=True)
 
     frame(0)
     torch.cuda.synchronize()
-    segs = 0
+    segs, ms_trace, launches = 0, 0.0, 0
     t0 = time.perf_counter()
     for f in range(1, frames + 1):
         st = frame(f)
         segs += st.primary_segments + st.continuation_segments
+        ms_trace += st.ms_trace
+        launches += st.trace_launches
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    st_c = frame(1, count=True)
+    cont = max(1, st_c.continuation_segments)
+    node_bytes = NODE_BYTES[int(tuning["bvh_width"])]
+    bytes_per_seg = (st_c.node_visits * node_bytes + st_c.triangle_tests * TRI_BYTES) / cont + RAY_IO_BYTES
+    cont_timed = segs - frames * W * H
+    avg_ms = ms_trace / max(1, launches)
+    roof = roofline(f"k_trace<false, SrcQueue, SinkQueue, false, {int(tuning['bvh_width'])}>", avg_ms, launches,
+                    bytes_per_seg * cont_timed / max(1, launches), SMALL_TRAFFIC,
+                    match={"tuning": tuning, "workload": SMALL_WORKLOAD},
+                    extra={"bytes_per_segment": round(bytes_per_seg, 1)})
     ds.destroy()
     res = {"workload": f"{sc.triangle_count} triangles, 512x512, 4 spp, max path length 4, constant environment",
            "mseg_per_s": round(segs / dt / 1e6, 1), "ms_per_frame": round(dt * 1e3 / frames, 3),
-           "segments_per_frame": segs // frames, "frames": frames, "tuning": tuning,
-           "tuning_source": f"mrt_tune before timing ({tune_s:.1f} s)"}
+           "segments_per_frame": segs // frames, "frames": frames, "tuning": tuning, "tuning_source": source,
+           "roofline": roof}
     log(f"small leg: {res}")
     return res
 
@@ -157,82 +228,125 @@ def cpu_ray_query(threads: int) -> dict:
     return out
 
 
-def guided_leg(ctx, scene, L: int) -> dict:
-    """Equal-time relMSE of guided vs plain pt on a 480x270 view of the 1080p scene (same camera,
-    lower resolution), against a 16384-spp plain render of that view. The guided side trains
-    online during its frames; both sides get the same wall time (timed around whole frames).
-    The Markov-chain guiding (MCPG, the non-learned comparator) is measured the same way, with
-    the MCPG example graph's grid settings, learning online from an empty state."""
+def guided_leg(ctx, scene, L: int, spp: int = 64, ref_frames: int = 256, budget_s: float = 3.0,
+               train_frames: int = 4) -> dict:
+    """Learned guiding at the 1080p workload's settings (1920x1080, 64 spp frames, L = 8): relMSE at
+    equal device time against a 16384-spp plain render (256 frames of 64 spp).
+
+    The reference's own noise is measured and subtracted: per pixel and channel its variance is the
+    scatter of its frame means over the number of frames, and for an image x independent of the
+    reference E[(x - ref)^2] = E[(x - mu)^2] + var(ref), so relMSE = mean(((x - ref)^2 - var(ref)) /
+    (ref^2 + 1e-2)); the raw figures are reported beside it.
+
+    Protocol: the guide trains online on its first `train_frames` frames (16 Adam steps each) and
+    then renders with the weights frozen; every frame's samples are kept (one-sample MIS keeps each
+    frame unbiased whatever the weights). Frames render on one stream with device-side accumulation;
+    device time is taken with HIP events around each frame. The plain pt and the Markov-chain
+    guiding comparator (MCPG, learning online from an empty state with the MCPG example graph's
+    grid settings) render frames until they have used the same device time."""
     import numpy as np
     import torch
     from merian_amd import abi, scenes
     from merian_amd.runtime import Guide, Mcpg, guide_config, mcpg_config, pt_params
-    w, h = 480, 270
-    spec = scenes.CameraSpec(scene.camera.eye, scene.camera.target, scene.camera.up, scene.camera.vfov, w, h)
+    spec = scene.camera
+    w, h = spec.width, spec.height
     ds = ctx.scene(scene)
+    with open(TUNING_CACHE) as f:
+        _apply(ds, json.load(f)["tuning"])
     cam = ctx.camera(spec, 0)
+    strm = torch.cuda.current_stream().cuda_stream
     gb = torch.empty(w * h * 16, dtype=torch.uint8, device="cuda")
     out = torch.empty(w * h * 4, dtype=torch.float32, device="cuda")
-    acc = torch.zeros(w * h * 4, dtype=torch.float64, device="cuda")
     ctx.gbuffer_dev(ds, cam, w, h, gb.data_ptr())
+    ctx.synchronize()
 
-    def run(frames, params, guide=None, train=True, base=0, mcpg=None, keep=True):
-        """Renders `frames` frames on the library's context stream; the device is synchronised
-        after each frame before torch (another stream) accumulates it. Returns the mean image
-        (None with keep=False: the frames' samples are discarded) and the wall time."""
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        acc.zero_()
-        for f in range(frames):
-            if mcpg is not None:
-                mcpg.render_dev(ds, params, cam, base + f, w, h, gb.data_ptr(), out.data_ptr())
-            elif guide is None:
-                ctx.pt_render_dev(ds, params, cam, base + f, w, h, gb.data_ptr(), out.data_ptr())
-            else:
-                guide.render_dev(ds, params, cam, base + f, w, h, gb.data_ptr(), out.data_ptr(), train=train)
-            torch.cuda.synchronize()
-            if keep:
-                acc.add_(out)
-        torch.cuda.synchronize()
-        img = (acc / frames).cpu().numpy().reshape(h, w, 4) if keep else None
-        return img, time.perf_counter() - t0
+    def frame(f, params, guide=None, train=False, mcpg=None):
+        if mcpg is not None:
+            mcpg.render_dev(ds, params, cam, f, w, h, gb.data_ptr(), out.data_ptr(), stream=strm)
+        elif guide is None:
+            ctx.pt_render_dev(ds, params, cam, f, w, h, gb.data_ptr(), out.data_ptr(), stream=strm)
+        else:
+            guide.render_dev(ds, params, cam, f, w, h, gb.data_ptr(), out.data_ptr(), train=train, stream=strm)
 
-    plain = pt_params(spp=64, max_path_length=L, record_signature=False)
-    ref, t_ref = run(256, plain, base=100000)  # 16384 spp
-    _, t8 = run(8, plain, base=300000, keep=False)
-    ms_plain = 1e3 * t8 / 8
-    # guided: TRAIN online-training frames (timed, samples discarded), then RENDER frames with
-    # the guide frozen; plain gets the same total wall time
-    TRAIN, RENDER = 16, 96
+    def run(params, base, budget=None, frames=None, **kw):
+        """Frames until `frames` are done or `budget` seconds of device time are used; returns the
+        mean image, frames and device seconds."""
+        acc = torch.zeros(w * h * 4, dtype=torch.float64, device="cuda")
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        used, n = 0.0, 0
+        while (frames is None or n < frames) and (budget is None or used < budget):
+            e0.record()
+            frame(base + n, params, **kw)
+            acc += out
+            e1.record()
+            e1.synchronize()
+            used += e0.elapsed_time(e1) * 1e-3
+            n += 1
+        return (acc / n).cpu().numpy().reshape(h, w, 4), n, used
+
+    plain = pt_params(spp=spp, max_path_length=L, record_signature=False)
+    # reference: mean and per-pixel variance of the mean from the frames' scatter
+    t0 = time.perf_counter()
+    s1 = torch.zeros(w * h * 4, dtype=torch.float64, device="cuda")
+    s2 = torch.zeros_like(s1)
+    for f in range(ref_frames):
+        frame(100000 + f, plain)
+        o = out.double()
+        s1 += o
+        s2 += o * o
+    torch.cuda.synchronize()
+    t_ref = time.perf_counter() - t0
+    mean = s1 / ref_frames
+    var_mean = (s2 / ref_frames - mean * mean).clamp_min(0) / (ref_frames - 1)
+    ref = mean.cpu().numpy().reshape(h, w, 4)[..., :3]
+    var_ref = var_mean.cpu().numpy().reshape(h, w, 4)[..., :3]
+    del s1, s2, mean, var_mean
+    den = ref ** 2 + 1e-2
+
+    def rel(x):
+        d = (x[..., :3] - ref) ** 2
+        return float(np.mean(d / den)), float(np.mean((d - var_ref) / den))
+
+    # guided: online training on the first frames, then frozen; every frame kept
     g = Guide(ctx, ds, guide_config(seed=1, train_steps=16, guided_bounces=3))
-    gp = pt_params(spp=64, max_path_length=L, mode=abi.MODE_GUIDED, record_signature=False)
-    _, t_train = run(TRAIN, gp, guide=g, train=True, keep=False)
-    gimg, t_render = run(RENDER, gp, guide=g, train=False, base=1000)
+    gp = pt_params(spp=spp, max_path_length=L, mode=abi.MODE_GUIDED, record_signature=False)
+    gacc = torch.zeros(w * h * 4, dtype=torch.float64, device="cuda")
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t_g, n_g, t_train = 0.0, 0, 0.0
+    while n_g < train_frames or t_g < budget_s:
+        train = n_g < train_frames
+        e0.record()
+        frame(1000 + n_g, gp, guide=g, train=train)
+        gacc += out
+        e1.record()
+        e1.synchronize()
+        dt = e0.elapsed_time(e1) * 1e-3
+        t_g += dt
+        t_train += dt if train else 0.0
+        n_g += 1
+    gimg = (gacc / n_g).cpu().numpy().reshape(h, w, 4)
     g.close()
-    t_g = t_train + t_render
-    frames_p = max(1, int(round(t_g / (t8 / 8))))
-    pimg, t_p = run(frames_p, plain)
-    # MCPG: learning frames (timed, samples discarded), then frames that keep learning, as the
-    # node runs; the MCPG example graph's grid settings
+    del gacc
+    pimg, n_p, t_p = run(plain, 200000, budget=t_g)
     m = Mcpg(ctx, mcpg_config(mc_tan_alpha_half=0.003, mc_distribution_dimension=1.5, lc_tan_alpha_half=0.012,
                               lc_buffer_size=8000009, lc_probe_count=2))
-    _, tm_learn = run(4, plain, mcpg=m, base=5000, keep=False)
-    mimg, tm_render = run(8, plain, mcpg=m, base=6000)
+    mimg, n_m, t_m = run(plain, 5000, budget=t_g, mcpg=m)
     m.close()
-    t_m = tm_learn + tm_render
-    frames_pm = max(1, int(round(t_m / (t8 / 8))))
-    pmimg, t_pm = run(frames_pm, plain, base=20000)
     ds.destroy()
-    rel = lambda x: float(np.mean((x[..., :3] - ref[..., :3]) ** 2 / (ref[..., :3] ** 2 + 1e-2)))
-    res = {"workload": f"{w}x{h} view of the 1080p scene, L={L}, 64 spp per frame; reference 16384 spp plain",
-           "protocol": f"guided (vertices 0-2): {TRAIN} online-training frames (train_steps 16, samples discarded) "
-                       f"then {RENDER} frames with the guide frozen, all timed; plain: every frame of the same wall time",
-           "relmse_guided": rel(gimg), "relmse_plain": rel(pimg), "seconds_guided": round(t_g, 3),
-           "seconds_guided_training": round(t_train, 3), "seconds_plain": round(t_p, 3),
-           "ms_per_frame_guided": round(1e3 * t_render / RENDER, 2), "ms_per_frame_plain": round(ms_plain, 2),
-           "frames_guided": RENDER, "frames_plain": frames_p,
-           "relmse_mcpg": rel(mimg), "seconds_mcpg": round(t_m, 3), "frames_mcpg": 8,
-           "relmse_plain_at_mcpg_time": rel(pmimg), "frames_plain_at_mcpg_time": frames_pm,
+    (rg_raw, rg), (rp_raw, rp), (rm_raw, rm) = rel(gimg), rel(pimg), rel(mimg)
+    ref_noise = float(np.mean(var_ref / den))
+    res = {"workload": f"{w}x{h}, {spp} spp per frame, L={L}, {scene.triangle_count:,} triangles (the 1080p scene); "
+                       f"reference {ref_frames * spp} spp plain",
+           "protocol": f"equal device time (HIP events per frame): guided trains online on its first {train_frames} "
+                       f"frames (16 Adam steps each, guide on vertices 0-2) then renders frozen, all frames kept; "
+                       f"plain and MCPG render frames until the same device time",
+           "relmse_guided": rg, "relmse_plain": rp, "guided_over_plain": round(rg / rp, 4) if rp > 0 else None,
+           "relmse_guided_raw": rg_raw, "relmse_plain_raw": rp_raw, "reference_noise_relmse": ref_noise,
+           "seconds_guided": round(t_g, 3), "seconds_guided_training": round(t_train, 3),
+           "seconds_plain": round(t_p, 3), "frames_guided": n_g, "frames_plain": n_p,
+           "ms_per_frame_guided_frozen": round(1e3 * (t_g - t_train) / max(1, n_g - train_frames), 2),
+           "ms_per_frame_plain": round(1e3 * t_p / n_p, 2),
+           "relmse_mcpg": rm, "relmse_mcpg_raw": rm_raw, "seconds_mcpg": round(t_m, 3), "frames_mcpg": n_m,
            "mcpg_note": "MCPG guides rough glass: the GGX BTDF evaluation it shares with the plain pt is non-zero "
                         "where its pdf is 0, so guided refractions there carry unbounded weights (fireflies)",
            "reference_seconds": round(t_ref, 2)}
@@ -240,27 +354,27 @@ def guided_leg(ctx, scene, L: int) -> dict:
     return res
 
 
-def serving_leg(ctx, tune: bool = True) -> dict:
-    """Serving mix: Poisson arrivals over 4 resident scenes (82k..1.0M triangles), resolutions
-    {256^2, 512^2, 1024x768, 1920x1080}, spp {1, 4, 16, 64}, L in 4..8; arrival rate 80 % of the
-    capacity measured with a closed batch. Latency = submit -> result in the host buffer."""
+def serving_leg(ctx, single_frame_mseg: float, n_jobs: int = 400, tune_ms: float = 1500.0) -> dict:
+    """Serving mix: Poisson arrivals over 8 resident scenes (seeds 101-108, 103,682 to 1,999,896
+    triangles), resolutions {256^2, 512^2, 1024x768, 1920x1080}, spp {1, 4, 16, 64}, L in 4..8,
+    admitted by the server's HBM budget, at 80 % of the capacity a closed batch of 32 jobs reaches.
+    Latency = submit -> result in the caller's host buffer; p50 / p99 are nearest-rank over the
+    `n_jobs` timed jobs. A job's host buffer is dropped as soon as it is done."""
     import numpy as np
     from merian_amd import scenes
     from merian_amd.runtime import Server, pt_params
     rng = np.random.default_rng(3)
-    resident = [scenes.plain_small(seed=101), scenes.full_scene(seed=102, instances=26, env_size=(512, 256)),
-                scenes.plain_small(seed=105, subdiv=4), scenes.full_scene(seed=103, instances=128, env_size=(512, 256))]
+    t0 = time.perf_counter()
+    resident = scenes.serving_scenes(8)
     dss = [ctx.scene(s) for s in resident]
+    log(f"serving: 8 scenes resident ({time.perf_counter() - t0:.1f} s): {[s.triangle_count for s in resident]}")
     res_choices = [(256, 256), (512, 512), (1024, 768), (1920, 1080)]
-    tune_s = 0.0
-    if tune:
-        # each resident scene is autotuned once, on a 1024x768 view at 16 spp, L = 6 (untimed)
-        t0 = time.perf_counter()
-        for sc, d in zip(resident, dss):
-            spec = scenes.CameraSpec(sc.camera.eye, sc.camera.target, sc.camera.up, sc.camera.vfov, 1024, 768)
-            d.tune(ctx.camera(spec, 0), 1024, 768, pt_params(spp=16, max_path_length=6, record_signature=False),
-                   budget_ms=5000.0)
-        tune_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for sc, d in zip(resident, dss):  # each scene autotuned once on a 1024x768, 16 spp, L = 6 view (untimed)
+        spec = scenes.CameraSpec(sc.camera.eye, sc.camera.target, sc.camera.up, sc.camera.vfov, 1024, 768)
+        d.tune(ctx.camera(spec, 0), 1024, 768, pt_params(spp=16, max_path_length=6, record_signature=False),
+               budget_ms=tune_ms)
+    tune_s = time.perf_counter() - t0
 
     def job():
         k = int(rng.integers(len(dss)))
@@ -272,37 +386,42 @@ def serving_leg(ctx, tune: bool = True) -> dict:
         f = int(rng.integers(1 << 16))
         return dss[k], pt_params(spp=spp, max_path_length=L, record_signature=False), ctx.camera(spec, f), f, w, h
 
-    with Server(ctx, max_slots=4) as srv:
-        batch = [job() for _ in range(16)]
+    with Server(ctx, max_slots=8) as srv:
+        batch = [job() for _ in range(32)]
         t0 = time.perf_counter()
-        subs = [srv.submit(*j) for j in batch]
-        for s_ in subs:
+        for s_ in [srv.submit(*j) for j in batch]:
             s_.wait()
         capacity = len(batch) / (time.perf_counter() - t0)
         srv.reset_stats()
         lam = 0.8 * capacity
-        jobs = [job() for _ in range(48)]
+        jobs = [job() for _ in range(n_jobs)]
         arrivals = np.cumsum(rng.exponential(1.0 / lam, len(jobs)))
-        subs = []
+        pending = []
         t0 = time.perf_counter()
         for a, j in zip(arrivals, jobs):
-            dt = a - (time.perf_counter() - t0)
-            if dt > 0:
-                time.sleep(dt)
-            subs.append(srv.submit(*j))
-        for s_ in subs:
-            s_.wait()
+            while True:  # drop finished jobs (and their host buffers) while waiting for the next arrival
+                pending = [p_ for p_ in pending if p_.poll().state < 2]
+                dt = a - (time.perf_counter() - t0)
+                if dt <= 0:
+                    break
+                time.sleep(min(dt, 0.002))
+            pending.append(srv.submit(*j))
+        for p_ in pending:
+            p_.wait()
         wall = time.perf_counter() - t0
         st = srv.stats()
     for d in dss:
         d.destroy()
-    res = {"workload": "48 Poisson-arrival jobs over 4 resident scenes (81,922 / 203,120 / 327,682 / "
-                       "999,944 triangles), res {256^2, 512^2, 1024x768, 1920x1080}, spp {1,4,16,64}, L 4..8",
+    mseg = st.segments / wall / 1e6
+    res = {"workload": f"{n_jobs} Poisson-arrival jobs over 8 resident scenes "
+                       f"({', '.join(f'{s.triangle_count:,}' for s in resident)} triangles), res {{256^2, 512^2, "
+                       "1024x768, 1920x1080}, spp {1,4,16,64}, L 4..8, 8 slots",
            "arrival_rate_per_s": round(lam, 2), "capacity_jobs_per_s": round(capacity, 2),
            "p50_ms": round(st.p50_ms, 2), "p99_ms": round(st.p99_ms, 2), "max_ms": round(st.max_ms, 2),
-           "jobs_per_s": round(st.completed / wall, 2), "mseg_per_s": round(st.segments / wall / 1e6, 1),
-           "peak_in_flight": st.peak_in_flight, "completed": st.completed,
-           "tuning_source": f"mrt_tune per scene before the mix ({tune_s:.1f} s)" if tune else "defaults"}
+           "mean_ms": round(st.mean_ms, 2), "jobs_per_s": round(st.completed / wall, 2),
+           "mseg_per_s": round(mseg, 1), "fraction_of_single_frame": round(mseg / single_frame_mseg, 3),
+           "peak_in_flight": st.peak_in_flight, "completed": st.completed, "wall_s": round(wall, 2),
+           "tuning_source": f"mrt_tune per scene before the mix ({tune_s:.1f} s)"}
     log(f"serving leg: {res}")
     return res
 
@@ -383,9 +502,13 @@ def ray_query_leg(ctx, reps: int = 3, retune: bool = False) -> dict:
         nodes, tris = ctx.trace_count_dev(ds, d_r.data_ptr(), n, d_hits.data_ptr())
         ms = _time_rays(ctx, ds, d_r, n, d_hits, stream, reps)
         bytes_per_ray = (nodes * NODE_BYTES[settings["bvh_width"]] + tris * TRI_BYTES) / n + RAY_IO_BYTES
+        roof = roofline(f"k_trace<false, SrcRays, SinkHits, false, {settings['bvh_width']}>", ms, reps,
+                        bytes_per_ray * n, os.path.join(ROOT, "profiles", f"k_trace_rays_{name}.traffic.json"),
+                        match={"tuning": settings, "workload": f"ray_query {name}"})
         res[name] = {"mrays_per_s": round(n / ms / 1e3, 1), "ms_per_launch": round(ms, 3),
                      "node_visits_per_ray": round(nodes / n, 2), "triangle_tests_per_ray": round(tris / n, 2),
-                     "algorithmic_gbs": round(bytes_per_ray * n / ms / 1e6, 1), "tuning": settings}
+                     "algorithmic_gbs": round(bytes_per_ray * n / ms / 1e6, 1), "tuning": settings,
+                     "roofline": roof}
     ds.destroy()
     log(f"ray-query leg: {res}")
     return res
@@ -406,10 +529,16 @@ def main():
     ap.add_argument("--default-tuning", action="store_true", help="render with the untuned defaults")
     ap.add_argument("--no-guided", action="store_true", help="skip the equal-time guided vs plain leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-mix leg")
+    ap.add_argument("--serving-jobs", type=int, default=400, help="timed jobs of the serving mix")
     ap.add_argument("--no-ray-query", action="store_true", help="skip the ray-query microbenchmark leg")
     ap.add_argument("--no-small", action="store_true", help="skip the small plain-pt leg (512^2, 4 spp, L = 4)")
     ap.add_argument("--retune-rays", action="store_true",
                     help="grid-search the ray-query leg's traversal settings instead of using RAY_QUERY_TUNING")
+    ap.add_argument("--only", choices=["guided", "serving", "rays", "small"],
+                    help="run one leg alone and print its JSON (profiling passes)")
+    ap.add_argument("--small-frames", type=int, default=64)
+    ap.add_argument("--ray-reps", type=int, default=3)
+    ap.add_argument("--ref-frames", type=int, default=256, help="64-spp frames of the guided leg's reference")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; --gpus N > 1 is not supported", file=sys.stderr)
@@ -420,6 +549,21 @@ def main():
 
     from merian_amd import abi, scenes
     from merian_amd.runtime import Context, pt_params
+
+    if args.only:
+        torch.cuda.init()
+        ctx = Context(0)
+        if args.only == "small":
+            leg = small_leg(ctx, frames=args.small_frames, retune=args.retune)
+        elif args.only == "rays":
+            leg = ray_query_leg(ctx, reps=args.ray_reps, retune=args.retune_rays)
+        elif args.only == "serving":
+            leg = serving_leg(ctx, 1.0, n_jobs=args.serving_jobs)
+        else:
+            leg = guided_leg(ctx, scenes.full_scene(seed=2), args.max_path_length, ref_frames=args.ref_frames)
+        ctx.close()
+        print(json.dumps({args.only: leg}), flush=True)
+        return
 
     t0 = time.perf_counter()
     scene = scenes.full_scene(seed=2)
@@ -499,50 +643,32 @@ def main():
     cont_timed = segs - args.steps * W * H
     bytes_per_launch = bytes_per_seg * cont_timed / max(1, launches)
     avg_ms = ms_trace / max(1, launches)
-    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    traffic, prof = None, {}
-    if os.path.exists(TRAFFIC_FILE):
-        with open(TRAFFIC_FILE) as f:
-            tj = json.load(f)
-        # the profile holds for the tuning it was taken with
-        prof_t = tj.get("tuning")
-        same = (prof_t is None) == (tuning is None) and (
-            prof_t is None or all(k in tuning and abs(float(prof_t[k]) - float(tuning[k])) < 1e-6 for k in prof_t))
-        if same:
-            traffic = tj["traffic_bytes_per_dispatch"]
-            # measured HBM rate of the same kernel, and its issue figures, from the same profile
-            prof = {"hbm_gbs": round(traffic / (avg_ms * 1e-3) / 1e9, 1),
-                    "hbm_frac": round(traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                    "l2_hit_pct": tj.get("l2_hit_pct"), "valu_busy_pct": tj.get("valu_busy_pct"),
-                    "valu_lane_util_pct": tj.get("valu_lane_util_pct"),
-                    "profile": os.path.relpath(TRAFFIC_FILE, ROOT)}
-    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "algorithmic_bytes_per_launch": round(bytes_per_launch),
-            "kernel": "k_trace_queue", "avg_launch_ms": round(avg_ms, 4), "launches": launches,
-            "bytes_per_segment": round(bytes_per_seg, 1), "node_bytes": node_bytes,
-            "node_visits_per_segment": round(st_c.node_visits / cont, 2),
-            "triangle_tests_per_segment": round(st_c.triangle_tests / cont, 2), **prof}
+    roof = roofline(f"k_trace<false, SrcQueue, SinkQueue, false, {int(ds.tuning().bvh_width)}>", avg_ms, launches,
+                    bytes_per_launch, TRAFFIC_FILE,
+                    match={"tuning": tuning, "workload": TRAFFIC_WORKLOAD},
+                    extra={"bytes_per_segment": round(bytes_per_seg, 1), "node_bytes": node_bytes,
+                           "node_visits_per_segment": round(st_c.node_visits / cont, 2),
+                           "triangle_tests_per_segment": round(st_c.triangle_tests / cont, 2)})
 
     line = {"metric": METRIC, "value": round(value, 2), "unit": "Mpath-segments/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1e3 / args.steps, 2),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
             "data": "synthetic",
             "config": {"workload": f"full_1080p: {W}x{H}, {args.spp} spp, max path length {args.max_path_length}, "
-                                   f"{scene.triangle_count} triangles (256 displaced tori + open room), "
+                                   f"{scene.triangle_count} triangles (256 displaced tori in an enclosing room with a window and a skylight), "
                                    f"{len(scene.materials)} materials, 2048x1024 lat-long sky, BSDF-only pt",
                        "segments_per_step": segs // args.steps, "trace_ms_per_step": round(ms_trace / args.steps, 2),
                        "tuning": tuning, "tuning_source": tuning_source},
             "roofline": roof, "pcie_inclusive": pcie}
     ds.destroy()
     if not args.no_guided:
-        line["guided"] = guided_leg(ctx, scene, args.max_path_length)
+        line["guided"] = guided_leg(ctx, scene, args.max_path_length, ref_frames=args.ref_frames)
     if not args.no_serving:
-        line["serving"] = serving_leg(ctx)
+        line["serving"] = serving_leg(ctx, value, n_jobs=args.serving_jobs)
     if not args.no_ray_query:
-        line["ray_query"] = ray_query_leg(ctx, retune=args.retune_rays)
+        line["ray_query"] = ray_query_leg(ctx, reps=args.ray_reps, retune=args.retune_rays)
     if not args.no_small:
-        line["small"] = small_leg(ctx)
+        line["small"] = small_leg(ctx, frames=args.small_frames, retune=args.retune)
     ctx.close()
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)

@@ -624,6 +624,23 @@ MRT_API void mrt_string_free(char* s);
  * material entry, the environment, the camera (eye, forward, up, vfov radians, width,
  * height) and the loader's warnings. */
 MRT_API mrt_status mrt_pbrt_inspect(const char* path, char** out_json, char** errors);
+/* The loaded scene as a host description (no device work), owned by the library until
+ * mrt_pbrt_desc_free: what mrt_pbrt_scene_create uploads, for callers that inspect or edit a
+ * scene before uploading it. The camera is the file's (camera_from_file) or the default camera
+ * framing the scene's bounds (pbrt_scene.cpp:994-1003), whose image size is the caller's and whose
+ * aspect ratio is fixed. */
+typedef struct mrt_pbrt_desc {
+    mrt_scene_desc scene;
+    int32_t has_camera;
+    int32_t camera_from_file;
+    float eye[3], forward[3], up[3];
+    float vfov;             /* radians, vertical */
+    float aspect;           /* default camera: fixed aspect ratio; 0 = the image's */
+    uint32_t width, height; /* the file's film resolution */
+    uint32_t warning_count; /* loader warnings (listed by mrt_pbrt_inspect) */
+} mrt_pbrt_desc;
+MRT_API mrt_status mrt_pbrt_load_desc(const char* path, mrt_pbrt_desc** out, char** errors);
+MRT_API void mrt_pbrt_desc_free(mrt_pbrt_desc* desc);
 /* Loads and uploads the scene (as mrt_scene_create). */
 MRT_API mrt_status mrt_pbrt_scene_create(mrt_ctx* ctx, const char* path, mrt_scene** out);
 /* The graph runner (merian-graph-run's command line over this library's node host);

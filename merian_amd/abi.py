@@ -89,6 +89,14 @@ class SceneDesc(C.Structure):
                 ("env", Env)]
 
 
+class PbrtDesc(C.Structure):
+    """mrt_pbrt_desc: a loaded pbrt scene as a host description plus its camera."""
+    _fields_ = [("scene", SceneDesc), ("has_camera", C.c_int32), ("camera_from_file", C.c_int32),
+                ("eye", C.c_float * 3), ("forward", C.c_float * 3), ("up", C.c_float * 3), ("vfov", C.c_float),
+                ("aspect", C.c_float), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("warning_count", C.c_uint32)]
+
+
 class Camera(C.Structure):
     _fields_ = [("position", C.c_float * 3), ("U", C.c_float * 3), ("V", C.c_float * 3),
                 ("W", C.c_float * 3), ("near_plane", C.c_float), ("far_plane", C.c_float),
@@ -277,6 +285,8 @@ EXPORTS = [
     ("mrt_graph_cli_help", C.c_int32, [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p)]),
     ("mrt_string_free", None, [C.c_void_p]),
     ("mrt_pbrt_inspect", C.c_int32, [C.c_char_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
+    ("mrt_pbrt_load_desc", C.c_int32, [C.c_char_p, C.POINTER(C.POINTER(PbrtDesc)), C.POINTER(C.c_void_p)]),
+    ("mrt_pbrt_desc_free", None, [C.POINTER(PbrtDesc)]),
     ("mrt_pbrt_scene_create", C.c_int32, [C.c_void_p, C.c_char_p, C.POINTER(C.c_void_p)]),
     ("mrt_graph_run_main", C.c_int32, [C.c_int32, C.POINTER(C.c_char_p)]),
     ("mrt_alloc_dev", C.c_int32, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
@@ -305,6 +315,12 @@ def load_library(path: os.PathLike | None = None) -> C.CDLL:
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
+    if path is None and (PKG_DIR / "csrc").is_dir():
+        # the library embeds a hash of the sources it was built from: a stale build fails loudly
+        from .build import source_hash
+        version = lib.mrt_version().decode()
+        if not version.endswith("sources " + source_hash()):
+            raise RuntimeError(f"{p} ({version}) was built from other sources: run __graft_entry__.build()")
     if path is None:
         _LIB = lib
     return lib

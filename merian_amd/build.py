@@ -29,9 +29,22 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-pthread", "-fvisibility=hidden", "-ffp
           "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-value", "-Wno-unused-result"]
 
 
-def _compile(src: Path) -> Path:
+def source_hash() -> str:
+    """SHA-256 (first 16 hex digits) over the library's sources, headers and this build script:
+    the library embeds it (mrt_version) so a stale build is detected at load time."""
+    import hashlib
+    h = hashlib.sha256()
+    files = sorted([*CSRC.glob("*.hip"), *CSRC.glob("*.cpp"), *CSRC.glob("*.h"), *(PKG / "host").glob("*.cpp"),
+                    *(PKG / "host").glob("*.h"), PKG.parent / "include" / "merian_amd.h", Path(__file__).resolve()])
+    for f in files:
+        h.update(f.name.encode())
+        h.update(f.read_bytes())
+    return h.hexdigest()[:16]
+
+
+def _compile(src: Path, defines: list[str] = ()) -> Path:
     obj = OBJ / (src.parent.name + "_" + src.name + ".o")
-    cmd = [HIPCC, *COMMON, "-c", str(src), "-o", str(obj)]
+    cmd = [HIPCC, *COMMON, *defines, "-c", str(src), "-o", str(obj)]
     if src.suffix == ".cpp":
         cmd[1:1] = ["-x", "hip"]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -44,10 +57,11 @@ def _compile(src: Path) -> Path:
 def build(jobs: int = 8) -> Path:
     OBJ.mkdir(exist_ok=True)
     srcs = [PKG / s for s in SOURCES]
+    defines = [f'-DMRT_SOURCE_HASH="{source_hash()}"']
     with ThreadPoolExecutor(max_workers=max(1, min(jobs, 16))) as ex:
-        objs = list(ex.map(_compile, srcs))
+        objs = list(ex.map(lambda s: _compile(s, defines), srcs))
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", str(tmp), *map(str, objs)]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", str(tmp), *map(str, objs), "-lz"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)

@@ -170,7 +170,12 @@ hipError_t scratch_end(mrt_ctx* c, hipStream_t st) {
 
 extern "C" {
 
-MRT_API const char* mrt_version(void) { return "merian_amd 0.1 (gfx950, HIP)"; }
+#ifndef MRT_SOURCE_HASH
+#define MRT_SOURCE_HASH "unknown"
+#endif
+// The build script passes a hash of the library's sources; loaders compare it with the sources
+// beside the library to detect a stale build.
+MRT_API const char* mrt_version(void) { return "merian_amd 0.2 (gfx950, HIP) sources " MRT_SOURCE_HASH; }
 
 MRT_API uint32_t merian_node_plugin_abi_version(void) { return 1u; }
 

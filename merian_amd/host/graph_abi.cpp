@@ -191,6 +191,8 @@ MRT_API mrt_status mrt_pbrt_inspect(const char* path, char** out_json, char** er
             cam["forward"] = a3(sc.camera.forward);
             cam["up"] = a3(sc.camera.up);
             cam["vfov"] = double(sc.camera.vfov);
+            cam["from_file"] = sc.camera.has_resolution;
+            if (sc.camera.aspect > 0.f) cam["aspect"] = double(sc.camera.aspect);
         }
         cam["width"] = int64_t(sc.camera.width);
         cam["height"] = int64_t(sc.camera.height);
@@ -205,6 +207,44 @@ MRT_API mrt_status mrt_pbrt_inspect(const char* path, char** out_json, char** er
         return MRT_ERR_INVALID;
     }
 }
+
+namespace {
+// mrt_pbrt_desc followed by the loaded scene its pointers refer to
+struct PbrtDescHolder {
+    mrt_pbrt_desc pub;
+    pbrt::Scene scene;
+};
+} // namespace
+
+MRT_API mrt_status mrt_pbrt_load_desc(const char* path, mrt_pbrt_desc** out, char** errors) {
+    if (!path || !out) return MRT_ERR_INVALID;
+    *out = nullptr;
+    try {
+        auto* h = new PbrtDescHolder();
+        h->scene = pbrt::load(path);
+        const pbrt::Camera& c = h->scene.camera;
+        h->pub.scene = h->scene.desc();
+        h->pub.has_camera = c.present ? 1 : 0;
+        h->pub.camera_from_file = c.present && c.has_resolution ? 1 : 0;
+        for (int k = 0; k < 3; k++) {
+            h->pub.eye[k] = c.present ? c.eye[k] : 0.f;
+            h->pub.forward[k] = c.present ? c.forward[k] : 0.f;
+            h->pub.up[k] = c.present ? c.up[k] : 0.f;
+        }
+        h->pub.vfov = c.vfov;
+        h->pub.aspect = c.aspect;
+        h->pub.width = c.width;
+        h->pub.height = c.height;
+        h->pub.warning_count = uint32_t(h->scene.warnings.size());
+        *out = &h->pub;
+        return MRT_OK;
+    } catch (const std::exception& e) {
+        put_errors(errors, {e.what()});
+        return MRT_ERR_INVALID;
+    }
+}
+
+MRT_API void mrt_pbrt_desc_free(mrt_pbrt_desc* desc) { delete reinterpret_cast<PbrtDescHolder*>(desc); }
 
 MRT_API mrt_status mrt_pbrt_scene_create(mrt_ctx* ctx, const char* path, mrt_scene** out) {
     if (!ctx || !path || !out) return MRT_ERR_INVALID;

@@ -156,9 +156,9 @@ class PbrtSceneNode : public Node {
     SceneResource res_;
 };
 
-// Primary rays + closest hit into the 16-byte GBuffer (gbuffer.cpp:103-146). The extent is the
-// scene camera's resolution when the scene has a camera (gbuffer.cpp:104-112), else
-// width x height; pbrt cameras use the R2 jitter sequence.
+// Primary rays + closest hit into the 16-byte GBuffer (gbuffer.cpp:103-146) and the albedo
+// texture. The extent is the scene camera's resolution when the file has a camera
+// (gbuffer.cpp:104-112), else width x height; pbrt cameras use the R2 jitter sequence.
 class GBufferNode : public Node {
   public:
     std::vector<std::string> inputs() const override { return {"scene"}; }
@@ -178,8 +178,9 @@ class GBufferNode : public Node {
         const SceneResource& s = std::get<SceneResource>(*it->second);
         GBufferResource g;
         g.scene = s.scene;
-        g.width = s.camera.present ? s.camera.width : width_;
-        g.height = s.camera.present ? s.camera.height : height_;
+        const bool file_extent = s.camera.present && s.camera.has_resolution;
+        g.width = file_extent ? s.camera.width : width_;
+        g.height = file_extent ? s.camera.height : height_;
         float eye[3] = {3, 3, 3}, target[3] = {0, 0, 0}, up[3] = {0, 1, 0};
         float vfov = float(60.0 * M_PI / 180.0);
         if (s.camera.present) {
@@ -190,9 +191,10 @@ class GBufferNode : public Node {
             }
             vfov = s.camera.vfov;
         }
+        const float aspect = s.camera.present && s.camera.aspect > 0.f ? s.camera.aspect : float(g.width) / float(g.height);
         check(info.ctx,
-              mrt_camera_look_at(eye, target, up, vfov, float(g.width) / float(g.height), 0.01f, 1000.f,
-                                 MRT_JITTER_R2, uint32_t(info.iteration), &g.camera),
+              mrt_camera_look_at(eye, target, up, vfov, aspect, 0.01f, 1000.f, MRT_JITTER_R2, uint32_t(info.iteration),
+                                 &g.camera),
               "camera");
         buf_.ensure(info.ctx, uint64_t(g.width) * g.height * sizeof(mrt_packed_hit));
         albedo_.ensure(info.ctx, uint64_t(g.width) * g.height * 16);

@@ -1,6 +1,9 @@
 // pbrt-v4 subset loader (pbrt.h).
 #include "pbrt.h"
 
+#include <zlib.h>
+
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -119,9 +122,10 @@ struct M4 {
         const double a = m[0], b = m[1], c = m[2], d = m[4], e = m[5], f = m[6], g = m[8], h = m[9], i = m[10];
         const double co[9] = {e * i - f * h, -(d * i - f * g), d * h - e * g, -(b * i - c * h), a * i - c * g,
                               -(a * h - b * g), b * f - c * e, -(a * f - c * d), a * e - b * d};
-        // cofactor matrix = det * inverse transpose
+        // cofactor matrix = det * inverse transpose: the sign of det restores the orientation
+        const double sgn = det3() < 0.0 ? -1.0 : 1.0;
         double o[3];
-        for (int r = 0; r < 3; r++) o[r] = co[3 * r] * in[0] + co[3 * r + 1] * in[1] + co[3 * r + 2] * in[2];
+        for (int r = 0; r < 3; r++) o[r] = sgn * (co[3 * r] * in[0] + co[3 * r + 1] * in[1] + co[3 * r + 2] * in[2]);
         const double l = std::sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]);
         for (int r = 0; r < 3; r++) out[r] = l > 1e-30 ? float(o[r] / l) : 0.f;
     }
@@ -149,6 +153,219 @@ struct M4 {
         return true;
     }
 };
+
+// ------------------------------------------------------------------------------------------
+// PLY meshes (ascii, binary little / big endian; .gz through zlib)
+
+struct PlyMesh {
+    std::vector<float> pos, nrm, uv;
+    std::vector<uint32_t> tri;
+};
+
+bool read_all(const std::string& path, std::string& out) {
+    const bool gz = path.size() > 3 && path.compare(path.size() - 3, 3, ".gz") == 0;
+    if (!gz) {
+        std::ifstream in(path, std::ios::binary);
+        if (!in) return false;
+        out.assign((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+        return true;
+    }
+    gzFile f = gzopen(path.c_str(), "rb");
+    if (!f) return false;
+    out.clear();
+    char buf[1 << 16];
+    int n;
+    while ((n = gzread(f, buf, sizeof buf)) > 0) out.append(buf, size_t(n));
+    const bool ok = n == 0;
+    gzclose(f);
+    return ok;
+}
+
+struct PlyProperty {
+    std::string name;
+    char type = 'f';      // c C s S i I f d: int8 uint8 int16 uint16 int32 uint32 float32 float64
+    char count_type = 0;  // list properties: the type of the element count
+};
+struct PlyElement {
+    std::string name;
+    size_t count = 0;
+    std::vector<PlyProperty> props;
+};
+
+char ply_type(const std::string& t) {
+    static const std::map<std::string, char> types = {
+        {"char", 'c'},  {"int8", 'c'},   {"uchar", 'C'}, {"uint8", 'C'},  {"short", 's'},   {"int16", 's'},
+        {"ushort", 'S'}, {"uint16", 'S'}, {"int", 'i'},   {"int32", 'i'},  {"uint", 'I'},    {"uint32", 'I'},
+        {"float", 'f'}, {"float32", 'f'}, {"double", 'd'}, {"float64", 'd'}};
+    auto it = types.find(t);
+    return it == types.end() ? 0 : it->second;
+}
+size_t ply_size(char t) { return t == 'c' || t == 'C' ? 1 : t == 's' || t == 'S' ? 2 : t == 'd' ? 8 : 4; }
+
+// Values of the PLY body in file order.
+struct PlyBody {
+    const std::string& data;
+    size_t p;
+    int format; // 0 ascii, 1 binary little endian, 2 binary big endian
+    bool failed = false;
+    double next(char t) {
+        if (format == 0) {
+            while (p < data.size() && std::isspace(static_cast<unsigned char>(data[p]))) p++;
+            const char* b = data.c_str() + p;
+            char* e = nullptr;
+            const double v = std::strtod(b, &e);
+            if (e == b) failed = true;
+            p += size_t(e - b);
+            return v;
+        }
+        const size_t n = ply_size(t);
+        if (p + n > data.size()) {
+            failed = true;
+            return 0.0;
+        }
+        unsigned char raw[8];
+        std::memcpy(raw, data.data() + p, n);
+        p += n;
+        if (format == 2) std::reverse(raw, raw + n);
+        switch (t) {
+        case 'c': return double(int8_t(raw[0]));
+        case 'C': return double(raw[0]);
+        case 's': { int16_t v; std::memcpy(&v, raw, 2); return v; }
+        case 'S': { uint16_t v; std::memcpy(&v, raw, 2); return v; }
+        case 'i': { int32_t v; std::memcpy(&v, raw, 4); return v; }
+        case 'I': { uint32_t v; std::memcpy(&v, raw, 4); return v; }
+        case 'f': { float v; std::memcpy(&v, raw, 4); return v; }
+        default: { double v; std::memcpy(&v, raw, 8); return v; }
+        }
+    }
+};
+
+bool read_ply(const std::string& path, PlyMesh& m, std::string& why) {
+    std::string data;
+    if (!read_all(path, data)) {
+        why = "cannot read the file";
+        return false;
+    }
+    std::istringstream header(data);
+    std::string line;
+    std::getline(header, line);
+    if (line.compare(0, 3, "ply") != 0) {
+        why = "not a PLY file";
+        return false;
+    }
+    int format = -1;
+    std::vector<PlyElement> elems;
+    size_t body = std::string::npos;
+    while (std::getline(header, line)) {
+        if (!line.empty() && line.back() == '\r') line.pop_back();
+        std::istringstream ls(line);
+        std::string kw;
+        ls >> kw;
+        if (kw == "format") {
+            std::string f;
+            ls >> f;
+            format = f == "ascii" ? 0 : f == "binary_little_endian" ? 1 : f == "binary_big_endian" ? 2 : -1;
+        } else if (kw == "element") {
+            PlyElement e;
+            ls >> e.name >> e.count;
+            elems.push_back(e);
+        } else if (kw == "property" && !elems.empty()) {
+            std::string t;
+            ls >> t;
+            PlyProperty pr;
+            if (t == "list") {
+                std::string ct, vt;
+                ls >> ct >> vt >> pr.name;
+                pr.count_type = ply_type(ct);
+                pr.type = ply_type(vt);
+                if (!pr.count_type) pr.type = 0;
+            } else {
+                pr.type = ply_type(t);
+                ls >> pr.name;
+            }
+            if (!pr.type) {
+                why = "unknown property type in '" + line + "'";
+                return false;
+            }
+            elems.back().props.push_back(pr);
+        } else if (kw == "end_header") {
+            body = size_t(header.tellg());
+            break;
+        }
+    }
+    if (format < 0 || body == std::string::npos) {
+        why = "no format or no end_header";
+        return false;
+    }
+    PlyBody in{data, body, format};
+    size_t nv = 0;
+    bool got_vertices = false, got_faces = false;
+    for (const PlyElement& e : elems) {
+        const bool is_vertex = e.name == "vertex", is_face = e.name == "face";
+        int ip[3] = {-1, -1, -1}, in_[3] = {-1, -1, -1}, it[2] = {-1, -1}, iv = -1;
+        for (size_t k = 0; k < e.props.size(); k++) {
+            const std::string& n = e.props[k].name;
+            const int kk = int(k);
+            if (n == "x") ip[0] = kk; else if (n == "y") ip[1] = kk; else if (n == "z") ip[2] = kk;
+            else if (n == "nx") in_[0] = kk; else if (n == "ny") in_[1] = kk; else if (n == "nz") in_[2] = kk;
+            else if (n == "u" || n == "s" || n == "texture_u" || n == "texture_s") it[0] = kk;
+            else if (n == "v" || n == "t" || n == "texture_v" || n == "texture_t") it[1] = kk;
+            else if (n == "vertex_indices" || n == "vertex_index") iv = kk;
+        }
+        if (is_vertex && (ip[0] < 0 || ip[1] < 0 || ip[2] < 0)) {
+            why = "vertices without x, y, z";
+            return false;
+        }
+        const bool with_n = in_[0] >= 0 && in_[1] >= 0 && in_[2] >= 0, with_uv = it[0] >= 0 && it[1] >= 0;
+        std::vector<double> row(e.props.size());
+        std::vector<uint32_t> poly;
+        for (size_t r = 0; r < e.count && !in.failed; r++) {
+            poly.clear();
+            for (size_t k = 0; k < e.props.size(); k++) {
+                const PlyProperty& pr = e.props[k];
+                if (pr.count_type) {
+                    const size_t cnt = size_t(in.next(pr.count_type));
+                    for (size_t q = 0; q < cnt && !in.failed; q++) {
+                        const double v = in.next(pr.type);
+                        if (is_face && int(k) == iv) poly.push_back(uint32_t(v));
+                    }
+                } else {
+                    row[k] = in.next(pr.type);
+                }
+            }
+            if (is_vertex) {
+                for (int a = 0; a < 3; a++) m.pos.push_back(float(row[size_t(ip[a])]));
+                if (with_n)
+                    for (int a = 0; a < 3; a++) m.nrm.push_back(float(row[size_t(in_[a])]));
+                if (with_uv)
+                    for (int a = 0; a < 2; a++) m.uv.push_back(float(row[size_t(it[a])]));
+            } else if (is_face) {
+                for (size_t q = 1; q + 1 < poly.size(); q++) // polygons as triangle fans
+                    if (poly[0] < nv && poly[q] < nv && poly[q + 1] < nv) m.tri.insert(m.tri.end(), {poly[0], poly[q], poly[q + 1]});
+            }
+        }
+        if (in.failed) {
+            why = "the body ends early or holds a malformed value (element '" + e.name + "')";
+            return false;
+        }
+        if (is_vertex) {
+            nv = e.count;
+            got_vertices = true;
+        }
+        if (is_face) {
+            if (!got_vertices || iv < 0) {
+                why = "faces without vertex_indices, or before the vertices";
+                return false;
+            }
+            got_faces = true;
+        }
+    }
+    if (!got_vertices || !got_faces) {
+        why = "no vertex or no face element";
+        return false;
+    }
+    return true;
+}
 
 void normalize3(double v[3]) {
     const double l = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
@@ -315,10 +532,12 @@ struct Loader {
         return m;
     }
 
+    // Shape "trianglemesh" (and "loopsubdiv", loaded unsubdivided): P, indices (optional for a
+    // single triangle), N, uv or st (pbrt_scene.cpp:533-575).
     void trianglemesh(const Params& p) {
         auto P = p.find("P");
         if (P == p.end() || P->second.nums.size() < 9) {
-            warn("tm_invalid", "invalid trianglemesh skipped");
+            warn("tm_invalid", "a trianglemesh without at least one triangle of \"P\" was skipped");
             return;
         }
         const size_t nv = P->second.nums.size() / 3;
@@ -328,37 +547,112 @@ struct Loader {
             for (size_t k = 0; k + 2 < I->second.nums.size(); k += 3) {
                 const uint32_t a = uint32_t(I->second.nums[k]), b = uint32_t(I->second.nums[k + 1]),
                                c = uint32_t(I->second.nums[k + 2]);
-                if (a < nv && b < nv && c < nv) {
-                    tri.push_back(a);
-                    tri.push_back(b);
-                    tri.push_back(c);
-                }
+                if (a < nv && b < nv && c < nv) tri.insert(tri.end(), {a, b, c});
             }
         } else if (nv == 3) {
             tri = {0, 1, 2};
         } else {
-            warn("tm_invalid", "invalid trianglemesh skipped");
+            warn("tm_invalid", "a trianglemesh without \"indices\" and more than three points was skipped");
             return;
         }
-        if (tri.empty()) return;
-        // world-space positions (pretransformed)
-        std::vector<float> pos(3 * nv);
-        for (size_t v = 0; v < nv; v++) {
-            const float in[3] = {float(P->second.nums[3 * v]), float(P->second.nums[3 * v + 1]),
-                                 float(P->second.nums[3 * v + 2])};
-            st.ctm.point(in, &pos[3 * v]);
-        }
-        std::vector<float> nrm(3 * nv, 0.f);
+        std::vector<float> pos(3 * nv), nrm, uv;
+        for (size_t k = 0; k < 3 * nv; k++) pos[k] = float(P->second.nums[k]);
         auto N = p.find("N");
         if (N != p.end() && N->second.nums.size() == 3 * nv) {
-            for (size_t v = 0; v < nv; v++) {
-                const float in[3] = {float(N->second.nums[3 * v]), float(N->second.nums[3 * v + 1]),
-                                     float(N->second.nums[3 * v + 2])};
-                st.ctm.normal(in, &nrm[3 * v]);
+            nrm.resize(3 * nv);
+            for (size_t k = 0; k < 3 * nv; k++) nrm[k] = float(N->second.nums[k]);
+        }
+        auto UV = p.find("uv");
+        if (UV == p.end()) UV = p.find("st");
+        if (UV != p.end() && UV->second.nums.size() >= 2 * nv) {
+            uv.resize(2 * nv);
+            for (size_t k = 0; k < 2 * nv; k++) uv[k] = float(UV->second.nums[k]);
+        }
+        emit_mesh(pos, nrm, uv, tri);
+    }
+
+    // Shape "sphere": radius, zmin, zmax, phimax tessellated into 64 slices x 32 stacks with
+    // analytic normals (pbrt_scene.cpp:692-733).
+    void sphere(const Params& p) {
+        const float r = pf(p, "radius", 1.f);
+        const float zmin = std::clamp(pf(p, "zmin", -r), -r, r), zmax = std::clamp(pf(p, "zmax", r), -r, r);
+        const float phi_max = std::clamp(pf(p, "phimax", 360.f), 0.f, 360.f) * (float(M_PI) / 180.f);
+        const float th0 = std::acos(std::clamp(zmax / r, -1.f, 1.f)), th1 = std::acos(std::clamp(zmin / r, -1.f, 1.f));
+        constexpr int S = 64, T = 32;
+        std::vector<float> pos, nrm, uv;
+        for (int j = 0; j <= T; j++) {
+            const float theta = th0 + (th1 - th0) * float(j) / T;
+            for (int i = 0; i <= S; i++) {
+                const float phi = phi_max * float(i) / S;
+                const float n[3] = {std::sin(theta) * std::cos(phi), std::sin(theta) * std::sin(phi), std::cos(theta)};
+                for (int k = 0; k < 3; k++) {
+                    pos.push_back(r * n[k]);
+                    nrm.push_back(n[k]);
+                }
+                uv.push_back(float(i) / S);
+                uv.push_back(1.f - float(j) / T);
             }
-        } else {
-            // area-weighted vertex normals (unnormalised face cross products)
-            for (size_t t = 0; t < tri.size(); t += 3) {
+        }
+        std::vector<uint32_t> tri;
+        for (int j = 0; j < T; j++)
+            for (int i = 0; i < S; i++) {
+                const uint32_t a = uint32_t(j * (S + 1) + i), b = a + S + 1;
+                tri.insert(tri.end(), {a, b, a + 1, a + 1, b, b + 1});
+            }
+        emit_mesh(pos, nrm, uv, tri);
+    }
+
+    // Shape "disk": height, radius, innerradius, phimax as a ring of 64 slices (pbrt_scene.cpp:735-765).
+    void disk(const Params& p) {
+        const float h = pf(p, "height", 0.f), r = pf(p, "radius", 1.f);
+        const float ri = std::clamp(pf(p, "innerradius", 0.f), 0.f, r);
+        const float phi_max = std::clamp(pf(p, "phimax", 360.f), 0.f, 360.f) * (float(M_PI) / 180.f);
+        constexpr int S = 64;
+        std::vector<float> pos, nrm, uv;
+        for (int i = 0; i <= S; i++) {
+            const float phi = phi_max * float(i) / S;
+            const float dx = std::cos(phi), dy = std::sin(phi);
+            pos.insert(pos.end(), {r * dx, r * dy, h, ri * dx, ri * dy, h});
+            nrm.insert(nrm.end(), {0.f, 0.f, 1.f, 0.f, 0.f, 1.f});
+            uv.insert(uv.end(), {float(i) / S, 0.f, float(i) / S, 1.f});
+        }
+        std::vector<uint32_t> tri;
+        for (uint32_t i = 0; i < uint32_t(S); i++) {
+            const uint32_t a = 2 * i;
+            tri.insert(tri.end(), {a, a + 2, a + 1, a + 1, a + 2, a + 3});
+        }
+        emit_mesh(pos, nrm, uv, tri);
+    }
+
+    // Shape "plymesh": a PLY file (ascii or binary, optionally gzip-compressed) relative to the
+    // scene file's directory (pbrt_scene.cpp:577-690,790-798).
+    void plymesh(const Params& p, const std::string& base) {
+        const std::string file = ps_(p, "filename", "");
+        if (file.empty()) {
+            warn("ply_nofile", "a plymesh without \"filename\" was skipped");
+            return;
+        }
+        const std::string path = file[0] == '/' ? file : base + "/" + file;
+        PlyMesh m;
+        std::string why;
+        if (!read_ply(path, m, why)) {
+            warn("ply_" + path, "plymesh '" + path + "' was skipped: " + why);
+            return;
+        }
+        emit_mesh(m.pos, m.nrm, m.uv, m.tri);
+    }
+
+    // Adds one shape: object-space positions, normals (empty: area-weighted from the positions,
+    // pbrt_scene.cpp:55-70) and uvs (pbrt's convention, origin bottom-left; empty: zero), placed
+    // by the current transform with the current material and area light (pbrt_scene.cpp:72-86,
+    // 776-830). Positions are baked into world space.
+    void emit_mesh(const std::vector<float>& pos, std::vector<float> nrm, const std::vector<float>& uv,
+                   const std::vector<uint32_t>& tri) {
+        if (tri.empty()) return;
+        const size_t nv = pos.size() / 3;
+        if (nrm.size() != 3 * nv) {
+            nrm.assign(3 * nv, 0.f);
+            for (size_t t = 0; t < tri.size(); t += 3) { // unnormalised face cross products
                 const float* a = &pos[3 * tri[t]];
                 const float* b = &pos[3 * tri[t + 1]];
                 const float* c = &pos[3 * tri[t + 2]];
@@ -370,20 +664,24 @@ struct Loader {
                     for (int a2 = 0; a2 < 3; a2++) nrm[3 * tri[t + k] + a2] += f[a2];
             }
         }
-        auto UV = p.find("uv");
-        if (UV == p.end()) UV = p.find("st");
-        const bool has_uv = UV != p.end() && UV->second.nums.size() >= 2 * nv;
         const uint32_t vbase = uint32_t(sc.vertices.size());
         for (size_t v = 0; v < nv; v++) {
             mrt_vertex vx{};
-            std::memcpy(vx.position, &pos[3 * v], 12);
-            f3 n{nrm[3 * v], nrm[3 * v + 1], nrm[3 * v + 2]};
+            st.ctm.point(&pos[3 * v], vx.position);
+            float on[3] = {nrm[3 * v], nrm[3 * v + 1], nrm[3 * v + 2]};
+            const float ol = std::sqrt(on[0] * on[0] + on[1] * on[1] + on[2] * on[2]);
+            float wn[3] = {0.f, 1.f, 0.f};
+            if (ol > 1e-20f) {
+                for (int k = 0; k < 3; k++) on[k] /= ol;
+                st.ctm.normal(on, wn);
+            }
+            f3 n{wn[0], wn[1], wn[2]};
             const float l = std::sqrt(n.x * n.x + n.y * n.y + n.z * n.z);
             n = l > 1e-20f ? f3{n.x / l, n.y / l, n.z / l} : f3{0.f, 1.f, 0.f};
             vx.encoded_normal = encode_normal(n);
-            if (has_uv) { // pbrt's uv origin is bottom-left
-                vx.uv[0] = float_to_half(float(UV->second.nums[2 * v]));
-                vx.uv[1] = float_to_half(1.f - float(UV->second.nums[2 * v + 1]));
+            if (uv.size() >= 2 * nv) { // pbrt's uv origin is bottom-left, the texture's top-left
+                vx.uv[0] = float_to_half(uv[2 * v]);
+                vx.uv[1] = float_to_half(1.f - uv[2 * v + 1]);
             }
             sc.vertices.push_back(vx);
         }
@@ -623,7 +921,14 @@ struct Loader {
                 const std::string type = string_arg();
                 const Params p = params();
                 if (type == "trianglemesh" || type == "loopsubdiv") {
+                    if (type == "loopsubdiv") warn("loopsubdiv", "loopsubdiv control meshes are loaded unsubdivided");
                     trianglemesh(p);
+                } else if (type == "sphere") {
+                    sphere(p);
+                } else if (type == "disk") {
+                    disk(p);
+                } else if (type == "plymesh") {
+                    plymesh(p, base);
                 } else {
                     warn("shape_" + type, "shape '" + type + "' is unsupported, skipping");
                 }
@@ -672,8 +977,66 @@ struct Loader {
             const double fov = fov_deg * M_PI / 180.0;
             c.vfov = float(film_w >= film_h ? fov : 2.0 * std::atan(std::tan(0.5 * fov) / aspect));
             c.present = true;
+        } else if (!sc.vertices.empty()) {
+            default_camera(c);
         }
         if (sc.materials.empty()) sc.materials.push_back(openpbr_defaults());
+    }
+
+    // No camera in the file: a camera at (3, 3, 3) looking at the origin, 60 degrees, 16:9, then
+    // fitted to the scene's bounds (pbrt_scene.cpp:994-1003) as Camera::look_at_bounding_box does
+    // (camera.cpp:373-423): start above the box looking at its centre, then three times pan so the
+    // projected box is centred and back off until all eight corners are inside 85 % of the field
+    // of view. Float arithmetic, as the reference camera's.
+    void default_camera(Camera& c) {
+        float lo[3], hi[3];
+        for (int k = 0; k < 3; k++) lo[k] = hi[k] = sc.vertices[0].position[k];
+        for (const mrt_vertex& v : sc.vertices)
+            for (int k = 0; k < 3; k++) {
+                lo[k] = std::min(lo[k], v.position[k]);
+                hi[k] = std::max(hi[k], v.position[k]);
+            }
+        const f3 up{0.f, 1.f, 0.f};
+        const float vfov = float(60.0 * M_PI / 180.0), aspect = 1920.f / 1080.f;
+        const float tan_hy = std::tan(vfov * 0.85f * 0.5f), tan_hx = tan_hy * aspect;
+        const f3 centre{(lo[0] + hi[0]) * 0.5f, (lo[1] + hi[1]) * 0.5f, (lo[2] + hi[2]) * 0.5f};
+        f3 pos = f3{1.3f, 1.3f, 1.3f} * (hi[1] + 1e-3f), target = centre;
+        const auto corner = [&](int i) { return f3{(i & 1) ? hi[0] : lo[0], (i & 2) ? hi[1] : lo[1], (i & 4) ? hi[2] : lo[2]}; };
+        for (int iter = 0; iter < 3; iter++) {
+            const f3 fwd = normalize(target - pos);
+            const f3 r = normalize(cross(fwd, up));
+            const f3 u = cross(r, fwd);
+            float px0 = 3.4e38f, px1 = -3.4e38f, py0 = 3.4e38f, py1 = -3.4e38f;
+            for (int i = 0; i < 8; i++) {
+                const f3 v = corner(i) - pos;
+                const float d = dot(v, fwd), px = dot(v, r) / d, py = dot(v, u) / d;
+                px0 = std::min(px0, px);
+                px1 = std::max(px1, px);
+                py0 = std::min(py0, py);
+                py1 = std::max(py1, py);
+            }
+            const float depth = dot(target - pos, fwd);
+            const f3 shift = ((px0 + px1) * 0.5f * depth) * r + ((py0 + py1) * 0.5f * depth) * u;
+            pos = pos + shift;
+            target = target + shift;
+            float dist = 0.f;
+            for (int i = 0; i < 8; i++) {
+                const f3 v = corner(i) - target;
+                const float z = dot(v, fwd);
+                dist = std::max(dist, std::max(std::fabs(dot(v, r)) / tan_hx - z, std::fabs(dot(v, u)) / tan_hy - z));
+            }
+            pos = target - fwd * dist;
+        }
+        const f3 fwd = normalize(target - pos);
+        for (int k = 0; k < 3; k++) {
+            c.eye[k] = (&pos.x)[k];
+            c.forward[k] = (&fwd.x)[k];
+            c.up[k] = (&up.x)[k];
+        }
+        c.vfov = vfov;
+        c.aspect = aspect;
+        c.has_resolution = false;
+        c.present = true;
     }
 };
 

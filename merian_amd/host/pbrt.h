@@ -7,8 +7,10 @@
 // conductor (reflectance or eta/k), dielectric and mix mapped onto OpenPBR (:365-487, roughness
 // remap :332-364); AreaLightSource "diffuse" (emission); LightSource "infinite" with "rgb L"
 // or a PFM image (equal-area octahedral, clamp to edge, :924-968); Shape "trianglemesh" with
-// P, indices, N, uv (normals computed by area weighting when absent, :55-70, :533-575);
-// Include / Import. Geometry is visible from both sides unless it is a two-sided emitter
+// P, indices, N, uv (normals computed by area weighting when absent, :55-70, :533-575),
+// "loopsubdiv" (unsubdivided), "sphere" and "disk" (tessellated, :692-765), "plymesh" (ascii or
+// binary PLY, optionally .gz, :577-690); Include / Import. A file without a camera gets the
+// default camera framing the scene's bounds (:994-1003). Geometry is visible from both sides unless it is a two-sided emitter
 // (:802-815); ReverseOrientation xor a mirroring transform flips facing (:817).
 // Anything else is skipped with a warning.
 #pragma once
@@ -20,11 +22,15 @@
 
 namespace mrt::pbrt {
 
+// The scene's camera: the file's, or, for a file without one, a default camera framing the
+// scene's bounds (pbrt_scene.cpp:994-1003, camera.cpp:373-423).
 struct Camera {
     bool present = false;
     float eye[3], forward[3], up[3];
     float vfov = 0.f; // radians, vertical
     uint32_t width = 1920, height = 1080;
+    bool has_resolution = true; // false: the default camera, the image size is the GBuffer node's
+    float aspect = 0.f;         // the default camera's fixed aspect ratio (0: that of the image)
 };
 
 struct Scene {

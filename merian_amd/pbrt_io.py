@@ -26,6 +26,39 @@ def inspect(path: str) -> dict:
     return json.loads(text)
 
 
+class LoadedScene:
+    """A pbrt file loaded into a host scene description (mrt_pbrt_load_desc), owned by the library.
+
+    desc() returns the mrt_scene_desc, valid while this object lives; `camera` the file's camera
+    (or the default camera framing the scene) as a dict."""
+
+    def __init__(self, path: str):
+        self.lib = abi.load_library()
+        self.p = C.POINTER(abi.PbrtDesc)()
+        err = C.c_void_p()
+        st = self.lib.mrt_pbrt_load_desc(str(path).encode(), C.byref(self.p), C.byref(err))
+        if st != abi.MRT_OK:
+            msg = C.cast(err, C.c_char_p).value.decode() if err.value else "pbrt load failed"
+            if err.value:
+                self.lib.mrt_string_free(err)
+            raise RuntimeError(msg.strip())
+        d = self.p.contents
+        self.camera = {"present": bool(d.has_camera), "from_file": bool(d.camera_from_file),
+                       "eye": list(d.eye), "forward": list(d.forward), "up": list(d.up), "vfov": d.vfov,
+                       "aspect": d.aspect, "width": d.width, "height": d.height}
+
+    def desc(self) -> abi.SceneDesc:
+        return self.p.contents.scene
+
+    def close(self):
+        if self.p:
+            self.lib.mrt_pbrt_desc_free(self.p)
+            self.p = None
+
+    def __del__(self):
+        self.close()
+
+
 def read_pfm(path: str) -> np.ndarray:
     """PFM (PF / Pf) as float32 [h, w, 3], top row first."""
     with open(path, "rb") as f:

@@ -11,7 +11,7 @@ octahedral texture, clamp-to-edge, pbrt_scene.cpp:924-969; --env-map -> lat-long
 Workloads (sizes as named):
   plain_small       8x8 icospheres (1,280 tris each) + ground, 512x512, 4 spp, L = 4
   ray_query         1M random triangles in the unit cube, 16M incoherent + 4096^2 coherent rays
-  full_1080p        256 displaced tori instances (~2M tris) + open room, 256 materials,
+  full_1080p        256 displaced tori instances (~2M tris) in an enclosing room lit through a window and a skylight, 256 materials,
                     2048x1024 lat-long sky + sun, 1920x1080, 64 spp, L = 8
   serving scenes    8 scenes of 100k..2M triangles built like the two above
 """
@@ -411,9 +411,25 @@ def _displaced_torus(rng, nu=63, nv=62):
     return P, T, vertex_normals(P, T)
 
 
+def _rect_with_hole(o, U, V, hole):
+    """Quads covering the rectangle o + s U + t V, s, t in [0, 1], minus the hole
+    (s0, s1, t0, t1) if given; fronts along cross(U, V)."""
+    def rect(s0, s1, t0, t1):
+        return quad(o + s0 * U + t0 * V, o + s1 * U + t0 * V, o + s1 * U + t1 * V, o + s0 * U + t1 * V)
+    if hole is None:
+        return [rect(0, 1, 0, 1)]
+    s0, s1, t0, t1 = hole
+    return [rect(0, 1, 0, t0), rect(0, 1, t1, 1), rect(0, s0, t0, t1), rect(s1, 1, t0, t1)]
+
+
 def full_scene(seed: int = 2, instances: int = 256, res=(1920, 1080), env_size=(2048, 1024),
-               nu: int = 63, nv: int = 62, name: str = "full_1080p") -> Scene:
-    """Displaced tori instances in an open room (floor + 3 walls), 256 materials, sky env."""
+               nu: int = 63, nv: int = 62, name: str = "full_1080p", enclosed: bool = True) -> Scene:
+    """Displaced tori instances in an enclosing room lit through a window and a skylight, 256
+    materials, sky env with a sun lobe.
+
+    enclosed=False gives the open set instead (floor and three walls under the open sky, camera
+    outside): the sun reaches most surfaces directly, which tests of estimators that rely on
+    sampling the sun by the BSDF alone need."""
     rng = np.random.default_rng(seed)
     b = SceneBuilder(f"{name}_s{seed}")
     bases = [_displaced_torus(rng, nu, nv) for _ in range(4)]
@@ -440,28 +456,47 @@ def full_scene(seed: int = 2, instances: int = 256, res=(1920, 1080), env_size=(
         flags = abi.GEOM_TWO_SIDED if mats[k]["features"][0] & abi.PBR_TRANSMISSION else 0
         b.add_mesh((P * s) @ R.T + t, N @ R.T, T, mat_ids[k], flags)
     half = side * 0.8 + 2.0
+    H = 6.0
     floor_m = b.add_material(pbrt_diffuse((0.6, 0.6, 0.6)))
     wall_m = b.add_material(pbrt_diffuse((0.7, 0.55, 0.45)))
-    P, N, T = quad((-half, 0, -half), (-half, 0, half), (half, 0, half), (half, 0, -half))
-    b.add_mesh(P, N, T, floor_m)
-    H = 6.0
-    for quad_pts in (((-half, 0, -half), (half, 0, -half), (half, H, -half), (-half, H, -half)),   # back
-                     ((-half, 0, half), (-half, 0, -half), (-half, H, -half), (-half, H, half)),  # left
-                     ((half, 0, -half), (half, 0, half), (half, H, half), (half, H, -half))):     # right
-        P, N, T = quad(*quad_pts)
-        b.add_mesh(P, N, T, wall_m)
-    cam = CameraSpec((0.0, 7.0, half + 9.0), (0.0, 0.8, 0.0), (0.0, 1.0, 0.0), math.radians(45.0), res[0], res[1])
+    if not enclosed:
+        P, N, T = quad((-half, 0, -half), (-half, 0, half), (half, 0, half), (half, 0, -half))
+        b.add_mesh(P, N, T, floor_m)
+        for q in (((-half, 0, -half), (half, 0, -half), (half, H, -half), (-half, H, -half)),   # back
+                  ((-half, 0, half), (-half, 0, -half), (-half, H, -half), (-half, H, half)),   # left
+                  ((half, 0, -half), (half, 0, half), (half, H, half), (half, H, -half))):      # right
+            P, N, T = quad(*q)
+            b.add_mesh(P, N, T, wall_m)
+        cam = CameraSpec((0.0, 7.0, half + 9.0), (0.0, 0.8, 0.0), (0.0, 1.0, 0.0), math.radians(45.0),
+                         res[0], res[1])
+        return b.build(camera=cam, **sky_env(env_size[0], env_size[1]))
+    # the six faces of the room, fronts facing inwards (walls are one-sided): rect(o, U, V) spans
+    # o + s U + t V with cross(U, V) as its front; the back wall (towards the sun) has a window and
+    # the ceiling a skylight, the only ways light gets in
+    X, Y, Z = np.array([2 * half, 0, 0]), np.array([0, H, 0]), np.array([0, 0, 2 * half])
+    faces = [((-half, 0, -half), Z, X, None),                          # floor, front +y
+             ((-half, H, -half), X, Z, (0.32, 0.68, 0.2, 0.55)),         # ceiling, -y, skylight
+             ((-half, 0, -half), X, Y, (0.22, 0.78, 0.2, 0.8)),          # back wall z = -half, +z, window
+             ((-half, 0, half), Y, X, None),                             # front wall, -z
+             ((-half, 0, half), -Z, Y, None),                            # left wall x = -half, +x
+             ((half, 0, -half), Z, Y, None)]                             # right wall, -x
+    for k, (o, U, V, hole) in enumerate(faces):
+        for P, N, T in _rect_with_hole(np.array(o, np.float64), U, V, hole):
+            b.add_mesh(P, N, T, floor_m if k == 0 else wall_m)
+    cam = CameraSpec((0.0, 4.0, half - 0.8), (0.0, 0.8, -0.25 * half), (0.0, 1.0, 0.0), math.radians(60.0),
+                     res[0], res[1])
     return b.build(camera=cam, **sky_env(env_size[0], env_size[1]))
 
 
 def serving_scenes(count: int = 8):
-    """Resident scenes of the serving mix (seeds 101..108): small grids and tori rooms."""
+    """Resident scenes of the serving mix (seeds 101..108, 103,682 to 1,999,896 triangles): icosphere
+    grids (as the small plain-pt scene, 9x9) and tori in enclosing rooms (as the 1080p scene)."""
     scenes = []
     sizes = [13, 26, 52, 100, 160, 200, 230, 256]  # torus instances -> ~100k .. 2M triangles
     for k in range(count):
         seed = 101 + k
         if k % 4 == 0:
-            scenes.append(plain_small(seed=seed, subdiv=3 + (k // 4), grid=8))
+            scenes.append(plain_small(seed=seed, subdiv=3 + (k // 4), grid=9))
         else:
             scenes.append(full_scene(seed=seed, instances=sizes[k], env_size=(512, 256), name="serve"))
     return scenes

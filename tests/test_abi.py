@@ -44,7 +44,8 @@ def test_ctypes_layouts_match_the_header(tmp_path):
                    "sizeof(mrt_packed_hit),sizeof(mrt_pt_params),sizeof(mrt_pt_stats),"
                    "offsetof(mrt_pt_stats,node_visits),sizeof(mrt_guide_config),"
                    "offsetof(mrt_guide_config,guided_bounces),sizeof(mrt_mcpg_grid_config),"
-                   "sizeof(mrt_mcpg_config),offsetof(mrt_mcpg_config,lc_min_pdf));return 0;}\n")
+                   "sizeof(mrt_mcpg_config),offsetof(mrt_mcpg_config,lc_min_pdf));"
+                   "printf(\"%zu %zu\\n\",sizeof(mrt_pbrt_desc),offsetof(mrt_pbrt_desc,warning_count));return 0;}\n")
     subprocess.run(["gcc", "-std=c11", "-I", str(ROOT), "-o", str(exe), str(src)], check=True)
     sizes = list(map(int, subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()))
     ours = [abi.VERTEX_DTYPE.itemsize, abi.GEOMETRY_DTYPE.itemsize, abi.MATERIAL_DTYPE.itemsize,
@@ -52,7 +53,7 @@ def test_ctypes_layouts_match_the_header(tmp_path):
             abi.HIT_DTYPE.itemsize, abi.PACKED_HIT_DTYPE.itemsize, C.sizeof(abi.PtParams),
             C.sizeof(abi.PtStats), abi.PtStats.node_visits.offset, C.sizeof(abi.GuideConfig),
             abi.GuideConfig.guided_bounces.offset, C.sizeof(abi.McpgGridConfig), C.sizeof(abi.McpgConfig),
-            abi.McpgConfig.lc_min_pdf.offset]
+            abi.McpgConfig.lc_min_pdf.offset, C.sizeof(abi.PbrtDesc), abi.PbrtDesc.warning_count.offset]
     assert sizes == ours
 
 
@@ -92,3 +93,11 @@ def test_camera_is_host_side_and_matches_oracle():
             assert lib.mrt_camera_look_at(f3(*spec.eye), f3(*spec.target), f3(*spec.up), spec.vfov, spec.aspect,
                                           spec.near, spec.far, spec.jitter, frame, C.byref(cam)) == 0
             assert bytes(cam) == bytes(O.camera_look_at(spec, frame))
+
+
+def test_library_is_built_from_these_sources():
+    # mrt_version carries the hash of the sources the library was built from (merian_amd/build.py);
+    # load_library refuses a library built from other sources
+    from merian_amd.build import source_hash
+    lib = abi.load_library()
+    assert lib.mrt_version().decode().endswith("sources " + source_hash())

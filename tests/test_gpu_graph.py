@@ -119,3 +119,39 @@ def test_runner_mcpg_variant_matches_direct_mcpg(ctx, tmp_path):
     assert rc == 0
     got2 = pbrt_io.read_pfm(str(out2) + ".pfm")
     assert np.isfinite(got2).all() and abs(got2.mean() - ref.mean()) < 0.1 * ref.mean()
+
+
+REFERENCE_GRAPH = GRAPH.parent / "reference" / "pbrt.json"
+HEADLESS = GRAPH.parent / "headless_pfm.json"
+
+
+def test_reference_example_graph_matches_the_oracle(ctx, tmp_path):
+    # the reference's unmodified examples/pbrt.json with merian-graph-run's command line
+    # (--renderer pt --spp 4 --max-path-length 4, --merge a headless Image Write fragment) renders
+    # the frame the CPU oracle renders from the same pbrt file, loaded by the same loader
+    from merian_amd.runtime import DeviceScene  # noqa: F401  (context fixture keeps the library loaded)
+    from oracle import oracle as O
+    sc = scenes.plain_small(seed=12, subdiv=2, grid=3, res=96)
+    p = tmp_path / "scene.pbrt"
+    scenes.write_pbrt(sc, str(p))
+    out = tmp_path / "frame"
+    rc = pbrt_io.run_graph([str(REFERENCE_GRAPH), str(p), "--renderer", "pt", "--spp", "4", "--max-path-length", "4",
+                            "--merge", str(HEADLESS), "--out", str(out), "--max-iterations=1"])
+    assert rc == 0
+    got = pbrt_io.read_pfm(str(out) + ".pfm")
+    ls = pbrt_io.LoadedScene(p)
+    cam_i = ls.camera
+    assert cam_i["present"] and cam_i["from_file"]
+    w, h = cam_i["width"], cam_i["height"]
+    eye = np.array(cam_i["eye"], np.float32)
+    target = (eye + np.array(cam_i["forward"], np.float32)).astype(np.float32)
+    spec = scenes.CameraSpec(tuple(eye), tuple(target), tuple(cam_i["up"]), cam_i["vfov"], w, h)
+    osc = O.OracleScene(ls)
+    ocam = O.camera_look_at(spec, 0)
+    gb = osc.gbuffer(ocam, w, h, mask=0xFF & ~(1 << 6))  # the example's GBuffer has mask bit "6" off
+    ref, _, _ = osc.render(O.pt_params(4, 4, mask=0xFF), ocam, 0, w, h, gb)
+    ls.close()
+    assert got.shape == (h, w, 3)
+    exact = (got.view(np.uint32) == np.ascontiguousarray(ref[..., :3]).view(np.uint32)).all(axis=-1)
+    assert exact.mean() >= 0.995, exact.mean()
+    assert np.abs(got - ref[..., :3]).mean() < 1e-2 * max(1e-3, float(np.abs(ref[..., :3]).mean()))

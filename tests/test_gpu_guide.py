@@ -7,7 +7,6 @@
 - at equal wall time the trained guide beats the plain pt in relMSE against a converged render.
 """
 import math
-import time
 
 import numpy as np
 import pytest
@@ -135,7 +134,11 @@ def test_training_lowers_the_loss(ctx, box_scene):
 
 @pytest.fixture(scope="module")
 def sky(ctx):
-    sc = scenes.full_scene(seed=5, instances=16, env_size=(256, 128))
+    # the open set: the sun reaches most surfaces directly. In the enclosing room it comes in
+    # through the openings only, where BSDF- and guide-sampled paths find it so rarely that 2048
+    # samples per pixel underestimate both the mean and the spread (the block z-test is then
+    # meaningless, not failed).
+    sc = scenes.full_scene(seed=5, instances=16, env_size=(256, 128), enclosed=False)
     ds = ctx.scene(sc)
     yield sc, ds
     ds.destroy()
@@ -144,26 +147,38 @@ def sky(ctx):
 W_, H_ = 192, 108
 
 
-def _frames(ctx, ds, spec, cam_frame, params, frames, guide=None, train=True):
-    """Mean of `frames` frames (frame index varies the RNG; the camera stays fixed)."""
+def _frames(ctx, ds, spec, cam_frame, params, frames, guide=None, train=True, base=1000):
+    """Mean of `frames` frames (frame index base + f varies the RNG; the camera stays fixed).
+    Frames render on torch's current stream and are summed on the device, so the sequence has no
+    host round trip between frames."""
     import torch
+    strm = torch.cuda.current_stream().cuda_stream
     cam = ctx.camera(spec, cam_frame)
     gb = torch.empty(W_ * H_ * 16, dtype=torch.uint8, device="cuda")
     out = torch.empty(W_ * H_ * 4, dtype=torch.float32, device="cuda")
+    acc = torch.zeros(W_ * H_ * 4, dtype=torch.float64, device="cuda")
     ctx.gbuffer_dev(ds, cam, W_, H_, gb.data_ptr())
-    acc = np.zeros((H_, W_, 4), np.float64)
+    ctx.synchronize()
     for f in range(frames):
         if guide is None:
-            ctx.pt_render_dev(ds, params, cam, 1000 + f, W_, H_, gb.data_ptr(), out.data_ptr())
+            ctx.pt_render_dev(ds, params, cam, base + f, W_, H_, gb.data_ptr(), out.data_ptr(), stream=strm)
         else:
-            guide.render_dev(ds, params, cam, 1000 + f, W_, H_, gb.data_ptr(), out.data_ptr(), train=train)
-        torch.cuda.synchronize()
-        acc += out.cpu().numpy().reshape(H_, W_, 4)
-    return acc / frames
+            guide.render_dev(ds, params, cam, base + f, W_, H_, gb.data_ptr(), out.data_ptr(), train=train,
+                             stream=strm)
+        acc += out
+    return (acc / frames).cpu().numpy().reshape(H_, W_, 4)
 
 
 def _lum(img):
     return img[..., 0] * 0.299 + img[..., 1] * 0.587 + img[..., 2] * 0.114
+
+
+def _reference(ctx, ds, spec, L, frames):
+    # The reference is the NEE estimator (env importance sampling + MIS, unbiased, tested in
+    # test_gpu_nee.py), whose sun contribution is well sampled, on RNG frames (50000 + f)
+    # disjoint from every guided frame's.
+    return _frames(ctx, ds, spec, 0, pt_params(spp=1024, max_path_length=L, mode=abi.MODE_NEE,
+                                               record_signature=False), frames, base=50000)
 
 
 def test_guided_is_unbiased_and_wins_at_equal_time(ctx, sky):
@@ -171,14 +186,14 @@ def test_guided_is_unbiased_and_wins_at_equal_time(ctx, sky):
     sc, ds = sky
     spec = scenes.CameraSpec(sc.camera.eye, sc.camera.target, sc.camera.up, sc.camera.vfov, W_, H_)
     L = 6
-    ref = _frames(ctx, ds, spec, 0, pt_params(spp=1024, max_path_length=L, record_signature=False), 16)
+    ref = _reference(ctx, ds, spec, L, 16)
     g = Guide(ctx, ds, guide_config(seed=4))
     try:
         gp = pt_params(spp=16, max_path_length=L, mode=abi.MODE_GUIDED, record_signature=False)
-        _frames(ctx, ds, spec, 0, gp, 24, guide=g, train=True)  # online training
+        _frames(ctx, ds, spec, 0, gp, 24, guide=g, train=True, base=2000)  # online training
         # unbiased: guided (frozen) block means against the reference within their errors
         gp.spp = 512
-        gimg = _frames(ctx, ds, spec, 0, gp, 4, guide=g, train=False)
+        gimg = _frames(ctx, ds, spec, 0, gp, 4, guide=g, train=False, base=3000)
         def blocks(img, spp):
             l = _lum(img)
             var = np.maximum(img[..., 3] - l * l, 0) / spp
@@ -191,11 +206,14 @@ def test_guided_is_unbiased_and_wins_at_equal_time(ctx, sky):
         assert abs(z.mean()) < 0.6
         # equal time: plain and guided (training included) get the same wall time
         def timed(fn):
+            # device time of the frame sequence on torch's stream (HIP events), not host time
             torch.cuda.synchronize()
-            t0 = time.perf_counter()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
             r = fn()
+            e1.record()
             torch.cuda.synchronize()
-            return r, time.perf_counter() - t0
+            return r, e0.elapsed_time(e1) * 1e-3
         # 40 guided frames (about 0.1 s): with 8, one sun firefly or a weaker training run (the
         # grid gradients are summed by atomics in no fixed order) decided the comparison; at 40
         # four guide seeds gave 0.022-0.027 against the plain pt's 0.057
@@ -221,13 +239,13 @@ def test_guided_bounces_limit_stays_unbiased(ctx, sky):
     sc, ds = sky
     spec = scenes.CameraSpec(sc.camera.eye, sc.camera.target, sc.camera.up, sc.camera.vfov, W_, H_)
     L = 6
-    ref = _frames(ctx, ds, spec, 0, pt_params(spp=1024, max_path_length=L, record_signature=False), 8)
+    ref = _reference(ctx, ds, spec, L, 8)
     g = Guide(ctx, ds, guide_config(seed=6, guided_bounces=1))
     try:
         gp = pt_params(spp=16, max_path_length=L, mode=abi.MODE_GUIDED, record_signature=False)
-        _frames(ctx, ds, spec, 0, gp, 8, guide=g, train=True)
+        _frames(ctx, ds, spec, 0, gp, 8, guide=g, train=True, base=2000)
         gp.spp = 256
-        img = _frames(ctx, ds, spec, 0, gp, 4, guide=g, train=False)
+        img = _frames(ctx, ds, spec, 0, gp, 4, guide=g, train=False, base=3000)
         lum_g, lum_r = _lum(img), _lum(ref)
         var_g = np.maximum(img[..., 3] - lum_g * lum_g, 0) / (256 * 4)
         var_r = np.maximum(ref[..., 3] - lum_r * lum_r, 0) / (1024 * 8)

@@ -15,8 +15,8 @@ def test_plain_small_sizes_and_determinism():
 
 def test_full_scene_sizes():
     s = scenes.full_scene(seed=2, instances=256, env_size=(64, 32))
-    assert s.triangle_count == 256 * 7812 + 8 == 1999880
-    assert len(s.geometries) == 260
+    assert s.triangle_count == 256 * 7812 + 24 == 1999896  # 12 room quads: 4 faces + 2 with an opening
+    assert len(s.geometries) == 256 + 12
     feats = s.materials["features"][:256]
     metal = s.materials["metalness"][:256]
     assert (feats & abi.PBR_TRANSMISSION).sum() > 20 and (metal == 1).sum() > 20
@@ -24,6 +24,12 @@ def test_full_scene_sizes():
     # dielectrics are TwoSided (pbrt_scene.cpp:440-466), nothing else is
     assert np.array_equal(two[:256], (feats & abi.PBR_TRANSMISSION) != 0)
     assert (s.camera.width, s.camera.height) == (1920, 1080)
+    # the open set: the same tori and materials, a floor and three walls, the camera outside
+    o = scenes.full_scene(seed=2, instances=256, env_size=(64, 32), enclosed=False)
+    assert o.triangle_count == 256 * 7812 + 8 and len(o.geometries) == 256 + 4
+    nv = int(s.geometries["vertex_offset"][256])  # the tori's vertices come first
+    assert nv == int(o.geometries["vertex_offset"][256]) and o.vertices[:nv].tobytes() == s.vertices[:nv].tobytes()
+    assert o.camera.eye[2] > o.bounds()[1][2]
 
 
 def test_sky_dynamic_range():
