@@ -281,6 +281,19 @@ MRT_API mrt_status mrt_trace_closest_dev(mrt_ctx* ctx, mrt_scene* scene, const m
  * bandwidth roofline. */
 MRT_API mrt_status mrt_trace_count_dev(mrt_ctx* ctx, mrt_scene* scene, const mrt_ray* d_rays, uint64_t n,
                                        mrt_hit* d_hits, uint64_t counts[2]);
+/* Allocation-free closest-hit queries. No counterpart in the reference: this entry point is
+ * for callers that cannot allocate device memory (code under hipGraph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). Same results as mrt_trace_closest_dev, bit for bit.
+ * Its traversal scratch (a work cursor and the traversal stacks' overflow columns) is a fixed
+ * array of MRT_NOALLOC_SCRATCH_BYTES in device memory, held by every GPU the library runs on
+ * until the process ends; its address is looked up per device by mrt_create. The persistent grid
+ * is capped at what that array holds for the scene's tree depth; a tree too deep for even one
+ * block returns MRT_ERR_UNSUPPORTED. Every call on one GPU shares the array: calls that overlap
+ * on one GPU (other streams, other contexts, other threads) race and must be ordered by the
+ * caller. Calls no hipMalloc, hipFree or allocating library routine. */
+#define MRT_NOALLOC_SCRATCH_BYTES (64u * 1024u * 1024u)
+MRT_API mrt_status mrt_trace_closest_noalloc_dev(mrt_ctx* ctx, mrt_scene* scene, const mrt_ray* d_rays,
+                                                 uint64_t n, mrt_hit* d_hits, void* stream);
 /* Pinhole rays of a w x h image on the device (camera.slang:45-56), for coherent
  * ray-query workloads. */
 MRT_API mrt_status mrt_camera_rays_dev(mrt_ctx* ctx, const mrt_camera* cam, uint32_t width,

@@ -229,6 +229,8 @@ EXPORTS = [
     ("mrt_trace_closest", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
     ("mrt_trace_closest_dev", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64,
                                           C.c_void_p, C.c_void_p]),
+    ("mrt_trace_closest_noalloc_dev", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64,
+                                                  C.c_void_p, C.c_void_p]),
     ("mrt_trace_count_dev", C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p,
                                         C.POINTER(C.c_uint64)]),
     ("mrt_camera_rays_dev", C.c_int32, [C.c_void_p, C.POINTER(Camera), C.c_uint32, C.c_uint32,

@@ -18,7 +18,7 @@ LIB = PKG / "libmerian_amd.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["csrc/kernels.hip", "csrc/api.cpp", "csrc/bvh.cpp", "csrc/serve.cpp", "csrc/guide.hip",
-           "csrc/guide_api.cpp", "csrc/tune.cpp", "csrc/mcpg_api.cpp", "host/graph_description.cpp", "host/graph_abi.cpp", "host/pbrt.cpp",
+           "csrc/guide_api.cpp", "csrc/tune.cpp", "csrc/mcpg_api.cpp", "csrc/noalloc.cpp", "host/graph_description.cpp", "host/graph_abi.cpp", "host/pbrt.cpp",
            "host/graph.cpp", "host/nodes.cpp", "host/graph_run.cpp"]
 RUNNER = PKG / "merian-amd-graph-run"
 

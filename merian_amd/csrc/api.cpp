@@ -200,6 +200,11 @@ MRT_API mrt_status mrt_create(int32_t device, uint64_t hbm_budget, mrt_ctx** out
         return MRT_ERR_DEVICE;
     }
     c->budget = hbm_budget ? hbm_budget : uint64_t(double(free_b) * 0.9);
+    if (noalloc_setup(c) != MRT_OK) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return MRT_ERR_DEVICE;
+    }
     const char* mp = std::getenv("MRT_MAX_PATHS");
     c->max_paths = mp ? std::strtoull(mp, nullptr, 10) : (uint64_t(1) << 24);
     *out = c;

@@ -57,6 +57,9 @@ TraceScratch scene_scratch(const mrt_ctx* c, const mrt_scene* s);
 // recorded outside a capture cannot be waited on inside it): captured launches are ordered by
 // the caller.
 hipError_t scratch_begin(mrt_ctx* c, hipStream_t st);
+// Looks up the device address of the allocation-free entry point's fixed scratch on the
+// context's device (noalloc.cpp); mrt_create calls it with that device current.
+mrt_status noalloc_setup(mrt_ctx* c);
 hipError_t scratch_end(mrt_ctx* c, hipStream_t st);
 // Rebuilds and uploads the scene's BVH with the given node width (4 or 8), leaf size and SAH
 // traversal cost.

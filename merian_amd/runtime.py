@@ -119,6 +119,11 @@ class Context:
         self._check(self.lib.mrt_trace_closest_dev(self.h, scene.h, C.c_void_p(d_rays), n, C.c_void_p(d_hits),
                                                    C.c_void_p(stream) if stream else None))
 
+    def trace_noalloc_dev(self, scene: DeviceScene, d_rays: int, n: int, d_hits: int, stream: int | None = None):
+        """mrt_trace_closest_noalloc_dev: as trace_dev, with the library's fixed scratch (no allocation)."""
+        self._check(self.lib.mrt_trace_closest_noalloc_dev(self.h, scene.h, C.c_void_p(d_rays), n,
+                                                           C.c_void_p(d_hits), C.c_void_p(stream) if stream else None))
+
     def trace_count_dev(self, scene: DeviceScene, d_rays: int, n: int, d_hits: int) -> tuple[int, int]:
         """As trace_dev with the counting kernel (synchronous): (node visits, triangle tests)."""
         counts = (C.c_uint64 * 2)()
