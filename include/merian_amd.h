@@ -9,7 +9,9 @@
  *     maps a negative status onto that.
  *   - Host buffers are owned by the caller; device memory behind the opaque handles is
  *     owned by the library. Entry points suffixed _dev take device pointers that the
- *     caller owns and an optional hipStream_t (NULL = the context's stream).
+ *     caller owns and an optional hipStream_t: NULL = the context's own stream (created
+ *     non-blocking, so it does not wait for the HIP null stream); hipStreamLegacy = the HIP
+ *     null stream; anything else = that stream.
  *   - One host thread per context (the reference graph is single-threaded,
  *     src/merian-graph/graph/graph.cpp:68-253). Work is asynchronous on the context's
  *     stream; host-buffer entry points synchronise before returning.

@@ -64,7 +64,14 @@ void dev_free(mrt_ctx* c, void* p) {
     if (e != hipSuccess) c->err = std::string("hipFree: ") + hipGetErrorString(e);
 }
 
-hipStream_t pick_stream(mrt_ctx* c, void* stream) { return stream ? static_cast<hipStream_t>(stream) : c->stream; }
+// NULL selects the context's own stream; hipStreamLegacy selects the HIP null stream (the
+// default stream of most callers, torch's included, whose handle is 0 and so cannot be passed
+// as itself).
+hipStream_t pick_stream(mrt_ctx* c, void* stream) {
+    if (!stream) return c->stream;
+    if (static_cast<hipStream_t>(stream) == hipStreamLegacy) return hipStream_t(nullptr);
+    return static_cast<hipStream_t>(stream);
+}
 
 void free_path_buffers(mrt_ctx* c) {
     PathBuffers& p = c->paths;

@@ -14,6 +14,12 @@ WRITE_SIZE counters) and summaries of rocprofv3's CSV output.
         SQ_INSTS_VALU, SQ_ACTIVE_INST_VALU and SQ_THREAD_CYCLES_VALU
     python -m merian_amd.profiling l2 DIR
         per-kernel L2 hit rate from a TCC_HIT_sum / TCC_MISS_sum pass
+    python -m merian_amd.profiling dram DIR
+        per-kernel L2-to-fabric requests and the share destined for DRAM, from a pass with
+        TCC_EA0_RDREQ_sum, TCC_EA0_RDREQ_DRAM_sum, TCC_EA0_WRREQ_sum and TCC_EA0_WRREQ_DRAM_sum
+
+gfx950's rocprofv3 lists no MALL (Infinity Cache) counter: the L2's fabric requests are the last
+level it can see, and a request "destined for DRAM" still passes the MALL on its way.
 """
 from __future__ import annotations
 
@@ -203,6 +209,54 @@ def l2(d: str) -> dict:
             for k, c in acc.items() if c.get("TCC_HIT_sum", 0) + c.get("TCC_MISS_sum", 0) > 0}
 
 
+def dram(d: str) -> dict:
+    """L2-to-fabric (EA) read and write requests per dispatch and the percentage of them
+    destined for DRAM (the memory controller, behind the MALL) rather than IO/GMI."""
+    acc, disp = _sums(d)
+    out = {}
+    for k, c in acc.items():
+        n, ms = disp[k]
+        rd, wr = c.get("TCC_EA0_RDREQ_sum", 0.0), c.get("TCC_EA0_WRREQ_sum", 0.0)
+        if rd + wr <= 0:
+            continue
+        out[k] = {"dispatches": n, "avg_ms": round(ms / n, 4), "ea_rdreq_per_dispatch": round(rd / n),
+                  "ea_wrreq_per_dispatch": round(wr / n),
+                  "rd_to_dram_pct": round(100 * c.get("TCC_EA0_RDREQ_DRAM_sum", 0.0) / rd, 1) if rd > 0 else None,
+                  "wr_to_dram_pct": round(100 * c.get("TCC_EA0_WRREQ_DRAM_sum", 0.0) / wr, 1) if wr > 0 else None}
+    return out
+
+
+def traffic_record(args) -> dict:
+    """The traffic of one kernel (traffic()) with the workload and tuning it was measured on and
+    the same kernel's figures from the other passes (kernel-trace average, VALU, L2, DRAM
+    requests), as bench.py's roofline reads it."""
+    r = traffic(args.fetch_dir, args.write_dir, args.copy_fetch_dir, args.copy_write_dir, args.kernel,
+                args.copy_mib << 20)
+    if args.workload:
+        r["workload"] = args.workload
+    if args.tuning_file:
+        with open(args.tuning_file) as f:
+            r["tuning"] = json.load(f)["tuning"]
+    def pick(rows):
+        hits = [v for k, v in rows.items() if k.startswith(args.kernel)]
+        return hits[0] if len(hits) == 1 else None
+    if args.stats_dir:
+        ks = [x for x in kernel_stats(args.stats_dir) if short_name(x["name"]).startswith(args.kernel)]
+        if len(ks) == 1:
+            r["avg_ms_kernel_trace"] = round(ks[0]["avg_ns"] / 1e6, 4)
+            r["hbm_gbs_at_kernel_trace_avg"] = round(r["traffic_bytes_per_dispatch"] / (ks[0]["avg_ns"] * 1e-9) / 1e9, 1)
+    if args.valu_dir and (v := pick(valu(args.valu_dir))):
+        r.update({k: v[k] for k in ("valu_busy_pct", "valu_lane_util_pct", "valu_issue_ms")})
+    if args.l2_dir and (v := pick(l2(args.l2_dir))):
+        r["l2_hit_pct"] = v["l2_hit_pct"]
+    if args.dram_dir and (v := pick(dram(args.dram_dir))):
+        r.update({k: v[k] for k in ("ea_rdreq_per_dispatch", "ea_wrreq_per_dispatch", "rd_to_dram_pct",
+                                    "wr_to_dram_pct")})
+    if args.command:
+        r["commands"] = args.command
+    return r
+
+
 def copy_workload(mib: int, reps: int) -> None:
     import torch
     n = mib << 20
@@ -276,6 +330,13 @@ def main(argv=None) -> int:
     for a in ("fetch_dir", "write_dir", "copy_fetch_dir", "copy_write_dir", "kernel"):
         t.add_argument(a)
     t.add_argument("--copy-mib", type=int, default=1024)
+    t.add_argument("--workload", default="")
+    t.add_argument("--tuning-file", default="", help="JSON file whose 'tuning' object the profile ran with")
+    for a in ("--stats-dir", "--valu-dir", "--l2-dir", "--dram-dir"):
+        t.add_argument(a, default="")
+    t.add_argument("--command", action="append", default=[], help="a command of the profile (repeatable)")
+    dr = sub.add_parser("dram")
+    dr.add_argument("dir")
     v = sub.add_parser("valu")
     v.add_argument("dir")
     l = sub.add_parser("l2")
@@ -288,10 +349,9 @@ def main(argv=None) -> int:
     elif args.cmd == "guided":
         guided_workload(args.width, args.height, args.spp, args.train_frames, args.frames, args.guided_bounces)
     elif args.cmd == "traffic":
-        print(json.dumps(traffic(args.fetch_dir, args.write_dir, args.copy_fetch_dir, args.copy_write_dir,
-                                 args.kernel, args.copy_mib << 20), indent=1))
-    elif args.cmd in ("valu", "l2", "mfma"):
-        rows = valu(args.dir) if args.cmd == "valu" else (l2(args.dir) if args.cmd == "l2" else mfma(args.dir))
+        print(json.dumps(traffic_record(args), indent=1))
+    elif args.cmd in ("valu", "l2", "mfma", "dram"):
+        rows = {"valu": valu, "l2": l2, "mfma": mfma, "dram": dram}[args.cmd](args.dir)
         keys = list(next(iter(rows.values())).keys()) if rows else []
         print("| kernel | " + " | ".join(keys) + " |\n|---|" + "---:|" * len(keys))
         for k, r in sorted(rows.items(), key=lambda kv: -kv[1]["dispatches"] * kv[1].get("avg_ms", 1)):

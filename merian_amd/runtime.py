@@ -21,6 +21,18 @@ def _p(a):
     return None if a is None else C.c_void_p(a.ctypes.data)
 
 
+HIP_STREAM_LEGACY = 1  # hipStreamLegacy: the HIP null stream
+
+
+def _stream(stream: int | None):
+    """The stream argument of a _dev entry point. None selects the context's own stream; a HIP
+    stream handle selects that stream, where handle 0 (the null stream, torch's default stream)
+    is passed as hipStreamLegacy, because a NULL argument would select the context's stream."""
+    if stream is None:
+        return None
+    return C.c_void_p(HIP_STREAM_LEGACY if int(stream) == 0 else int(stream))
+
+
 class DeviceScene:
     def __init__(self, ctx: "Context", handle: C.c_void_p, host: Scene | None):
         self.ctx, self.h, self.host = ctx, handle, host
@@ -117,12 +129,12 @@ class Context:
 
     def trace_dev(self, scene: DeviceScene, d_rays: int, n: int, d_hits: int, stream: int | None = None):
         self._check(self.lib.mrt_trace_closest_dev(self.h, scene.h, C.c_void_p(d_rays), n, C.c_void_p(d_hits),
-                                                   C.c_void_p(stream) if stream else None))
+                                                   _stream(stream)))
 
     def trace_noalloc_dev(self, scene: DeviceScene, d_rays: int, n: int, d_hits: int, stream: int | None = None):
         """mrt_trace_closest_noalloc_dev: as trace_dev, with the library's fixed scratch (no allocation)."""
         self._check(self.lib.mrt_trace_closest_noalloc_dev(self.h, scene.h, C.c_void_p(d_rays), n,
-                                                           C.c_void_p(d_hits), C.c_void_p(stream) if stream else None))
+                                                           C.c_void_p(d_hits), _stream(stream)))
 
     def trace_count_dev(self, scene: DeviceScene, d_rays: int, n: int, d_hits: int) -> tuple[int, int]:
         """As trace_dev with the counting kernel (synchronous): (node visits, triangle tests)."""
@@ -130,29 +142,32 @@ class Context:
         self._check(self.lib.mrt_trace_count_dev(self.h, scene.h, C.c_void_p(d_rays), n, C.c_void_p(d_hits), counts))
         return int(counts[0]), int(counts[1])
 
-    def camera_rays_dev(self, cam: abi.Camera, w: int, h: int, d_rays: int):
-        self._check(self.lib.mrt_camera_rays_dev(self.h, C.byref(cam), w, h, C.c_void_p(d_rays), None))
+    def camera_rays_dev(self, cam: abi.Camera, w: int, h: int, d_rays: int, stream: int | None = None):
+        self._check(self.lib.mrt_camera_rays_dev(self.h, C.byref(cam), w, h, C.c_void_p(d_rays), _stream(stream)))
 
-    def gbuffer_dev(self, scene: DeviceScene, cam: abi.Camera, w: int, h: int, d_out: int, mask: int = 0xFF):
-        self._check(self.lib.mrt_gbuffer_dev(self.h, scene.h, C.byref(cam), w, h, mask, C.c_void_p(d_out), None))
+    def gbuffer_dev(self, scene: DeviceScene, cam: abi.Camera, w: int, h: int, d_out: int, mask: int = 0xFF,
+                    stream: int | None = None):
+        self._check(self.lib.mrt_gbuffer_dev(self.h, scene.h, C.byref(cam), w, h, mask, C.c_void_p(d_out),
+                                             _stream(stream)))
 
     def gbuffer_albedo_dev(self, scene: DeviceScene, cam: abi.Camera, w: int, h: int, d_gbuffer: int, d_albedo: int,
                            stream: int | None = None):
         """mrt_gbuffer_albedo_dev: the GBuffer albedo texture (float4 per pixel) of a hit texture."""
         self._check(self.lib.mrt_gbuffer_albedo_dev(self.h, scene.h, C.byref(cam), w, h, C.c_void_p(d_gbuffer),
-                                                    C.c_void_p(d_albedo), C.c_void_p(stream) if stream else None))
+                                                    C.c_void_p(d_albedo), _stream(stream)))
 
     def pt_render_dev(self, scene: DeviceScene, params: abi.PtParams, cam: abi.Camera, frame: int, w: int, h: int,
                       d_gbuffer: int, d_out: int, d_sig: int | None = None, stats: bool = False,
                       stream: int | None = None):
-        """Enqueues a frame on `stream` (a HIP stream handle; None = the context's own stream,
-        which does not synchronise with torch's)."""
+        """Enqueues a frame on `stream`: a HIP stream handle such as
+        torch.cuda.current_stream().cuda_stream (0, torch's default stream, is the HIP null
+        stream), or None for the context's own stream, which does not wait for torch's."""
         st = abi.PtStats() if stats else None
         self._check(self.lib.mrt_pt_render_dev(self.h, scene.h, C.byref(params), C.byref(cam), frame, w, h,
                                                C.c_void_p(d_gbuffer), C.c_void_p(d_out),
                                                C.c_void_p(d_sig) if d_sig else None,
                                                C.byref(st) if st is not None else None,
-                                               C.c_void_p(stream) if stream else None))
+                                               _stream(stream)))
         return st
 
     def render(self, scene: DeviceScene, params: abi.PtParams, cam: abi.Camera, frame: int, w: int, h: int,
@@ -340,7 +355,7 @@ class Guide:
         self.ctx._check(self.lib.mrt_guided_render_dev(self.ctx.h, scene.h, self.h, C.byref(params), C.byref(cam),
                                                        frame, w, h, C.c_void_p(d_gbuffer), C.c_void_p(d_out),
                                                        int(train), C.byref(st) if st is not None else None,
-                                                       C.c_void_p(stream) if stream else None))
+                                                       _stream(stream)))
         return st
 
 
@@ -385,7 +400,7 @@ class Mcpg:
         self.ctx._check(self.lib.mrt_mcpg_render_dev(self.ctx.h, scene.h, self.h, C.byref(params), C.byref(cam),
                                                      frame, w, h, C.c_void_p(d_gbuffer), C.c_void_p(d_out),
                                                      C.byref(st) if st is not None else None,
-                                                     C.c_void_p(stream) if stream else None))
+                                                     _stream(stream)))
         return st
 
     def grids(self):

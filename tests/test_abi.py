@@ -101,3 +101,12 @@ def test_library_is_built_from_these_sources():
     from merian_amd.build import source_hash
     lib = abi.load_library()
     assert lib.mrt_version().decode().endswith("sources " + source_hash())
+
+
+def test_stream_argument_mapping():
+    # None selects the context's own stream (NULL); HIP stream handles pass through, except the
+    # null stream (handle 0, torch's default stream), which is passed as hipStreamLegacy
+    from merian_amd.runtime import HIP_STREAM_LEGACY, _stream
+    assert _stream(None) is None
+    assert _stream(0).value == HIP_STREAM_LEGACY == 1
+    assert _stream(0x7f00deadbeef).value == 0x7f00deadbeef

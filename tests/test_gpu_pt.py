@@ -176,6 +176,33 @@ def test_frame_split_into_passes(ctx, small):
     _compare(split, sig2, ref, ref_sig)
 
 
+@pytest.mark.parametrize("which", ["default", "side"])
+def test_frames_on_torch_streams(ctx, small, which):
+    # frames enqueued on a torch stream (the default one, handle 0 = the HIP null stream, or a
+    # side stream) and summed there by torch with no synchronisation in between: the sum equals
+    # that of the same frames rendered synchronously
+    import torch
+    sc, ds, _ = small
+    W, H, n = 64, 48, 4
+    cam = ctx.camera(sc.camera, 2)
+    p = pt_params(spp=4, max_path_length=5, record_signature=False)
+    ref = np.zeros((H, W, 4))
+    for f in range(n):
+        ref += ctx.render(ds, p, cam, 10 + f, W, H, signature=False)[0]
+    s = torch.cuda.current_stream() if which == "default" else torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        gb = torch.empty(W * H * 16, dtype=torch.uint8, device="cuda")
+        out = torch.empty(W * H * 4, dtype=torch.float32, device="cuda")
+        acc = torch.zeros(W * H * 4, dtype=torch.float64, device="cuda")
+        ctx.gbuffer_dev(ds, cam, W, H, gb.data_ptr(), stream=s.cuda_stream)
+        for f in range(n):
+            ctx.pt_render_dev(ds, p, cam, 10 + f, W, H, gb.data_ptr(), out.data_ptr(), stream=s.cuda_stream)
+            acc += out
+            out.fill_(float("nan"))  # a frame that read stale or half-written output shows up
+        got = acc.cpu().numpy().reshape(H, W, 4)
+    assert got.tobytes() == ref.tobytes()
+
+
 def test_zero_sized_image_is_a_no_op(ctx, small):
     sc, ds, _ = small
     cam = ctx.camera(sc.camera, 0)
