@@ -32,7 +32,7 @@ NODE_BYTES = {4: 128, 8: 80}  # BVH4Node (fp32 boxes), BVH8Node (8-bit quantized
 TRI_BYTES = 48        # TriRecord
 RAY_IO_BYTES = 32 + 16  # queued ray in, hit out
 # HBM bytes per dispatch of the intersect kernel, measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
-# passes over this workload and calibrated against a plain copy (python -m merian_amd.profiling traffic)
+# passes over this workload, calibrated against gather and copy yardsticks (python -m merian_amd.profiling traffic)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_trace_queue_full_1080p.traffic.json")
 TRAFFIC_WORKLOAD = "full_1080p enclosed room: 1920x1080, 64 spp, max path length 8, 1999896 triangles, BSDF-only pt"
 # autotune result of the workload, applied before timing (re-run the search with --retune)
@@ -47,12 +47,14 @@ def roofline(kernel: str, avg_ms: float, launches: int, algorithmic_bytes: float
              match: dict | None = None, extra: dict | None = None) -> dict:
     """Bandwidth roofline of one kernel. `achieved` is the HBM-side traffic per dispatch that a
     rocprofv3 profile of the same workload measured (FETCH_SIZE + WRITE_SIZE, each scaled by what
-    the counter reads for a plain copy of known size; profiles/*.traffic.json) over the kernel's
+    the counter reads for a yardstick of known size; profiles/*.traffic.json) over the kernel's
     average launch time measured here with HIP events: the bytes the kernel actually moves. The
     algorithmic bytes (every node, triangle and ray the traversal touches, most of them served by
     L2) stay beside it as a diagnostic. The profile applies only if it was taken with the settings
     in `match` (tuning, workload); otherwise `achieved` falls back to the algorithmic rate and says
-    so."""
+    so. Reads are scaled by what FETCH_SIZE shows for random whole-line gathers of known size
+    (exact: 64 bytes per request, two per 128-byte line fill); `traffic_upper` scales them by the
+    copy's factor instead (streaming reads count one request per 128-byte line)."""
     alg_gbs = algorithmic_bytes / (avg_ms * 1e-3) / 1e9
     r = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s", "kernel": kernel, "avg_launch_ms": round(avg_ms, 4),
          "launches": launches, "algorithmic_bytes_per_launch": round(algorithmic_bytes),
@@ -74,9 +76,14 @@ def roofline(kernel: str, avg_ms: float, launches: int, algorithmic_bytes: float
         traffic = float(tj["traffic_bytes_per_dispatch"])
         achieved = traffic / (avg_ms * 1e-3) / 1e9
         r.update({"achieved": round(achieved, 1), "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": round(traffic),
-                  "basis": "HBM-side bytes per dispatch (FETCH_SIZE + WRITE_SIZE, copy-calibrated) / launch time",
+                  "basis": "HBM-side bytes per dispatch / launch time: FETCH_SIZE scaled by a random line-gather "
+                           "yardstick, WRITE_SIZE by a copy" if "gather_factor_fetch" in tj else
+                           "HBM-side bytes per dispatch (FETCH_SIZE + WRITE_SIZE, copy-calibrated) / launch time",
                   "traffic_over_algorithmic": round(traffic / algorithmic_bytes, 3),
                   "profile": os.path.relpath(profile, ROOT)})
+        if tj.get("traffic_upper_bytes_per_dispatch") is not None:
+            up = float(tj["traffic_upper_bytes_per_dispatch"])
+            r.update({"traffic_upper": round(up), "frac_upper": round(up / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)})
         for k in ("l2_hit_pct", "valu_busy_pct", "valu_lane_util_pct", "mall_hit_pct", "dram_read_gbs"):
             if tj.get(k) is not None:
                 r[k] = tj[k]
@@ -229,7 +236,7 @@ def cpu_ray_query(threads: int) -> dict:
 
 
 def guided_leg(ctx, scene, L: int, spp: int = 64, ref_frames: int = 256, budget_s: float = 3.0,
-               train_frames: int = 4) -> dict:
+               train_frames: int | None = None, log2_table: int = 19, guided_bounces: int = 3) -> dict:
     """Learned guiding at the 1080p workload's settings (1920x1080, 64 spp frames, L = 8): relMSE at
     equal device time against a 16384-spp plain render (256 frames of 64 spp).
 
@@ -238,8 +245,9 @@ def guided_leg(ctx, scene, L: int, spp: int = 64, ref_frames: int = 256, budget_
     reference E[(x - ref)^2] = E[(x - mu)^2] + var(ref), so relMSE = mean(((x - ref)^2 - var(ref)) /
     (ref^2 + 1e-2)); the raw figures are reported beside it.
 
-    Protocol: the guide trains online on its first `train_frames` frames (16 Adam steps each) and
-    then renders with the weights frozen; every frame's samples are kept (one-sample MIS keeps each
+    Protocol: the guide trains online after each of its first `train_frames` frames (None: every
+    frame; 16 Adam steps of the frame's records each) and then renders with the weights frozen;
+    every frame's samples are kept (one-sample MIS keeps each
     frame unbiased whatever the weights). Frames render on one stream with device-side accumulation;
     device time is taken with HIP events around each frame. The plain pt and the Markov-chain
     guiding comparator (MCPG, learning online from an empty state with the MCPG example graph's
@@ -308,13 +316,13 @@ def guided_leg(ctx, scene, L: int, spp: int = 64, ref_frames: int = 256, budget_
         return float(np.mean(d / den)), float(np.mean((d - var_ref) / den))
 
     # guided: online training on the first frames, then frozen; every frame kept
-    g = Guide(ctx, ds, guide_config(seed=1, train_steps=16, guided_bounces=3))
+    g = Guide(ctx, ds, guide_config(seed=1, train_steps=16, guided_bounces=guided_bounces, log2_table=log2_table))
     gp = pt_params(spp=spp, max_path_length=L, mode=abi.MODE_GUIDED, record_signature=False)
     gacc = torch.zeros(w * h * 4, dtype=torch.float64, device="cuda")
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t_g, n_g, t_train = 0.0, 0, 0.0
-    while n_g < train_frames or t_g < budget_s:
-        train = n_g < train_frames
+    while (train_frames is not None and n_g < train_frames) or t_g < budget_s:
+        train = train_frames is None or n_g < train_frames
         e0.record()
         frame(1000 + n_g, gp, guide=g, train=train)
         gacc += out
@@ -337,14 +345,16 @@ def guided_leg(ctx, scene, L: int, spp: int = 64, ref_frames: int = 256, budget_
     ref_noise = float(np.mean(var_ref / den))
     res = {"workload": f"{w}x{h}, {spp} spp per frame, L={L}, {scene.triangle_count:,} triangles (the 1080p scene); "
                        f"reference {ref_frames * spp} spp plain",
-           "protocol": f"equal device time (HIP events per frame): guided trains online on its first {train_frames} "
-                       f"frames (16 Adam steps each, guide on vertices 0-2) then renders frozen, all frames kept; "
-                       f"plain and MCPG render frames until the same device time",
+           "protocol": "equal device time (HIP events per frame): guided trains online after "
+                       + ("every frame" if train_frames is None else f"its first {train_frames} frames, then frozen")
+                       + f" (16 Adam steps each, guide on vertices 0-{guided_bounces - 1}, 2^{log2_table}-entry "
+                       "hash tables), all frames kept; "
+                       "plain and MCPG render frames until the same device time",
            "relmse_guided": rg, "relmse_plain": rp, "guided_over_plain": round(rg / rp, 4) if rp > 0 else None,
            "relmse_guided_raw": rg_raw, "relmse_plain_raw": rp_raw, "reference_noise_relmse": ref_noise,
            "seconds_guided": round(t_g, 3), "seconds_guided_training": round(t_train, 3),
            "seconds_plain": round(t_p, 3), "frames_guided": n_g, "frames_plain": n_p,
-           "ms_per_frame_guided_frozen": round(1e3 * (t_g - t_train) / max(1, n_g - train_frames), 2),
+           "ms_per_frame_guided": round(1e3 * t_g / max(1, n_g), 2),
            "ms_per_frame_plain": round(1e3 * t_p / n_p, 2),
            "relmse_mcpg": rm, "relmse_mcpg_raw": rm_raw, "seconds_mcpg": round(t_m, 3), "frames_mcpg": n_m,
            "mcpg_note": "MCPG guides rough glass: the GGX BTDF evaluation it shares with the plain pt is non-zero "
@@ -354,14 +364,18 @@ def guided_leg(ctx, scene, L: int, spp: int = 64, ref_frames: int = 256, budget_
     return res
 
 
-def serving_leg(ctx, single_frame_mseg: float, n_jobs: int = 400, tune_ms: float = 1500.0) -> dict:
+MAX_PATHS_PER_JOB = 1 << 26  # paths in flight per serving job: a 1080p 64-spp job runs in 2 passes
+
+
+def serving_leg(ctx, single_frame_mseg: float, n_jobs: int = 400, tune_ms: float = 1500.0, slots: int = 8) -> dict:
     """Serving mix: Poisson arrivals over 8 resident scenes (seeds 101-108, 103,682 to 1,999,896
     triangles), resolutions {256^2, 512^2, 1024x768, 1920x1080}, spp {1, 4, 16, 64}, L in 4..8,
-    admitted by the server's HBM budget, at 80 % of the capacity a closed batch of 32 jobs reaches.
+    admitted by the server's HBM budget, at 80 % of the capacity a closed batch of 96 jobs of the
+    mix reaches (submitted at once, timed until the last is done).
     Latency = submit -> result in the caller's host buffer; p50 / p99 are nearest-rank over the
     `n_jobs` timed jobs. A job's host buffer is dropped as soon as it is done."""
     import numpy as np
-    from merian_amd import scenes
+    from merian_amd import abi, scenes
     from merian_amd.runtime import Server, pt_params
     rng = np.random.default_rng(3)
     t0 = time.perf_counter()
@@ -386,12 +400,17 @@ def serving_leg(ctx, single_frame_mseg: float, n_jobs: int = 400, tune_ms: float
         f = int(rng.integers(1 << 16))
         return dss[k], pt_params(spp=spp, max_path_length=L, record_signature=False), ctx.camera(spec, f), f, w, h
 
-    with Server(ctx, max_slots=8) as srv:
-        batch = [job() for _ in range(32)]
+    with Server(ctx, max_slots=slots, max_paths_per_job=MAX_PATHS_PER_JOB) as srv:
+        # capacity: a closed batch of 96 jobs of the mix, submitted at once, over the time until
+        # the last one is done (the drain of the last jobs included, so the Poisson arrivals at
+        # 80 % of it keep the queue stable)
+        batch = [job() for _ in range(96)]
         t0 = time.perf_counter()
-        for s_ in [srv.submit(*j) for j in batch]:
-            s_.wait()
+        for h_ in [srv.submit(*j) for j in batch]:
+            h_.wait()
         capacity = len(batch) / (time.perf_counter() - t0)
+        cap_st = srv.stats()
+        cap_mseg = cap_st.segments / (time.perf_counter() - t0) / 1e6
         srv.reset_stats()
         lam = 0.8 * capacity
         jobs = [job() for _ in range(n_jobs)]
@@ -415,8 +434,11 @@ def serving_leg(ctx, single_frame_mseg: float, n_jobs: int = 400, tune_ms: float
     mseg = st.segments / wall / 1e6
     res = {"workload": f"{n_jobs} Poisson-arrival jobs over 8 resident scenes "
                        f"({', '.join(f'{s.triangle_count:,}' for s in resident)} triangles), res {{256^2, 512^2, "
-                       "1024x768, 1920x1080}, spp {1,4,16,64}, L 4..8, 8 slots",
+                       f"1024x768, 1920x1080}}, spp {{1,4,16,64}}, L 4..8, {slots} slots, "
+                       f"{MAX_PATHS_PER_JOB} paths in flight per job",
            "arrival_rate_per_s": round(lam, 2), "capacity_jobs_per_s": round(capacity, 2),
+           "capacity_mseg_per_s": round(cap_mseg, 1),
+           "capacity_fraction_of_single_frame": round(cap_mseg / single_frame_mseg, 3),
            "p50_ms": round(st.p50_ms, 2), "p99_ms": round(st.p99_ms, 2), "max_ms": round(st.max_ms, 2),
            "mean_ms": round(st.mean_ms, 2), "jobs_per_s": round(st.completed / wall, 2),
            "mseg_per_s": round(mseg, 1), "fraction_of_single_frame": round(mseg / single_frame_mseg, 3),
@@ -530,6 +552,9 @@ def main():
     ap.add_argument("--no-guided", action="store_true", help="skip the equal-time guided vs plain leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-mix leg")
     ap.add_argument("--serving-jobs", type=int, default=400, help="timed jobs of the serving mix")
+    ap.add_argument("--serving-slots", type=int, default=8, help="jobs in flight in the serving mix")
+    ap.add_argument("--single-frame-mseg", type=float, default=0.0,
+                    help="--only serving: the single-frame rate its fractions are quoted against")
     ap.add_argument("--no-ray-query", action="store_true", help="skip the ray-query microbenchmark leg")
     ap.add_argument("--no-small", action="store_true", help="skip the small plain-pt leg (512^2, 4 spp, L = 4)")
     ap.add_argument("--retune-rays", action="store_true",
@@ -539,6 +564,8 @@ def main():
     ap.add_argument("--small-frames", type=int, default=64)
     ap.add_argument("--ray-reps", type=int, default=3)
     ap.add_argument("--ref-frames", type=int, default=256, help="64-spp frames of the guided leg's reference")
+    ap.add_argument("--guide-table", type=int, default=19, help="log2 entries per hash-grid level of the guide")
+    ap.add_argument("--guided-bounces", type=int, default=3, help="vertices [0, n) sample the guide")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; --gpus N > 1 is not supported", file=sys.stderr)
@@ -558,9 +585,10 @@ def main():
         elif args.only == "rays":
             leg = ray_query_leg(ctx, reps=args.ray_reps, retune=args.retune_rays)
         elif args.only == "serving":
-            leg = serving_leg(ctx, 1.0, n_jobs=args.serving_jobs)
+            leg = serving_leg(ctx, args.single_frame_mseg or 1.0, n_jobs=args.serving_jobs, slots=args.serving_slots)
         else:
-            leg = guided_leg(ctx, scenes.full_scene(seed=2), args.max_path_length, ref_frames=args.ref_frames)
+            leg = guided_leg(ctx, scenes.full_scene(seed=2), args.max_path_length, ref_frames=args.ref_frames,
+                             log2_table=args.guide_table, guided_bounces=args.guided_bounces)
         ctx.close()
         print(json.dumps({args.only: leg}), flush=True)
         return
@@ -662,9 +690,10 @@ def main():
             "roofline": roof, "pcie_inclusive": pcie}
     ds.destroy()
     if not args.no_guided:
-        line["guided"] = guided_leg(ctx, scene, args.max_path_length, ref_frames=args.ref_frames)
+        line["guided"] = guided_leg(ctx, scene, args.max_path_length, ref_frames=args.ref_frames,
+                                    log2_table=args.guide_table, guided_bounces=args.guided_bounces)
     if not args.no_serving:
-        line["serving"] = serving_leg(ctx, value, n_jobs=args.serving_jobs)
+        line["serving"] = serving_leg(ctx, value, n_jobs=args.serving_jobs, slots=args.serving_slots)
     if not args.no_ray_query:
         line["ray_query"] = ray_query_leg(ctx, reps=args.ray_reps, retune=args.retune_rays)
     if not args.no_small:

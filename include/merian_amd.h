@@ -403,7 +403,7 @@ typedef struct mrt_server_config {
     uint64_t hbm_budget;         /* bytes for the buffers of jobs in flight; reserved from the
                                     context's budget at creation (0 = half of what is left) */
     uint32_t max_slots;          /* jobs in flight at once (0 = 4, at most 16) */
-    uint32_t max_paths_per_job;  /* paths in flight per job (0 = 2^22); more spp -> chunks */
+    uint32_t max_paths_per_job;  /* paths in flight per job (0 = 2^24); more spp -> chunks */
     int32_t use_graphs;          /* 1: capture each job in a hipGraph; 0: launch directly */
     uint32_t _pad;
 } mrt_server_config;

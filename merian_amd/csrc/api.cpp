@@ -733,6 +733,13 @@ mrt_status render_frame_dev(mrt_ctx* c, mrt_scene* s, const mrt_pt_params* prm, 
     if (guide) {
         st = guide_prepare_frame(c, guide, npix * chunk, pb, a.guide_fraction, a.guided_bounces, &net);
         if (st != MRT_OK) return st;
+        // one record per guided vertex of the first pass's sample-0 paths: when more would
+        // arrive than the buffer holds, keep a pseudo-random 2^-shift of the pixels rather than
+        // the first pixels of the queue (which would be the top of the image)
+        const uint64_t verts = std::min<uint64_t>(a.guided_bounces, uint64_t(prm->max_path_length - 1));
+        uint32_t shift = 0;
+        while (shift < 31 && ((npix * verts) >> shift) > uint64_t(pb.grec_cap)) shift++;
+        a.guide_record_shift = shift;
     }
     FrameTiming t;
     uint64_t segs = 0;

@@ -14,6 +14,9 @@ constexpr int GB = 256;        // threads per block (4 waves)
 constexpr int NT = 2;          // 16-query tiles per wave iteration
 constexpr int WQ = 16 * NT;    // queries per wave iteration
 constexpr int BQ = 4 * WQ;     // queries per block iteration
+constexpr int INT_ = 1;        // inference: 16-query tiles per wave iteration
+constexpr int IWQ = 16 * INT_;
+constexpr int IBQ = 4 * IWQ;
 
 __device__ __forceinline__ floatx4 mfma(const half8_t a, const half8_t b, floatx4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
@@ -107,12 +110,12 @@ __device__ __forceinline__ void encode(const GuideNet& net, f3 p, f3 wo, int g, 
     for (int s = 4; s < 8; s++) x[1][s] = _Float16(0.f);
 }
 
-// acc[t][ot] = W_layer (rows 16 ot..) * x[t] for ot < OT.
-template <int OT>
-__device__ __forceinline__ void layer_mfma(const half8_t* frag, int layer, const half8_t (&x)[NT][2],
-                                           floatx4 (&acc)[NT][4], int lane) {
+// acc[t][ot] = W_layer (rows 16 ot..) * x[t] for ot < OT and each of the T tiles.
+template <int OT, int T = NT>
+__device__ __forceinline__ void layer_mfma(const half8_t* frag, int layer, const half8_t (&x)[T][2],
+                                           floatx4 (&acc)[T][4], int lane) {
 #pragma unroll
-    for (int t = 0; t < NT; t++)
+    for (int t = 0; t < T; t++)
 #pragma unroll
         for (int ot = 0; ot < 4; ot++) acc[t][ot] = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -121,15 +124,16 @@ __device__ __forceinline__ void layer_mfma(const half8_t* frag, int layer, const
         for (int ot = 0; ot < OT; ot++) {
             const half8_t a = frag[((layer * 4 + ot) * 2 + c) * 64 + lane];
 #pragma unroll
-            for (int t = 0; t < NT; t++) acc[t][ot] = mfma(a, x[t][c], acc[t][ot]);
+            for (int t = 0; t < T; t++) acc[t][ot] = mfma(a, x[t][c], acc[t][ot]);
         }
     }
 }
 
 // D layout -> B operand of the next layer (ReLU, fp16).
-__device__ __forceinline__ void relu_to_operand(const floatx4 (&acc)[NT][4], half8_t (&x)[NT][2]) {
+template <int T = NT>
+__device__ __forceinline__ void relu_to_operand(const floatx4 (&acc)[T][4], half8_t (&x)[T][2]) {
 #pragma unroll
-    for (int t = 0; t < NT; t++)
+    for (int t = 0; t < T; t++)
 #pragma unroll
         for (int c = 0; c < 2; c++)
 #pragma unroll
@@ -176,82 +180,101 @@ struct ArrayQueries {
     }
 };
 
-// Sinks of the 48 outputs of one query (gathered in LDS): raw outputs or packed lobes.
+// Output rows of the inference kernels' last layer: lane group g (lanes 16g..16g+15) holds D rows
+// 16 ot + 4 g + r (ot < 3, r < 4) of its query, so the last layer's rows are permuted when its
+// weights are staged: slot k = 4 ot + r of group g computes output 10 g + k for k < 10 (lobes 2g
+// and 2g+1, five outputs each), a zero row for k = 10, 11. Each lane then holds two whole
+// lobes and no output has to move between lanes through LDS.
+__device__ __forceinline__ int out_row_of(int row) {
+    const int ot = row >> 4, g = (row & 15) >> 2, k = 4 * ot + (row & 3);
+    return k < 10 ? 10 * g + k : -1;
+}
+
+// Sinks of one query's ten outputs held by lane group g: raw outputs or packed lobes. Every lane
+// of the wave calls store (the lobe softmax reduces across the query's four lane groups).
 struct RawSink {
     float* out;
-    // lanes 2q, 2q+1 handle query q: each writes 20 of the 40 outputs
-    __device__ __forceinline__ void store(uint32_t i, const float* o, int half_) const {
-        for (int k = 0; k < 20; k++) out[size_t(i) * G_OUT + 20 * half_ + k] = o[20 * half_ + k];
+    __device__ __forceinline__ void store(uint32_t i, bool valid, const float (&o)[10], int g) const {
+        if (!valid) return;
+#pragma unroll
+        for (int k = 0; k < 10; k++) out[size_t(i) * G_OUT + 10 * g + k] = o[k];
     }
 };
 struct LobeSink {
     uint4* lobes;
-    __device__ __forceinline__ void store(uint32_t i, const float* o, int half_) const {
-        float zmax = o[0];
+    __device__ __forceinline__ void store(uint32_t i, bool valid, const float (&o)[10], int g) const {
+        // softmax over the 8 lobe logits: two per lane, the other six in lanes +-16, +-32
+        float zmax = fmaxf(o[0], o[5]);
+        zmax = fmaxf(zmax, __shfl_xor(zmax, 16));
+        zmax = fmaxf(zmax, __shfl_xor(zmax, 32));
+        const float e0 = expf(o[0] - zmax), e1 = expf(o[5] - zmax);
+        float sum = e0 + e1;
+        sum += __shfl_xor(sum, 16);
+        sum += __shfl_xor(sum, 32);
+        if (!valid) return;
+        uint32_t w[4];
 #pragma unroll
-        for (int j = 1; j < G_LOBES; j++) zmax = fmaxf(zmax, o[5 * j]);
-        float e[G_LOBES], sum = 0.f;
-#pragma unroll
-        for (int j = 0; j < G_LOBES; j++) {
-            e[j] = expf(o[5 * j] - zmax);
-            sum += e[j];
-        }
-        uint32_t w[8];
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int j = 4 * half_ + k;
-            const float pi = e[j] / sum;
-            const float kappa = expf(fminf(G_KAPPA_LOG_MAX, fmaxf(G_KAPPA_LOG_MIN, o[5 * j + 1])));
-            f3 v{o[5 * j + 2], o[5 * j + 3], o[5 * j + 4]};
+        for (int h = 0; h < 2; h++) {
+            const float pi = (h ? e1 : e0) / sum;
+            const float kappa = expf(fminf(G_KAPPA_LOG_MAX, fmaxf(G_KAPPA_LOG_MIN, o[5 * h + 1])));
+            f3 v{o[5 * h + 2], o[5 * h + 3], o[5 * h + 4]};
             const float len = sqrtf(dot(v, v));
             const f3 mu = len > 1e-6f ? v * (1.f / len) : f3{0.f, 0.f, 1.f};
-            w[2 * k] = uint32_t(float_to_half(pi)) | (uint32_t(float_to_half(kappa)) << 16);
-            w[2 * k + 1] = encode_normal(mu);
+            w[2 * h] = uint32_t(float_to_half(pi)) | (uint32_t(float_to_half(kappa)) << 16);
+            w[2 * h + 1] = encode_normal(mu);
         }
-        uint4* dst = lobes + size_t(i) * 4 + 2 * half_;
-        dst[0] = make_uint4(w[0], w[1], w[2], w[3]);
-        dst[1] = make_uint4(w[4], w[5], w[6], w[7]);
+        lobes[size_t(i) * 4 + g] = make_uint4(w[0], w[1], w[2], w[3]);
     }
 };
 
+// Inference: one 16-query tile per wave iteration, the forward weights in LDS (the last layer's
+// rows permuted, out_row_of), 4 waves per block. LDS holds nothing else (40 KB: four blocks per
+// CU), a query's outputs stay in the registers of the lanes that computed them, and one tile
+// keeps the kernel within 128 registers without spills: four waves per SIMD hide the latency of
+// the hash-grid gathers. The grid is capped at the resident blocks (1024), so the weights are
+// staged once per block.
 template <class Q, class Sink>
-__global__ __launch_bounds__(GB) void k_guide_infer(GuideNet net, Q q, Sink sink) {
-    __shared__ half8_t frag[G_FRAGS * 64];          // 40 KB: forward fragments of all layers
-    __shared__ float outs[4][WQ][48];               // 24 KB: per wave, the outputs of its queries
-    for (int k = threadIdx.x; k < G_FRAGS * 64; k += GB) frag[k] = net.frag_fwd[k];
+__global__ __launch_bounds__(GB) __attribute__((amdgpu_waves_per_eu(4))) void k_guide_infer(GuideNet net, Q q, Sink sink) {
+    __shared__ half8_t frag[G_FRAGS * 64]; // 40 KB: forward fragments of all layers
+    for (int k = threadIdx.x; k < G_FRAGS * 64; k += GB) {
+        const int layer = k / (8 * 64);
+        if (layer < G_LAYERS - 1) {
+            frag[k] = net.frag_fwd[k];
+        } else {
+            // [layer][ot][c][lane']: lane' = 16 kg + rr holds row 16 ot + rr of k-group kg
+            const int ot = (k >> 7) & 3, c = (k >> 6) & 1, l = k & 63;
+            const int o = out_row_of(16 * ot + (l & 15));
+            frag[k] = o < 0 ? half8_t{} : net.frag_fwd[((layer * 4 + (o >> 4)) * 2 + c) * 64 + (l & ~15) + (o & 15)];
+        }
+    }
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4;
     const uint32_t n = q.n();
-    for (uint32_t base = blockIdx.x * BQ; base < n; base += gridDim.x * BQ) { // block-uniform trip count
-        half8_t x[NT][2];
-        floatx4 acc[NT][4];
+    const uint32_t stride = gridDim.x * IBQ;
+    for (uint32_t base = blockIdx.x * IBQ + wave * IWQ; base < n; base += stride) { // wave-uniform
+        half8_t x[INT_][2];
+        floatx4 acc[INT_][4];
+        bool valid[INT_];
 #pragma unroll
-        for (int t = 0; t < NT; t++) {
-            const uint32_t i = base + wave * WQ + t * 16 + (lane & 15);
+        for (int t = 0; t < INT_; t++) {
+            const uint32_t i = base + t * 16 + (lane & 15);
             f3 p{0.f, 0.f, 0.f}, wo{0.f, 0.f, 1.f};
-            if (i < n) q.load(i, p, wo);
+            valid[t] = i < n && q.load(i, p, wo);
             encode(net, p, wo, g, x[t]);
         }
 #pragma unroll
         for (int layer = 0; layer < 4; layer++) {
-            layer_mfma<4>(frag, layer, x, acc, lane);
-            relu_to_operand(acc, x);
+            layer_mfma<4, INT_>(frag, layer, x, acc, lane);
+            relu_to_operand<INT_>(acc, x);
         }
-        layer_mfma<3>(frag, 4, x, acc, lane);
+        layer_mfma<3, INT_>(frag, 4, x, acc, lane);
 #pragma unroll
-        for (int t = 0; t < NT; t++)
+        for (int t = 0; t < INT_; t++) {
+            float o[10];
 #pragma unroll
-            for (int ot = 0; ot < 3; ot++)
-#pragma unroll
-                for (int r = 0; r < 4; r++) outs[wave][t * 16 + (lane & 15)][16 * ot + 4 * g + r] = acc[t][ot][r];
-        __syncthreads();
-        {
-            const int qi = lane >> 1, half_ = lane & 1;
-            const uint32_t i = base + wave * WQ + qi;
-            f3 p, wo;
-            if (i < n && q.load(i, p, wo)) sink.store(i, outs[wave][qi], half_);
+            for (int k = 0; k < 10; k++) o[k] = acc[t][k >> 2][k & 3];
+            sink.store(base + t * 16 + (lane & 15), valid[t], o, g);
         }
-        __syncthreads();
     }
 }
 
@@ -573,7 +596,7 @@ hipError_t launch_guide_infer_queue(const GuideNet& net, const DeviceScene& s, c
                                     hipStream_t st) {
     if (n_max == 0) return hipSuccess;
     const QueueQueries q{s, qo, qd, hits, count, n_max};
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_guide_infer<QueueQueries, LobeSink>), dim3(blocks_for(n_max, BQ, 2048)),
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_guide_infer<QueueQueries, LobeSink>), dim3(blocks_for(n_max, IBQ, 1024)),
                        dim3(GB), 0, st, net, q, LobeSink{lobes});
     return hipGetLastError();
 }
@@ -582,7 +605,7 @@ hipError_t launch_guide_forward(const GuideNet& net, const float4* pos, const fl
                                 hipStream_t st) {
     if (n == 0) return hipSuccess;
     const ArrayQueries q{pos, wo, n};
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_guide_infer<ArrayQueries, RawSink>), dim3(blocks_for(n, BQ, 2048)), dim3(GB),
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_guide_infer<ArrayQueries, RawSink>), dim3(blocks_for(n, IBQ, 1024)), dim3(GB),
                        0, st, net, q, RawSink{out});
     return hipGetLastError();
 }

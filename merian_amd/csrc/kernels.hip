@@ -184,9 +184,10 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DeviceScene S, Src src, S
                                                        uint32_t* __restrict__ work, uint32_t mask, TraceScratch ts,
                                                        unsigned long long* counters) {
     __shared__ uint32_t stack[TRACE_LDS_STACK * TRACE_BLOCK];
+    __shared__ uint32_t ents[(W == 8 ? 8 : 1) * TRACE_BLOCK]; // BVH8 node step: child entries by slot
     const uint32_t n = count_ptr ? *count_ptr : n_fixed;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    const LaneStack stk{stack + threadIdx.x, ts.ovf + gtid, gridDim.x * blockDim.x};
+    const LaneStack stk{stack + threadIdx.x, ts.ovf + gtid, gridDim.x * blockDim.x, ents + threadIdx.x};
     TraversalCount cnt;
     if constexpr (W == 8) {
         trace_persistent<TRACE_LDS_STACK, COUNT, Src, Sink, ANY>(DeviceBVH8{S.nodes8, S.tris}, src, sink, n, work,
@@ -736,8 +737,13 @@ __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(MOD
                             const f3 f = sp.mat.eval(wi_local, wl_local);
                             const float wl_ng = dot(wl, ng);
                             const bool trans = wi_local.z * wl_local.z < 0.f;
-                            if ((f.x > 0.f || f.y > 0.f || f.z > 0.f) && ((wi_below != (wl_ng < 0.f)) == trans)) {
-                                const float w = power_heuristic(env_pdf(S, wl), sp.mat.pdf(wi_local, wl_local));
+                            // Only directions the BSDF sampler can produce (pdf > 0): the BSDF-only
+                            // estimator integrates f over that support (the GGX BTDF's eval is
+                            // non-zero outside it), and NEE must estimate the same integral.
+                            const float pb = sp.mat.pdf(wi_local, wl_local);
+                            if (pb > 0.f && (f.x > 0.f || f.y > 0.f || f.z > 0.f) &&
+                                ((wi_below != (wl_ng < 0.f)) == trans)) {
+                                const float w = power_heuristic(env_pdf(S, wl), pb);
                                 sh_c = ((thr * f) * env_eval(S, wl)) * (w / pdf_l);
                                 sh_o = offset_ray_origin(sp.pos, wl_ng < 0.f ? -ng : ng);
                                 sh_d = wl;
@@ -768,16 +774,22 @@ __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(MOD
                         if (ok) {
                             const f3 wo = sp.frame.to_world(wo_l);
                             const f3 f = sp.mat.eval(wi_local, wo_l);
-                            const float p = alpha * lobes_pdf(L, wo) + (1.f - alpha) * sp.mat.pdf(wi_local, wo_l);
+                            const float pb = sp.mat.pdf(wi_local, wo_l);
+                            const float p = alpha * lobes_pdf(L, wo) + (1.f - alpha) * pb;
                             const float wo_ng = dot(wo, ng);
                             const bool transmission = wi_local.z * wo_l.z < 0.f;
-                            if (p > 0.f && (f.x > 0.f || f.y > 0.f || f.z > 0.f) &&
+                            // as NEE: only directions inside the BSDF sampler's support (pb > 0),
+                            // the domain the BSDF-only estimator integrates over; this also bounds
+                            // the weight by f / ((1 - alpha) pb)
+                            if (pb > 0.f && p > 0.f && (f.x > 0.f || f.y > 0.f || f.z > 0.f) &&
                                 (wi_below != (wo_ng < 0.f)) == transmission) {
                                 thr = thr * (f / p);
                                 next_o = offset_ray_origin(sp.pos, wo_ng < 0.f ? -ng : ng);
                                 next_d = wo;
                                 push = finite3(thr);
-                                record = push && A.s0 == 0 && pid < A.npix;
+                                record = push && A.s0 == 0 && pid < A.npix &&
+                                         (A.guide_record_shift == 0u ||
+                                          ((pid * 2654435761u) >> (32u - A.guide_record_shift)) == 0u);
                                 rec_pos = sp.pos;
                                 rec_rad = rad;
                                 rec_pdf = p;

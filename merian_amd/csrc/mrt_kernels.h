@@ -82,6 +82,10 @@ struct PtArgs {
     int32_t mode;     // MRT_MODE_*
     float guide_fraction; // MRT_MODE_GUIDED: probability of sampling the guide at an eligible vertex
     uint32_t guided_bounces; // MRT_MODE_GUIDED: vertices [0, guided_bounces) use the guide
+    // MRT_MODE_GUIDED: training records come from the pixels p with
+    // (p * 2654435761) >> (32 - shift) == 0 (all pixels when 0), a pseudo-random 2^-shift of the
+    // image, so that a frame whose records outnumber the record buffer still trains on all of it
+    uint32_t guide_record_shift;
     int32_t demodulate; // divide every sample by the pixel's floored GBuffer albedo
     uint32_t s0;      // first sample index of this chunk
     uint32_t chunk;   // samples in this chunk

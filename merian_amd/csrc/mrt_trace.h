@@ -75,6 +75,7 @@ struct LaneStack {
     uint32_t* lds; // this lane's column
     uint32_t* ovf;
     uint32_t ovf_stride;
+    uint32_t* ents; // BVH8: this lane's column of the node step's 8 child entries (LDS)
 };
 template <int SL> __device__ __forceinline__ void stack_push(const LaneStack& s, int& sp, uint32_t e) {
     if (sp < SL) {
@@ -226,54 +227,64 @@ struct DeviceBVH8 {
     const float4* tris; // TriRecord as 3 x float4
 };
 
-// One inner-node step over the 8-wide quantised BVH: decode the eight child boxes
-// (fma(q, 2^e, origin), conservative by construction), slab-test them as the BVH4 step does,
-// sort the hit children near to far with a 19-comparator network, push all but the nearest,
-// continue with it.
+// One inner-node step over the 8-wide quantised BVH: slab-test the eight child boxes, sort the
+// hit children near to far, push all but the nearest, continue with it.
+//
+// Slab distances are taken in the node's grid: a plane at grid coordinate q lies at q 2^e + p,
+// so its distance is q (2^e / d) + (p / d - o / d) = fma(q, a, b) with a = 2^e ix (exact: a
+// power-of-two scale) and b = fma(p, ix, -ox) once per node and axis, one FMA per plane after
+// the byte conversion. The sort key of a hit child is its entry distance's bit pattern with the
+// child slot in the low three bits (misses: all ones), so a 19-comparator network of integer
+// min/max sorts keys alone; the child entries wait in an LDS column and are read back by slot.
+// Only the visiting order depends on the keys; which children are visited does not.
 template <int SL, bool COUNT>
 __device__ __forceinline__ void node_step(const DeviceBVH8 bvh, RayState& r, const LaneStack& stk,
                                           TraversalCount& count) {
     if constexpr (COUNT) count.nodes++;
     const uint4* np = reinterpret_cast<const uint4*>(bvh.nodes + r.cur);
     const uint4 h0 = np[0], h1 = np[1], qx = np[2], qy = np[3], qz = np[4];
-    const float px = __uint_as_float(h0.x), py = __uint_as_float(h0.y), pz = __uint_as_float(h0.z);
     const int ex = int(int8_t(h0.w & 0xFFu)), ey = int(int8_t((h0.w >> 8) & 0xFFu)), ez = int(int8_t((h0.w >> 16) & 0xFFu));
     const uint32_t imask = h0.w >> 24;
-    const float sx = __uint_as_float(uint32_t(ex + 127) << 23);
-    const float sy = __uint_as_float(uint32_t(ey + 127) << 23);
-    const float sz = __uint_as_float(uint32_t(ez + 127) << 23);
+    const float ax = __uint_as_float(uint32_t(ex + 127) << 23) * r.ix;
+    const float ay = __uint_as_float(uint32_t(ey + 127) << 23) * r.iy;
+    const float az = __uint_as_float(uint32_t(ez + 127) << 23) * r.iz;
+    const float bx = __builtin_fmaf(__uint_as_float(h0.x), r.ix, -r.ox);
+    const float by = __builtin_fmaf(__uint_as_float(h0.y), r.iy, -r.oy);
+    const float bz = __builtin_fmaf(__uint_as_float(h0.z), r.iz, -r.oz);
     const float tlim = r.tlim;
     const bool nx = r.neg & 1u, ny = r.neg & 2u, nz = r.neg & 4u;
-    const float inf = __builtin_inff();
     // near / far plane bytes per axis, selected by the ray's octant once per node (children 0-3
     // in the low word, 4-7 in the high word)
     const uint32_t nx0 = nx ? qx.z : qx.x, nx1 = nx ? qx.w : qx.y, fx0 = nx ? qx.x : qx.z, fx1 = nx ? qx.y : qx.w;
     const uint32_t ny0 = ny ? qy.z : qy.x, ny1 = ny ? qy.w : qy.y, fy0 = ny ? qy.x : qy.z, fy1 = ny ? qy.y : qy.w;
     const uint32_t nz0 = nz ? qz.z : qz.x, nz1 = nz ? qz.w : qz.y, fz0 = nz ? qz.x : qz.z, fz1 = nz ? qz.y : qz.w;
-    float key[8];
-    uint32_t ent[8];
+    constexpr uint32_t MISS = 0xFFFFFFFFu;
+    uint32_t key[8];
     uint32_t inner_next = h1.x, tri_next = h1.y;
 #pragma unroll
     for (int c = 0; c < 8; c++) {
         const uint32_t sh = 8u * uint32_t(c & 3);
-        const float NX = __builtin_fmaf(float(((c < 4 ? nx0 : nx1) >> sh) & 0xFFu), sx, px);
-        const float FX = __builtin_fmaf(float(((c < 4 ? fx0 : fx1) >> sh) & 0xFFu), sx, px);
-        const float NY = __builtin_fmaf(float(((c < 4 ? ny0 : ny1) >> sh) & 0xFFu), sy, py);
-        const float FY = __builtin_fmaf(float(((c < 4 ? fy0 : fy1) >> sh) & 0xFFu), sy, py);
-        const float NZ = __builtin_fmaf(float(((c < 4 ? nz0 : nz1) >> sh) & 0xFFu), sz, pz);
-        const float FZ = __builtin_fmaf(float(((c < 4 ? fz0 : fz1) >> sh) & 0xFFu), sz, pz);
-        const float tn = fmaxf(fmaxf(__builtin_fmaf(NX, r.ix, -r.ox), __builtin_fmaf(NY, r.iy, -r.oy)),
-                               fmaxf(__builtin_fmaf(NZ, r.iz, -r.oz), r.t0));
-        const float tf = fminf(fminf(__builtin_fmaf(FX, r.ix, -r.ox), __builtin_fmaf(FY, r.iy, -r.oy)),
-                               fminf(__builtin_fmaf(FZ, r.iz, -r.oz), tlim));
+        const float NX = __builtin_fmaf(float(((c < 4 ? nx0 : nx1) >> sh) & 0xFFu), ax, bx);
+        const float FX = __builtin_fmaf(float(((c < 4 ? fx0 : fx1) >> sh) & 0xFFu), ax, bx);
+        const float NY = __builtin_fmaf(float(((c < 4 ? ny0 : ny1) >> sh) & 0xFFu), ay, by);
+        const float FY = __builtin_fmaf(float(((c < 4 ? fy0 : fy1) >> sh) & 0xFFu), ay, by);
+        const float NZ = __builtin_fmaf(float(((c < 4 ? nz0 : nz1) >> sh) & 0xFFu), az, bz);
+        const float FZ = __builtin_fmaf(float(((c < 4 ? fz0 : fz1) >> sh) & 0xFFu), az, bz);
+        const float tn = fmaxf(fmaxf(NX, NY), fmaxf(NZ, r.t0));
+        const float tf = fminf(fminf(FX, FY), fminf(FZ, tlim));
         const uint32_t cnt = (h1.z >> (4 * c)) & 15u;
         const bool inner = (imask >> c) & 1u;
-        key[c] = (tn <= tf && (inner || cnt)) ? tn : inf;
-        ent[c] = inner ? inner_next : (LEAF_FLAG | (tri_next << 3) | ((cnt - 1u) & 7u));
+        key[c] = (tn <= tf && (inner || cnt)) ? ((__float_as_uint(tn) & ~7u) | uint32_t(c)) : MISS;
+        stk.ents[c * TRACE_BLOCK] = inner ? inner_next : (LEAF_FLAG | (tri_next << 3) | ((cnt - 1u) & 7u));
         inner_next += inner ? 1u : 0u;
         tri_next += cnt;
     }
-#define MRT_CS(a, b) cswap(key[a], ent[a], key[b], ent[b]);
+#define MRT_CS(a, b)                                                                                           \
+    {                                                                                                          \
+        const uint32_t lo_ = min(key[a], key[b]);                                                              \
+        key[b] = max(key[a], key[b]);                                                                          \
+        key[a] = lo_;                                                                                          \
+    }
     MRT_CS(0, 1) MRT_CS(2, 3) MRT_CS(4, 5) MRT_CS(6, 7)
     MRT_CS(0, 2) MRT_CS(1, 3) MRT_CS(4, 6) MRT_CS(5, 7)
     MRT_CS(1, 2) MRT_CS(5, 6)
@@ -281,11 +292,12 @@ __device__ __forceinline__ void node_step(const DeviceBVH8 bvh, RayState& r, con
     MRT_CS(2, 4) MRT_CS(3, 5)
     MRT_CS(1, 2) MRT_CS(3, 4) MRT_CS(5, 6)
 #undef MRT_CS
+    typedef const __attribute__((address_space(3))) uint32_t* lds_ptr;
 #pragma unroll
     for (int c = 7; c >= 1; c--)
-        if (key[c] != inf) stack_push<SL>(stk, r.sp, ent[c]);
-    if (key[0] != inf) {
-        r.cur = ent[0];
+        if (key[c] != MISS) stack_push<SL>(stk, r.sp, ((lds_ptr)stk.ents)[(key[c] & 7u) * TRACE_BLOCK]);
+    if (key[0] != MISS) {
+        r.cur = ((lds_ptr)stk.ents)[(key[0] & 7u) * TRACE_BLOCK];
     } else {
         r.cur = r.sp > 0 ? stack_pop<SL>(stk, r.sp) : TRAVERSAL_DONE;
     }

@@ -400,7 +400,7 @@ MRT_API mrt_status mrt_server_create(mrt_ctx* c, const mrt_server_config* cfg, m
     mrt_server_config k = cfg ? *cfg : mrt_server_config{};
     if (k.max_slots == 0) k.max_slots = 4;
     if (k.max_slots > MAX_SLOTS) return fail(c, MRT_ERR_INVALID, "max_slots above 16");
-    if (k.max_paths_per_job == 0) k.max_paths_per_job = 1u << 22;
+    if (k.max_paths_per_job == 0) k.max_paths_per_job = 1u << 24;
     const uint64_t left = c->budget > c->in_use ? c->budget - c->in_use : 0;
     if (k.hbm_budget == 0) k.hbm_budget = left / 2;
     if (k.hbm_budget > left) return fail(c, MRT_ERR_BUDGET, "server budget exceeds what the context has left");

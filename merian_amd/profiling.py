@@ -3,11 +3,14 @@ WRITE_SIZE counters) and summaries of rocprofv3's CSV output.
 
     python -m merian_amd.profiling copy --mib 1024 --reps 4
         one plain device-to-device copy of `mib` MiB per rep (run under rocprofv3 --pmc)
+    python -m merian_amd.profiling gather --mib 4096 --rows 16777216 --reps 2
+        random 128-byte line gathers of known count (run under rocprofv3 --pmc)
     python -m merian_amd.profiling stats DIR
         per-kernel table from DIR's *_kernel_stats.csv (rocprofv3 --kernel-trace --stats)
     python -m merian_amd.profiling pmc DIR COUNTER
         per-kernel mean of COUNTER per dispatch from DIR's *_counter_collection.csv
     python -m merian_amd.profiling traffic FETCH_DIR WRITE_DIR COPY_FETCH_DIR COPY_WRITE_DIR KERNEL
+            [--gather-fetch-dir DIR]
         calibrated HBM bytes per dispatch of KERNEL (JSON)
     python -m merian_amd.profiling valu DIR
         per-kernel VALU busy, lane utilisation and issue time from a pass with GRBM_GUI_ACTIVE,
@@ -78,10 +81,18 @@ def short_name(name: str) -> str:
 
 
 def traffic(fetch_dir: str, write_dir: str, copy_fetch_dir: str, copy_write_dir: str, kernel: str,
-            copy_bytes: int) -> dict:
-    """HBM bytes per dispatch of the kernels whose short name starts with `kernel`, from a
-    FETCH_SIZE run and a WRITE_SIZE run, each scaled by what the same counter reads for a plain
-    copy of `copy_bytes` bytes (which reads and writes exactly that many)."""
+            copy_bytes: int, gather_fetch_dir: str = "", gather_bytes: int = 0) -> dict:
+    """HBM-side bytes per dispatch of the kernels whose short name starts with `kernel`, from a
+    FETCH_SIZE run and a WRITE_SIZE run.
+
+    Yardsticks: a plain copy of `copy_bytes` (reads and writes exactly that many) and, when
+    given, a random gather of whole 128-byte lines (`gather_bytes` of lines). On gfx950
+    FETCH_SIZE counts 64 bytes per L2 read request: exact for scattered line fills (two requests
+    per 128-byte line; the gather reads back its line bytes), half the bytes of the copy's
+    streaming reads (one request per line). Reads are scaled by the gather's factor when it is
+    given (the traversal's reads are scattered node and triangle fetches; its ray-queue stream
+    would count low, so `traffic_upper_bytes_per_dispatch` scales the reads by the copy's factor
+    instead), else by the copy's. Writes are scaled by the copy's write factor."""
     def sel(d, counter):
         tot, n = 0.0, 0
         for k, v in counter_per_kernel(d, counter).items():
@@ -96,14 +107,24 @@ def traffic(fetch_dir: str, write_dir: str, copy_fetch_dir: str, copy_write_dir:
         vals = [v["mean"] for k, v in counter_per_kernel(d, counter).items() if "copy" in k.lower()]
         return max(vals)
 
+    def gather_kib(d):
+        vals = [v["mean"] for k, v in counter_per_kernel(d, "FETCH_SIZE").items() if "gather" in k.lower()]
+        return max(vals)
+
     f_kib, nf = sel(fetch_dir, "FETCH_SIZE")
     w_kib, nw = sel(write_dir, "WRITE_SIZE")
     cf = copy_bytes / (copy_kib(copy_fetch_dir, "FETCH_SIZE") * 1024.0)
     cw = copy_bytes / (copy_kib(copy_write_dir, "WRITE_SIZE") * 1024.0)
-    return {"kernel": kernel, "dispatches": [nf, nw], "fetch_kib_per_dispatch": round(f_kib, 1),
-            "write_kib_per_dispatch": round(w_kib, 1), "copy_bytes": copy_bytes,
-            "copy_factor_fetch": round(cf, 4), "copy_factor_write": round(cw, 4),
-            "traffic_bytes_per_dispatch": round(f_kib * 1024.0 * cf + w_kib * 1024.0 * cw)}
+    r = {"kernel": kernel, "dispatches": [nf, nw], "fetch_kib_per_dispatch": round(f_kib, 1),
+         "write_kib_per_dispatch": round(w_kib, 1), "copy_bytes": copy_bytes,
+         "copy_factor_fetch": round(cf, 4), "copy_factor_write": round(cw, 4)}
+    gf = cf
+    if gather_fetch_dir:
+        gf = gather_bytes / (gather_kib(gather_fetch_dir) * 1024.0)
+        r.update({"gather_line_bytes": gather_bytes, "gather_factor_fetch": round(gf, 4)})
+    r["traffic_bytes_per_dispatch"] = round(f_kib * 1024.0 * gf + w_kib * 1024.0 * cw)
+    r["traffic_upper_bytes_per_dispatch"] = round(f_kib * 1024.0 * max(cf, gf) + w_kib * 1024.0 * cw)
+    return r
 
 
 def agent(d: str) -> dict:
@@ -185,6 +206,12 @@ def guided_workload(width: int, height: int, spp: int, train_frames: int, frames
     scene = scenes.full_scene()
     spec = scenes.CameraSpec(scene.camera.eye, scene.camera.target, scene.camera.up, scene.camera.vfov, width, height)
     ds = ctx.scene(scene)
+    # the 1080p workload's cached settings (as bench.py applies them)
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning", "full_1080p.json")) as f:
+        t = ds.tuning()
+        for k, v in json.load(f)["tuning"].items():
+            setattr(t, k, v)
+        ds.set_tuning(t)
     cam = ctx.camera(spec, 0)
     gb = torch.empty(width * height * 16, dtype=torch.uint8, device="cuda")
     out = torch.empty(width * height * 4, dtype=torch.float32, device="cuda")
@@ -197,6 +224,10 @@ def guided_workload(width: int, height: int, spp: int, train_frames: int, frames
         torch.cuda.synchronize()
         print(json.dumps({"frame": f, "train": train, "ms": round(st.ms_total, 3),
                           "segments": st.continuation_segments}), flush=True)
+    st = ctx.pt_render_dev(ds, pt_params(spp=spp, max_path_length=8, record_signature=False), cam, 0, width, height,
+                           gb.data_ptr(), out.data_ptr(), stats=True)
+    torch.cuda.synchronize()
+    print(json.dumps({"plain_ms": round(st.ms_total, 3), "segments": st.continuation_segments}), flush=True)
     g.close()
     ds.destroy()
     ctx.close()
@@ -231,7 +262,7 @@ def traffic_record(args) -> dict:
     the same kernel's figures from the other passes (kernel-trace average, VALU, L2, DRAM
     requests), as bench.py's roofline reads it."""
     r = traffic(args.fetch_dir, args.write_dir, args.copy_fetch_dir, args.copy_write_dir, args.kernel,
-                args.copy_mib << 20)
+                args.copy_mib << 20, args.gather_fetch_dir, args.gather_rows * 128)
     if args.workload:
         r["workload"] = args.workload
     if args.tuning_file:
@@ -268,6 +299,26 @@ def copy_workload(mib: int, reps: int) -> None:
         dst.copy_(src)
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes": n, "reps": reps}), flush=True)
+
+
+def gather_workload(mib: int, rows: int, reps: int) -> None:
+    """Random 128-byte row gathers from a buffer of `mib` MiB (far larger than L2 and MALL): each
+    rep reads `rows` rows at random 128-byte-aligned offsets (out = table[idx]), one whole
+    128-byte L2 line each. A yardstick for the counters under scattered reads, as the traversal
+    makes them: the lines read are known (`rows` x 128 bytes, plus the `rows` x 8-byte index
+    stream)."""
+    import torch
+    table = torch.empty((mib << 20) // 128, 32, dtype=torch.float32, device="cuda:0")
+    table.fill_(1.0)
+    g = torch.Generator(device="cuda:0")
+    g.manual_seed(1)
+    idx = torch.randint(0, table.shape[0], (rows,), device="cuda:0", generator=g)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        out = table.index_select(0, idx)
+    torch.cuda.synchronize()
+    print(json.dumps({"table_bytes": table.numel() * 4, "rows": rows, "row_bytes": 128, "reps": reps,
+                      "checksum": float(out.sum())}), flush=True)
 
 
 def mcpg_workload(width: int, height: int, spp: int, frames: int, max_path_length: int) -> None:
@@ -318,6 +369,10 @@ def main(argv=None) -> int:
     g.add_argument("--spp", type=int, default=4)
     g.add_argument("--frames", type=int, default=3)
     g.add_argument("--max-path-length", type=int, default=8)
+    ga = sub.add_parser("gather")
+    ga.add_argument("--mib", type=int, default=4096)
+    ga.add_argument("--rows", type=int, default=1 << 24)
+    ga.add_argument("--reps", type=int, default=2)
     c = sub.add_parser("copy")
     c.add_argument("--mib", type=int, default=1024)
     c.add_argument("--reps", type=int, default=4)
@@ -330,6 +385,8 @@ def main(argv=None) -> int:
     for a in ("fetch_dir", "write_dir", "copy_fetch_dir", "copy_write_dir", "kernel"):
         t.add_argument(a)
     t.add_argument("--copy-mib", type=int, default=1024)
+    t.add_argument("--gather-fetch-dir", default="", help="FETCH_SIZE run of the gather yardstick")
+    t.add_argument("--gather-rows", type=int, default=1 << 24, help="rows (128-byte lines) the gather read")
     t.add_argument("--workload", default="")
     t.add_argument("--tuning-file", default="", help="JSON file whose 'tuning' object the profile ran with")
     for a in ("--stats-dir", "--valu-dir", "--l2-dir", "--dram-dir"):
@@ -344,6 +401,8 @@ def main(argv=None) -> int:
     args = ap.parse_args(argv)
     if args.cmd == "copy":
         copy_workload(args.mib, args.reps)
+    elif args.cmd == "gather":
+        gather_workload(args.mib, args.rows, args.reps)
     elif args.cmd == "mcpg":
         mcpg_workload(args.width, args.height, args.spp, args.frames, args.max_path_length)
     elif args.cmd == "guided":

@@ -423,13 +423,15 @@ def _rect_with_hole(o, U, V, hole):
 
 
 def full_scene(seed: int = 2, instances: int = 256, res=(1920, 1080), env_size=(2048, 1024),
-               nu: int = 63, nv: int = 62, name: str = "full_1080p", enclosed: bool = True) -> Scene:
+               nu: int = 63, nv: int = 62, name: str = "full_1080p", enclosed: bool = True,
+               dynamic_range: float = 5e4) -> Scene:
     """Displaced tori instances in an enclosing room lit through a window and a skylight, 256
     materials, sky env with a sun lobe.
 
     enclosed=False gives the open set instead (floor and three walls under the open sky, camera
     outside): the sun reaches most surfaces directly, which tests of estimators that rely on
-    sampling the sun by the BSDF alone need."""
+    sampling the sun by the BSDF alone need. dynamic_range is the sky's max/min radiance (a
+    weaker sun for tests whose estimators would otherwise be heavy-tailed)."""
     rng = np.random.default_rng(seed)
     b = SceneBuilder(f"{name}_s{seed}")
     bases = [_displaced_torus(rng, nu, nv) for _ in range(4)]
@@ -469,7 +471,7 @@ def full_scene(seed: int = 2, instances: int = 256, res=(1920, 1080), env_size=(
             b.add_mesh(P, N, T, wall_m)
         cam = CameraSpec((0.0, 7.0, half + 9.0), (0.0, 0.8, 0.0), (0.0, 1.0, 0.0), math.radians(45.0),
                          res[0], res[1])
-        return b.build(camera=cam, **sky_env(env_size[0], env_size[1]))
+        return b.build(camera=cam, **sky_env(env_size[0], env_size[1], dynamic_range=dynamic_range))
     # the six faces of the room, fronts facing inwards (walls are one-sided): rect(o, U, V) spans
     # o + s U + t V with cross(U, V) as its front; the back wall (towards the sun) has a window and
     # the ceiling a skylight, the only ways light gets in
@@ -485,7 +487,7 @@ def full_scene(seed: int = 2, instances: int = 256, res=(1920, 1080), env_size=(
             b.add_mesh(P, N, T, floor_m if k == 0 else wall_m)
     cam = CameraSpec((0.0, 4.0, half - 0.8), (0.0, 0.8, -0.25 * half), (0.0, 1.0, 0.0), math.radians(60.0),
                      res[0], res[1])
-    return b.build(camera=cam, **sky_env(env_size[0], env_size[1]))
+    return b.build(camera=cam, **sky_env(env_size[0], env_size[1], dynamic_range=dynamic_range))
 
 
 def serving_scenes(count: int = 8):

@@ -67,6 +67,19 @@ def test_traffic_is_copy_calibrated(prof):
     assert t["traffic_bytes_per_dispatch"] == 9 << 30
 
 
+def test_traffic_reads_scaled_by_the_gather_yardstick(prof, tmp_path):
+    # the gather read 2^20 whole lines (128 MiB) and FETCH_SIZE shows exactly that: reads scale
+    # by 1 (scattered line fills), the upper bound by the copy's factor 2
+    d, _ = prof
+    G = "void at::native::vectorized_gather_kernel<16, long>(char*, char*, long)"
+    _counters(tmp_path / "gfetch", [(1, G, "FETCH_SIZE", 1 << 17, 1_000_000)])
+    t = P.traffic(str(d / "fetch"), str(d / "write"), str(d / "cfetch"), str(d / "cwrite"), "k_trace<false", 1 << 30,
+                  str(tmp_path / "gfetch"), (1 << 20) * 128)
+    assert t["gather_factor_fetch"] == 1.0 and t["copy_factor_fetch"] == 2.0
+    assert t["traffic_bytes_per_dispatch"] == (4 << 30) + (1 << 30)
+    assert t["traffic_upper_bytes_per_dispatch"] == (8 << 30) + (1 << 30)
+
+
 def test_l2_and_dram_summaries(prof):
     d, _ = prof
     (k, v), = P.l2(str(d / "tcc")).items()
