@@ -236,7 +236,7 @@ def cpu_ray_query(threads: int) -> dict:
 
 
 def guided_leg(ctx, scene, L: int, spp: int = 64, ref_frames: int = 256, budget_s: float = 3.0,
-               train_frames: int | None = None, log2_table: int = 19, guided_bounces: int = 3) -> dict:
+               train_frames: int | None = None, log2_table: int = 14, guided_bounces: int = 3) -> dict:
     """Learned guiding at the 1080p workload's settings (1920x1080, 64 spp frames, L = 8): relMSE at
     equal device time against a 16384-spp plain render (256 frames of 64 spp).
 
@@ -336,8 +336,11 @@ def guided_leg(ctx, scene, L: int, spp: int = 64, ref_frames: int = 256, budget_
     g.close()
     del gacc
     pimg, n_p, t_p = run(plain, 200000, budget=t_g)
+    # the MCPG example graph's grid settings, counting guided directions only inside the BSDF
+    # sampler's support (as guiding and NEE here do): with the reference's rule a guided
+    # refraction where the GGX BTDF's pdf is 0 carries an unbounded weight
     m = Mcpg(ctx, mcpg_config(mc_tan_alpha_half=0.003, mc_distribution_dimension=1.5, lc_tan_alpha_half=0.012,
-                              lc_buffer_size=8000009, lc_probe_count=2))
+                              lc_buffer_size=8000009, lc_probe_count=2, bsdf_support_only=1))
     mimg, n_m, t_m = run(plain, 5000, budget=t_g, mcpg=m)
     m.close()
     ds.destroy()
@@ -357,8 +360,9 @@ def guided_leg(ctx, scene, L: int, spp: int = 64, ref_frames: int = 256, budget_
            "ms_per_frame_guided": round(1e3 * t_g / max(1, n_g), 2),
            "ms_per_frame_plain": round(1e3 * t_p / n_p, 2),
            "relmse_mcpg": rm, "relmse_mcpg_raw": rm_raw, "seconds_mcpg": round(t_m, 3), "frames_mcpg": n_m,
-           "mcpg_note": "MCPG guides rough glass: the GGX BTDF evaluation it shares with the plain pt is non-zero "
-                        "where its pdf is 0, so guided refractions there carry unbounded weights (fireflies)",
+           "mcpg_note": "MCPG with bsdf_support_only (a guided direction counts only where the BSDF's sampling pdf "
+                        "is > 0): with the reference's rule the GGX BTDF's non-zero evaluation outside that support "
+                        "gives guided refractions unbounded weights",
            "reference_seconds": round(t_ref, 2)}
     log(f"guided leg: {res}")
     return res
@@ -417,13 +421,17 @@ def serving_leg(ctx, single_frame_mseg: float, n_jobs: int = 400, tune_ms: float
         arrivals = np.cumsum(rng.exponential(1.0 / lam, len(jobs)))
         pending = []
         t0 = time.perf_counter()
+        t_sweep = 0.0
         for a, j in zip(arrivals, jobs):
-            while True:  # drop finished jobs (and their host buffers) while waiting for the next arrival
+            now = time.perf_counter() - t0
+            if now - t_sweep > 0.05:
+                # drop finished jobs (and their host buffers) every 50 ms: polling every pending
+                # job in a tight loop would contend for the server's lock with its scheduler
                 pending = [p_ for p_ in pending if p_.poll().state < 2]
-                dt = a - (time.perf_counter() - t0)
-                if dt <= 0:
-                    break
-                time.sleep(min(dt, 0.002))
+                t_sweep = now
+            dt = a - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
             pending.append(srv.submit(*j))
         for p_ in pending:
             p_.wait()
@@ -550,6 +558,7 @@ def main():
                     help="search the tuning again instead of applying the cached autotune result")
     ap.add_argument("--default-tuning", action="store_true", help="render with the untuned defaults")
     ap.add_argument("--no-guided", action="store_true", help="skip the equal-time guided vs plain leg")
+    ap.add_argument("--no-nee", action="store_true", help="skip the NEE frames of the 1080p workload")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-mix leg")
     ap.add_argument("--serving-jobs", type=int, default=400, help="timed jobs of the serving mix")
     ap.add_argument("--serving-slots", type=int, default=8, help="jobs in flight in the serving mix")
@@ -564,7 +573,7 @@ def main():
     ap.add_argument("--small-frames", type=int, default=64)
     ap.add_argument("--ray-reps", type=int, default=3)
     ap.add_argument("--ref-frames", type=int, default=256, help="64-spp frames of the guided leg's reference")
-    ap.add_argument("--guide-table", type=int, default=19, help="log2 entries per hash-grid level of the guide")
+    ap.add_argument("--guide-table", type=int, default=14, help="log2 entries per hash-grid level of the guide")
     ap.add_argument("--guided-bounces", type=int, default=3, help="vertices [0, n) sample the guide")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -649,6 +658,30 @@ def main():
     value = segs / dt / 1e6
     log(f"timed {args.steps} frames: {dt:.3f} s, {segs} segments, {value:.1f} Mseg/s")
 
+    # NEE mode on the same frames (environment importance sampling with MIS, one shadow ray per
+    # vertex; no counterpart in the reference's pt): reported beside `value`, the BSDF-only pt
+    p_nee = pt_params(spp=args.spp, max_path_length=args.max_path_length, mode=abi.MODE_NEE, record_signature=False)
+    n_segs, n_shadow = 0, 0
+    nee = None
+    frame_nee = lambda f: ctx.pt_render_dev(ds, p_nee, ctx.camera(spec, f), f, W, H, gb.data_ptr(), out.data_ptr(),
+                                            stats=True)
+    if not args.no_nee:
+        ctx.gbuffer_dev(ds, ctx.camera(spec, args.warmup), W, H, gb.data_ptr())
+        frame_nee(args.warmup)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            ctx.gbuffer_dev(ds, ctx.camera(spec, args.warmup + k), W, H, gb.data_ptr())
+            st_n = frame_nee(args.warmup + k)
+            n_segs += st_n.primary_segments + st_n.continuation_segments
+            n_shadow += st_n.shadow_rays
+        torch.cuda.synchronize()
+        dt_n = time.perf_counter() - t0
+        nee = {"mseg_per_s": round(n_segs / dt_n / 1e6, 1), "ms_per_frame": round(dt_n * 1e3 / args.steps, 2),
+               "shadow_rays_per_frame": n_shadow // args.steps,
+               "note": "path segments per second as `value`; shadow rays are traced on top and not counted"}
+        log(f"NEE frames: {nee}")
+
     # PCIe-inclusive rate (reported beside `value`, never as it): the host-buffer entry point
     # mrt_render_frame, whose frame ends with the 33 MB irradiance copy into pageable host memory
     h_segs = 0
@@ -688,6 +721,8 @@ def main():
                        "segments_per_step": segs // args.steps, "trace_ms_per_step": round(ms_trace / args.steps, 2),
                        "tuning": tuning, "tuning_source": tuning_source},
             "roofline": roof, "pcie_inclusive": pcie}
+    if nee is not None:
+        line["nee"] = nee
     ds.destroy()
     if not args.no_guided:
         line["guided"] = guided_leg(ctx, scene, args.max_path_length, ref_frames=args.ref_frames,

@@ -492,7 +492,10 @@ enum { MRT_GUIDE_LEVELS = 16, MRT_GUIDE_WIDTH = 64, MRT_GUIDE_LAYERS = 5, MRT_GU
 typedef struct mrt_guide_config {
     float bbox_min[3], bbox_max[3]; /* position normalisation box (all zero: scene bounds) */
     uint32_t seed;                  /* weight initialisation */
-    uint32_t log2_table;            /* hash-table entries per level (0 = 19) */
+    uint32_t log2_table;            /* hash-table entries per level (0 = 14: the sixteen levels'
+                                       tables, 1 MB, stay in L2 while inference gathers from
+                                       them; at the 1080p workload 2^14 was both faster and,
+                                       with short online training, more accurate than 2^16-2^19) */
     uint32_t base_resolution;       /* coarsest grid resolution (0 = 16) */
     uint32_t max_resolution;        /* finest grid resolution (0 = 2048) */
     uint32_t train_batch;           /* records per training step (0 = 65536) */
@@ -585,6 +588,10 @@ typedef struct mrt_mcpg_config {
     int32_t missing_light_heuristic; /* "missing light heuristic" */
     int32_t use_light_cache_tail;    /* "surf: use LC" */
     float lc_min_pdf;                /* "LC min pdf" */
+    int32_t bsdf_support_only;       /* no counterpart in the node (0 there): 1 counts a guided
+                                        direction only where the BSDF's own sampling pdf is > 0,
+                                        the domain the plain pt integrates over (the GGX BTDF
+                                        evaluates to non-zero outside it) */
 } mrt_mcpg_config;
 
 /* The node's defaults (render_pt_mcpg.hpp:125-152, hash_grid.hpp:42-44). */

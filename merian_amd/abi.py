@@ -187,7 +187,7 @@ class McpgConfig(C.Structure):
     _fields_ = [("mc", McpgGridConfig), ("lc", McpgGridConfig), ("mc_samples", C.c_int32), ("p_guiding", C.c_float),
                 ("sampling_type", C.c_int32), ("guiding_alpha_threshold", C.c_float),
                 ("dir_guide_prior", C.c_float), ("missing_light_heuristic", C.c_int32),
-                ("use_light_cache_tail", C.c_int32), ("lc_min_pdf", C.c_float)]
+                ("use_light_cache_tail", C.c_int32), ("lc_min_pdf", C.c_float), ("bsdf_support_only", C.c_int32)]
 
 
 MCPG_MIS, MCPG_MIS_ROUGHNESS, MCPG_MAX_SAMPLES = 0, 1, 8
@@ -198,7 +198,7 @@ LC_ENTRY_DTYPE = np.dtype([("irr", "<u2", 3), ("N", "<u2")])
 assert C.sizeof(SceneDesc) == 128 and C.sizeof(Camera) == 64 and C.sizeof(PtParams) == 32
 assert C.sizeof(ServerConfig) == 24 and C.sizeof(JobDesc) == 128 and C.sizeof(JobStatus) == 40
 assert C.sizeof(ServerStats) == 104 and C.sizeof(GuideConfig) == 64
-assert C.sizeof(McpgGridConfig) == 32 and C.sizeof(McpgConfig) == 96 and MCPG_STATE_DTYPE.itemsize == 32
+assert C.sizeof(McpgGridConfig) == 32 and C.sizeof(McpgConfig) == 100 and MCPG_STATE_DTYPE.itemsize == 32
 
 
 def ptr(a: np.ndarray) -> C.c_void_p:

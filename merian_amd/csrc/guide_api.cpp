@@ -38,7 +38,7 @@ namespace {
 
 mrt_guide_config with_defaults(const mrt_guide_config* in) {
     mrt_guide_config c = in ? *in : mrt_guide_config{};
-    if (c.log2_table == 0) c.log2_table = 19;
+    if (c.log2_table == 0) c.log2_table = 14;
     if (c.base_resolution == 0) c.base_resolution = 16;
     if (c.max_resolution == 0) c.max_resolution = 2048;
     if (c.train_batch == 0) c.train_batch = 1u << 16;

@@ -1116,8 +1116,10 @@ __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(3))
                             mix /= score_sum;
                         }
                         const float bsdf_w = can_guide ? (1.0f - p_guide) : 1.0f;
-                        const float wo_p = bsdf_w * sp.mat.pdf(wi_local, wo_local) + (can_guide ? p_guide : 0.f) * mix;
-                        const f3 bsdf = sp.mat.eval(wi_local, wo_local);
+                        const float pb = sp.mat.pdf(wi_local, wo_local);
+                        const float wo_p = bsdf_w * pb + (can_guide ? p_guide : 0.f) * mix;
+                        const f3 bsdf = M.bsdf_support_only && !(pb > 0.f) ? f3{0.f, 0.f, 0.f}
+                                                                           : sp.mat.eval(wi_local, wo_local);
                         const bool transmission = wi_local.z * wo_local.z < 0.f;
                         bool has_next = false;
                         if (bsdf.x > 0.f || bsdf.y > 0.f || bsdf.z > 0.f) { // ShadingPoint.make_ray

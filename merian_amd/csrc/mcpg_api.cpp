@@ -39,7 +39,8 @@ bool config_ok(const mrt_mcpg_config& c) {
            c.p_guiding >= 0.f && c.p_guiding <= 1.f && (c.sampling_type == MRT_MCPG_MIS ||
                                                          c.sampling_type == MRT_MCPG_MIS_ROUGHNESS) &&
            c.guiding_alpha_threshold >= 0.f && std::isfinite(c.guiding_alpha_threshold) && c.dir_guide_prior >= 0.f &&
-           std::isfinite(c.dir_guide_prior) && c.lc_min_pdf >= 0.f && std::isfinite(c.lc_min_pdf);
+           std::isfinite(c.dir_guide_prior) && c.lc_min_pdf >= 0.f && std::isfinite(c.lc_min_pdf) &&
+           (c.bsdf_support_only == 0 || c.bsdf_support_only == 1);
 }
 
 // Zero-fills both grids with a fill kernel on the context stream and waits for it. Frames may
@@ -113,6 +114,7 @@ mrt_status mcpg_prepare_frame(mrt_ctx* c, mrt_mcpg* m, uint64_t paths, const mrt
     a.missing_light = k.missing_light_heuristic ? 1 : 0;
     a.lc_tail = k.use_light_cache_tail ? 1 : 0;
     a.lc_min_pdf = k.lc_min_pdf;
+    a.bsdf_support_only = k.bsdf_support_only;
     for (int i = 0; i < 3; i++) {
         a.cam[i] = cam.position[i];
         a.prev_cam[i] = m->has_prev ? m->prev_cam[i] : cam.position[i];

@@ -149,6 +149,7 @@ struct McpgArgs {
     int32_t missing_light;   // missing_light_heuristic
     int32_t lc_tail;         // use_light_cache_tail
     float lc_min_pdf;
+    int32_t bsdf_support_only; // a direction with BSDF sampling pdf 0 contributes nothing
     float cam[3], prev_cam[3];
     float time;
 };

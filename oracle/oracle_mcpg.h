@@ -292,6 +292,7 @@ struct McpgCfg {
     float alpha_threshold, prior;
     bool missing_light, lc_tail;
     float lc_min_pdf;
+    bool bsdf_support_only;
 };
 
 struct McpgState {
@@ -434,8 +435,11 @@ inline f3 mcpg_trace_path(const Scene& sc, McpgState& M, const mrt_pt_params& pr
             wo_p /= score_sum;
         }
         const float bsdf_w = can_guide ? (1.0f - p_guide) : 1.0f;
-        wo_p = bsdf_w * bsdf_m.pdf(wi_local, wo_local) + (can_guide ? p_guide : 0.0f) * wo_p;
-        const f3 bsdf = bsdf_m.eval(wi_local, wo_local);
+        const float pb = bsdf_m.pdf(wi_local, wo_local);
+        wo_p = bsdf_w * pb + (can_guide ? p_guide : 0.0f) * wo_p;
+        // bsdf_support_only (no counterpart in the reference): directions outside the BSDF
+        // sampler's support contribute nothing
+        const f3 bsdf = k.bsdf_support_only && !(pb > 0.0f) ? f3{0, 0, 0} : bsdf_m.eval(wi_local, wo_local);
         const bool transmission = wi_local.z * wo_local.z < 0.f;
         Ray next;
         bool has_next = false;
@@ -498,6 +502,7 @@ struct orc_mcpg_cfg {
     float alpha_threshold, prior;
     int32_t missing_light, lc_tail;
     float lc_min_pdf;
+    int32_t bsdf_support_only;
 };
 
 __attribute__((visibility("default"))) void* orc_mcpg_create(const orc_mcpg_cfg* k) {
@@ -506,7 +511,8 @@ __attribute__((visibility("default"))) void* orc_mcpg_create(const orc_mcpg_cfg*
     m->lc.c = GridCfg{k->lc.size,       k->lc.probes,   k->lc.locality, k->lc.stochastic != 0, false,
                       false,            k->lc.tan_half, k->lc.bias,     k->lc.dim};
     m->cfg = McpgCfg{k->samples,       k->p_guiding, k->roughness != 0, k->alpha_threshold,
-                     k->prior,         k->missing_light != 0, k->lc_tail != 0, k->lc_min_pdf};
+                     k->prior,         k->missing_light != 0, k->lc_tail != 0, k->lc_min_pdf,
+                     k->bsdf_support_only != 0};
     m->mc.reset();
     m->lc.reset();
     return m;

@@ -3,8 +3,11 @@
 - forward outputs and loss gradients (weights and hash grid) of the MFMA kernels equal the
   oracle's mirror mode (the kernels' fp16 rounding points, fp32 accumulation) within tolerance;
 - training on records drawn from a position-dependent vMF lowers the loss toward its optimum;
-- guided rendering is unbiased: block means agree with the BSDF-only reference;
-- at equal wall time the trained guide beats the plain pt in relMSE against a converged render.
+- guided rendering is unbiased: block means agree with the NEE reference (itself tested against
+  the BSDF-only estimator), every vertex guided and only vertex 0 guided, under a sky whose sun is
+  100x its dimmest texel (a 5e4x sun makes the guided estimator heavy-tailed, which a block
+  z-test cannot tell from bias);
+- at equal device time the trained guide beats the plain pt in relMSE under the 5e4x sun.
 """
 import math
 
@@ -181,7 +184,42 @@ def _reference(ctx, ds, spec, L, frames):
                                                record_signature=False), frames, base=50000)
 
 
-def test_guided_is_unbiased_and_wins_at_equal_time(ctx, sky):
+def _blocks(img, spp):
+    l = _lum(img)
+    var = np.maximum(img[..., 3] - l * l, 0) / spp
+    b = lambda a: a.reshape(H_ // 6, 6, W_ // 6, 6).mean((1, 3))
+    return b(l), np.sqrt(b(var) / 36)
+
+
+def test_guided_is_unbiased(ctx, soft_sky):
+    # every vertex may sample the guide; block means of the frozen guide's frames against the
+    # NEE reference within their errors. Under the soft sky: with a 5e4x sun the guided
+    # estimator's rare large contributions (sun hits by BSDF-sampled directions before the
+    # guide has learnt them) skew its block means low and its spread estimate small, which a
+    # z-test reads as bias.
+    sc, ds = soft_sky
+    spec = scenes.CameraSpec(sc.camera.eye, sc.camera.target, sc.camera.up, sc.camera.vfov, W_, H_)
+    L = 6
+    ref = _reference(ctx, ds, spec, L, 16)
+    g = Guide(ctx, ds, guide_config(seed=4))
+    try:
+        gp = pt_params(spp=16, max_path_length=L, mode=abi.MODE_GUIDED, record_signature=False)
+        _frames(ctx, ds, spec, 0, gp, 24, guide=g, train=True, base=2000)  # online training
+        gp.spp = 512
+        gimg = _frames(ctx, ds, spec, 0, gp, 4, guide=g, train=False, base=3000)
+    finally:
+        g.close()
+    m_r, s_r = _blocks(ref, 1024 * 16)
+    m_g, s_g = _blocks(gimg, 512 * 4)
+    z = (m_g - m_r) / np.sqrt(s_r ** 2 + s_g ** 2 + 1e-12)
+    print(f"guided: max |z| {np.abs(z).max():.2f}, mean z {z.mean():.3f}")
+    assert np.abs(z).max() < 6.0, (np.abs(z).max(), z.mean())
+    assert abs(z.mean()) < 0.3, z.mean()
+
+
+def test_guided_wins_at_equal_time(ctx, sky):
+    # under the sun: after online training the guide lowers relMSE against the NEE reference
+    # below the plain pt's at equal device time (training included)
     import torch
     sc, ds = sky
     spec = scenes.CameraSpec(sc.camera.eye, sc.camera.target, sc.camera.up, sc.camera.vfov, W_, H_)
@@ -191,21 +229,7 @@ def test_guided_is_unbiased_and_wins_at_equal_time(ctx, sky):
     try:
         gp = pt_params(spp=16, max_path_length=L, mode=abi.MODE_GUIDED, record_signature=False)
         _frames(ctx, ds, spec, 0, gp, 24, guide=g, train=True, base=2000)  # online training
-        # unbiased: guided (frozen) block means against the reference within their errors
-        gp.spp = 512
-        gimg = _frames(ctx, ds, spec, 0, gp, 4, guide=g, train=False, base=3000)
-        def blocks(img, spp):
-            l = _lum(img)
-            var = np.maximum(img[..., 3] - l * l, 0) / spp
-            b = lambda a: a.reshape(H_ // 6, 6, W_ // 6, 6).mean((1, 3))
-            return b(l), np.sqrt(b(var) / 36)
-        m_r, s_r = blocks(ref, 1024 * 16)
-        m_g, s_g = blocks(gimg, 512 * 4)
-        z = (m_g - m_r) / np.sqrt(s_r ** 2 + s_g ** 2 + 1e-12)
-        print(f"guided: max |z| {np.abs(z).max():.2f}, mean z {z.mean():.3f}")
-        assert np.abs(z).max() < 6.0, (np.abs(z).max(), z.mean())
-        assert abs(z.mean()) < 0.6
-        # equal time: plain and guided (training included) get the same wall time
+
         def timed(fn):
             # device time of the frame sequence on torch's stream (HIP events), not host time
             torch.cuda.synchronize()
@@ -218,27 +242,27 @@ def test_guided_is_unbiased_and_wins_at_equal_time(ctx, sky):
         # 40 guided frames (about 0.1 s): with 8, one sun firefly or a weaker training run (the
         # grid gradients are summed by atomics in no fixed order) decided the comparison; at 40
         # four guide seeds gave 0.022-0.027 against the plain pt's 0.057
-        gp.spp = 16
         gimg, t_g = timed(lambda: _frames(ctx, ds, spec, 0, gp, 40, guide=g, train=True))
         pp = pt_params(spp=16, max_path_length=L, record_signature=False)
         _, t_p1 = timed(lambda: _frames(ctx, ds, spec, 0, pp, 16))
         frames_p = max(16, int(round(16 * t_g / t_p1)))
         pimg, t_p = timed(lambda: _frames(ctx, ds, spec, 0, pp, frames_p))
-        def rel_mse(x):
-            return float(np.mean((x[..., :3] - ref[..., :3]) ** 2 / (ref[..., :3] ** 2 + 1e-2)))
-        e_g, e_p = rel_mse(gimg), rel_mse(pimg)
-        print(f"equal time: guided {t_g:.3f} s relMSE {e_g:.4g}; plain {t_p:.3f} s ({frames_p} frames) relMSE {e_p:.4g}")
-        assert t_p >= 0.8 * t_g
-        assert e_g < e_p, (e_g, e_p)
     finally:
         g.close()
+
+    def rel_mse(x):
+        return float(np.mean((x[..., :3] - ref[..., :3]) ** 2 / (ref[..., :3] ** 2 + 1e-2)))
+    e_g, e_p = rel_mse(gimg), rel_mse(pimg)
+    print(f"equal time: guided {t_g:.3f} s relMSE {e_g:.4g}; plain {t_p:.3f} s ({frames_p} frames) relMSE {e_p:.4g}")
+    assert t_p >= 0.8 * t_g
+    assert e_g < e_p, (e_g, e_p)
 
 
 @pytest.fixture(scope="module")
 def soft_sky(ctx):
-    # the open set under a sky whose sun is 100x its dimmest texel rather than 5e4x: with only
-    # vertex 0 guided, later vertices find a 5e4x sun by BSDF sampling alone, rarely and with
-    # large weights, and the block z-test then measures that heavy tail rather than a bias
+    # the open set under a sky whose sun is 100x its dimmest texel rather than 5e4x: directions
+    # the guide does not cover find a 5e4x sun by BSDF sampling alone, rarely and with large
+    # weights, and the block z-test then measures that heavy tail rather than a bias
     sc = scenes.full_scene(seed=5, instances=16, env_size=(256, 128), enclosed=False, dynamic_range=100.0)
     ds = ctx.scene(sc)
     yield sc, ds

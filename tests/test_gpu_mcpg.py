@@ -65,6 +65,9 @@ CONFIGS = [
     dict(mc_buffer_size=1531, mc_probe_count=3, mc_locality_bits=0, lc_buffer_size=509, lc_probe_count=3,
          lc_locality_bits=0, lc_stochastic_interpolation=1, use_light_cache_tail=1, sampling_type=abi.MCPG_MIS,
          missing_light_heuristic=0, mc_samples=3, p_guiding=0.9),
+    # as the first, counting guided directions only inside the BSDF sampler's support (an option
+    # of this build, off in the node)
+    dict(mc_buffer_size=4096, lc_buffer_size=1024, p_guiding=0.9, sampling_type=abi.MCPG_MIS, bsdf_support_only=1),
 ]
 
 
@@ -104,6 +107,7 @@ def test_config_validation_and_reset(ctx, room):
     sc, ds, _ = room
     for bad in (dict(mc_samples=9), dict(mc_probe_count=0), dict(lc_locality_bits=6), dict(p_guiding=1.5),
                 dict(mc_tan_alpha_half=0.0), dict(sampling_type=2), dict(lc_distribution_dimension=0.0),
+                dict(bsdf_support_only=2),
                 dict(mc_buffer_size=(1 << 28) + 1)):
         with pytest.raises(MrtError):
             Mcpg(ctx, mcpg_config(**(dict(mc_buffer_size=1024, lc_buffer_size=1024) | bad)))

@@ -159,5 +159,5 @@ def test_mirror_mode_close_to_exact():
 
 def test_guide_layout():
     res, table = guide_layout(guide_config())
-    assert res[0] == 16 and res[-1] == 2048 and table == 1 << 19
+    assert res[0] == 16 and res[-1] == 2048 and table == 1 << 14
     assert all(b > a for a, b in zip(res, res[1:]))
