@@ -11,6 +11,8 @@
 
 #include <chrono>
 #include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <memory>
@@ -49,7 +51,6 @@ struct JobBuffers {
     float4* out = nullptr;
     PathBuffers pb{};
     TraceScratch ts{};
-    std::vector<void*> allocs;
 };
 
 struct Slot {
@@ -58,6 +59,11 @@ struct Slot {
     hipGraphExec_t exec = nullptr;
     uint32_t* h_counters = nullptr; // pinned
     uint64_t h_counters_cap = 0;
+    void* h_result = nullptr; // pinned staging of the job's irradiance
+    uint64_t h_result_cap = 0;
+    char* block = nullptr;   // device block the slot's jobs run in, kept from job to job
+    uint64_t block_bytes = 0;
+    double t_alloc_ms = 0; // host time of the last job's block resize (MRT_SERVE_TRACE)
     Job* job = nullptr;
     JobBuffers buf;
 };
@@ -95,6 +101,7 @@ struct mrt_server {
     std::vector<double> latencies;
     double first_submit = -1, last_done = 0;
     uint64_t peak_bytes = 0;
+    uint64_t held = 0; // bytes of the slots' blocks (written by the scheduler thread)
     uint32_t peak_in_flight = 0, graph_inst = 0, graph_upd = 0;
     std::string err;
 };
@@ -137,16 +144,18 @@ mrt_status validate(const mrt_job_desc* d, std::string* why) {
     return MRT_OK;
 }
 
-hipError_t alloc_job(hipMemPool_t pool, Slot& sl, const Job& j, const JobShape& sh) {
+// The job's buffers carved from the slot's block in the order job_layout counts them (each
+// 256-byte aligned); the block holds at least the job's size class.
+hipError_t carve_job(Slot& sl, const Job& j, const JobShape& sh) {
     JobBuffers& b = sl.buf;
     b = JobBuffers{};
     const JobLayout& l = j.layout;
-    hipError_t e = hipSuccess;
+    if (!sl.block || sl.block_bytes < l.bytes) return hipErrorInvalidValue;
+    char* base = sl.block;
+    uint64_t off = 0;
     auto A = [&](uint64_t bytes) -> void* {
-        if (e != hipSuccess) return nullptr;
-        void* p = nullptr;
-        e = hipMallocFromPoolAsync(&p, round256(bytes), pool, sl.stream);
-        if (e == hipSuccess) b.allocs.push_back(p);
+        void* p = base + off;
+        off += round256(bytes);
         return p;
     };
     const uint64_t P = l.paths, N = sh.npix;
@@ -167,10 +176,10 @@ hipError_t alloc_job(hipMemPool_t pool, Slot& sl, const Job& j, const JobShape& 
     }
     b.pb.counters = static_cast<uint32_t*>(A(4 * l.counters));
     b.pb.accum = static_cast<float4*>(A(16 * N));
-    if (sh.demod) b.pb.demod = static_cast<float4*>(A(16 * N));
     b.pb.sig_accum = static_cast<uint32_t*>(A(4 * N));
+    if (sh.demod) b.pb.demod = static_cast<float4*>(A(16 * N));
     b.pb.capacity = P;
-    b.ts.ovf = static_cast<uint32_t*>(A(uint64_t(l.threads) * sh.stack_entries * 4));
+    b.ts.ovf = static_cast<uint32_t*>(A(uint64_t(l.threads) * std::max<uint32_t>(1, sh.stack_entries) * 4));
     b.ts.work = static_cast<uint32_t*>(A(64));
     b.ts.threads = l.threads;
     b.ts.entries = sh.stack_entries;
@@ -178,18 +187,31 @@ hipError_t alloc_job(hipMemPool_t pool, Slot& sl, const Job& j, const JobShape& 
     b.ts.refill = tuned.refill;
     b.ts.grid_pct = tuned.grid_pct;
     b.ts.leaf_lanes = tuned.leaf_lanes;
+    // job_layout counts the same buffers: the carve never passes the block
+    return off <= l.bytes ? hipSuccess : hipErrorInvalidValue;
+}
+
+// Returns the slot's block to the server's pool (stream-ordered after the slot's work).
+hipError_t release_block(Slot& sl) {
+    hipError_t e = hipSuccess;
+    if (sl.block) e = hipFreeAsync(sl.block, sl.stream);
+    sl.block = nullptr;
+    sl.block_bytes = 0;
     return e;
 }
 
-// Returns every buffer of the slot's job to the pool; the first failure is reported.
-hipError_t free_job(Slot& sl) {
-    hipError_t first = hipSuccess;
-    for (void* p : sl.buf.allocs) {
-        const hipError_t e = hipFreeAsync(p, sl.stream);
-        if (first == hipSuccess) first = e;
+// Makes the slot's block exactly `bytes` (a size class) unless it is that already.
+hipError_t resize_block(hipMemPool_t pool, Slot& sl, uint64_t bytes) {
+    if (sl.block && sl.block_bytes == bytes) return hipSuccess;
+    hipError_t e = release_block(sl);
+    if (e != hipSuccess) return e;
+    e = hipMallocFromPoolAsync(reinterpret_cast<void**>(&sl.block), bytes, pool, sl.stream);
+    if (e != hipSuccess) {
+        sl.block = nullptr;
+        return e;
     }
-    sl.buf = JobBuffers{};
-    return first;
+    sl.block_bytes = bytes;
+    return hipSuccess;
 }
 
 // Enqueues one job's frame on the slot's stream (captured in a graph when enabled).
@@ -269,11 +291,24 @@ hipError_t start_job(mrt_server* s, Slot& sl, Job& j) {
         if (e != hipSuccess) return e;
         sl.h_counters_cap = nz;
     }
-    hipError_t e = alloc_job(s->pool, sl, j, sh);
+    // the result goes device -> pinned staging on the slot stream, so the completion event
+    // covers it and the scheduler thread only copies host memory (no pageable DMA)
+    const uint64_t out_bytes = 16ull * j.d.width * j.d.height;
+    if (out_bytes > sl.h_result_cap) {
+        hipError_t e = sl.h_result ? hipHostFree(sl.h_result) : hipSuccess;
+        sl.h_result = nullptr;
+        sl.h_result_cap = 0;
+        if (e != hipSuccess) return e;
+        e = hipHostMalloc(&sl.h_result, out_bytes, hipHostMallocDefault);
+        if (e != hipSuccess) return e;
+        sl.h_result_cap = out_bytes;
+    }
+    hipError_t e = carve_job(sl, j, sh);
     if (e == hipSuccess) e = enqueue_frame(s, sl, j);
-    // queue sizes of the frame -> pinned host memory, then the completion event
+    // queue sizes and the result -> pinned host memory, then the completion event
     if (e == hipSuccess)
         e = hipMemcpyAsync(sl.h_counters, sl.buf.pb.counters, nz * 4, hipMemcpyDeviceToHost, sl.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(sl.h_result, sl.buf.out, out_bytes, hipMemcpyDeviceToHost, sl.stream);
     if (e == hipSuccess) e = hipEventRecord(sl.done, sl.stream);
     return e;
 }
@@ -293,19 +328,11 @@ void finish_job(mrt_server* s, Slot& sl, hipError_t status) {
             std::lock_guard<std::mutex> lk(s->m);
             cancel = j.cancel;
         }
-        if (!cancel) {
-            const hipError_t e = hipMemcpy(j.d.irradiance, sl.buf.out, 16ull * j.d.width * j.d.height,
-                                           hipMemcpyDeviceToHost);
-            if (e != hipSuccess) result = MRT_ERR_DEVICE;
-        }
+        if (!cancel) std::memcpy(j.d.irradiance, sl.h_result, 16ull * j.d.width * j.d.height);
     } else {
         result = MRT_ERR_DEVICE;
     }
-    const hipError_t fe = free_job(sl);
-    if (fe != hipSuccess && result == MRT_OK) {
-        result = MRT_ERR_DEVICE;
-        status = fe;
-    }
+    sl.buf = JobBuffers{}; // the block stays with the slot for its next job
     std::lock_guard<std::mutex> lk(s->m);
     j.t_done = now_ms(s->t0);
     if (result != MRT_OK) {
@@ -325,7 +352,7 @@ void finish_job(mrt_server* s, Slot& sl, hipError_t status) {
         s->latencies.push_back(j.t_done - j.t_submit);
         s->last_done = j.t_done;
     }
-    s->adm.release(j.layout.bytes);
+    s->adm.release(j.layout.block);
     s->finished.push_back(j.id);
     while (s->finished.size() > KEEP_FINISHED) {
         s->jobs.erase(s->finished.front());
@@ -338,26 +365,56 @@ void finish_job(mrt_server* s, Slot& sl, hipError_t status) {
 void scheduler(mrt_server* s) {
     // a thread that cannot select the device fails every job it admits with that error
     const hipError_t dev = hipSetDevice(s->device);
+    // MRT_SERVE_TRACE=1 prints, per job, the host time spent starting it (allocation, capture,
+    // launch) and finishing it, and the pool's reserved bytes, to stderr
+    const char* tr = std::getenv("MRT_SERVE_TRACE");
+    const bool trace = tr && tr[0] == '1';
     std::unique_lock<std::mutex> lk(s->m);
     while (true) {
-        // admit while the head of the queue fits
-        for (Slot& sl : s->slots) {
-            if (sl.job) continue;
+        // admit while the head of the queue fits: beside the jobs in flight (budget), in a free
+        // slot whose block fits or can grow within the budget (plan_slot)
+        while (true) {
             uint64_t id = 0, bytes = 0;
-            if (!s->adm.admit(&id, &bytes)) break;
+            if (!s->adm.peek(&id, &bytes)) break;
+            std::vector<uint64_t> held(s->slots.size());
+            std::vector<bool> busy(s->slots.size());
+            for (size_t k = 0; k < s->slots.size(); k++) {
+                held[k] = s->slots[k].block_bytes;
+                busy[k] = s->slots[k].job != nullptr;
+            }
+            SlotPlan plan;
+            if (!plan_slot(held, busy, s->adm.budget(), bytes, &plan)) break;
+            (void)s->adm.admit(&id, &bytes);
+            Slot& sl = s->slots[size_t(plan.slot)];
             Job& j = *s->jobs.at(id);
             j.state = MRT_JOB_RUNNING;
             j.t_admit = now_ms(s->t0);
             sl.job = &j;
-            s->peak_bytes = std::max(s->peak_bytes, s->adm.in_use());
             s->peak_in_flight = std::max(s->peak_in_flight, s->adm.busy());
             lk.unlock();
-            const hipError_t e = dev != hipSuccess ? dev : start_job(s, sl, j);
+            const auto t_start = std::chrono::steady_clock::now();
+            hipError_t e = dev;
+            for (int r : plan.release)
+                if (e == hipSuccess) e = release_block(s->slots[size_t(r)]);
+            if (e == hipSuccess) e = resize_block(s->pool, sl, plan.block);
+            sl.t_alloc_ms = now_ms(t_start);
+            uint64_t held_now = 0;
+            for (const Slot& t : s->slots) held_now += t.block_bytes;
+            if (e == hipSuccess) e = start_job(s, sl, j);
+            if (trace) {
+                uint64_t reserved = 0;
+                (void)hipMemPoolGetAttribute(s->pool, hipMemPoolAttrReservedMemCurrent, &reserved);
+                std::fprintf(stderr, "serve: job %llu %ux%u spp %d start %.3f ms (alloc %.3f ms), pool reserved %.2f GB\n",
+                             (unsigned long long)j.id, j.d.width, j.d.height, j.d.params.spp, now_ms(t_start),
+                             sl.t_alloc_ms, double(reserved) / 1e9);
+            }
             if (e != hipSuccess) {
                 (void)hipStreamSynchronize(sl.stream); // drain what was queued; the job fails with e
                 finish_job(s, sl, e);
             }
             lk.lock();
+            s->held = held_now;
+            s->peak_bytes = std::max(s->peak_bytes, held_now);
         }
         const bool in_flight = s->adm.busy() > 0;
         if (!in_flight) {
@@ -371,7 +428,10 @@ void scheduler(mrt_server* s) {
             if (!sl.job) continue;
             const hipError_t q = hipEventQuery(sl.done);
             if (q == hipErrorNotReady) continue;
+            const auto t_fin = std::chrono::steady_clock::now();
+            const uint64_t fin_id = sl.job->id;
             finish_job(s, sl, q);
+            if (trace) std::fprintf(stderr, "serve: job %llu finish %.3f ms\n", (unsigned long long)fin_id, now_ms(t_fin));
             progressed = true;
         }
         if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(50));
@@ -383,7 +443,7 @@ void fill_status(const Job& j, mrt_job_status* st) {
     st->state = j.state;
     st->error = j.error;
     st->segments = j.segments;
-    st->device_bytes = j.layout.bytes;
+    st->device_bytes = j.layout.block;
     st->queued_ms = j.state == MRT_JOB_QUEUED ? 0.0 : j.t_admit - j.t_submit;
     st->latency_ms = (j.state == MRT_JOB_DONE || j.state == MRT_JOB_FAILED || j.state == MRT_JOB_CANCELLED)
                          ? j.t_done - j.t_submit
@@ -473,6 +533,9 @@ MRT_API mrt_status mrt_server_destroy(mrt_server* s) {
         ok &= hipStreamSynchronize(sl.stream) == hipSuccess;
         if (sl.exec) ok &= hipGraphExecDestroy(sl.exec) == hipSuccess;
         if (sl.h_counters) ok &= hipHostFree(sl.h_counters) == hipSuccess;
+        if (sl.h_result) ok &= hipHostFree(sl.h_result) == hipSuccess;
+        ok &= release_block(sl) == hipSuccess;
+        ok &= hipStreamSynchronize(sl.stream) == hipSuccess;
         ok &= hipEventDestroy(sl.done) == hipSuccess;
         ok &= hipStreamDestroy(sl.stream) == hipSuccess;
     }
@@ -503,7 +566,7 @@ MRT_API mrt_status mrt_job_bytes(mrt_server* s, const mrt_job_desc* d, uint64_t*
         s->err = why;
         return v;
     }
-    *bytes = job_layout(shape_of(s, *d)).bytes;
+    *bytes = job_layout(shape_of(s, *d)).block;
     return MRT_OK;
 }
 
@@ -523,8 +586,8 @@ MRT_API mrt_status mrt_job_submit(mrt_server* s, const mrt_job_desc* d, uint64_t
     std::unique_ptr<Job> j(new Job());
     j->d = *d;
     j->layout = job_layout(shape_of(s, *d));
-    if (!s->adm.can_ever_fit(j->layout.bytes)) {
-        s->err = "job needs " + std::to_string(j->layout.bytes) + " bytes, more than the server budget of " +
+    if (!s->adm.can_ever_fit(j->layout.block)) {
+        s->err = "job needs " + std::to_string(j->layout.block) + " bytes, more than the server budget of " +
                  std::to_string(s->adm.budget());
         return MRT_ERR_BUDGET;
     }
@@ -533,7 +596,7 @@ MRT_API mrt_status mrt_job_submit(mrt_server* s, const mrt_job_desc* d, uint64_t
     if (s->first_submit < 0) s->first_submit = j->t_submit;
     s->submitted++;
     *id = j->id;
-    s->adm.push(j->id, j->layout.bytes);
+    s->adm.push(j->id, j->layout.block);
     s->jobs.emplace(j->id, std::move(j));
     s->cv_work.notify_one();
     return MRT_OK;
@@ -620,7 +683,7 @@ MRT_API mrt_status mrt_server_stats_reset(mrt_server* s) {
     s->latencies.clear();
     s->first_submit = -1;
     s->last_done = 0;
-    s->peak_bytes = s->adm.in_use();
+    s->peak_bytes = s->held;
     s->peak_in_flight = s->adm.busy();
     s->graph_inst = s->graph_upd = 0;
     return MRT_OK;

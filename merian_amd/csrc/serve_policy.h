@@ -40,16 +40,22 @@ class Admission {
         return false;
     }
 
-    // Pops and charges the head of the queue if it can start now.
-    bool admit(uint64_t* id, uint64_t* bytes) {
+    // The head of the queue, if a slot is free and its bytes fit beside the jobs in flight.
+    bool peek(uint64_t* id, uint64_t* bytes) const {
         if (q_.empty() || busy_ >= slots_) return false;
         const Entry e = q_.front();
         if (in_use_ + e.bytes > budget_) return false;
-        q_.pop_front();
-        in_use_ += e.bytes;
-        busy_++;
         *id = e.id;
         *bytes = e.bytes;
+        return true;
+    }
+
+    // Pops and charges the head of the queue if it can start now.
+    bool admit(uint64_t* id, uint64_t* bytes) {
+        if (!peek(id, bytes)) return false;
+        q_.pop_front();
+        in_use_ += *bytes;
+        busy_++;
         return true;
     }
 
@@ -67,6 +73,57 @@ class Admission {
     uint64_t budget_ = 0, in_use_ = 0;
     uint32_t slots_ = 1, busy_ = 0;
 };
+
+// Slots keep their device block from job to job, so the steady state allocates nothing (a
+// stream-ordered pool that has to grow synchronises with the other streams' work: measured, a
+// 1080p job's allocation then waited up to 2.5 s for the jobs in flight). plan_slot picks the
+// slot a job of `need` bytes runs in:
+//   1. a free slot whose block fits, the smallest such;
+//   2. else the free slot with the largest block, grown to `need`; while the blocks held would
+//      then exceed the budget, the blocks of other free slots are released, largest first.
+// False when no free slot exists or the job cannot fit beside the blocks busy slots hold (it
+// waits until one of them finishes).
+struct SlotPlan {
+    int slot = -1;
+    uint64_t block = 0;       // the slot's block for this job (its current one if it fits)
+    std::vector<int> release; // other free slots whose blocks are released first
+};
+
+inline bool plan_slot(const std::vector<uint64_t>& held, const std::vector<bool>& busy, uint64_t budget,
+                      uint64_t need, SlotPlan* plan) {
+    *plan = SlotPlan{};
+    int fit = -1, largest = -1;
+    uint64_t total = 0;
+    for (size_t k = 0; k < held.size(); k++) {
+        total += held[k];
+        if (busy[k]) continue;
+        if (held[k] >= need && (fit < 0 || held[k] < held[size_t(fit)])) fit = int(k);
+        if (largest < 0 || held[k] > held[size_t(largest)]) largest = int(k);
+    }
+    if (fit >= 0) {
+        plan->slot = fit;
+        plan->block = held[size_t(fit)];
+        return true;
+    }
+    if (largest < 0) return false;
+    uint64_t after = total - held[size_t(largest)] + need;
+    std::vector<int> idle;
+    for (size_t k = 0; k < held.size(); k++)
+        if (!busy[k] && int(k) != largest && held[k] > 0) idle.push_back(int(k));
+    std::sort(idle.begin(), idle.end(), [&](int a, int b) { return held[size_t(a)] > held[size_t(b)]; });
+    for (int k : idle) {
+        if (after <= budget) break;
+        after -= held[size_t(k)];
+        plan->release.push_back(k);
+    }
+    if (after > budget) {
+        plan->release.clear();
+        return false;
+    }
+    plan->slot = largest;
+    plan->block = need;
+    return true;
+}
 
 // Nearest-rank percentile (p in [0, 100]) of a sample; 0 for an empty sample.
 inline double percentile(std::vector<double> v, double p) {
@@ -95,10 +152,24 @@ struct JobLayout {
     uint64_t paths;     // npix * chunk
     uint64_t counters;  // u32 counters (4 arrays of n_chunks * (L + 1))
     uint32_t threads;   // traversal-stack columns
-    uint64_t bytes;     // total device bytes
+    uint64_t bytes;     // device bytes the job's buffers need
+    uint64_t block;     // bytes allocated and charged: `bytes` rounded up to a size class
 };
 
 inline uint64_t round256(uint64_t b) { return (std::max<uint64_t>(b, 1) + 255) & ~uint64_t(255); }
+
+// Size classes of job blocks: 1 MiB, then 2^k x {1, 1.25, 1.5, 1.75} (at most 25 % over the
+// need). A job's buffers are one block from the server's pool, so blocks of a class freed by
+// finished jobs serve later jobs of that class without the pool mapping new memory.
+inline uint64_t size_class(uint64_t b) {
+    const uint64_t min_block = uint64_t(1) << 20;
+    if (b <= min_block) return min_block;
+    uint64_t p = min_block;
+    while (p * 2 < b) p *= 2; // p < b <= 2p
+    for (uint64_t q = 5; q <= 8; q++) // 1.25p, 1.5p, 1.75p, 2p
+        if (b <= p * q / 4) return p * q / 4;
+    return 2 * p;
+}
 
 inline JobLayout job_layout(const JobShape& s) {
     JobLayout l{};
@@ -123,6 +194,7 @@ inline JobLayout job_layout(const JobShape& s) {
     b += round256(uint64_t(l.threads) * std::max<uint32_t>(1, s.stack_entries) * 4); // stack overflow
     b += round256(64);                            // ray cursor
     l.bytes = b;
+    l.block = size_class(b);
     return l;
 }
 

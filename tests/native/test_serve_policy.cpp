@@ -89,6 +89,40 @@ int main() {
         s.max_paths = 50; // fewer than one sample of every pixel: still one sample per pass
         l = job_layout(s);
         CHECK(l.chunk == 1 && l.n_chunks == 3 && l.paths == 100);
+        CHECK(l.block == (uint64_t(1) << 20) && l.block >= l.bytes);
+    }
+    {   // slot blocks kept from job to job
+        SlotPlan p;
+        // best fit among free slots whose block fits; busy slots are never chosen
+        CHECK(plan_slot({100, 40, 60, 0}, {false, false, false, true}, 1000, 50, &p) && p.slot == 2 && p.block == 60 &&
+              p.release.empty());
+        CHECK(plan_slot({100, 40, 60, 0}, {true, false, false, false}, 1000, 50, &p) && p.slot == 2);
+        // nothing fits: the largest free block grows
+        CHECK(plan_slot({10, 40, 0}, {false, false, true}, 1000, 50, &p) && p.slot == 1 && p.block == 50 &&
+              p.release.empty());
+        // growing would pass the budget: other free blocks are released, largest first
+        CHECK(plan_slot({30, 40, 20, 25}, {false, false, false, true}, 120, 70, &p) && p.slot == 1 && p.block == 70 &&
+              p.release.size() == 1 && p.release[0] == 0);
+        CHECK(plan_slot({30, 40, 20, 25}, {false, false, false, true}, 100, 70, &p) && p.release.size() == 2);
+        CHECK(!plan_slot({30, 40, 20, 25}, {false, false, false, true}, 90, 70, &p)); // 25 busy + 70 > 90
+        // busy blocks leave too little: wait
+        CHECK(!plan_slot({80, 10}, {true, false}, 100, 30, &p) && p.slot == -1 && p.release.empty());
+        // no free slot
+        CHECK(!plan_slot({10, 10}, {true, true}, 100, 5, &p));
+        // an empty free slot with nothing held
+        CHECK(plan_slot({0, 0}, {false, false}, 100, 5, &p) && p.block == 5);
+    }
+    {   // size classes: 1 MiB, then 2^k x {1, 1.25, 1.5, 1.75}; never below the need, at most 25 % above
+        const uint64_t M = uint64_t(1) << 20;
+        CHECK(size_class(1) == M && size_class(M) == M);
+        CHECK(size_class(M + 1) == M * 5 / 4 && size_class(M * 5 / 4) == M * 5 / 4);
+        CHECK(size_class(M * 5 / 4 + 1) == M * 3 / 2 && size_class(M * 7 / 4 + 1) == 2 * M);
+        CHECK(size_class(3 * M) == 3 * M && size_class(12ull << 30) == 12ull << 30);
+        for (uint64_t b = 1; b < (uint64_t(1) << 36); b = b * 3 + 7) {
+            const uint64_t c = size_class(b);
+            CHECK(c >= b && (b <= M || double(c) <= 1.25 * double(b) + 1.0));
+            CHECK(size_class(c) == c); // a class is its own class
+        }
     }
     if (failures) return 1;
     std::printf("serve policy: all checks passed\n");
