@@ -474,7 +474,8 @@ MRT_API mrt_status mrt_server_stats_reset(mrt_server* server);
  * guided one-sample MIS (render_pt_mcpg.slang:104-136,181) and its von Mises-Fisher lobes
  * (von-mises-fisher.slang:29-45). A network maps a shading position and the outgoing
  * direction to an 8-lobe vMF mixture over incident directions:
- *   input   64 = multiresolution hash grid (16 levels x 2 features, fp16 tables)
+ *   input   64 = multiresolution hash grid (8 levels x 4 features, fp16 tables; a cell's
+ *                two x-neighbour entries are adjacent, one 16-byte load per cell edge)
  *                + 16 real spherical harmonics (degree <= 3) of the direction to the viewer
  *                + 16 zeros
  *   layers  4 hidden layers of width 64 (ReLU), linear output of 40 = 8 x (logit, log kappa,
@@ -486,16 +487,17 @@ MRT_API mrt_status mrt_server_stats_reset(mrt_server* server);
  * ---------------------------------------------------------------------------------- */
 typedef struct mrt_guide mrt_guide;
 
-enum { MRT_GUIDE_LEVELS = 16, MRT_GUIDE_WIDTH = 64, MRT_GUIDE_LAYERS = 5, MRT_GUIDE_OUTPUTS = 40,
-       MRT_GUIDE_LOBES = 8 };
+enum { MRT_GUIDE_LEVELS = 8, MRT_GUIDE_FEATURES = 4, MRT_GUIDE_WIDTH = 64, MRT_GUIDE_LAYERS = 5,
+       MRT_GUIDE_OUTPUTS = 40, MRT_GUIDE_LOBES = 8 };
 
 typedef struct mrt_guide_config {
     float bbox_min[3], bbox_max[3]; /* position normalisation box (all zero: scene bounds) */
     uint32_t seed;                  /* weight initialisation */
-    uint32_t log2_table;            /* hash-table entries per level (0 = 14: the sixteen levels'
+    uint32_t log2_table;            /* hash-table entries per level (0 = 14: the eight levels'
                                        tables, 1 MB, stay in L2 while inference gathers from
                                        them; at the 1080p workload 2^14 was both faster and,
-                                       with short online training, more accurate than 2^16-2^19) */
+                                       with short online training, more accurate than larger
+                                       tables); must hold max_resolution + 1 entries */
     uint32_t base_resolution;       /* coarsest grid resolution (0 = 16) */
     uint32_t max_resolution;        /* finest grid resolution (0 = 2048) */
     uint32_t train_batch;           /* records per training step (0 = 65536) */
@@ -508,7 +510,7 @@ typedef struct mrt_guide_config {
 } mrt_guide_config;
 
 /* Per-level grid resolutions and table size the guide uses (host-side computation). */
-MRT_API mrt_status mrt_guide_layout(const mrt_guide_config* config, uint32_t resolutions[16],
+MRT_API mrt_status mrt_guide_layout(const mrt_guide_config* config, uint32_t resolutions[MRT_GUIDE_LEVELS],
                                     uint32_t* table_entries);
 MRT_API mrt_status mrt_guide_create(mrt_ctx* ctx, mrt_scene* scene, const mrt_guide_config* config,
                                     mrt_guide** out);
@@ -516,7 +518,7 @@ MRT_API mrt_status mrt_guide_destroy(mrt_ctx* ctx, mrt_guide* guide);
 /* Position normalisation in use: p' = clamp((p - offset) * scale, 0, 1) per axis. */
 MRT_API mrt_status mrt_guide_normalisation(mrt_guide* guide, float offset[3], float scale[3]);
 /* Parameters as fp32 (host arrays): weights [5][64][64] (row = output; layer 5 rows >= 40 are
- * zero), grid [16][table][2]. Either pointer may be NULL. */
+ * zero), grid [8][table][4]. Either pointer may be NULL. */
 MRT_API mrt_status mrt_guide_get_params(mrt_ctx* ctx, mrt_guide* guide, float* weights, float* grid);
 /* Replaces the parameters (rounded to fp16 for the kernels) and clears the Adam state. */
 MRT_API mrt_status mrt_guide_set_params(mrt_ctx* ctx, mrt_guide* guide, const float* weights,
@@ -526,7 +528,7 @@ MRT_API mrt_status mrt_guide_forward(mrt_ctx* ctx, mrt_guide* guide, const float
                                      const float* directions, uint32_t n, float* outputs);
 /* Training records: float[12*n] = (position.xyz, weight, wo.xyz, 0, dir.xyz, 0); weight =
  * incident luminance / sampling pdf. Gradients of the loss (sum over records, scaled by
- * n / sum of weights) without an update: weights [5][64][64], grid [16][table][2] (host
+ * n / sum of weights) without an update: weights [5][64][64], grid [8][table][4] (host
  * fp32, either may be NULL); *loss = mean weighted -log q. */
 MRT_API mrt_status mrt_guide_gradient(mrt_ctx* ctx, mrt_guide* guide, const float* records,
                                       uint32_t n, float* d_weights, float* d_grid, float* loss);

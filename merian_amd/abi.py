@@ -174,7 +174,7 @@ class GuideConfig(C.Structure):
                 ("learning_rate", C.c_float), ("guide_fraction", C.c_float), ("guided_bounces", C.c_uint32)]
 
 
-GUIDE_LEVELS, GUIDE_WIDTH, GUIDE_LAYERS, GUIDE_OUTPUTS, GUIDE_LOBES = 16, 64, 5, 40, 8
+GUIDE_LEVELS, GUIDE_FEATURES, GUIDE_WIDTH, GUIDE_LAYERS, GUIDE_OUTPUTS, GUIDE_LOBES = 8, 4, 64, 5, 40, 8
 
 
 class McpgGridConfig(C.Structure):

@@ -45,7 +45,10 @@ __device__ __forceinline__ float sh_coef(int k, float x, float y, float z) {
 }
 
 // Corners of grid level l around normalised position pn: table indices and trilinear weights
-// (corner bit 0 = x, 1 = y, 2 = z).
+// (corner bit 0 = x, 1 = y, 2 = z). Both layouts keep a grid row along x in consecutive
+// entries: dense levels are x-major, and a hashed level places row (y, z) at an offset in
+// [0, table - side] drawn from a hash of (y, z) (multiply-high range reduction), so
+// idx[2r + 1] == idx[2r] + 1 < table and one 8-byte load fetches each x-edge of the cell.
 __device__ __forceinline__ void grid_corners(const GuideNet& net, int l, const float pn[3], uint32_t idx[8], float w[8]) {
     const uint32_t res = net.res[l];
     int ci[3];
@@ -60,12 +63,13 @@ __device__ __forceinline__ void grid_corners(const GuideNet& net, int l, const f
     }
     const uint64_t side = uint64_t(res) + 1;
     const bool dense = side * side * side <= uint64_t(net.table);
+    const uint32_t span = net.table - uint32_t(side) + 1u; // row offsets; side <= table (checked at create)
 #pragma unroll
     for (int c = 0; c < 8; c++) {
         const uint32_t cx = uint32_t(ci[0] + (c & 1)), cy = uint32_t(ci[1] + ((c >> 1) & 1)),
                        cz = uint32_t(ci[2] + ((c >> 2) & 1));
         idx[c] = dense ? uint32_t(cx + side * (cy + side * cz))
-                       : ((cx * 1u) ^ (cy * 2654435761u) ^ (cz * 805459861u)) & (net.table - 1u);
+                       : uint32_t((uint64_t((cy * 2654435761u) ^ (cz * 805459861u)) * span) >> 32) + cx;
         const float wx = (c & 1) ? fr[0] : 1.f - fr[0];
         const float wy = (c & 2) ? fr[1] : 1.f - fr[1];
         const float wz = (c & 4) ? fr[2] : 1.f - fr[2];
@@ -79,30 +83,46 @@ __device__ __forceinline__ void normalise(const GuideNet& net, f3 p, float pn[3]
     pn[2] = fminf(1.f, fmaxf(0.f, (p.z - net.offset[2]) * net.scale[2]));
 }
 
-// Grid level of slot pair s/2 in chunk 0 for lane group g: features feat_of(0, g, s) = 2 * level + f.
-__device__ __forceinline__ int slot_level(int g, int pair) { return (pair < 2 ? 2 * g : 8 + 2 * g) + (pair & 1); }
+// Entries i and i + 1 of one level's table (the two x-corners of a cell edge, G_FEATS fp16
+// features each) in one 16-byte load (8-byte aligned).
+struct __attribute__((packed, aligned(8))) EntryPair {
+    uint32_t a0, a1, b0, b1;
+};
+static_assert(G_FEATS == 4, "one entry = two 32-bit words");
 
-// Encoding of one query by lane group g: chunk 0 = grid levels {2g, 2g+1, 8+2g, 9+2g},
-// chunk 1 = SH coefficients 4g..4g+3, then zeros.
+// Grid levels of lane group g: chunk-0 slots s < 4 hold features 4g + s (level g), slots s >= 4
+// hold features 16 + 4g + s - 4 (level 4 + g), as feat_of(0, g, s) = G_FEATS * level + feature.
+__device__ __forceinline__ int slot_level(int g, int h) { return 4 * h + g; }
+
+__device__ __forceinline__ void add_entry(float f[4], float w, uint32_t lo, uint32_t hi) {
+    f[0] += w * half_to_float(uint16_t(lo & 0xFFFFu));
+    f[1] += w * half_to_float(uint16_t(lo >> 16));
+    f[2] += w * half_to_float(uint16_t(hi & 0xFFFFu));
+    f[3] += w * half_to_float(uint16_t(hi >> 16));
+}
+
+// Encoding of one query by lane group g: chunk 0 = grid levels g and 4 + g (four features
+// each), chunk 1 = SH coefficients 4g..4g+3, then zeros.
 __device__ __forceinline__ void encode(const GuideNet& net, f3 p, f3 wo, int g, half8_t x[2]) {
     float pn[3];
     normalise(net, p, pn);
     const uint32_t* tab = reinterpret_cast<const uint32_t*>(net.grid);
 #pragma unroll
-    for (int pair = 0; pair < 4; pair++) {
-        const int l = slot_level(g, pair);
+    for (int h = 0; h < 2; h++) {
+        const int l = slot_level(g, h);
         uint32_t idx[8];
         float w[8];
         grid_corners(net, l, pn, idx, w);
-        float f0 = 0.f, f1 = 0.f;
+        const uint32_t* lt = tab + size_t(l) * net.table * 2;
+        float f[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int c = 0; c < 8; c++) {
-            const uint32_t v = tab[size_t(l) * net.table + idx[c]];
-            f0 += w[c] * half_to_float(uint16_t(v & 0xFFFFu));
-            f1 += w[c] * half_to_float(uint16_t(v >> 16));
+        for (int e = 0; e < 4; e++) {
+            const EntryPair v = *reinterpret_cast<const EntryPair*>(lt + 2 * size_t(idx[2 * e]));
+            add_entry(f, w[2 * e], v.a0, v.a1);
+            add_entry(f, w[2 * e + 1], v.b0, v.b1);
         }
-        x[0][2 * pair] = _Float16(f0);
-        x[0][2 * pair + 1] = _Float16(f1);
+#pragma unroll
+        for (int k = 0; k < 4; k++) x[0][4 * h + k] = _Float16(f[k]);
     }
 #pragma unroll
     for (int s = 0; s < 4; s++) x[1][s] = _Float16(sh_coef(4 * g + s, wo.x, wo.y, wo.z));
@@ -443,8 +463,8 @@ __global__ __launch_bounds__(GB) void k_guide_fwd_bwd(GuideNet net, const float4
                             d[t][c][s] = _Float16(float(x[layer][t][c][s]) > 0.f ? acc[t][2 * c + (s >> 2)][s & 3] : 0.f);
             }
         }
-        // acc = gradient at the encoding: in-tile 0 holds features 4g+r (levels 2g, 2g+1),
-        // in-tile 1 holds 16+4g+r (levels 8+2g, 9+2g)
+        // acc = gradient at the encoding: in-tile 0 holds features 4g+r (level g), in-tile 1
+        // holds 16+4g+r (level 4+g)
 #pragma unroll
         for (int t = 0; t < NT; t++) {
             if (rid[t] >= n) continue;
@@ -452,17 +472,17 @@ __global__ __launch_bounds__(GB) void k_guide_fwd_bwd(GuideNet net, const float4
             float pn[3];
             normalise(net, f3{a.x, a.y, a.z}, pn);
 #pragma unroll
-            for (int pair = 0; pair < 4; pair++) {
-                const int l = slot_level(g, pair);
-                const float d0 = acc[t][pair >> 1][2 * (pair & 1)], d1 = acc[t][pair >> 1][2 * (pair & 1) + 1];
+            for (int h = 0; h < 2; h++) {
+                const int l = slot_level(g, h);
                 uint32_t idx[8];
                 float w[8];
                 grid_corners(net, l, pn, idx, w);
 #pragma unroll
                 for (int c = 0; c < 8; c++) {
-                    float* gg = B.grid_grad + (size_t(l) * net.table + idx[c]) * 2;
-                    if (d0 != 0.f) unsafeAtomicAdd(gg, w[c] * d0);
-                    if (d1 != 0.f) unsafeAtomicAdd(gg + 1, w[c] * d1);
+                    float* gg = B.grid_grad + (size_t(l) * net.table + idx[c]) * G_FEATS;
+#pragma unroll
+                    for (int k = 0; k < G_FEATS; k++)
+                        if (acc[t][h][k] != 0.f) unsafeAtomicAdd(gg + k, w[c] * acc[t][h][k]);
                 }
             }
         }

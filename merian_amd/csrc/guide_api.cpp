@@ -54,12 +54,14 @@ mrt_status check_config(const mrt_guide_config& c) {
     if (c.log2_table < 8 || c.log2_table > 24) return MRT_ERR_INVALID;
     if (c.base_resolution < 1 || c.max_resolution < c.base_resolution || c.max_resolution > (1u << 20))
         return MRT_ERR_INVALID;
+    // a hashed level stores each grid row along x whole: the finest row must fit one table
+    if (uint64_t(c.max_resolution) + 1 > (uint64_t(1) << c.log2_table)) return MRT_ERR_INVALID;
     if (c.train_batch > (1u << 22) || c.max_records > (1u << 26)) return MRT_ERR_INVALID;
     if (!(c.guide_fraction > 0.f && c.guide_fraction < 1.f)) return MRT_ERR_INVALID;
     return MRT_OK;
 }
 
-void layout(const mrt_guide_config& c, uint32_t res[16]) {
+void layout(const mrt_guide_config& c, uint32_t res[MRT_GUIDE_LEVELS]) {
     const double b = std::log(double(c.max_resolution) / double(c.base_resolution)) / double(MRT_GUIDE_LEVELS - 1);
     for (int l = 0; l < MRT_GUIDE_LEVELS; l++)
         res[l] = uint32_t(std::floor(double(c.base_resolution) * std::exp(b * double(l)) + 0.5));
@@ -216,7 +218,7 @@ mrt_status guide_train_frame(mrt_ctx* c, mrt_guide* g, hipStream_t st) {
 
 extern "C" {
 
-MRT_API mrt_status mrt_guide_layout(const mrt_guide_config* config, uint32_t resolutions[16], uint32_t* table) {
+MRT_API mrt_status mrt_guide_layout(const mrt_guide_config* config, uint32_t resolutions[MRT_GUIDE_LEVELS], uint32_t* table) {
     const mrt_guide_config c = with_defaults(config);
     if (check_config(c) != MRT_OK) return MRT_ERR_INVALID;
     if (resolutions) layout(c, resolutions);
@@ -246,7 +248,7 @@ MRT_API mrt_status mrt_guide_create(mrt_ctx* c, mrt_scene* s, const mrt_guide_co
     }
     mrt_status st = MRT_OK;
     const size_t nw = size_t(G_LAYERS) * G_WIDTH * G_WIDTH;
-    const uint64_t ng = uint64_t(G_LEVELS) * g->net.table * 2;
+    const uint64_t ng = uint64_t(G_LEVELS) * g->net.table * G_FEATS;
     g->P.w = galloc<float>(c, g, nw, &st);
     g->P.w_m = galloc<float>(c, g, nw, &st);
     g->P.w_v = galloc<float>(c, g, nw, &st);

@@ -18,7 +18,7 @@
 
 namespace mrt {
 
-constexpr int G_LEVELS = 16, G_WIDTH = 64, G_LAYERS = 5, G_OUT = 40, G_LOBES = 8;
+constexpr int G_LEVELS = 8, G_FEATS = 4, G_WIDTH = 64, G_LAYERS = 5, G_OUT = 40, G_LOBES = 8; // G_FEATS per entry
 constexpr float G_KAPPA_LOG_MIN = -2.302585093f; // kappa in [0.1, 5000]
 constexpr float G_KAPPA_LOG_MAX = 8.517193191f;
 constexpr float G_DOUT_CLAMP = 4096.f;
@@ -33,7 +33,7 @@ __host__ __device__ inline int feat_of(int c, int g, int s) { return 16 * (2 * c
 struct GuideNet {
     const half8_t* frag_fwd; // W_k fragments: [layer][out tile][chunk][lane]
     const half8_t* frag_bwd; // W_k^T fragments: [layer][in tile][chunk][lane]
-    const _Float16* grid;    // [level][table][2]
+    const _Float16* grid;    // [level][table][G_FEATS]
     uint32_t res[G_LEVELS];
     uint32_t table;
     float offset[3], scale[3];
@@ -122,7 +122,7 @@ struct GuideTrainBuffers {
     _Float16* H;       // [layer][cap][64] inputs of each layer
     _Float16* D;       // [layer][cap][64] gradients at each layer's outputs
     float* partials;   // [layer][kblocks][64][64]
-    float* grid_grad;  // [level][table][2]
+    float* grid_grad;  // [level][table][G_FEATS]
     float* scalars;    // [0] weight sum, [1] loss sum
     uint32_t cap, kblocks;
 };
@@ -142,7 +142,7 @@ struct GuideParams {
     float* w_v;
     half8_t* frag_fwd;
     half8_t* frag_bwd;
-    float* g;      // master grid [16][table][2]
+    float* g;      // master grid [G_LEVELS][table][G_FEATS]
     float* g_m;
     float* g_v;
     _Float16* grid;

@@ -287,7 +287,7 @@ def guide_config(seed: int = 1, log2_table: int = 0, base_resolution: int = 0, m
 def guide_layout(cfg: abi.GuideConfig):
     """(per-level resolutions, table entries) of a guide configuration (host computation)."""
     lib = abi.load_library()
-    res = (C.c_uint32 * 16)()
+    res = (C.c_uint32 * abi.GUIDE_LEVELS)()
     t = C.c_uint32()
     st = lib.mrt_guide_layout(C.byref(cfg), res, C.byref(t))
     if st != abi.MRT_OK:
@@ -318,7 +318,7 @@ class Guide:
 
     def params(self):
         W = np.zeros((abi.GUIDE_LAYERS, 64, 64), np.float32)
-        G = np.zeros((abi.GUIDE_LEVELS, self.table, 2), np.float32)
+        G = np.zeros((abi.GUIDE_LEVELS, self.table, abi.GUIDE_FEATURES), np.float32)
         self.ctx._check(self.lib.mrt_guide_get_params(self.ctx.h, self.h, _p(W), _p(G)))
         return W, G
 
@@ -337,7 +337,7 @@ class Guide:
     def gradient(self, records: np.ndarray):
         R = np.ascontiguousarray(records, np.float32)
         dW = np.zeros((abi.GUIDE_LAYERS, 64, 64), np.float32)
-        dG = np.zeros((abi.GUIDE_LEVELS, self.table, 2), np.float32)
+        dG = np.zeros((abi.GUIDE_LEVELS, self.table, abi.GUIDE_FEATURES), np.float32)
         loss = C.c_float()
         self.ctx._check(self.lib.mrt_guide_gradient(self.ctx.h, self.h, _p(R), len(R), _p(dW), _p(dG),
                                                     C.byref(loss)))

@@ -32,7 +32,7 @@ class BsdfCfg(C.Structure):
 
 class GuideSpec(C.Structure):
     """Guide encoding layout: per-level grid resolutions, table entries, position normalisation."""
-    _fields_ = [("res", C.c_uint32 * 16), ("table", C.c_uint32), ("offset", C.c_float * 3),
+    _fields_ = [("res", C.c_uint32 * 8), ("table", C.c_uint32), ("offset", C.c_float * 3),
                 ("scale", C.c_float * 3)]
 
 
@@ -273,7 +273,7 @@ def pt_params(spp=4, max_path_length=4, emission_on_primary=True, mask=0xFF, mod
 
 def guide_spec(res, table: int, offset, scale) -> GuideSpec:
     g = GuideSpec()
-    for k in range(16):
+    for k in range(8):
         g.res[k] = int(res[k])
     g.table = int(table)
     for a in range(3):

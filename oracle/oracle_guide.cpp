@@ -19,7 +19,7 @@
 
 namespace orc_guide {
 
-constexpr int LEVELS = 16, WIDTH = 64, LAYERS = 5, OUTPUTS = 40, LOBES = 8;
+constexpr int LEVELS = 8, FEATS = 4, WIDTH = 64, LAYERS = 5, OUTPUTS = 40, LOBES = 8; // FEATS per grid entry
 constexpr float KAPPA_LOG_MIN = -2.302585093f; // kappa in [0.1, 5000]
 constexpr float KAPPA_LOG_MAX = 8.517193191f;
 constexpr float DOUT_CLAMP = 4096.f;           // output gradients are clamped to +-4096 (fp16 range)
@@ -64,6 +64,9 @@ template <class T> void sh16(T x, T y, T z, T* o) {
 }
 
 // Grid corner of one level: 8 table indices and trilinear weights (corner bit 0 = x, 1 = y, 2 = z).
+// Dense levels are x-major; a hashed level stores grid row (y, z) as a run of side entries at
+// offset floor(hash(y, z) * (table - side + 1) / 2^32), so a cell's two x-corners are
+// consecutive entries.
 template <class T> void corners(const Spec& s, int l, const float* pn, uint32_t* idx, T* w) {
     const uint32_t res = s.res[l];
     int ci[3];
@@ -78,11 +81,12 @@ template <class T> void corners(const Spec& s, int l, const float* pn, uint32_t*
     }
     const uint64_t side = uint64_t(res) + 1;
     const bool dense = side * side * side <= s.table;
+    const uint32_t span = s.table - uint32_t(side) + 1u;
     for (int c = 0; c < 8; c++) {
         const uint32_t cx = uint32_t(ci[0] + (c & 1)), cy = uint32_t(ci[1] + ((c >> 1) & 1)),
                        cz = uint32_t(ci[2] + ((c >> 2) & 1));
         idx[c] = dense ? uint32_t(cx + side * (cy + side * cz))
-                       : ((cx * 1u) ^ (cy * 2654435761u) ^ (cz * 805459861u)) & (s.table - 1u);
+                       : uint32_t((uint64_t((cy * 2654435761u) ^ (cz * 805459861u)) * span) >> 32) + cx;
         const T wx = (c & 1) ? fr[0] : T(1) - fr[0];
         const T wy = (c & 2) ? fr[1] : T(1) - fr[1];
         const T wz = (c & 4) ? fr[2] : T(1) - fr[2];
@@ -110,10 +114,10 @@ void forward(const Spec& s, const std::vector<typename Num<MIRROR>::T>& W, const
     normalise(s, p, pn);
     for (int l = 0; l < LEVELS; l++) {
         corners<T>(s, l, pn, a.idx[l], a.cw[l]);
-        for (int f = 0; f < 2; f++) {
+        for (int f = 0; f < FEATS; f++) {
             T acc = 0;
-            for (int c = 0; c < 8; c++) acc += a.cw[l][c] * G[(size_t(l) * s.table + a.idx[l][c]) * 2 + f];
-            a.h[0][2 * l + f] = Num<MIRROR>::round(acc);
+            for (int c = 0; c < 8; c++) acc += a.cw[l][c] * G[(size_t(l) * s.table + a.idx[l][c]) * FEATS + f];
+            a.h[0][FEATS * l + f] = Num<MIRROR>::round(acc);
         }
     }
     T sh[16];
@@ -207,7 +211,7 @@ template <class T> void output_gradient(const T* out, const T* d, T s, T* g) {
 template <bool MIRROR>
 float gradient(const Spec& s, const float* Wf, const float* Gf, const float* rec, uint32_t n, float* dW, float* dG) {
     using T = typename Num<MIRROR>::T;
-    std::vector<T> W(size_t(LAYERS) * WIDTH * WIDTH), G(size_t(LEVELS) * s.table * 2);
+    std::vector<T> W(size_t(LAYERS) * WIDTH * WIDTH), G(size_t(LEVELS) * s.table * FEATS);
     for (size_t i = 0; i < W.size(); i++) W[i] = Num<MIRROR>::round(T(Wf[i]));
     for (size_t i = 0; i < G.size(); i++) G[i] = Num<MIRROR>::round(T(Gf[i]));
     std::vector<T> gW(W.size(), T(0)), gG(G.size(), T(0));
@@ -243,8 +247,8 @@ float gradient(const Spec& s, const float* Wf, const float* Gf, const float* rec
         // dh = gradient at the encoding; scatter the grid part to the 8 corners of each level
         for (int l = 0; l < LEVELS; l++)
             for (int c = 0; c < 8; c++)
-                for (int f = 0; f < 2; f++)
-                    gG[(size_t(l) * s.table + a.idx[l][c]) * 2 + f] += a.cw[l][c] * dh[2 * l + f];
+                for (int f = 0; f < FEATS; f++)
+                    gG[(size_t(l) * s.table + a.idx[l][c]) * FEATS + f] += a.cw[l][c] * dh[FEATS * l + f];
     }
     if (dW)
         for (size_t i = 0; i < gW.size(); i++) dW[i] = float(gW[i]);
@@ -259,11 +263,11 @@ using namespace orc_guide;
 
 extern "C" {
 
-// Spec layout: res[16], table, offset[3], scale[3] (host struct of 23 words).
+// Spec layout: res[8], table, offset[3], scale[3] (host struct of 15 words).
 void orc_guide_forward(const Spec* s, const float* Wf, const float* Gf, const float* pos, const float* wo, uint32_t n,
                        float* out, float* encoding, int exact) {
     if (exact) {
-        std::vector<double> W(Wf, Wf + size_t(LAYERS) * WIDTH * WIDTH), G(Gf, Gf + size_t(LEVELS) * s->table * 2);
+        std::vector<double> W(Wf, Wf + size_t(LAYERS) * WIDTH * WIDTH), G(Gf, Gf + size_t(LEVELS) * s->table * FEATS);
         Act<double> a;
         for (uint32_t q = 0; q < n; q++) {
             forward<false>(*s, W, G, pos + 3 * q, wo + 3 * q, a);
@@ -273,7 +277,7 @@ void orc_guide_forward(const Spec* s, const float* Wf, const float* Gf, const fl
         }
         return;
     }
-    std::vector<float> W(size_t(LAYERS) * WIDTH * WIDTH), G(size_t(LEVELS) * s->table * 2);
+    std::vector<float> W(size_t(LAYERS) * WIDTH * WIDTH), G(size_t(LEVELS) * s->table * FEATS);
     for (size_t i = 0; i < W.size(); i++) W[i] = r16(Wf[i]);
     for (size_t i = 0; i < G.size(); i++) G[i] = r16(Gf[i]);
     Act<float> a;

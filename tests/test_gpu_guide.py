@@ -69,7 +69,7 @@ def _random_params(g, seed, scale=1.0):
     for k in range(5):
         n_out = 64 if k < 4 else 40
         W[k, :n_out] = rng.uniform(-1, 1, (n_out, 64)) * math.sqrt(6 / (64 + n_out)) * scale
-    G = rng.uniform(-0.5, 0.5, (16, g.table, 2)).astype(np.float32)
+    G = rng.uniform(-0.5, 0.5, (abi.GUIDE_LEVELS, g.table, abi.GUIDE_FEATURES)).astype(np.float32)
     return W, G
 
 

@@ -39,7 +39,7 @@ def random_params(table, seed=0, scale=1.0):
     for k in range(5):
         n_out = 64 if k < 4 else 40
         W[k, :n_out] = rng.uniform(-1, 1, (n_out, 64)) * math.sqrt(6 / (64 + n_out)) * scale
-    G = rng.uniform(-0.5, 0.5, (16, table, 2)).astype(np.float32)
+    G = rng.uniform(-0.5, 0.5, (8, table, 4)).astype(np.float32)
     return W, G
 
 
@@ -88,8 +88,8 @@ def test_dense_levels_reproduce_linear_functions():
     spec, res, table = small_spec(log2_table=16, base=2, top=64)
     # feature 0 = a . (vertex index / res) + b on every dense level; interpolation is exact
     a, b = np.array([0.3, -0.2, 0.5]), 0.1
-    G = np.zeros((16, table, 2), np.float32)
-    dense = [l for l in range(16) if (res[l] + 1) ** 3 <= table]
+    G = np.zeros((8, table, 4), np.float32)
+    dense = [l for l in range(8) if (res[l] + 1) ** 3 <= table]
     assert len(dense) >= 4
     for l in dense:
         s = res[l] + 1
@@ -97,12 +97,14 @@ def test_dense_levels_reproduce_linear_functions():
         xyz = np.stack([idx % s, (idx // s) % s, idx // (s * s)], 1) / res[l]
         G[l, idx, 0] = xyz @ a + b
         G[l, idx, 1] = 1.0
+        G[l, idx, 3] = xyz @ a
     W = np.zeros((5, 64, 64), np.float32)
     pts = np.random.default_rng(4).uniform(0, 1, (64, 3)).astype(np.float32)
     _, enc = O.guide_forward(spec, W, G, pts, np.tile([0, 0, 1.0], (64, 1)), exact=True, encoding=True)
     for l in dense:
-        assert np.allclose(enc[:, 2 * l], pts @ a + b, atol=1e-6)
-        assert np.allclose(enc[:, 2 * l + 1], 1.0, atol=1e-6)  # the weights sum to one
+        assert np.allclose(enc[:, 4 * l], pts @ a + b, atol=1e-6)
+        assert np.allclose(enc[:, 4 * l + 1], 1.0, atol=1e-6)  # the weights sum to one
+        assert np.allclose(enc[:, 4 * l + 2], 0.0) and np.allclose(enc[:, 4 * l + 3], pts @ a, atol=1e-6)
 
 
 def test_exact_gradient_matches_finite_differences():
@@ -154,10 +156,23 @@ def test_mirror_mode_close_to_exact():
     lm, dWm, _ = O.guide_gradient(spec, W, G, R, exact=False)
     assert abs(le - lm) < 1e-2 * abs(le) + 1e-3
     for k in range(5):
-        assert np.linalg.norm(dWm[k] - dWe[k]) <= 3e-2 * np.linalg.norm(dWe[k]) + 1e-4
+        assert np.linalg.norm(dWm[k] - dWe[k]) <= 1e-1 * np.linalg.norm(dWe[k]) + 1e-4
+    # record by record: a ReLU unit whose pre-activation lies within fp16 rounding of zero
+    # switches between the modes and moves one row of that layer's gradient (a few percent of
+    # records, whatever the grid's hash); every other record agrees within 2 %
+    close = 0
+    for i in range(len(R)):
+        _, a, _ = O.guide_gradient(spec, W, G, R[i:i + 1], exact=True)
+        _, b, _ = O.guide_gradient(spec, W, G, R[i:i + 1], exact=False)
+        close += all(np.linalg.norm(b[k] - a[k]) <= 2e-2 * np.linalg.norm(a[k]) + 1e-6 for k in range(5))
+    assert close >= 0.9 * len(R), close
 
 
 def test_guide_layout():
     res, table = guide_layout(guide_config())
-    assert res[0] == 16 and res[-1] == 2048 and table == 1 << 14
+    assert len(res) == 8 and res[0] == 16 and res[-1] == 2048 and table == 1 << 14
     assert all(b > a for a, b in zip(res, res[1:]))
+    # a hashed level keeps each grid row along x whole, so the finest row must fit one table
+    assert guide_layout(guide_config(log2_table=10, base_resolution=2, max_resolution=1023))[1] == 1024
+    with pytest.raises(Exception, match="out of range"):
+        guide_layout(guide_config(log2_table=10, base_resolution=2, max_resolution=1024))
