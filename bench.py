@@ -236,7 +236,7 @@ def cpu_ray_query(threads: int) -> dict:
 
 
 def guided_leg(ctx, scene, L: int, spp: int = 64, ref_frames: int = 256, budget_s: float = 3.0,
-               train_frames: int | None = None, log2_table: int = 14, guided_bounces: int = 3) -> dict:
+               train_frames: int | None = None, log2_table: int = 14, guided_bounces: int = 4) -> dict:
     """Learned guiding at the 1080p workload's settings (1920x1080, 64 spp frames, L = 8): relMSE at
     equal device time against a 16384-spp plain render (256 frames of 64 spp).
 
@@ -574,7 +574,7 @@ def main():
     ap.add_argument("--ray-reps", type=int, default=3)
     ap.add_argument("--ref-frames", type=int, default=256, help="64-spp frames of the guided leg's reference")
     ap.add_argument("--guide-table", type=int, default=14, help="log2 entries per hash-grid level of the guide")
-    ap.add_argument("--guided-bounces", type=int, default=3, help="vertices [0, n) sample the guide")
+    ap.add_argument("--guided-bounces", type=int, default=4, help="vertices [0, n) sample the guide")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; --gpus N > 1 is not supported", file=sys.stderr)

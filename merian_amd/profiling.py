@@ -17,6 +17,10 @@ WRITE_SIZE counters) and summaries of rocprofv3's CSV output.
         SQ_INSTS_VALU, SQ_ACTIVE_INST_VALU and SQ_THREAD_CYCLES_VALU
     python -m merian_amd.profiling l2 DIR
         per-kernel L2 hit rate from a TCC_HIT_sum / TCC_MISS_sum pass
+    python -m merian_amd.profiling l1 DIR
+        per-kernel L1-to-L2 read requests (count, rate, L1 hit share) and texture-addresser busy
+        and stall shares, from a pass with TCP_TCC_READ_REQ_sum, TCP_TOTAL_CACHE_ACCESSES_sum,
+        TA_BUSY_avr, TA_ADDR_STALLED_BY_TC_CYCLES_sum and GRBM_GUI_ACTIVE
     python -m merian_amd.profiling dram DIR
         per-kernel L2-to-fabric requests and the share destined for DRAM, from a pass with
         TCC_EA0_RDREQ_sum, TCC_EA0_RDREQ_DRAM_sum, TCC_EA0_WRREQ_sum and TCC_EA0_WRREQ_DRAM_sum
@@ -168,6 +172,30 @@ def valu(d: str) -> dict:
                   "valu_busy_pct": round(100 * c["SQ_ACTIVE_INST_VALU"] / cus / gui, 1),
                   "valu_lane_util_pct": round(100 * c.get("SQ_THREAD_CYCLES_VALU", 0) / (64 * c["SQ_ACTIVE_INST_VALU"]), 1),
                   "valu_issue_ms": round(c.get("SQ_INSTS_VALU", 0) / n * 4 / simds / clk_hz * 1e3, 4)}
+    return out
+
+
+def l1(d: str) -> dict:
+    """L1 (TCP) figures per kernel: read requests it sends to the L2 per dispatch and per second
+    of the kernel's own duration, the share of its accesses that did not become L2 reads, the
+    texture addresser's busy share and the share of cycles it stalled on the L1 (both per CU,
+    over the GPU's busy cycles)."""
+    a = agent(d)
+    cus, xcc = int(a["Cu_Count"]), int(a["Num_Xcc"])
+    acc, disp = _sums(d)
+    out = {}
+    for k, c in acc.items():
+        n, ms = disp[k]
+        req = c.get("TCP_TCC_READ_REQ_sum", 0.0)
+        gui = c.get("GRBM_GUI_ACTIVE", 0.0) / xcc
+        if req <= 0 or gui <= 0:
+            continue
+        tot = c.get("TCP_TOTAL_CACHE_ACCESSES_sum", 0.0)
+        out[k] = {"dispatches": n, "avg_ms": round(ms / n, 4), "l2_read_req_per_dispatch": round(req / n),
+                  "l2_read_greq_per_s": round(req / (ms * 1e-3) / 1e9, 1),
+                  "l1_hit_pct": round(100 * (1 - req / tot), 1) if tot > 0 else None,
+                  "ta_busy_pct": round(100 * c.get("TA_BUSY_avr", 0.0) / gui, 1),
+                  "ta_stalled_on_l1_pct": round(100 * c.get("TA_ADDR_STALLED_BY_TC_CYCLES_sum", 0.0) / cus / gui, 1)}
     return out
 
 
@@ -360,7 +388,7 @@ def main(argv=None) -> int:
     gd.add_argument("--spp", type=int, default=64)
     gd.add_argument("--train-frames", type=int, default=4)
     gd.add_argument("--frames", type=int, default=4)
-    gd.add_argument("--guided-bounces", type=int, default=3)
+    gd.add_argument("--guided-bounces", type=int, default=4)
     mf = sub.add_parser("mfma")
     mf.add_argument("dir")
     g = sub.add_parser("mcpg")
@@ -398,6 +426,8 @@ def main(argv=None) -> int:
     v.add_argument("dir")
     l = sub.add_parser("l2")
     l.add_argument("dir")
+    l1p = sub.add_parser("l1")
+    l1p.add_argument("dir")
     args = ap.parse_args(argv)
     if args.cmd == "copy":
         copy_workload(args.mib, args.reps)
@@ -409,8 +439,8 @@ def main(argv=None) -> int:
         guided_workload(args.width, args.height, args.spp, args.train_frames, args.frames, args.guided_bounces)
     elif args.cmd == "traffic":
         print(json.dumps(traffic_record(args), indent=1))
-    elif args.cmd in ("valu", "l2", "mfma", "dram"):
-        rows = {"valu": valu, "l2": l2, "mfma": mfma, "dram": dram}[args.cmd](args.dir)
+    elif args.cmd in ("valu", "l2", "l1", "mfma", "dram"):
+        rows = {"valu": valu, "l2": l2, "l1": l1, "mfma": mfma, "dram": dram}[args.cmd](args.dir)
         keys = list(next(iter(rows.values())).keys()) if rows else []
         print("| kernel | " + " | ".join(keys) + " |\n|---|" + "---:|" * len(keys))
         for k, r in sorted(rows.items(), key=lambda kv: -kv[1]["dispatches"] * kv[1].get("avg_ms", 1)):

@@ -117,3 +117,23 @@ def test_bench_roofline_uses_the_profile_only_for_its_tuning(prof, tmp_path):
                 {"tuning": {"bvh_width": 8, "max_paths": 1 << 28}, "workload": "other"}):
         r = bench.roofline("k", 1.0, 10, 8e9, str(prof_json), match=bad)
         assert r["traffic"] is None and r["frac"] == 1.0 and "algorithmic" in r["basis"]
+
+
+def test_l1_request_summary(tmp_path):
+    # one dispatch of 10 ms on a 256-CU, 8-XCD agent busy for 24 M cycles per XCD: 2 G L1->L2
+    # reads of 3 G accesses; the addresser busy 18 M cycles per CU and stalled 12 M per CU
+    _counters(tmp_path / "l1", [(1, K, "TCP_TCC_READ_REQ_sum", 2e9, 10_000_000),
+                                (1, K, "TCP_TOTAL_CACHE_ACCESSES_sum", 3e9, 10_000_000),
+                                (1, K, "TA_BUSY_avr", 18e6, 10_000_000),
+                                (1, K, "TA_ADDR_STALLED_BY_TC_CYCLES_sum", 256 * 12e6, 10_000_000),
+                                (1, K, "GRBM_GUI_ACTIVE", 8 * 24e6, 10_000_000)])
+    with open(tmp_path / "l1" / "x_agent_info.csv", "w", newline="") as f:
+        w = csv.DictWriter(f, ["Agent_Type", "Name", "Cu_Count", "Simd_Count", "Num_Xcc", "Max_Engine_Clk_Fcompute"])
+        w.writeheader()
+        w.writerow({"Agent_Type": "CPU", "Name": "EPYC", "Cu_Count": 16, "Simd_Count": 0, "Num_Xcc": 0,
+                    "Max_Engine_Clk_Fcompute": 0})
+        w.writerow({"Agent_Type": "GPU", "Name": "gfx950", "Cu_Count": 256, "Simd_Count": 1024, "Num_Xcc": 8,
+                    "Max_Engine_Clk_Fcompute": 2400})
+    (k, v), = P.l1(str(tmp_path / "l1")).items()
+    assert k.startswith("k_trace<") and v["l2_read_req_per_dispatch"] == 2e9 and v["l2_read_greq_per_s"] == 200.0
+    assert v["l1_hit_pct"] == 33.3 and v["ta_busy_pct"] == 75.0 and v["ta_stalled_on_l1_pct"] == 50.0
