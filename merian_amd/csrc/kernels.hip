@@ -295,7 +295,7 @@ __device__ __forceinline__ f3 env_to_world(const DeviceScene& S, f3 l) { // to_l
 }
 
 // Solid-angle density with which env_sample produces world direction `dir`.
-__device__ float env_pdf(const DeviceScene& S, f3 dir) {
+__device__ __forceinline__ float env_pdf(const DeviceScene& S, f3 dir) {
     if (S.env_type == MRT_ENV_NONE) return 0.f;
     const float* m = S.env_to_local;
     const f3 d = normalize(f3{m[0] * dir.x + m[1] * dir.y + m[2] * dir.z, m[3] * dir.x + m[4] * dir.y + m[5] * dir.z,
@@ -331,7 +331,7 @@ __device__ float env_pdf(const DeviceScene& S, f3 dir) {
 }
 
 // Samples a world direction toward the environment; returns false for an empty environment.
-__device__ bool env_sample(const DeviceScene& S, PCG& rng, f3& dir, float& pdf) {
+__device__ __forceinline__ bool env_sample(const DeviceScene& S, PCG& rng, f3& dir, float& pdf) {
     if (S.env_type == MRT_ENV_NONE) return false;
     const uint32_t n = S.env_w * S.env_h;
     const float r0 = rng.next_float(), r1 = rng.next_float();
@@ -581,7 +581,7 @@ __device__ __forceinline__ void stage_flush(const RayStage& st, float4* __restri
 // Groups the tile's paths [tile, tile + SHADE_TILE) by shading class (counting sort in LDS,
 // stable within a class): order[k] = queue slot of the k-th path in class order; slots past
 // the queue end sort last. Ends with the block synchronised.
-__device__ void shade_sort_tile(const DeviceScene& S, const float4* __restrict__ hits, uint32_t tile, uint32_t n,
+__device__ __forceinline__ void shade_sort_tile(const DeviceScene& S, const float4* __restrict__ hits, uint32_t tile, uint32_t n,
                                 uint32_t* order, uint32_t* cnt, uint32_t* total) {
     constexpr uint32_t K = SHADE_CLASSES + 1;
     constexpr uint32_t WAVES = SHADE_BLOCK / 64;

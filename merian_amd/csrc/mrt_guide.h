@@ -80,24 +80,21 @@ __device__ __forceinline__ float lobes_pdf(const GuideLobes& L, f3 d) {
 // Samples the mixture: lobe by weight (u0), then the vMF lobe (u1, u2):
 // cos = 1 + log(u1 + (1 - u1) exp(-2 kappa)) / kappa, azimuth 2 pi u2 around mu.
 __device__ __forceinline__ f3 lobes_sample(const GuideLobes& L, float u0, float u1, float u2) {
+    // The chosen lobe's kappa and mu ride along in selects while the weights are walked, so no
+    // lobe index exists that the compiler could turn into an indexed (scratch-resident) array.
     float t = u0 * L.pi_sum;
-    int j = G_LOBES - 1;
+    float kappa = L.kappa[G_LOBES - 1];
+    f3 mu = L.mu[G_LOBES - 1];
+    bool found = false;
 #pragma unroll
     for (int k = 0; k < G_LOBES - 1; k++) {
-        if (t < L.pi[k]) {
-            j = k;
-            break;
-        }
-        t -= L.pi[k];
-    }
-    float kappa = L.kappa[0];
-    f3 mu = L.mu[0];
-#pragma unroll
-    for (int k = 1; k < G_LOBES; k++) {
-        if (k == j) {
-            kappa = L.kappa[k];
-            mu = L.mu[k];
-        }
+        const bool take = !found && t < L.pi[k];
+        kappa = take ? L.kappa[k] : kappa;
+        mu.x = take ? L.mu[k].x : mu.x;
+        mu.y = take ? L.mu[k].y : mu.y;
+        mu.z = take ? L.mu[k].z : mu.z;
+        if (!found && !take) t -= L.pi[k];
+        found = found || take;
     }
     float c = 1.f + logf(u1 + (1.f - u1) * expf(-2.f * kappa)) / kappa;
     c = fminf(1.f, fmaxf(-1.f, c));
