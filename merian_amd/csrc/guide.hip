@@ -4,6 +4,8 @@
 // over the batch, Adam). Layout conventions: mrt_guide.h.
 #include "mrt_guide.h"
 
+#include <rocprim/device/device_radix_sort.hpp>
+
 namespace mrt {
 
 namespace {
@@ -479,10 +481,16 @@ __global__ __launch_bounds__(GB) void k_guide_fwd_bwd(GuideNet net, const float4
                 grid_corners(net, l, pn, idx, w);
 #pragma unroll
                 for (int c = 0; c < 8; c++) {
-                    float* gg = B.grid_grad + (size_t(l) * net.table + idx[c]) * G_FEATS;
+                    unsigned long long* gg = B.grid_grad + (size_t(l) * net.table + idx[c]) * G_FEATS;
 #pragma unroll
-                    for (int k = 0; k < G_FEATS; k++)
-                        if (acc[t][h][k] != 0.f) unsafeAtomicAdd(gg + k, w[c] * acc[t][h][k]);
+                    for (int k = 0; k < G_FEATS; k++) {
+                        const float v = w[c] * acc[t][h][k];
+                        // non-finite contributions are dropped (they would poison Adam's moments)
+                        if (v != 0.f && fabsf(v) < __builtin_inff()) {
+                            const float cv = fminf(fmaxf(v, -G_GRID_FX_CLAMP), G_GRID_FX_CLAMP);
+                            atomicAdd(gg + k, (unsigned long long)__float2ll_rn(cv * G_GRID_FX));
+                        }
+                    }
                 }
             }
         }
@@ -582,14 +590,14 @@ __global__ void k_guide_adam_w(GuideParams p, GuideTrainBuffers B, float lr, flo
     reinterpret_cast<_Float16*>(p.frag_bwd + bwd)[bs] = _Float16(w);
 }
 
-__global__ void k_guide_adam_grid(GuideParams p, float* __restrict__ grad, float lr, float scale, float bc1,
-                                  float bc2, int update) {
+__global__ void k_guide_adam_grid(GuideParams p, unsigned long long* __restrict__ grad, float lr, float scale,
+                                  float bc1, float bc2, int update) {
     const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
     for (uint64_t e = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < p.grid_count; e += stride) {
         float w = p.g[e];
         if (update) {
-            const float gr = grad[e] * scale;
-            grad[e] = 0.f;
+            const float gr = float((long long)grad[e]) * (1.f / G_GRID_FX) * scale;
+            grad[e] = 0ull;
             float m = p.g_m[e], v = p.g_v[e];
             if (gr != 0.f || m != 0.f) {
                 m = 0.9f * m + 0.1f * gr;
@@ -601,6 +609,31 @@ __global__ void k_guide_adam_grid(GuideParams p, float* __restrict__ grad, float
             }
         }
         p.grid[e] = _Float16(w);
+    }
+}
+
+// Sort keys (the third float4's w of each record) and their record indices.
+__global__ void k_guide_record_keys(const float4* __restrict__ in, uint32_t n, uint32_t* __restrict__ keys,
+                                    uint32_t* __restrict__ vals) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride) {
+        keys[r] = __float_as_uint(in[3 * size_t(r) + 2].w);
+        vals[r] = r;
+    }
+}
+
+// Record k of the sorted order -> slot (k * 2654435761) mod n (2654435761 is prime and n < 2^32,
+// so the map is a permutation); the key is cleared.
+__global__ void k_guide_record_place(const float4* __restrict__ in, const uint32_t* __restrict__ order, uint32_t n,
+                                     float4* __restrict__ out) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+        const float4* a = in + 3 * size_t(order[k]);
+        float4* b = out + 3 * ((uint64_t(k) * 2654435761ull) % n);
+        const float4 c = a[2];
+        b[0] = a[0];
+        b[1] = a[1];
+        b[2] = make_float4(c.x, c.y, c.z, 0.f);
     }
 }
 
@@ -664,6 +697,29 @@ hipError_t launch_guide_pack(const GuideParams& p, hipStream_t st) {
                        0.f, 0.f, 1.f, 1.f, 0);
     hipLaunchKernelGGL(k_guide_adam_grid, dim3(blocks_for(p.grid_count, 256, 8192)), dim3(256), 0, st, p, nullptr,
                        0.f, 0.f, 1.f, 1.f, 0);
+    return hipGetLastError();
+}
+
+size_t launch_guide_order_temp_bytes(uint32_t cap) {
+    size_t bytes = 0;
+    if (rocprim::radix_sort_pairs(nullptr, bytes, static_cast<uint32_t*>(nullptr), static_cast<uint32_t*>(nullptr),
+                                  static_cast<uint32_t*>(nullptr), static_cast<uint32_t*>(nullptr), cap, 0u, 32u) !=
+        hipSuccess)
+        return 0;
+    return bytes;
+}
+
+hipError_t launch_guide_order_records(const float4* in, uint32_t n, uint32_t* keys, uint32_t* vals, void* temp,
+                                      size_t temp_bytes, float4* out, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const int g = blocks_for(n, 256, 4096);
+    hipLaunchKernelGGL(k_guide_record_keys, dim3(g), dim3(256), 0, st, in, n, keys, vals);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    size_t bytes = temp_bytes;
+    e = rocprim::radix_sort_pairs(temp, bytes, keys, keys + n, vals, vals + n, n, 0u, 32u, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_guide_record_place, dim3(g), dim3(256), 0, st, in, vals + n, n, out);
     return hipGetLastError();
 }
 

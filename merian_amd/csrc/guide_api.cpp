@@ -28,6 +28,11 @@ struct mrt_guide {
     uint64_t lobes_cap = 0;   // queue entries
     float4* grec = nullptr;
     float4* gtrain = nullptr;
+    float4* gtrain_ordered = nullptr; // the frame's records in their fixed training order
+    uint32_t* order_keys = nullptr;   // 2 x max_records (sort input, output)
+    uint32_t* order_vals = nullptr;
+    void* order_temp = nullptr;
+    size_t order_temp_bytes = 0;
     uint32_t* grec_count = nullptr;
     uint32_t* h_count = nullptr; // pinned
     uint64_t steps = 0;       // Adam steps taken
@@ -199,6 +204,11 @@ mrt_status guide_train_frame(mrt_ctx* c, mrt_guide* g, hipStream_t st) {
     if (e != hipSuccess) return hip_fail(c, e, "guide record count");
     const uint32_t n = std::min(*g->h_count, g->cfg.max_records);
     if (n == 0) return MRT_OK;
+    // records arrive in the order blocks finish; sorted by (path, bounce) and permuted, the
+    // training order, hence the trained guide, is the same on every run
+    e = launch_guide_order_records(g->gtrain, n, g->order_keys, g->order_vals, g->order_temp, g->order_temp_bytes,
+                                   g->gtrain_ordered, st);
+    if (e != hipSuccess) return hip_fail(c, e, "guide record order");
     // the steps take disjoint slices of the (permuted) records while the frame has enough of
     // them: no record drives more than one step of a frame
     const uint32_t steps = g->cfg.train_steps;
@@ -208,7 +218,7 @@ mrt_status guide_train_frame(mrt_ctx* c, mrt_guide* g, hipStream_t st) {
     for (uint32_t k = 0; k < steps; k++) {
         const uint32_t off = uint32_t((uint64_t(k) * batch) % n);
         const uint32_t cnt = std::min(batch, n - off);
-        s = train_step(c, g, g->gtrain + 3 * size_t(off), cnt, st);
+        s = train_step(c, g, g->gtrain_ordered + 3 * size_t(off), cnt, st);
         if (s != MRT_OK) return s;
     }
     return MRT_OK;
@@ -259,11 +269,17 @@ MRT_API mrt_status mrt_guide_create(mrt_ctx* c, mrt_scene* s, const mrt_guide_co
     g->P.g_v = galloc<float>(c, g, ng, &st);
     g->P.grid = galloc<_Float16>(c, g, ng, &st);
     g->P.grid_count = ng;
-    g->T.grid_grad = galloc<float>(c, g, ng, &st);
+    g->T.grid_grad = galloc<unsigned long long>(c, g, ng, &st);
     g->T.scalars = galloc<float>(c, g, 4, &st);
     g->dW = galloc<float>(c, g, nw, &st);
     g->grec = galloc<float4>(c, g, size_t(5) * cfg.max_records, &st);
     g->gtrain = galloc<float4>(c, g, size_t(3) * cfg.max_records, &st);
+    g->gtrain_ordered = galloc<float4>(c, g, size_t(3) * cfg.max_records, &st);
+    g->order_keys = galloc<uint32_t>(c, g, size_t(2) * cfg.max_records, &st);
+    g->order_vals = galloc<uint32_t>(c, g, size_t(2) * cfg.max_records, &st);
+    g->order_temp_bytes = launch_guide_order_temp_bytes(cfg.max_records);
+    if (st == MRT_OK && g->order_temp_bytes == 0) st = fail(c, MRT_ERR_DEVICE, "guide record sort: no temporary size");
+    g->order_temp = galloc<char>(c, g, std::max<size_t>(g->order_temp_bytes, 1), &st);
     g->grec_count = galloc<uint32_t>(c, g, 1, &st);
     if (st == MRT_OK && hipHostMalloc(reinterpret_cast<void**>(&g->h_count), 4, hipHostMallocDefault) != hipSuccess) {
         g->h_count = nullptr;
@@ -344,7 +360,7 @@ MRT_API mrt_status mrt_guide_set_params(mrt_ctx* c, mrt_guide* g, const float* w
     if (e == hipSuccess) e = hipMemsetAsync(g->P.w_v, 0, nw * 4, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(g->P.g_m, 0, g->P.grid_count * 4, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(g->P.g_v, 0, g->P.grid_count * 4, c->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(g->T.grid_grad, 0, g->P.grid_count * 4, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(g->T.grid_grad, 0, g->P.grid_count * sizeof(unsigned long long), c->stream);
     if (e == hipSuccess) e = launch_guide_pack(g->P, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     g->steps = 0;
@@ -392,19 +408,21 @@ MRT_API mrt_status mrt_guide_gradient(mrt_ctx* c, mrt_guide* g, const float* rec
     if (st == MRT_OK) st = upload_records(c, g, records, n);
     if (st != MRT_OK) return st;
     GuideTrainBuffers tb = g->T;
-    hipError_t e = hipMemsetAsync(g->T.grid_grad, 0, g->P.grid_count * 4, c->stream);
+    const size_t gbytes = g->P.grid_count * sizeof(unsigned long long);
+    hipError_t e = hipMemsetAsync(g->T.grid_grad, 0, gbytes, c->stream);
     if (e == hipSuccess) e = launch_guide_gradient(g->net, g->host_records, n, tb, c->stream);
     if (e == hipSuccess) e = launch_guide_reduce_wgrad(tb, g->dW, c->stream);
     float sc[2] = {0.f, 0.f};
+    std::vector<unsigned long long> fx(d_grid ? g->P.grid_count : 0);
     if (e == hipSuccess && d_weights)
         e = hipMemcpyAsync(d_weights, g->dW, size_t(G_LAYERS) * G_WIDTH * G_WIDTH * 4, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess && d_grid)
-        e = hipMemcpyAsync(d_grid, g->T.grid_grad, g->P.grid_count * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && d_grid) e = hipMemcpyAsync(fx.data(), g->T.grid_grad, gbytes, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(sc, g->T.scalars, 8, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     // leave the gradient accumulator cleared for training
-    if (e == hipSuccess) e = hipMemsetAsync(g->T.grid_grad, 0, g->P.grid_count * 4, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(g->T.grid_grad, 0, gbytes, c->stream);
     if (e != hipSuccess) return hip_fail(c, e, "mrt_guide_gradient");
+    for (size_t k = 0; k < fx.size(); k++) d_grid[k] = float(double((long long)fx[k]) / double(G_GRID_FX));
     if (loss) *loss = sc[1] / float(n);
     return MRT_OK;
 }

@@ -818,7 +818,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(MOD
                     float4* g = P.grec + 5 * size_t(r);
                     g[0] = make_float4(rec_pos.x, rec_pos.y, rec_pos.z, __uint_as_float(pid));
                     g[1] = make_float4(-rd_keep.x, -rd_keep.y, -rd_keep.z, rec_pdf);
-                    g[2] = make_float4(next_d.x, next_d.y, next_d.z, 0.f);
+                    g[2] = make_float4(next_d.x, next_d.y, next_d.z, __uint_as_float(uint32_t(bounce)));
                     g[3] = make_float4(rec_rad.x, rec_rad.y, rec_rad.z, 0.f);
                     g[4] = make_float4(thr_keep.x, thr_keep.y, thr_keep.z, 0.f);
                 }
@@ -1154,8 +1154,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(3))
 // Training records of the guide from the first chunk's sample-0 paths, once their radiance
 // is final: incident radiance along the sampled direction = (final radiance - radiance before
 // the vertex's continuation) / throughput after it; weight = its luminance / sampling pdf.
-// Record r is stored at (r * 2654435761) mod n, a permutation (the multiplier is a prime above
-// any record count), so consecutive training slices mix pixels and bounces.
+// Record r keeps its arrival slot; training reorders the records by key first.
 __global__ void k_guide_targets(PathBuffers P) {
     const uint32_t n = min(*P.grec_count, P.grec_cap);
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -1167,10 +1166,12 @@ __global__ void k_guide_targets(PathBuffers P) {
                    th.z > 0.f ? fmaxf(rf.z - rb.z, 0.f) / th.z : 0.f};
         float w = b.w > 0.f ? yuv_luminance(L) / b.w : 0.f;
         if (!(w >= 0.f && w < 3.0e38f)) w = 0.f;
-        float4* t = P.gtrain + 3 * ((uint64_t(r) * 2654435761ull) % n);
+        // in arrival order; the key (path, bounce) is unique per record (paths below 2^26, at most
+        // 64 vertices) and fixes the training order (launch_guide_order_records)
+        float4* t = P.gtrain + 3 * size_t(r);
         t[0] = make_float4(a.x, a.y, a.z, w);
         t[1] = make_float4(b.x, b.y, b.z, 0.f);
-        t[2] = make_float4(c.x, c.y, c.z, 0.f);
+        t[2] = make_float4(c.x, c.y, c.z, __uint_as_float(__float_as_uint(a.w) * 64u + __float_as_uint(c.w)));
     }
 }
 

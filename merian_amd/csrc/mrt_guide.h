@@ -119,11 +119,23 @@ struct GuideTrainBuffers {
     _Float16* H;       // [layer][cap][64] inputs of each layer
     _Float16* D;       // [layer][cap][64] gradients at each layer's outputs
     float* partials;   // [layer][kblocks][64][64]
-    float* grid_grad;  // [level][table][G_FEATS]
+    // [level][table][G_FEATS] grid gradient as 64-bit fixed point (G_GRID_FX units): integer
+    // atomics sum it exactly, so the sum does not depend on the order the records arrive in
+    unsigned long long* grid_grad;
     float* scalars;    // [0] weight sum, [1] loss sum
     uint32_t cap, kblocks;
 };
 constexpr uint32_t G_WGRAD_RECORDS = 2048; // records per weight-gradient block
+constexpr float G_GRID_FX = 16777216.f;      // grid-gradient fixed point: 2^24 units per 1
+constexpr float G_GRID_FX_CLAMP = 4096.f;    // one corner's contribution, as the output gradients
+
+// A frame's training records in a fixed order: the records (3 float4 each; the third's w holds
+// the unique key pid * 64 + bounce) sorted by key, then record k of the sorted order written to
+// slot (k * 2654435761) mod n of `out`, so consecutive training steps mix pixels and bounces.
+// `temp` holds temp_bytes (launch_guide_order_temp_bytes(cap)); keys/vals hold 2 * cap entries.
+size_t launch_guide_order_temp_bytes(uint32_t cap);
+hipError_t launch_guide_order_records(const float4* in, uint32_t n, uint32_t* keys, uint32_t* vals, void* temp,
+                                      size_t temp_bytes, float4* out, hipStream_t st);
 
 // Records: 3 float4 each, (pos, weight), (wo, 0), (dir, 0). Computes the weight sum, the
 // forward and backward passes, the grid gradient (added to grid_grad) and the weight-gradient

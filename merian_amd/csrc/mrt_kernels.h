@@ -110,8 +110,8 @@ struct PathBuffers {
     uint32_t* sig_accum;
     // MRT_MODE_GUIDED only
     uint4* lobes;      // packed guide lobes per queue entry (4 x uint4)
-    float4* grec;      // raw training records: (pos, path), (wo, pdf), (dir, -), (radiance before, -), (throughput after, -)
-    float4* gtrain;    // training records: (pos, weight), (wo, -), (dir, -)
+    float4* grec;      // raw training records: (pos, path), (wo, pdf), (dir, bounce), (radiance before, -), (throughput after, -)
+    float4* gtrain;    // training records in arrival order: (pos, weight), (wo, -), (dir, key = path * 64 + bounce)
     uint32_t* grec_count;
     uint32_t grec_cap;
     uint64_t capacity; // paths

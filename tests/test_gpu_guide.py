@@ -7,7 +7,8 @@
   the BSDF-only estimator), every vertex guided and only vertex 0 guided, under a sky whose sun is
   100x its dimmest texel (a 5e4x sun makes the guided estimator heavy-tailed, which a block
   z-test cannot tell from bias);
-- at equal device time the trained guide beats the plain pt in relMSE under the 5e4x sun.
+- at equal device time the trained guide beats the plain pt in relMSE under the 5e4x sun;
+- online training is deterministic: two guides from one seed end bit-identical.
 """
 import math
 
@@ -292,3 +293,28 @@ def test_guided_bounces_limit_stays_unbiased(ctx, soft_sky):
         assert abs(z.mean()) < 0.6, z.mean()
     finally:
         g.close()
+
+
+def test_online_training_is_deterministic(ctx, sky):
+    # Two guides from the same seed trained online over the same frames end bit-identical, and
+    # so do their frames: the records are put in a fixed order before each frame's steps (they
+    # arrive in the order blocks finish) and the grid gradients are summed in fixed point
+    # (integer atomics are exact in any order).
+    sc, ds = sky
+    spec = scenes.CameraSpec(sc.camera.eye, sc.camera.target, sc.camera.up, sc.camera.vfov, W_, H_)
+    runs = []
+    for run in range(2):
+        print(f"deterministic training: run {run}", flush=True)
+        g = Guide(ctx, ds, guide_config(seed=11, train_steps=4, train_batch=4096))
+        try:
+            img = _frames(ctx, ds, spec, 0, pt_params(spp=8, max_path_length=6, mode=abi.MODE_GUIDED,
+                                                      record_signature=False), 4, guide=g, train=True, base=7000)
+            W, G = g.params()
+        finally:
+            g.close()
+        runs.append((img, W, G))
+    (i0, w0, g0), (i1, w1, g1) = runs
+    assert np.abs(w0).max() > 0 and np.isfinite(i0).all()
+    assert w0.tobytes() == w1.tobytes()
+    assert g0.tobytes() == g1.tobytes()
+    assert i0.tobytes() == i1.tobytes()
