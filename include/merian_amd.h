@@ -536,7 +536,9 @@ MRT_API mrt_status mrt_guide_gradient(mrt_ctx* ctx, mrt_guide* guide, const floa
 MRT_API mrt_status mrt_guide_train(mrt_ctx* ctx, mrt_guide* guide, const float* records,
                                    uint32_t n, uint32_t steps, float* loss);
 /* One guided frame (MRT_MODE_GUIDED): as mrt_pt_render_dev, sampling the guide at eligible
- * vertices; with train != 0 the frame's records then train the guide (train_steps steps). */
+ * vertices; with train != 0 the frame's records then train the guide (train_steps steps), in
+ * an order fixed by (pixel, vertex), so training is deterministic. At most 2^26 pixels
+ * (MRT_ERR_UNSUPPORTED beyond). */
 MRT_API mrt_status mrt_guided_render_dev(mrt_ctx* ctx, mrt_scene* scene, mrt_guide* guide,
                                          const mrt_pt_params* params, const mrt_camera* cam,
                                          uint32_t frame, uint32_t width, uint32_t height,

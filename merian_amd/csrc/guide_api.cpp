@@ -459,6 +459,9 @@ MRT_API mrt_status mrt_guided_render_dev(mrt_ctx* c, mrt_scene* s, mrt_guide* g,
     if (prm->mode != MRT_MODE_GUIDED) return fail(c, MRT_ERR_INVALID, "mrt_guided_render_dev: mode must be GUIDED");
     if (prm->spp < 0 || prm->max_path_length < 1 || prm->max_path_length > 64)
         return fail(c, MRT_ERR_INVALID, "samples per pixel < 0 or max path length outside [1, 64]");
+    // training records are keyed pixel * 64 + vertex in 32 bits
+    if (uint64_t(w) * h > (uint64_t(1) << 26))
+        return fail(c, MRT_ERR_UNSUPPORTED, "mrt_guided_render_dev: more than 2^26 pixels");
     // without training the records are still written but no step runs
     const uint32_t steps = g->cfg.train_steps;
     if (!train) g->cfg.train_steps = 0;

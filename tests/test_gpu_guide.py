@@ -318,3 +318,19 @@ def test_online_training_is_deterministic(ctx, sky):
     assert w0.tobytes() == w1.tobytes()
     assert g0.tobytes() == g1.tobytes()
     assert i0.tobytes() == i1.tobytes()
+
+
+def test_guided_frame_size_limit(ctx, box_scene):
+    # training records are keyed pixel * 64 + vertex in 32 bits: more than 2^26 pixels is refused
+    # before any buffer is touched
+    from merian_amd.runtime import MrtError
+    sc, ds = box_scene
+    g = Guide(ctx, ds, guide_config(seed=1, **SMALL))
+    try:
+        c = sc.camera
+        cam = ctx.camera(scenes.CameraSpec(c.eye, c.target, c.up, c.vfov, 8193, 8192), 0)
+        with pytest.raises(MrtError) as e:
+            g.render_dev(ds, pt_params(spp=1, max_path_length=2, mode=abi.MODE_GUIDED), cam, 0, 8193, 8192, 0, 0)
+        assert e.value.status == abi.MRT_ERR_UNSUPPORTED
+    finally:
+        g.close()
