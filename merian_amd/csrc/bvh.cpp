@@ -397,6 +397,8 @@ struct Collapser8 {
         node.child_base = uint32_t(out.nodes.size());
         out.nodes.resize(out.nodes.size() + size_t(n_inner));
         node.tri_base = uint32_t(out.tris.size());
+        std::memset(node.code, BVH8_EMPTY, 8);
+        uint32_t rank = 0;
         for (int k = 0; k < nk; k++) {
             for (int a = 0; a < 3; a++) {
                 const float sc = std::ldexp(1.0f, g.e[a]);
@@ -406,8 +408,10 @@ struct Collapser8 {
             const Node2& c = n2[kids[k]];
             if (c.left >= 0) {
                 node.inner_mask |= uint8_t(1u << k);
+                node.code[k] = uint8_t(BVH8_INNER + rank++);
             } else {
-                node.leaf_counts |= (c.count & 15u) << (4 * k);
+                const uint32_t off = uint32_t(out.tris.size()) - node.tri_base;
+                node.code[k] = uint8_t(off << 3 | (c.count - 1u)); // count in [1, BVH8_MAX_LEAF]
                 for (uint32_t t = 0; t < c.count; t++) {
                     out.tris.emplace_back();
                     set_record(out.tris.back(), idx[c.first + t], v0, e1, e2, flags);
@@ -499,12 +503,13 @@ BVH8 build_bvh8(const float* v0, const float* e1, const float* e2, const uint32_
     const float mag = pr.mag;
     const uint32_t m = uint32_t(pr.order.size());
     Builder bld(pr.prims, std::move(pr.order), threads);
-    bld.max_leaf = uint32_t(std::min(std::max(max_leaf, 1), std::min(MAX_LEAF, 15)));
+    bld.max_leaf = uint32_t(std::min(std::max(max_leaf, 1), std::min(MAX_LEAF, BVH8_MAX_LEAF)));
     bld.traversal_cost = traversal_cost > 0.f ? traversal_cost : 1.0f;
     BVH8 out;
     if (m == 0) {
         BVH8Node node;
         std::memset(&node, 0, sizeof(node));
+        std::memset(node.code, BVH8_EMPTY, 8);
         std::memset(node.qlox, 255, 8);
         std::memset(node.qloy, 255, 8);
         std::memset(node.qloz, 255, 8);

@@ -48,17 +48,25 @@ struct BVH4 {
 // five 16-byte loads): child box = fma(q, 2^e, origin) per plane, every decoded box containing
 // the padded child box exactly as the BVH4 stores it. Inner children are stored consecutively
 // from child_base in slot order; the triangles of the leaf children consecutively from tri_base.
+// One code byte per slot makes the slot's traversal entry a select and an add:
+//   leaf:  (first triangle - tri_base) << 3 | (count - 1)   (<= BVH8_LEAF_CODE_MAX)
+//   empty: BVH8_EMPTY
+//   inner: BVH8_INNER + rank among the inner children
+// so entry = (inner ? child_base - BVH8_INNER : LEAF | tri_base << 3) + code.
+constexpr uint8_t BVH8_EMPTY = 0xF0, BVH8_INNER = 0xF8;
+constexpr int BVH8_MAX_LEAF = 4; // 7 leaves of 4 before a slot: offset 28, code 227
+constexpr uint32_t BVH8_LEAF_CODE_MAX = 28u << 3 | 3u;
 struct alignas(16) BVH8Node {
     float px, py, pz;       // grid origin
     int8_t ex, ey, ez;      // grid step 2^e per axis
     uint8_t inner_mask;     // bit k: slot k is an inner child
     uint32_t child_base;    // node index of the first inner child
     uint32_t tri_base;      // first triangle record of the leaf children
-    uint32_t leaf_counts;   // 4 bits per slot: triangles of a leaf child (0: inner or empty)
-    uint32_t _pad;
+    uint8_t code[8];        // per slot, as above
     uint8_t qlox[8], qhix[8], qloy[8], qhiy[8], qloz[8], qhiz[8]; // empty slot: lo 255, hi 0
 };
 static_assert(sizeof(BVH8Node) == 80, "BVH8Node must be 80 bytes");
+static_assert(BVH8_LEAF_CODE_MAX < BVH8_EMPTY, "leaf codes below the sentinels");
 
 struct BVH8 {
     std::vector<BVH8Node> nodes; // nodes[0] is the root

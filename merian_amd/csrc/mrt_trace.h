@@ -244,7 +244,6 @@ __device__ __forceinline__ void node_step(const DeviceBVH8 bvh, RayState& r, con
     const uint4* np = reinterpret_cast<const uint4*>(bvh.nodes + r.cur);
     const uint4 h0 = np[0], h1 = np[1], qx = np[2], qy = np[3], qz = np[4];
     const int ex = int(int8_t(h0.w & 0xFFu)), ey = int(int8_t((h0.w >> 8) & 0xFFu)), ez = int(int8_t((h0.w >> 16) & 0xFFu));
-    const uint32_t imask = h0.w >> 24;
     const float ax = __uint_as_float(uint32_t(ex + 127) << 23) * r.ix;
     const float ay = __uint_as_float(uint32_t(ey + 127) << 23) * r.iy;
     const float az = __uint_as_float(uint32_t(ez + 127) << 23) * r.iz;
@@ -260,7 +259,9 @@ __device__ __forceinline__ void node_step(const DeviceBVH8 bvh, RayState& r, con
     const uint32_t nz0 = nz ? qz.z : qz.x, nz1 = nz ? qz.w : qz.y, fz0 = nz ? qz.x : qz.z, fz1 = nz ? qz.y : qz.w;
     constexpr uint32_t MISS = 0xFFFFFFFFu;
     uint32_t key[8];
-    uint32_t inner_next = h1.x, tri_next = h1.y;
+    // a slot's entry = its node's base + its code byte (bvh.h): inner children from child_base,
+    // leaves from tri_base with (offset << 3 | count - 1) in the code
+    const uint32_t inner_base = h1.x - uint32_t(BVH8_INNER), leaf_base = LEAF_FLAG | (h1.y << 3);
 #pragma unroll
     for (int c = 0; c < 8; c++) {
         const uint32_t sh = 8u * uint32_t(c & 3);
@@ -272,12 +273,9 @@ __device__ __forceinline__ void node_step(const DeviceBVH8 bvh, RayState& r, con
         const float FZ = __builtin_fmaf(float(((c < 4 ? fz0 : fz1) >> sh) & 0xFFu), az, bz);
         const float tn = fmaxf(fmaxf(NX, NY), fmaxf(NZ, r.t0));
         const float tf = fminf(fminf(FX, FY), fminf(FZ, tlim));
-        const uint32_t cnt = (h1.z >> (4 * c)) & 15u;
-        const bool inner = (imask >> c) & 1u;
-        key[c] = (tn <= tf && (inner || cnt)) ? ((__float_as_uint(tn) & ~7u) | uint32_t(c)) : MISS;
-        stk.ents[c * TRACE_BLOCK] = inner ? inner_next : (LEAF_FLAG | (tri_next << 3) | ((cnt - 1u) & 7u));
-        inner_next += inner ? 1u : 0u;
-        tri_next += cnt;
+        const uint32_t code = ((c < 4 ? h1.z : h1.w) >> sh) & 0xFFu;
+        key[c] = (tn <= tf && code != BVH8_EMPTY) ? ((__float_as_uint(tn) & ~7u) | uint32_t(c)) : MISS;
+        stk.ents[c * TRACE_BLOCK] = (code >= BVH8_INNER ? inner_base : leaf_base) + code;
     }
 #define MRT_CS(a, b)                                                                                           \
     {                                                                                                          \

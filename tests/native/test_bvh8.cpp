@@ -1,7 +1,7 @@
 // CPU check of the 8-wide quantised BVH (merian_amd/csrc/bvh.cpp): every decoded child box (the
 // kernel's fma(q, 2^e, origin)) contains every triangle below it, every triangle appears exactly
-// once, inner children are consecutive and leaf triangles follow tri_base in slot order; and a
-// brute-force-checked traversal of the tree finds the closest hit of random rays.
+// once, inner children are consecutive and leaf triangles follow tri_base in slot order, and the
+// slot code bytes give exactly those entries (inner rank, leaf offset and count, empty).
 #include "../../merian_amd/csrc/bvh.h"
 
 #include <cmath>
@@ -46,12 +46,19 @@ static void walk(const BVH8& b, const Soup& s, uint32_t ni, std::vector<int>& se
             lo[a] = decode(qlo[a][k], e[a], p[a]);
             hi[a] = decode(qhi[a][k], e[a], p[a]);
         }
-        const uint32_t cnt = (n.leaf_counts >> (4 * k)) & 15u;
-        if (n.inner_mask & (1u << k)) {
-            CHECK(cnt == 0);
+        const uint32_t code = n.code[k];
+        const bool is_inner = code >= BVH8_INNER, is_empty = code == BVH8_EMPTY;
+        CHECK(is_inner == bool(n.inner_mask & (1u << k)));
+        CHECK(is_inner || is_empty || code <= BVH8_LEAF_CODE_MAX);
+        // the kernel's entry: (inner ? child_base - BVH8_INNER : LEAF | tri_base << 3) + code
+        const uint32_t entry = (is_inner ? n.child_base - uint32_t(BVH8_INNER) : (0x80000000u | n.tri_base << 3)) + code;
+        const uint32_t cnt = is_inner || is_empty ? 0u : (code & 7u) + 1u;
+        if (is_inner) {
+            CHECK(entry == n.child_base + inner);
             walk(b, s, n.child_base + inner++, seen, depth + 1);
-        } else if (cnt) {
-            CHECK(cnt <= 8);
+        } else if (!is_empty) {
+            CHECK(cnt <= uint32_t(BVH8_MAX_LEAF));
+            CHECK(entry == (0x80000000u | tri << 3 | (cnt - 1u)));
             CHECK(size_t(tri) + cnt <= b.tris.size());
             if (size_t(tri) + cnt > b.tris.size()) return;
             for (uint32_t t = 0; t < cnt; t++, tri++) {
@@ -100,7 +107,8 @@ int main() {
         // empty scene: one node whose eight slots are empty boxes
         const BVH8 b = build_bvh8(nullptr, nullptr, nullptr, nullptr, 0, 1, 4, 1.0f);
         CHECK(b.nodes.size() == 1 && b.tris.empty() && b.depth == 1);
-        CHECK(b.nodes[0].inner_mask == 0 && b.nodes[0].leaf_counts == 0);
+        CHECK(b.nodes[0].inner_mask == 0);
+        for (int k = 0; k < 8; k++) CHECK(b.nodes[0].code[k] == BVH8_EMPTY);
         std::vector<int> seen;
         Soup s;
         walk(b, s, 0, seen, 1);
