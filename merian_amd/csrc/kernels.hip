@@ -184,10 +184,9 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DeviceScene S, Src src, S
                                                        uint32_t* __restrict__ work, uint32_t mask, TraceScratch ts,
                                                        unsigned long long* counters) {
     __shared__ uint32_t stack[TRACE_LDS_STACK * TRACE_BLOCK];
-    __shared__ uint32_t ents[(W == 8 ? 8 : 1) * TRACE_BLOCK]; // BVH8 node step: child entries by slot
     const uint32_t n = count_ptr ? *count_ptr : n_fixed;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    const LaneStack stk{stack + threadIdx.x, ts.ovf + gtid, gridDim.x * blockDim.x, ents + threadIdx.x};
+    const LaneStack stk{stack + threadIdx.x, ts.ovf + gtid, gridDim.x * blockDim.x};
     TraversalCount cnt;
     if constexpr (W == 8) {
         trace_persistent<TRACE_LDS_STACK, COUNT, Src, Sink, ANY>(DeviceBVH8{S.nodes8, S.tris}, src, sink, n, work,

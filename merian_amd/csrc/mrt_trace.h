@@ -75,7 +75,6 @@ struct LaneStack {
     uint32_t* lds; // this lane's column
     uint32_t* ovf;
     uint32_t ovf_stride;
-    uint32_t* ents; // BVH8: this lane's column of the node step's 8 child entries (LDS)
 };
 template <int SL> __device__ __forceinline__ void stack_push(const LaneStack& s, int& sp, uint32_t e) {
     if (sp < SL) {
@@ -103,6 +102,9 @@ struct RayState {
     float tlim;   // far limit of the box tests: min(tmax, best.t) widened by 2 ulp
     uint32_t neg; // bit k: direction component k is negative
     uint32_t pn, pf; // BVH4: byte offsets of the near / far plane row of axis k in byte k
+    uint32_t psel[3]; // BVH8: v_perm selector per axis picking the near plane word of a node
+                      // (lo for a positive direction, hi for a negative one); swapped operands
+                      // pick the far word
     HitRec best;
     int sp;
     uint32_t cur;  // next entry to process; TRAVERSAL_DONE when the stack is exhausted
@@ -134,6 +136,9 @@ __device__ __forceinline__ void ray_begin(RayState& r, f3 o, f3 d, float tmin, f
     const uint32_t nx = r.neg & 1u, ny = (r.neg >> 1) & 1u, nz = (r.neg >> 2) & 1u;
     r.pn = (nx << 4) | ((32u + (ny << 4)) << 8) | ((64u + (nz << 4)) << 16);
     r.pf = ((16u - (nx << 4))) | ((48u - (ny << 4)) << 8) | ((80u - (nz << 4)) << 16);
+    r.psel[0] = nx ? 0x07060504u : 0x03020100u;
+    r.psel[1] = ny ? 0x07060504u : 0x03020100u;
+    r.psel[2] = nz ? 0x07060504u : 0x03020100u;
     r.sp = 0;
     r.cur = 0u; // root node (always an inner node)
     r.leaf = TRAVERSAL_DONE;
@@ -235,7 +240,7 @@ struct DeviceBVH8 {
 // power-of-two scale) and b = fma(p, ix, -ox) once per node and axis, one FMA per plane after
 // the byte conversion. The sort key of a hit child is its entry distance's bit pattern with the
 // child slot in the low three bits (misses: all ones), so a 19-comparator network of integer
-// min/max sorts keys alone; the child entries wait in an LDS column and are read back by slot.
+// min/max sorts keys alone; a pushed child's entry is rebuilt from its slot's code byte.
 // Only the visiting order depends on the keys; which children are visited does not.
 template <int SL, bool COUNT>
 __device__ __forceinline__ void node_step(const DeviceBVH8 bvh, RayState& r, const LaneStack& stk,
@@ -244,19 +249,20 @@ __device__ __forceinline__ void node_step(const DeviceBVH8 bvh, RayState& r, con
     const uint4* np = reinterpret_cast<const uint4*>(bvh.nodes + r.cur);
     const uint4 h0 = np[0], h1 = np[1], qx = np[2], qy = np[3], qz = np[4];
     const int ex = int(int8_t(h0.w & 0xFFu)), ey = int(int8_t((h0.w >> 8) & 0xFFu)), ez = int(int8_t((h0.w >> 16) & 0xFFu));
-    const float ax = __uint_as_float(uint32_t(ex + 127) << 23) * r.ix;
-    const float ay = __uint_as_float(uint32_t(ey + 127) << 23) * r.iy;
-    const float az = __uint_as_float(uint32_t(ez + 127) << 23) * r.iz;
+    const float ax = ldexpf(r.ix, ex), ay = ldexpf(r.iy, ey), az = ldexpf(r.iz, ez); // 2^e ix, exact
     const float bx = __builtin_fmaf(__uint_as_float(h0.x), r.ix, -r.ox);
     const float by = __builtin_fmaf(__uint_as_float(h0.y), r.iy, -r.oy);
     const float bz = __builtin_fmaf(__uint_as_float(h0.z), r.iz, -r.oz);
     const float tlim = r.tlim;
-    const bool nx = r.neg & 1u, ny = r.neg & 2u, nz = r.neg & 4u;
-    // near / far plane bytes per axis, selected by the ray's octant once per node (children 0-3
-    // in the low word, 4-7 in the high word)
-    const uint32_t nx0 = nx ? qx.z : qx.x, nx1 = nx ? qx.w : qx.y, fx0 = nx ? qx.x : qx.z, fx1 = nx ? qx.y : qx.w;
-    const uint32_t ny0 = ny ? qy.z : qy.x, ny1 = ny ? qy.w : qy.y, fy0 = ny ? qy.x : qy.z, fy1 = ny ? qy.y : qy.w;
-    const uint32_t nz0 = nz ? qz.z : qz.x, nz1 = nz ? qz.w : qz.y, fz0 = nz ? qz.x : qz.z, fz1 = nz ? qz.y : qz.w;
+    // near / far plane bytes per axis, picked by the ray's octant with one byte permute per word
+    // (children 0-3 in the low word, 4-7 in the high word)
+    const uint32_t sx = r.psel[0], sy = r.psel[1], sz = r.psel[2];
+    const uint32_t nx0 = __builtin_amdgcn_perm(qx.z, qx.x, sx), nx1 = __builtin_amdgcn_perm(qx.w, qx.y, sx);
+    const uint32_t fx0 = __builtin_amdgcn_perm(qx.x, qx.z, sx), fx1 = __builtin_amdgcn_perm(qx.y, qx.w, sx);
+    const uint32_t ny0 = __builtin_amdgcn_perm(qy.z, qy.x, sy), ny1 = __builtin_amdgcn_perm(qy.w, qy.y, sy);
+    const uint32_t fy0 = __builtin_amdgcn_perm(qy.x, qy.z, sy), fy1 = __builtin_amdgcn_perm(qy.y, qy.w, sy);
+    const uint32_t nz0 = __builtin_amdgcn_perm(qz.z, qz.x, sz), nz1 = __builtin_amdgcn_perm(qz.w, qz.y, sz);
+    const uint32_t fz0 = __builtin_amdgcn_perm(qz.x, qz.z, sz), fz1 = __builtin_amdgcn_perm(qz.y, qz.w, sz);
     constexpr uint32_t MISS = 0xFFFFFFFFu;
     uint32_t key[8];
     // a slot's entry = its node's base + its code byte (bvh.h): inner children from child_base,
@@ -275,8 +281,13 @@ __device__ __forceinline__ void node_step(const DeviceBVH8 bvh, RayState& r, con
         const float tf = fminf(fminf(FX, FY), fminf(FZ, tlim));
         const uint32_t code = ((c < 4 ? h1.z : h1.w) >> sh) & 0xFFu;
         key[c] = (tn <= tf && code != BVH8_EMPTY) ? ((__float_as_uint(tn) & ~7u) | uint32_t(c)) : MISS;
-        stk.ents[c * TRACE_BLOCK] = (code >= BVH8_INNER ? inner_base : leaf_base) + code;
     }
+    // entry of the slot in a sorted key's low bits: its code byte picked from the node's two code
+    // words by a byte permute (selector bytes 0x0C read as zero), then base + code
+    auto entry_of = [&](uint32_t k) {
+        const uint32_t code = __builtin_amdgcn_perm(h1.w, h1.z, (k & 7u) | 0x0C0C0C00u);
+        return (code >= BVH8_INNER ? inner_base : leaf_base) + code;
+    };
 #define MRT_CS(a, b)                                                                                           \
     {                                                                                                          \
         const uint32_t lo_ = min(key[a], key[b]);                                                              \
@@ -290,12 +301,11 @@ __device__ __forceinline__ void node_step(const DeviceBVH8 bvh, RayState& r, con
     MRT_CS(2, 4) MRT_CS(3, 5)
     MRT_CS(1, 2) MRT_CS(3, 4) MRT_CS(5, 6)
 #undef MRT_CS
-    typedef const __attribute__((address_space(3))) uint32_t* lds_ptr;
 #pragma unroll
     for (int c = 7; c >= 1; c--)
-        if (key[c] != MISS) stack_push<SL>(stk, r.sp, ((lds_ptr)stk.ents)[(key[c] & 7u) * TRACE_BLOCK]);
+        if (key[c] != MISS) stack_push<SL>(stk, r.sp, entry_of(key[c]));
     if (key[0] != MISS) {
-        r.cur = ((lds_ptr)stk.ents)[(key[0] & 7u) * TRACE_BLOCK];
+        r.cur = entry_of(key[0]);
     } else {
         r.cur = r.sp > 0 ? stack_pop<SL>(stk, r.sp) : TRAVERSAL_DONE;
     }
