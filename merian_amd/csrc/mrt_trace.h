@@ -120,9 +120,12 @@ __device__ __forceinline__ void ray_begin(RayState& r, f3 o, f3 d, float tmin, f
     const float dx = fabsf(d.x) < 1e-20f ? copysignf(1e-20f, d.x) : d.x;
     const float dy = fabsf(d.y) < 1e-20f ? copysignf(1e-20f, d.y) : d.y;
     const float dz = fabsf(d.z) < 1e-20f ? copysignf(1e-20f, d.z) : d.z;
-    r.ix = 1.0f / dx;
-    r.iy = 1.0f / dy;
-    r.iz = 1.0f / dz;
+    // the hardware reciprocal (1 ulp) instead of a correctly rounded division: 1/d only feeds
+    // the box tests, whose boxes the builder pads by extent/1024 + magnitude/65536, far above
+    // an ulp of the slab distances; the triangle test uses d itself
+    r.ix = __builtin_amdgcn_rcpf(dx);
+    r.iy = __builtin_amdgcn_rcpf(dy);
+    r.iz = __builtin_amdgcn_rcpf(dz);
     r.ox = o.x * r.ix;
     r.oy = o.y * r.iy;
     r.oz = o.z * r.iz;
