@@ -43,6 +43,13 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def trace_variant(tuning: dict) -> int:
+    """Template argument W of the intersect kernel the settings launch (4: BVH4; 8: BVH8 with
+    sorted entries; 9: BVH8 in octant order with grouped stack entries)."""
+    w = int(tuning.get("bvh_width", 4))
+    return 9 if w == 8 and int(tuning.get("bvh8_order", 0)) == 1 else w
+
+
 def roofline(kernel: str, avg_ms: float, launches: int, algorithmic_bytes: float, profile: str,
              match: dict | None = None, extra: dict | None = None) -> dict:
     """Bandwidth roofline of one kernel. `achieved` is the HBM-side traffic per dispatch that a
@@ -176,7 +183,7 @@ def small_leg(ctx, frames: int = 64, retune: bool = False) -> dict:
         with open(SMALL_TUNING) as f:
             _apply(ds, json.load(f)["tuning"])
         source = "cached mrt_tune result (merian_amd/tuning/small_512.json)"
-    tuning = {k: v for k, v in ds.tuning().as_dict().items() if k not in ("ms_per_frame", "_reserved")}
+    tuning = {k: v for k, v in ds.tuning().as_dict().items() if k != "ms_per_frame"}
 
     def frame(f, count=False):
         cam = ctx.camera(spec, f)
@@ -201,7 +208,7 @@ def small_leg(ctx, frames: int = 64, retune: bool = False) -> dict:
     bytes_per_seg = (st_c.node_visits * node_bytes + st_c.triangle_tests * TRI_BYTES) / cont + RAY_IO_BYTES
     cont_timed = segs - frames * W * H
     avg_ms = ms_trace / max(1, launches)
-    roof = roofline(f"k_trace<false, SrcQueue, SinkQueue, false, {int(tuning['bvh_width'])}>", avg_ms, launches,
+    roof = roofline(f"k_trace<false, SrcQueue, SinkQueue, false, {trace_variant(tuning)}>", avg_ms, launches,
                     bytes_per_seg * cont_timed / max(1, launches), SMALL_TRAFFIC,
                     match={"tuning": tuning, "workload": SMALL_WORKLOAD},
                     extra={"bytes_per_segment": round(bytes_per_seg, 1)})
@@ -462,7 +469,7 @@ def serving_leg(ctx, single_frame_mseg: float, n_jobs: int = 400, tune_ms: float
 # (re-run with --retune-rays). No setting changes a hit.
 RAY_QUERY_TUNING = {
     "incoherent": dict(bvh_width=8, bvh_max_leaf=2, bvh_traversal_cost=1.0, refill_lanes=16, leaf_lanes=4,
-                       grid_percent=100),
+                       grid_percent=100, bvh8_order=1),
     "coherent": dict(bvh_width=4, bvh_max_leaf=2, bvh_traversal_cost=1.0, refill_lanes=48, leaf_lanes=8,
                      grid_percent=100),
 }
@@ -532,7 +539,7 @@ def ray_query_leg(ctx, reps: int = 3, retune: bool = False) -> dict:
         nodes, tris = ctx.trace_count_dev(ds, d_r.data_ptr(), n, d_hits.data_ptr())
         ms = _time_rays(ctx, ds, d_r, n, d_hits, stream, reps)
         bytes_per_ray = (nodes * NODE_BYTES[settings["bvh_width"]] + tris * TRI_BYTES) / n + RAY_IO_BYTES
-        roof = roofline(f"k_trace<false, SrcRays, SinkHits, false, {settings['bvh_width']}>", ms, reps,
+        roof = roofline(f"k_trace<false, SrcRays, SinkHits, false, {trace_variant(settings)}>", ms, reps,
                         bytes_per_ray * n, os.path.join(ROOT, "profiles", f"k_trace_rays_{name}.traffic.json"),
                         match={"tuning": settings, "workload": f"ray_query {name}"})
         res[name] = {"mrays_per_s": round(n / ms / 1e3, 1), "ms_per_launch": round(ms, 3),
@@ -704,7 +711,7 @@ def main():
     cont_timed = segs - args.steps * W * H
     bytes_per_launch = bytes_per_seg * cont_timed / max(1, launches)
     avg_ms = ms_trace / max(1, launches)
-    roof = roofline(f"k_trace<false, SrcQueue, SinkQueue, false, {int(ds.tuning().bvh_width)}>", avg_ms, launches,
+    roof = roofline(f"k_trace<false, SrcQueue, SinkQueue, false, {trace_variant(ds.tuning().as_dict())}>", avg_ms, launches,
                     bytes_per_launch, TRAFFIC_FILE,
                     match={"tuning": tuning, "workload": TRAFFIC_WORKLOAD},
                     extra={"bytes_per_segment": round(bytes_per_seg, 1), "node_bytes": node_bytes,

@@ -351,7 +351,11 @@ typedef struct mrt_tuning {
     float ms_per_frame;       /* frame time measured for these settings (mrt_tune) */
     uint32_t leaf_lanes;      /* lanes of 64 left with nothing but a postponed leaf before a
                                  traversal wave runs a leaf-test phase, 1..64 (0 = 16) */
-    uint32_t _reserved;       /* zero */
+    uint32_t bvh8_order;      /* BVH8 visiting order: 0 = near-first by sorted entry distance,
+                                 one stack entry per child; 1 = octant order, one stack group
+                                 per node (fewer instructions per node, a node header re-read
+                                 per pop: faster on long incoherent traversals; at most 2^24
+                                 nodes). Results are the same either way. */
 } mrt_tuning;
 
 typedef struct mrt_tune_request {

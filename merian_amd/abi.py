@@ -125,7 +125,7 @@ class PtStats(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("max_paths", C.c_uint64), ("refill_lanes", C.c_uint32), ("grid_percent", C.c_uint32),
                 ("bvh_max_leaf", C.c_uint32), ("bvh_traversal_cost", C.c_float), ("bvh_width", C.c_uint32),
-                ("ms_per_frame", C.c_float), ("leaf_lanes", C.c_uint32), ("_reserved", C.c_uint32)]
+                ("ms_per_frame", C.c_float), ("leaf_lanes", C.c_uint32), ("bvh8_order", C.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

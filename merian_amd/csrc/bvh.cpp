@@ -362,6 +362,49 @@ struct Collapser8 {
     BVH8& out;
     float mag;
 
+    // Places the node's children in slots by corner: slot bit a set = the child lies toward the
+    // high end of axis a (children scored by their centre's offset from the node centre,
+    // relative to the node's extent, greedily best pair first). A ray whose direction signs are
+    // the bits o (bit a: negative along a) then meets the children roughly in slot order i ^ o.
+    void assign_slots(const int32_t* kids, int nk, int32_t* slot_kid) const {
+        for (int s = 0; s < 8; s++) slot_kid[s] = -1;
+        float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+        float ctr[8][3];
+        for (int k = 0; k < nk; k++) {
+            const AABB& b = n2[kids[k]].box;
+            for (int a = 0; a < 3; a++) {
+                ctr[k][a] = 0.5f * (b.lo[a] + b.hi[a]);
+                mn[a] = std::min(mn[a], b.lo[a]);
+                mx[a] = std::max(mx[a], b.hi[a]);
+            }
+        }
+        float score[8][8];
+        for (int k = 0; k < nk; k++)
+            for (int s = 0; s < 8; s++) {
+                float sc = 0.f;
+                for (int a = 0; a < 3; a++) {
+                    const float ext = mx[a] - mn[a];
+                    const float off = ext > 0.f ? (ctr[k][a] - 0.5f * (mn[a] + mx[a])) / ext : 0.f;
+                    sc += ((s >> a) & 1) ? off : -off;
+                }
+                score[k][s] = sc;
+            }
+        bool kid_done[8] = {}, slot_done[8] = {};
+        for (int n = 0; n < nk; n++) {
+            int bk = -1, bs = -1;
+            float best = -FLT_MAX;
+            for (int k = 0; k < nk; k++)
+                for (int s = 0; s < 8; s++)
+                    if (!kid_done[k] && !slot_done[s] && score[k][s] > best) {
+                        best = score[k][s];
+                        bk = k;
+                        bs = s;
+                    }
+            kid_done[bk] = slot_done[bs] = true;
+            slot_kid[bs] = kids[bk];
+        }
+    }
+
     // fills out.nodes[me] from BVH2 node ni (or a lone leaf when ni is a leaf)
     void emit(int32_t ni, uint32_t me, uint32_t depth) {
         out.depth = std::max(out.depth, depth);
@@ -373,6 +416,8 @@ struct Collapser8 {
         } else {
             nk = gather_children(n2, ni, 8, kids);
         }
+        int32_t sk[8];
+        assign_slots(kids, nk, sk);
         float lo[8][3], hi[8][3];
         for (int k = 0; k < nk; k++) padded(n2[kids[k]].box, mag, lo[k], hi[k]);
         const Quant g = node_grid(lo, hi, nk);
@@ -399,19 +444,22 @@ struct Collapser8 {
         node.tri_base = uint32_t(out.tris.size());
         std::memset(node.code, BVH8_EMPTY, 8);
         uint32_t rank = 0;
-        for (int k = 0; k < nk; k++) {
+        for (int s = 0; s < 8; s++) {
+            if (sk[s] < 0) continue;
+            const Node2& c = n2[sk[s]];
+            float clo[3], chi[3];
+            padded(c.box, mag, clo, chi);
             for (int a = 0; a < 3; a++) {
                 const float sc = std::ldexp(1.0f, g.e[a]);
-                qlo[a][k] = quant_lo(lo[k][a], g.p[a], sc);
-                qhi[a][k] = quant_hi(hi[k][a], g.p[a], sc);
+                qlo[a][s] = quant_lo(clo[a], g.p[a], sc);
+                qhi[a][s] = quant_hi(chi[a], g.p[a], sc);
             }
-            const Node2& c = n2[kids[k]];
             if (c.left >= 0) {
-                node.inner_mask |= uint8_t(1u << k);
-                node.code[k] = uint8_t(BVH8_INNER + rank++);
+                node.inner_mask |= uint8_t(1u << s);
+                node.code[s] = uint8_t(BVH8_INNER + rank++);
             } else {
                 const uint32_t off = uint32_t(out.tris.size()) - node.tri_base;
-                node.code[k] = uint8_t(off << 3 | (c.count - 1u)); // count in [1, BVH8_MAX_LEAF]
+                node.code[s] = uint8_t(off << 3 | (c.count - 1u)); // count in [1, BVH8_MAX_LEAF]
                 for (uint32_t t = 0; t < c.count; t++) {
                     out.tris.emplace_back();
                     set_record(out.tris.back(), idx[c.first + t], v0, e1, e2, flags);
@@ -420,8 +468,8 @@ struct Collapser8 {
         }
         out.nodes[me] = node;
         uint32_t slot = node.child_base;
-        for (int k = 0; k < nk; k++)
-            if (n2[kids[k]].left >= 0) emit(kids[k], slot++, depth + 1);
+        for (int s = 0; s < 8; s++)
+            if (sk[s] >= 0 && n2[sk[s]].left >= 0) emit(sk[s], slot++, depth + 1);
     }
 };
 

@@ -177,7 +177,8 @@ __device__ __forceinline__ void flush_counts(const TraversalCount& c, unsigned l
 // queue): a persistent grid, one ray per lane with refill, LDS stack + global overflow.
 // Ray count: *count_ptr when given (device-side queue size), else n_fixed.
 // ANY: visibility queries (first hit ends the traversal).
-// W: BVH node width of the scene (4: BVH4Node, 8: quantised BVH8Node).
+// W: traversal of the scene's tree (4: BVH4Node; 8: quantised BVH8Node, sorted entries; 9: the
+// same nodes in octant order with grouped stack entries).
 template <bool COUNT, class Src, class Sink, bool ANY = false, int W = 4>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DeviceScene S, Src src, Sink sink,
                                                        const uint32_t* __restrict__ count_ptr, uint32_t n_fixed,
@@ -188,7 +189,10 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DeviceScene S, Src src, S
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const LaneStack stk{stack + threadIdx.x, ts.ovf + gtid, gridDim.x * blockDim.x};
     TraversalCount cnt;
-    if constexpr (W == 8) {
+    if constexpr (W == 9) {
+        trace_persistent<TRACE_LDS_STACK, COUNT, Src, Sink, ANY>(DeviceBVH8G{S.nodes8, S.tris}, src, sink, n, work,
+                                                                 mask, stk, cnt, ts.refill, ts.leaf_lanes);
+    } else if constexpr (W == 8) {
         trace_persistent<TRACE_LDS_STACK, COUNT, Src, Sink, ANY>(DeviceBVH8{S.nodes8, S.tris}, src, sink, n, work,
                                                                  mask, stk, cnt, ts.refill, ts.leaf_lanes);
     } else {
@@ -1258,13 +1262,18 @@ hipError_t launch_trace(const DeviceScene& s, const TraceScratch& ts, const Src&
     if (n_max == 0) return hipSuccess;
     if (s.stack_need > TRACE_LDS_STACK + ts.entries) return hipErrorInvalidValue; // overflow area too small
     if (s.width == 8 ? !s.nodes8 : !s.nodes) return hipErrorInvalidValue;
-    const int resident = s.width == 8 ? trace_resident_blocks<COUNT, Src, Sink, ANY, 8>()
-                                      : trace_resident_blocks<COUNT, Src, Sink, ANY, 4>();
+    const int w = s.width == 8 ? (s.bvh8_order ? 9 : 8) : 4;
+    const int resident = w == 9   ? trace_resident_blocks<COUNT, Src, Sink, ANY, 9>()
+                         : w == 8 ? trace_resident_blocks<COUNT, Src, Sink, ANY, 8>()
+                                  : trace_resident_blocks<COUNT, Src, Sink, ANY, 4>();
     const int cap = std::max(1, int(int64_t(resident) * int64_t(ts.grid_pct) / 100));
     int g = grid_for(n_max, TRACE_BLOCK, cap);
     if (uint64_t(g) * TRACE_BLOCK > ts.threads) g = int(ts.threads / TRACE_BLOCK);
     if (g <= 0) return hipErrorInvalidValue;
-    if (s.width == 8) {
+    if (w == 9) {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_trace<COUNT, Src, Sink, ANY, 9>), dim3(g), dim3(TRACE_BLOCK), 0, st, s,
+                           src, sink, count_ptr, n_max, work, mask, ts, counters);
+    } else if (w == 8) {
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_trace<COUNT, Src, Sink, ANY, 8>), dim3(g), dim3(TRACE_BLOCK), 0, st, s,
                            src, sink, count_ptr, n_max, work, mask, ts, counters);
     } else {
@@ -1287,6 +1296,9 @@ void trace_prepare() {
     (void)trace_resident_blocks<false, SrcCamera, SinkGBuffer, false, 8>();
     (void)trace_resident_blocks<false, SrcQueue, SinkQueue, false, 8>();
     (void)trace_resident_blocks<false, SrcShadow, SinkShadow, true, 8>();
+    (void)trace_resident_blocks<false, SrcCamera, SinkGBuffer, false, 9>();
+    (void)trace_resident_blocks<false, SrcQueue, SinkQueue, false, 9>();
+    (void)trace_resident_blocks<false, SrcShadow, SinkShadow, true, 9>();
 }
 
 uint32_t trace_max_threads() {

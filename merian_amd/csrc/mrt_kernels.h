@@ -15,6 +15,7 @@ struct DeviceScene {
     const BVH4Node* nodes;       // width 4
     const BVH8Node* nodes8;      // width 8 (quantised)
     uint32_t width;              // BVH node width in use: 4 or 8
+    uint32_t bvh8_order;         // BVH8 visiting order: 0 sorted entries, 1 octant groups (mrt_tuning)
     const float4* tris;          // TriRecord stream, leaf order
     const uint4* tri_verts;      // per global triangle id: 3 global vertex indices, geometry index
     const float4* tri_shade;     // per global triangle id, 64 B: (p0, n0), (p1, n1), (p2, n2),

@@ -93,6 +93,19 @@ template <int SL> __device__ __forceinline__ uint32_t stack_pop(const LaneStack&
     if (sp < SL) return ((lds_ptr)s.lds)[sp * TRACE_BLOCK];
     return ((global_ptr)s.ovf)[size_t(sp - SL) * s.ovf_stride];
 }
+template <int SL> __device__ __forceinline__ uint32_t stack_peek(const LaneStack& s, int i) {
+    typedef const __attribute__((address_space(3))) uint32_t* lds_ptr;
+    typedef const __attribute__((address_space(1))) uint32_t* global_ptr;
+    if (i < SL) return ((lds_ptr)s.lds)[i * TRACE_BLOCK];
+    return ((global_ptr)s.ovf)[size_t(i - SL) * s.ovf_stride];
+}
+template <int SL> __device__ __forceinline__ void stack_poke(const LaneStack& s, int i, uint32_t e) {
+    if (i < SL) {
+        s.lds[i * TRACE_BLOCK] = e;
+    } else {
+        s.ovf[size_t(i - SL) * s.ovf_stride] = e;
+    }
+}
 
 // Per-lane traversal state between steps.
 struct RayState {
@@ -172,17 +185,6 @@ __device__ __forceinline__ void leaf_test(const float4* __restrict__ tris, RaySt
     }
 }
 
-// Moves a leaf entry in `cur` into the (empty) postponed slot and pops the next entry, so the
-// lane can keep stepping through inner nodes while its leaf waits for the wave's next leaf
-// phase (speculative traversal, Aila & Laine 2009).
-template <int SL>
-__device__ __forceinline__ void postpone_leaf(RayState& r, const LaneStack& stk) {
-    if (r.leaf == TRAVERSAL_DONE && is_leaf(r.cur)) {
-        r.leaf = r.cur;
-        r.cur = r.sp > 0 ? stack_pop<SL>(stk, r.sp) : TRAVERSAL_DONE;
-    }
-}
-
 // One inner-node step (r.cur is an inner node): test its four children, push the hit ones far
 // to near, continue with the nearest.
 template <int SL, bool COUNT>
@@ -230,12 +232,55 @@ __device__ __forceinline__ void node_step(const DeviceBVH bvh, RayState& r, cons
     }
 }
 
-struct DeviceBVH8 {
+// Entry of BVH8 slot c from the node's second 16 bytes (child base, triangle base, code words):
+// the slot's code byte picked by a byte permute (selector bytes 0x0C read as zero), then
+// base + code (bvh.h).
+__device__ __forceinline__ uint32_t bvh8_entry(const uint4 h1, uint32_t c) {
+    const uint32_t code = __builtin_amdgcn_perm(h1.w, h1.z, c | 0x0C0C0C00u);
+    return (code >= BVH8_INNER ? h1.x - uint32_t(BVH8_INNER) : LEAF_FLAG | (h1.y << 3)) + code;
+}
+
+// Grouped BVH8 stack entries: a node index (< 2^24) and, in the top byte, the node's hit
+// children not yet visited as a mask over visiting positions (position i = slot i ^ r.neg).
+constexpr uint32_t BVH8_GROUP_NODE = 0xFFFFFFu;
+
+struct DeviceBVH8 { // near-first order by sorted distances, one stack entry per pushed child
     const BVH8Node* nodes;
     const float4* tris; // TriRecord as 3 x float4
 };
+struct DeviceBVH8G { // octant visiting order, one stack group per node
+    const BVH8Node* nodes;
+    const float4* tris;
+};
 
-// One inner-node step over the 8-wide quantised BVH: slab-test the eight child boxes, sort the
+// Next entry to visit: BVH4 and sorted BVH8 pop a ready entry; grouped BVH8 takes the first
+// remaining child of the top group (re-reading its node's 16-byte header) and drops the group
+// once it is empty.
+template <int SL>
+__device__ __forceinline__ uint32_t pop_next(const DeviceBVH&, RayState& r, const LaneStack& stk) {
+    return r.sp > 0 ? stack_pop<SL>(stk, r.sp) : TRAVERSAL_DONE;
+}
+template <int SL>
+__device__ __forceinline__ uint32_t pop_next(const DeviceBVH8&, RayState& r, const LaneStack& stk) {
+    return r.sp > 0 ? stack_pop<SL>(stk, r.sp) : TRAVERSAL_DONE;
+}
+template <int SL>
+__device__ __forceinline__ uint32_t pop_next(const DeviceBVH8G bvh, RayState& r, const LaneStack& stk) {
+    if (r.sp == 0) return TRAVERSAL_DONE;
+    const uint32_t g = stack_peek<SL>(stk, r.sp - 1);
+    const uint32_t node = g & BVH8_GROUP_NODE;
+    const uint32_t m = g >> 24;
+    const uint4 h1 = reinterpret_cast<const uint4*>(bvh.nodes + node)[1];
+    const uint32_t rest = m & (m - 1u);
+    if (rest == 0u) {
+        r.sp--;
+    } else {
+        stack_poke<SL>(stk, r.sp - 1, node | (rest << 24));
+    }
+    return bvh8_entry(h1, uint32_t(__builtin_ctz(m)) ^ r.neg);
+}
+
+// One inner-node step over the 8-wide quantised BVH, near-first by sorted entry distance: slab-test the eight child boxes, sort the
 // hit children near to far, push all but the nearest, continue with it.
 //
 // Slab distances are taken in the node's grid: a plane at grid coordinate q lies at q 2^e + p,
@@ -268,9 +313,6 @@ __device__ __forceinline__ void node_step(const DeviceBVH8 bvh, RayState& r, con
     const uint32_t fz0 = __builtin_amdgcn_perm(qz.x, qz.z, sz), fz1 = __builtin_amdgcn_perm(qz.y, qz.w, sz);
     constexpr uint32_t MISS = 0xFFFFFFFFu;
     uint32_t key[8];
-    // a slot's entry = its node's base + its code byte (bvh.h): inner children from child_base,
-    // leaves from tri_base with (offset << 3 | count - 1) in the code
-    const uint32_t inner_base = h1.x - uint32_t(BVH8_INNER), leaf_base = LEAF_FLAG | (h1.y << 3);
 #pragma unroll
     for (int c = 0; c < 8; c++) {
         const uint32_t sh = 8u * uint32_t(c & 3);
@@ -285,12 +327,8 @@ __device__ __forceinline__ void node_step(const DeviceBVH8 bvh, RayState& r, con
         const uint32_t code = ((c < 4 ? h1.z : h1.w) >> sh) & 0xFFu;
         key[c] = (tn <= tf && code != BVH8_EMPTY) ? ((__float_as_uint(tn) & ~7u) | uint32_t(c)) : MISS;
     }
-    // entry of the slot in a sorted key's low bits: its code byte picked from the node's two code
-    // words by a byte permute (selector bytes 0x0C read as zero), then base + code
-    auto entry_of = [&](uint32_t k) {
-        const uint32_t code = __builtin_amdgcn_perm(h1.w, h1.z, (k & 7u) | 0x0C0C0C00u);
-        return (code >= BVH8_INNER ? inner_base : leaf_base) + code;
-    };
+    // entry of the slot in a sorted key's low bits
+    auto entry_of = [&](uint32_t k) { return bvh8_entry(h1, k & 7u); };
 #define MRT_CS(a, b)                                                                                           \
     {                                                                                                          \
         const uint32_t lo_ = min(key[a], key[b]);                                                              \
@@ -314,6 +352,81 @@ __device__ __forceinline__ void node_step(const DeviceBVH8 bvh, RayState& r, con
     }
 }
 
+
+// One inner-node step over the 8-wide quantised BVH in octant order: slab-test the eight child
+// boxes, continue with the first hit child in the ray's visiting order and push the others as
+// one group. Fewer instructions per node than the sorted step, but a pop re-reads a node
+// header: it wins on long incoherent traversals and loses where pops are frequent.
+//
+// Slab distances are taken in the node's grid: a plane at grid coordinate q lies at q 2^e + p,
+// so its distance is q (2^e / d) + (p / d - o / d) = fma(q, a, b) with a = 2^e ix (exact: a
+// power-of-two scale) and b = fma(p, ix, -ox) once per node and axis, one FMA per plane after
+// the byte conversion. Visiting order: the builder puts a child in the slot of the node corner
+// it lies toward (bit a: high end of axis a), so a ray whose direction signs are r.neg meets
+// them roughly in the order slot = i ^ r.neg, i = 0..7 (the ordering of compressed wide BVHs,
+// Ylitie, Karras and Laine 2017); the hit mask is permuted into that order by three
+// conditional bit-group swaps, and no distances are sorted. Only the visiting order depends on
+// it; which children are visited does not.
+template <int SL, bool COUNT>
+__device__ __forceinline__ void node_step(const DeviceBVH8G bvh, RayState& r, const LaneStack& stk,
+                                          TraversalCount& count) {
+    if constexpr (COUNT) count.nodes++;
+    const uint4* np = reinterpret_cast<const uint4*>(bvh.nodes + r.cur);
+    const uint4 h0 = np[0], h1 = np[1], qx = np[2], qy = np[3], qz = np[4];
+    const int ex = int(int8_t(h0.w & 0xFFu)), ey = int(int8_t((h0.w >> 8) & 0xFFu)), ez = int(int8_t((h0.w >> 16) & 0xFFu));
+    const float ax = ldexpf(r.ix, ex), ay = ldexpf(r.iy, ey), az = ldexpf(r.iz, ez); // 2^e ix, exact
+    const float bx = __builtin_fmaf(__uint_as_float(h0.x), r.ix, -r.ox);
+    const float by = __builtin_fmaf(__uint_as_float(h0.y), r.iy, -r.oy);
+    const float bz = __builtin_fmaf(__uint_as_float(h0.z), r.iz, -r.oz);
+    const float tlim = r.tlim;
+    // near / far plane bytes per axis, picked by the ray's octant with one byte permute per word
+    // (children 0-3 in the low word, 4-7 in the high word)
+    const uint32_t sx = r.psel[0], sy = r.psel[1], sz = r.psel[2];
+    const uint32_t nx0 = __builtin_amdgcn_perm(qx.z, qx.x, sx), nx1 = __builtin_amdgcn_perm(qx.w, qx.y, sx);
+    const uint32_t fx0 = __builtin_amdgcn_perm(qx.x, qx.z, sx), fx1 = __builtin_amdgcn_perm(qx.y, qx.w, sx);
+    const uint32_t ny0 = __builtin_amdgcn_perm(qy.z, qy.x, sy), ny1 = __builtin_amdgcn_perm(qy.w, qy.y, sy);
+    const uint32_t fy0 = __builtin_amdgcn_perm(qy.x, qy.z, sy), fy1 = __builtin_amdgcn_perm(qy.y, qy.w, sy);
+    const uint32_t nz0 = __builtin_amdgcn_perm(qz.z, qz.x, sz), nz1 = __builtin_amdgcn_perm(qz.w, qz.y, sz);
+    const uint32_t fz0 = __builtin_amdgcn_perm(qz.x, qz.z, sz), fz1 = __builtin_amdgcn_perm(qz.y, qz.w, sz);
+    uint32_t hm = 0u; // bit c: slot c is hit
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        const uint32_t sh = 8u * uint32_t(c & 3);
+        const float NX = __builtin_fmaf(float(((c < 4 ? nx0 : nx1) >> sh) & 0xFFu), ax, bx);
+        const float FX = __builtin_fmaf(float(((c < 4 ? fx0 : fx1) >> sh) & 0xFFu), ax, bx);
+        const float NY = __builtin_fmaf(float(((c < 4 ? ny0 : ny1) >> sh) & 0xFFu), ay, by);
+        const float FY = __builtin_fmaf(float(((c < 4 ? fy0 : fy1) >> sh) & 0xFFu), ay, by);
+        const float NZ = __builtin_fmaf(float(((c < 4 ? nz0 : nz1) >> sh) & 0xFFu), az, bz);
+        const float FZ = __builtin_fmaf(float(((c < 4 ? fz0 : fz1) >> sh) & 0xFFu), az, bz);
+        const float tn = fmaxf(fmaxf(NX, NY), fmaxf(NZ, r.t0));
+        const float tf = fminf(fminf(FX, FY), fminf(FZ, tlim));
+        const uint32_t code = ((c < 4 ? h1.z : h1.w) >> sh) & 0xFFu;
+        hm |= (tn <= tf && code != BVH8_EMPTY) ? (1u << c) : 0u;
+    }
+    // position i of the visiting order is slot i ^ neg: swap bit pairs, pairs of pairs, nibbles
+    uint32_t pm = hm;
+    if (r.neg & 1u) pm = ((pm & 0x55u) << 1) | ((pm >> 1) & 0x55u);
+    if (r.neg & 2u) pm = ((pm & 0x33u) << 2) | ((pm >> 2) & 0x33u);
+    if (r.neg & 4u) pm = ((pm & 0x0Fu) << 4) | (pm >> 4);
+    if (pm == 0u) {
+        r.cur = pop_next<SL>(bvh, r, stk);
+        return;
+    }
+    const uint32_t rest = pm & (pm - 1u);
+    if (rest != 0u) stack_push<SL>(stk, r.sp, r.cur | (rest << 24));
+    r.cur = bvh8_entry(h1, uint32_t(__builtin_ctz(pm)) ^ r.neg);
+}
+
+// Moves a leaf entry in `cur` into the (empty) postponed slot and takes the next entry, so the
+// lane can keep stepping through inner nodes while its leaf waits for the wave's next leaf
+// phase (speculative traversal, Aila & Laine 2009).
+template <int SL, class BVH>
+__device__ __forceinline__ void postpone_leaf(const BVH& bvh, RayState& r, const LaneStack& stk) {
+    if (r.leaf == TRAVERSAL_DONE && is_leaf(r.cur)) {
+        r.leaf = r.cur;
+        r.cur = pop_next<SL>(bvh, r, stk);
+    }
+}
 
 // Persistent closest-hit loop of one wave: every lane runs one traversal step per iteration and,
 // when its ray is done, hands the hit to `sink` and later takes a fresh ray from `src`, so lanes
@@ -399,7 +512,7 @@ __device__ __forceinline__ void trace_persistent(const BVH bvh, const Src& src, 
             if (inner_ok) node_step<SL, COUNT>(bvh, r, stk, count);
         }
         if (active) {
-            postpone_leaf<SL>(r, stk);
+            postpone_leaf<SL>(bvh, r, stk);
             if (r.cur == TRAVERSAL_DONE && r.leaf == TRAVERSAL_DONE) {
                 sink.store(idx, r.best);
                 active = false;

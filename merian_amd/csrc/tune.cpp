@@ -81,6 +81,7 @@ mrt_status rebuild_bvh(mrt_ctx* c, mrt_scene* s, uint32_t width, uint32_t max_le
     s->allocs.push_back(nodes);
     s->allocs.push_back(tris);
     s->bytes += node_bytes + tri_bytes;
+    // at most width - 1 entries per level (a grouped BVH8 traversal needs one)
     s->ds.stack_need = (width - 1u) * depth + 4u;
     s->n_nodes = n_nodes;
     s->depth = depth;
@@ -97,7 +98,7 @@ namespace {
 bool valid(const mrt_tuning& t) {
     return t.refill_lanes >= 1 && t.refill_lanes <= 64 && t.grid_percent >= 1 && t.grid_percent <= 100 &&
            t.bvh_max_leaf >= 1 && t.bvh_max_leaf <= uint32_t(MAX_LEAF) && t.bvh_traversal_cost > 0.f &&
-           (t.bvh_width == 0 || t.bvh_width == 4 || t.bvh_width == 8) && t.leaf_lanes <= 64;
+           (t.bvh_width == 0 || t.bvh_width == 4 || t.bvh_width == 8) && t.leaf_lanes <= 64 && t.bvh8_order <= 1;
 }
 
 // Device time of one trial frame (gbuffer + pt) in ms, after one untimed frame.
@@ -166,10 +167,13 @@ MRT_API mrt_status mrt_scene_set_tuning(mrt_ctx* c, mrt_scene* s, const mrt_tuni
         const mrt_status st = rebuild_bvh(c, s, width, t->bvh_max_leaf, t->bvh_traversal_cost);
         if (st != MRT_OK) return st;
     }
+    // a grouped stack entry packs the node index into 24 bits (mrt_trace.h)
+    if (width == 8 && t->bvh8_order == 1 && s->n_nodes > (size_t(1) << 24))
+        return fail(c, MRT_ERR_UNSUPPORTED, "BVH8 octant order needs at most 2^24 nodes");
     s->tuning = *t;
     s->tuning.bvh_width = width;
     if (s->tuning.leaf_lanes == 0) s->tuning.leaf_lanes = 16;
-    s->tuning._reserved = 0;
+    s->ds.bvh8_order = width == 8 ? t->bvh8_order : 0u;
     return MRT_OK;
 }
 
@@ -207,9 +211,9 @@ MRT_API mrt_status mrt_tune(mrt_ctx* c, mrt_scene* s, const mrt_tune_request* r,
         cand.ms_per_frame = ms;
         if (verbose)
             std::fprintf(stderr,
-                         "mrt_tune: width %u leaf %u cost %.2f paths %llu refill %u leaf-phase %u grid %u%% -> %.3f ms "
-                         "(best %.3f)\n",
-                         cand.bvh_width, cand.bvh_max_leaf, double(cand.bvh_traversal_cost),
+                         "mrt_tune: width %u order %u leaf %u cost %.2f paths %llu refill %u leaf-phase %u grid %u%% -> "
+                         "%.3f ms (best %.3f)\n",
+                         cand.bvh_width, cand.bvh8_order, cand.bvh_max_leaf, double(cand.bvh_traversal_cost),
                          (unsigned long long)cand.max_paths, cand.refill_lanes, cand.leaf_lanes, cand.grid_percent,
                          double(ms), double(best.ms_per_frame));
         if (ms < best.ms_per_frame) best = cand;
@@ -260,6 +264,12 @@ MRT_API mrt_status mrt_tune(mrt_ctx* c, mrt_scene* s, const mrt_tune_request* r,
             mrt_tuning cand = best;
             cand.grid_percent = gp;
             if ((st = try_one(cand)) != MRT_OK) return st;
+        }
+        if (best.bvh_width == 8) {
+            mrt_tuning cand = best;
+            cand.bvh8_order ^= 1u;
+            if (cand.bvh8_order == 0u || s->n_nodes <= (size_t(1) << 24))
+                if ((st = try_one(cand)) != MRT_OK) return st;
         }
         if (std::memcmp(&start, &best, sizeof(best)) == 0) break;
     }

@@ -31,9 +31,9 @@ def _compare(img, sig, ref, ref_sig, min_match=0.999):
     return frac, worst
 
 
-def _with_tree(ds, width, leaf):
+def _with_tree(ds, width, leaf, order=0):
     t = ds.tuning()
-    t.bvh_width, t.bvh_max_leaf = width, leaf
+    t.bvh_width, t.bvh_max_leaf, t.bvh8_order = width, leaf, order
     ds.set_tuning(t)
     return ds
 
@@ -87,7 +87,7 @@ def test_pt_bvh8_matches_bvh4_bitwise(ctx):
         ds.destroy()
 
 
-@pytest.mark.parametrize("tree", [(4, 4), (4, 2), (8, 2), (8, 4)])
+@pytest.mark.parametrize("tree", [(4, 4), (4, 2), (8, 2), (8, 4), (8, 2, 1)])
 def test_pt_full_scene_crop(ctx, tree):
     # 256 displaced tori (reduced instance count), dielectrics, lat-long sky; 1080p camera.
     # Every BVH shape the autotuner may pick must give the same image.

@@ -12,19 +12,20 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 
-# (BVH width, max leaf size): the 4-wide tree and the 8-wide tree with quantized child boxes,
-# at leaf sizes the autotuner may choose.
-TREES = [(4, 4), (4, 2), (8, 2), (8, 8)]
+# (BVH width, max leaf size[, BVH8 visiting order]): the 4-wide tree and the 8-wide tree with
+# quantized child boxes, at leaf sizes the autotuner may choose, the 8-wide tree also in octant
+# order with grouped stack entries.
+TREES = [(4, 4), (4, 2), (8, 2), (8, 8), (8, 2, 1), (8, 4, 1)]
 
 
 def _tris(ctx, pos, idx, tree, two_sided=True):
     ds = ctx.triangles(pos, idx, two_sided=two_sided)
-    width, leaf = tree
+    width, leaf, order = (*tree, 0)[:3]
     t = ds.tuning()
-    t.bvh_width, t.bvh_max_leaf = width, leaf
+    t.bvh_width, t.bvh_max_leaf, t.bvh8_order = width, leaf, order
     ds.set_tuning(t)
     got = ds.tuning()
-    assert (got.bvh_width, got.bvh_max_leaf) == (width, leaf)
+    assert (got.bvh_width, got.bvh_max_leaf, got.bvh8_order) == (width, leaf, order)
     return ds
 
 
@@ -127,7 +128,7 @@ def _sample_hits(d_hits, sel):
     return rows.cpu().numpy().reshape(-1).view(abi.HIT_DTYPE)
 
 
-@pytest.mark.parametrize("tree", [(4, 2), (8, 2)])
+@pytest.mark.parametrize("tree", [(4, 2), (8, 2), (8, 2, 1)])
 def test_ray_query_workload_full_size(ctx, tree):
     # the ray-query microbenchmark at its full size: 1M random triangles, 16M incoherent rays and
     # 4096^2 coherent camera rays. A seeded sample of 65,536 rays of each set is bit-exact
