@@ -59,7 +59,7 @@ def roofline(kernel: str, avg_ms: float, launches: int, algorithmic_bytes: float
     algorithmic bytes (every node, triangle and ray the traversal touches, most of them served by
     L2) stay beside it as a diagnostic. The profile applies only if it was taken with the settings
     in `match` (tuning, workload); otherwise `achieved` falls back to the algorithmic rate and says
-    so. Reads are scaled by what FETCH_SIZE shows for random whole-line gathers of known size
+    so; a profile of another kernel does not apply either. Reads are scaled by what FETCH_SIZE shows for random whole-line gathers of known size
     (exact: 64 bytes per request, two per 128-byte line fill); `traffic_upper` scales them by the
     copy's factor instead (streaming reads count one request per 128-byte line)."""
     alg_gbs = algorithmic_bytes / (avg_ms * 1e-3) / 1e9
@@ -79,6 +79,9 @@ def roofline(kernel: str, avg_ms: float, launches: int, algorithmic_bytes: float
             elif pv != v:
                 tj = None
                 break
+    # a profile of another kernel variant (e.g. the other BVH8 visiting order) does not apply
+    if tj is not None and tj.get("kernel") not in (None, kernel):
+        tj = None
     if tj is not None:
         traffic = float(tj["traffic_bytes_per_dispatch"])
         achieved = traffic / (avg_ms * 1e-3) / 1e9

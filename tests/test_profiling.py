@@ -137,3 +137,16 @@ def test_l1_request_summary(tmp_path):
     (k, v), = P.l1(str(tmp_path / "l1")).items()
     assert k.startswith("k_trace<") and v["l2_read_req_per_dispatch"] == 2e9 and v["l2_read_greq_per_s"] == 200.0
     assert v["l1_hit_pct"] == 33.3 and v["ta_busy_pct"] == 75.0 and v["ta_stalled_on_l1_pct"] == 50.0
+
+
+def test_bench_roofline_ignores_a_profile_of_another_kernel(tmp_path):
+    sys.path.insert(0, str(ROOT))
+    import bench
+    prof_json = tmp_path / "t.json"
+    prof_json.write_text(json.dumps({"traffic_bytes_per_dispatch": 4e9, "workload": "w", "kernel": "k_trace<..., 8>",
+                                     "tuning": {"bvh_width": 8}}))
+    m = {"tuning": {"bvh_width": 8, "bvh8_order": 1}, "workload": "w"}
+    assert bench.roofline("k_trace<..., 8>", 1.0, 10, 8e9, str(prof_json), match=m)["traffic"] == 4e9
+    assert bench.roofline("k_trace<..., 9>", 1.0, 10, 8e9, str(prof_json), match=m)["traffic"] is None
+    assert bench.trace_variant({"bvh_width": 8, "bvh8_order": 1}) == 9 and bench.trace_variant({"bvh_width": 8}) == 8
+    assert bench.trace_variant({"bvh_width": 4, "bvh8_order": 1}) == 4
