@@ -355,7 +355,7 @@ typedef struct mrt_tuning {
                                  one stack entry per child; 1 = octant order, one stack group
                                  per node (fewer instructions per node, a node header re-read
                                  per pop: faster on long incoherent traversals; at most 2^24
-                                 nodes). Results are the same either way. */
+                                 triangles). Results are the same either way. */
 } mrt_tuning;
 
 typedef struct mrt_tune_request {

@@ -162,14 +162,15 @@ MRT_API mrt_status mrt_scene_set_tuning(mrt_ctx* c, mrt_scene* s, const mrt_tuni
     if (!valid(*t)) return fail(c, MRT_ERR_INVALID, "tuning values out of range");
     if (const hipError_t e = hipSetDevice(c->device); e != hipSuccess) return hip_fail(c, e, "mrt_scene_set_tuning");
     const uint32_t width = t->bvh_width ? t->bvh_width : 4u;
+    // a grouped stack entry packs the node index into 24 bits (mrt_trace.h); a tree has fewer
+    // nodes than triangles, so the check precedes any rebuild
+    if (width == 8 && t->bvh8_order == 1 && s->n_tris > (uint64_t(1) << 24))
+        return fail(c, MRT_ERR_UNSUPPORTED, "BVH8 octant order needs at most 2^24 triangles");
     if (t->bvh_max_leaf != s->tuning.bvh_max_leaf || t->bvh_traversal_cost != s->tuning.bvh_traversal_cost ||
         width != s->tuning.bvh_width) {
         const mrt_status st = rebuild_bvh(c, s, width, t->bvh_max_leaf, t->bvh_traversal_cost);
         if (st != MRT_OK) return st;
     }
-    // a grouped stack entry packs the node index into 24 bits (mrt_trace.h)
-    if (width == 8 && t->bvh8_order == 1 && s->n_nodes > (size_t(1) << 24))
-        return fail(c, MRT_ERR_UNSUPPORTED, "BVH8 octant order needs at most 2^24 nodes");
     s->tuning = *t;
     s->tuning.bvh_width = width;
     if (s->tuning.leaf_lanes == 0) s->tuning.leaf_lanes = 16;
@@ -268,7 +269,7 @@ MRT_API mrt_status mrt_tune(mrt_ctx* c, mrt_scene* s, const mrt_tune_request* r,
         if (best.bvh_width == 8) {
             mrt_tuning cand = best;
             cand.bvh8_order ^= 1u;
-            if (cand.bvh8_order == 0u || s->n_nodes <= (size_t(1) << 24))
+            if (cand.bvh8_order == 0u || s->n_tris <= (uint64_t(1) << 24))
                 if ((st = try_one(cand)) != MRT_OK) return st;
         }
         if (std::memcmp(&start, &best, sizeof(best)) == 0) break;
