@@ -502,12 +502,14 @@ def _time_rays(ctx, ds, d_r, n, d_hits, stream, reps):
 def _search_ray_tuning(ctx, ds, d_r, n, d_hits, stream) -> dict:
     import itertools
     best = None
-    trees = [(4, 2, 1.0), (4, 1, 1.0), (4, 4, 1.0), (8, 2, 1.0), (8, 4, 1.0), (4, 2, 2.0), (4, 2, 0.5)]
-    for width, leaf, cost in trees:
-        _apply(ds, dict(bvh_width=width, bvh_max_leaf=leaf, bvh_traversal_cost=cost))
+    # (width, leaf, SAH cost, BVH8 visiting order)
+    trees = [(4, 2, 1.0, 0), (4, 1, 1.0, 0), (4, 4, 1.0, 0), (8, 2, 1.0, 0), (8, 4, 1.0, 0), (8, 1, 1.0, 1),
+             (8, 2, 1.0, 1), (8, 4, 1.0, 1), (4, 2, 2.0, 0), (4, 2, 0.5, 0)]
+    for width, leaf, cost, order in trees:
+        _apply(ds, dict(bvh_width=width, bvh_max_leaf=leaf, bvh_traversal_cost=cost, bvh8_order=order))
         for refill, ll, grid in itertools.product([8, 16, 24, 32, 48], [4, 8, 16, 32], [50, 75, 100]):
             cand = dict(bvh_width=width, bvh_max_leaf=leaf, bvh_traversal_cost=cost, refill_lanes=refill,
-                        leaf_lanes=ll, grid_percent=grid)
+                        leaf_lanes=ll, grid_percent=grid, bvh8_order=order)
             _apply(ds, cand)
             ms = _time_rays(ctx, ds, d_r, n, d_hits, stream, 2)
             log(f"  {cand}: {ms:.2f} ms")
