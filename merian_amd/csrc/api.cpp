@@ -76,7 +76,7 @@ hipStream_t pick_stream(mrt_ctx* c, void* stream) {
 void free_path_buffers(mrt_ctx* c) {
     PathBuffers& p = c->paths;
     for (void* q : {(void*)p.state, (void*)p.sig, (void*)p.q_o[0], (void*)p.q_o[1], (void*)p.q_d[0],
-                    (void*)p.q_d[1], (void*)p.hits, (void*)p.aux, (void*)p.sq_o, (void*)p.sq_d, (void*)p.sq_c,
+                    (void*)p.q_d[1], (void*)p.q_t[0], (void*)p.q_t[1], (void*)p.q_r[0], (void*)p.q_r[1], (void*)p.hits, (void*)p.aux, (void*)p.sq_o, (void*)p.sq_d, (void*)p.sq_c,
                     (void*)p.counters, (void*)p.accum, (void*)p.demod, (void*)p.sig_accum})
         dev_free(c, q);
     p = PathBuffers{};
@@ -99,6 +99,8 @@ mrt_status ensure_path_buffers(mrt_ctx* c, uint64_t paths, uint64_t npix, uint64
     for (int k = 0; k < 2; k++) {
         p.q_o[k] = static_cast<float4*>(A(16 * paths));
         p.q_d[k] = static_cast<float4*>(A(16 * paths));
+        p.q_t[k] = static_cast<float4*>(A(16 * paths));
+        p.q_r[k] = static_cast<float4*>(A(16 * paths));
     }
     p.hits = static_cast<float4*>(A(16 * paths));
     p.aux = static_cast<float*>(A(4 * paths));

@@ -510,8 +510,12 @@ __global__ void k_pt_init(DeviceScene S, PtArgs A, const mrt_packed_hit* __restr
         st_stream(P.q_o[0] + i, make_float4(r.o.x, r.o.y, r.o.z, __uint_as_float(i)));
         st_stream(P.q_d[0] + i, make_float4(r.d.x, r.d.y, r.d.z, 0.f));
         st_stream(P.hits + i, make_float4(0.f, bu, bv, __uint_as_float(gid)));
-        st_stream(P.state + 2 * size_t(i), make_float4(1.f, 1.f, 1.f, 1.f));
-        st_stream(P.state + 2 * size_t(i) + 1, make_float4(0.f, 0.f, 0.f, __uint_as_float(pcg4d16(x, y, A.frame, s))));
+        // queue slot i holds path i: the state goes to either place at the same index
+        float4* const st = state_in_queue(A.mode) ? P.q_t[0] : P.state;
+        float4* const sr = state_in_queue(A.mode) ? P.q_r[0] : P.state + 1;
+        const size_t k = state_in_queue(A.mode) ? size_t(i) : 2 * size_t(i);
+        st_stream(st + k, make_float4(1.f, 1.f, 1.f, 1.f));
+        st_stream(sr + k, make_float4(0.f, 0.f, 0.f, __uint_as_float(pcg4d16(x, y, A.frame, s))));
         if (A.record_signature) P.sig[i] = sig_step(0x9E3779B9u * (s + 1u), gid);
         if (A.mode != MRT_MODE_BSDF) P.aux[i] = 0.f;
     }
@@ -522,13 +526,19 @@ __global__ void k_pt_init(DeviceScene S, PtArgs A, const mrt_packed_hit* __restr
 // staged rays out contiguously. One global atomic per SHADE_TILE rays instead of one per wave
 // keeps the queue counter (a single address every block of the grid appends to) off the
 // critical path.
+// STATE: the path's state (throughput and ior, radiance and RNG) is staged with the ray and
+// lands in the same queue slot.
 struct RayStage {
     float4* o;      // SHADE_TILE entries each
     float4* d;
+    float4* t;      // path state, STATE stages only
+    float4* r;
     uint32_t* meta; // [0] staged count, [1] reserved queue base, [2] count being flushed
 };
 
-__device__ __forceinline__ void stage_push(const RayStage& st, bool push, float4 o, float4 d) {
+template <bool STATE = false>
+__device__ __forceinline__ void stage_push(const RayStage& st, bool push, float4 o, float4 d, float4 t = float4{},
+                                           float4 r = float4{}) {
     const uint64_t b = __ballot(push);
     if (!b) return;
     const uint32_t lane = lane_id();
@@ -539,6 +549,10 @@ __device__ __forceinline__ void stage_push(const RayStage& st, bool push, float4
     if (push) {
         st.o[slot] = o;
         st.d[slot] = d;
+        if constexpr (STATE) {
+            st.t[slot] = t;
+            st.r[slot] = r;
+        }
     }
 }
 
@@ -563,8 +577,11 @@ __device__ __forceinline__ uint32_t block_append(bool push, uint32_t* __restrict
 }
 
 // Every thread of the block calls this; at most SHADE_TILE rays may be staged between calls.
+// nt, nr: the queue's path-state arrays (STATE stages only).
+template <bool STATE = false>
 __device__ __forceinline__ void stage_flush(const RayStage& st, float4* __restrict__ no, float4* __restrict__ nd,
-                                            uint32_t* __restrict__ count_out) {
+                                            uint32_t* __restrict__ count_out, float4* __restrict__ nt = nullptr,
+                                            float4* __restrict__ nr = nullptr) {
     __syncthreads();
     if (threadIdx.x == 0) {
         const uint32_t n = st.meta[0];
@@ -577,6 +594,10 @@ __device__ __forceinline__ void stage_flush(const RayStage& st, float4* __restri
     for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
         st_stream(no + base + k, st.o[k]);
         st_stream(nd + base + k, st.d[k]);
+        if constexpr (STATE) {
+            st_stream(nt + base + k, st.t[k]);
+            st_stream(nr + base + k, st.r[k]);
+        }
     }
     __syncthreads();
 }
@@ -666,20 +687,27 @@ __device__ __forceinline__ bool guide_eligible(const mrt_material& m) {
 // the guide's lobes with probability A.guide_fraction, else the BSDF, and weighs the direction
 // by f / (a q + (1 - a) p_bsdf) (one-sample MIS); sample-0 paths of the first chunk append a
 // training record per guided vertex.
+// In the state_in_queue modes the path's state is read from the input queue's slot (qt, qr) and
+// written with the continuation ray to the output queue's (nt, nr); a path that ends writes its
+// radiance to P.state once. NEE reads and writes the state by path index.
 // BSDF mode fits 168 VGPRs without spills: 3 waves per SIMD instead of 2.
 template <int MODE>
 __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(MODE == MRT_MODE_BSDF ? 3 : 1))) void k_shade(DeviceScene S, PtArgs A, int bounce, PathBuffers P,
                                                        const float4* __restrict__ qo, const float4* __restrict__ qd,
+                                                       const float4* __restrict__ qt, const float4* __restrict__ qr,
                                                        const uint32_t* __restrict__ count_in, float4* __restrict__ no,
-                                                       float4* __restrict__ nd, uint32_t* __restrict__ count_out,
+                                                       float4* __restrict__ nd, float4* __restrict__ nt,
+                                                       float4* __restrict__ nr, uint32_t* __restrict__ count_out,
                                                        uint32_t* __restrict__ shadow_count) {
+    constexpr bool queued = state_in_queue(MODE);
     __shared__ uint32_t s_order[SHADE_TILE];
     __shared__ uint32_t s_cnt[SHADE_ROUNDS * (SHADE_BLOCK / 64) * (SHADE_CLASSES + 1)];
     __shared__ uint32_t s_total[SHADE_CLASSES + 1];
     __shared__ float4 s_stage_o[SHADE_TILE], s_stage_d[SHADE_TILE];
+    __shared__ float4 s_stage_t[queued ? SHADE_TILE : 1], s_stage_r[queued ? SHADE_TILE : 1];
     __shared__ uint32_t s_meta[3];
     __shared__ uint32_t s_app[2]; // block_append scratch (records, shadow rays)
-    const RayStage stage{s_stage_o, s_stage_d, s_meta};
+    const RayStage stage{s_stage_o, s_stage_d, s_stage_t, s_stage_r, s_meta};
     if (threadIdx.x == 0) {
         s_meta[0] = 0u;
         s_app[0] = 0u;
@@ -703,13 +731,15 @@ __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(MOD
             f3 rec_pos{0, 0, 0}, rec_rad{0, 0, 0}, rd_keep{0, 0, 0}, thr_keep{0, 0, 0};
             float rec_pdf = 0.f;
             f3 sh_o{0, 0, 0}, sh_d{0, 0, 0}, sh_c{0, 0, 0};
+            float4 out_t{}, out_r{};
             uint32_t pid = 0;
             if (i < n) {
                 const float4 o4 = ld_stream(qo + i), d4 = ld_stream(qd + i), h4 = ld_stream(P.hits + i);
                 pid = __float_as_uint(o4.w);
                 const f3 ro{o4.x, o4.y, o4.z}, rd{d4.x, d4.y, d4.z};
                 const uint32_t gid = __float_as_uint(h4.w);
-                float4 sa = ld_stream(P.state + 2 * size_t(pid)), sb = ld_stream(P.state + 2 * size_t(pid) + 1);
+                const float4 sa = queued ? ld_stream(qt + i) : ld_stream(P.state + 2 * size_t(pid));
+                const float4 sb = queued ? ld_stream(qr + i) : ld_stream(P.state + 2 * size_t(pid) + 1);
                 f3 thr{sa.x, sa.y, sa.z};
                 float ior = sa.w;
                 f3 rad{sb.x, sb.y, sb.z};
@@ -810,11 +840,13 @@ __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(MOD
                         if (nee && push) P.aux[pid] = pdf;
                     }
                 }
-                st_stream(P.state + 2 * size_t(pid), make_float4(thr.x, thr.y, thr.z, ior));
-                st_stream(P.state + 2 * size_t(pid) + 1, make_float4(rad.x, rad.y, rad.z, __uint_as_float(rng.state)));
+                out_t = make_float4(thr.x, thr.y, thr.z, ior);
+                out_r = make_float4(rad.x, rad.y, rad.z, __uint_as_float(rng.state));
+                if (!queued) st_stream(P.state + 2 * size_t(pid), out_t);
+                if (!queued || !push) st_stream(P.state + 2 * size_t(pid) + 1, out_r);
             }
-            stage_push(stage, push, make_float4(next_o.x, next_o.y, next_o.z, __uint_as_float(pid)),
-                       make_float4(next_d.x, next_d.y, next_d.z, 0.f));
+            stage_push<queued>(stage, push, make_float4(next_o.x, next_o.y, next_o.z, __uint_as_float(pid)),
+                               make_float4(next_d.x, next_d.y, next_d.z, 0.f), out_t, out_r);
             if (guided) {
                 const uint32_t r = block_append(record, P.grec_count, s_app);
                 if (record && r < P.grec_cap) {
@@ -835,7 +867,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(MOD
                 }
             }
         }
-        stage_flush(stage, no, nd, count_out); // also keeps s_order intact until every round read it
+        stage_flush<queued>(stage, no, nd, count_out, nt, nr); // also keeps s_order intact until every round read it
     }
 }
 
@@ -845,10 +877,10 @@ __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(MOD
 // Results per path are those of k_pt_init + k_shade<MRT_MODE_BSDF> at bounce 0.
 __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(2))) void k_shade_primary(
     DeviceScene S, PtArgs A, const mrt_packed_hit* __restrict__ gbuffer, PathBuffers P, float4* __restrict__ no,
-    float4* __restrict__ nd, uint32_t* __restrict__ count_out) {
-    __shared__ float4 s_stage_o[SHADE_TILE], s_stage_d[SHADE_TILE];
+    float4* __restrict__ nd, float4* __restrict__ nt, float4* __restrict__ nr, uint32_t* __restrict__ count_out) {
+    __shared__ float4 s_stage_o[SHADE_TILE], s_stage_d[SHADE_TILE], s_stage_t[SHADE_TILE], s_stage_r[SHADE_TILE];
     __shared__ uint32_t s_meta[3];
-    const RayStage stage{s_stage_o, s_stage_d, s_meta};
+    const RayStage stage{s_stage_o, s_stage_d, s_stage_t, s_stage_r, s_meta};
     if (threadIdx.x == 0) s_meta[0] = 0u;
     __syncthreads();
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -880,20 +912,23 @@ __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(2))
             const uint32_t pid = sl * A.npix + p;
             bool push = false;
             f3 next_o{0.f, 0.f, 0.f}, next_d{0.f, 0.f, 0.f};
+            float4 out_t{}, out_r{};
             if (valid) {
                 const uint32_t s = A.s0 + sl;
                 PCG rng{pcg4d16(x, y, A.frame, s)};
                 f3 thr = one;
                 float ior = 1.f, pdf = 0.f;
                 if (cont) push = bsdf_continue(sp, wi_local, ng, wi_below, rng, thr, ior, next_o, next_d, pdf);
-                st_stream(P.state + 2 * size_t(pid), make_float4(thr.x, thr.y, thr.z, ior));
-                st_stream(P.state + 2 * size_t(pid) + 1, make_float4(rad.x, rad.y, rad.z, __uint_as_float(rng.state)));
+                // the state travels with the ray; a path that ends here leaves its radiance
+                out_t = make_float4(thr.x, thr.y, thr.z, ior);
+                out_r = make_float4(rad.x, rad.y, rad.z, __uint_as_float(rng.state));
+                if (!push) st_stream(P.state + 2 * size_t(pid) + 1, out_r);
                 if (A.record_signature) P.sig[pid] = sig_step(0x9E3779B9u * (s + 1u), gid);
             }
-            stage_push(stage, push, make_float4(next_o.x, next_o.y, next_o.z, __uint_as_float(pid)),
-                       make_float4(next_d.x, next_d.y, next_d.z, 0.f));
+            stage_push<true>(stage, push, make_float4(next_o.x, next_o.y, next_o.z, __uint_as_float(pid)),
+                             make_float4(next_d.x, next_d.y, next_d.z, 0.f), out_t, out_r);
             // the block stages up to SHADE_BLOCK rays per sample: flush every SHADE_ROUNDS samples
-            if ((sl + 1) % SHADE_ROUNDS == 0 || sl + 1 == A.chunk) stage_flush(stage, no, nd, count_out);
+            if ((sl + 1) % SHADE_ROUNDS == 0 || sl + 1 == A.chunk) stage_flush<true>(stage, no, nd, count_out, nt, nr);
         }
     }
 }
@@ -1017,7 +1052,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(3))
                                                             uint32_t* __restrict__ count_out) {
     __shared__ float4 s_stage_o[SHADE_TILE], s_stage_d[SHADE_TILE];
     __shared__ uint32_t s_meta[3];
-    const RayStage stage{s_stage_o, s_stage_d, s_meta};
+    const RayStage stage{s_stage_o, s_stage_d, nullptr, nullptr, s_meta};
     if (threadIdx.x == 0) s_meta[0] = 0u;
     __syncthreads();
     const uint32_t n = *count_in;
@@ -1361,6 +1396,7 @@ hipError_t launch_pt_frame(const DeviceScene& s, const TraceScratch& ts, PtArgs 
     const int L = a.max_path_length;
     const bool guided = a.mode == MRT_MODE_GUIDED;
     if (guided && (!guide || !pb.lobes || !pb.grec || !pb.gtrain || !pb.grec_count)) return hipErrorInvalidValue;
+    if (state_in_queue(a.mode) && (!pb.q_t[0] || !pb.q_t[1] || !pb.q_r[0] || !pb.q_r[1])) return hipErrorInvalidValue;
     const bool mc = a.mode == MRT_MODE_MCPG;
     if (mc && (!mcpg || !mcpg->carry || mcpg->samples < 0 || mcpg->samples > MCPG_MAX_SAMPLES))
         return hipErrorInvalidValue;
@@ -1425,7 +1461,7 @@ hipError_t launch_pt_frame(const DeviceScene& s, const TraceScratch& ts, PtArgs 
         } else if (fused_primary) {
             mark(-1);
             hipLaunchKernelGGL(k_shade_primary, dim3(grid_for(a.npix, SHADE_BLOCK, MAX_BLOCKS)), dim3(SHADE_BLOCK), 0,
-                               st, s, a, gbuffer, pb, pb.q_o[1], pb.q_d[1], cnt + 1);
+                               st, s, a, gbuffer, pb, pb.q_o[1], pb.q_d[1], pb.q_t[1], pb.q_r[1], cnt + 1);
             mark(1);
         } else {
             hipLaunchKernelGGL(k_pt_init, dim3(grid_for(a.n_paths, 256, MAX_BLOCKS)), dim3(256), 0, st, s, a, gbuffer,
@@ -1460,16 +1496,16 @@ hipError_t launch_pt_frame(const DeviceScene& s, const TraceScratch& ts, PtArgs 
                                    pb.q_d[qi], cnt + b, pb.q_o[qi ^ 1], pb.q_d[qi ^ 1], cnt + b + 1);
             } else if (nee) {
                 hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<MRT_MODE_NEE>), sg, dim3(SHADE_BLOCK), 0, st, s, a, b, pb,
-                                   pb.q_o[qi], pb.q_d[qi], cnt + b, pb.q_o[qi ^ 1], pb.q_d[qi ^ 1], cnt + b + 1,
-                                   scnt + b);
+                                   pb.q_o[qi], pb.q_d[qi], pb.q_t[qi], pb.q_r[qi], cnt + b, pb.q_o[qi ^ 1],
+                                   pb.q_d[qi ^ 1], pb.q_t[qi ^ 1], pb.q_r[qi ^ 1], cnt + b + 1, scnt + b);
             } else if (guided) {
                 hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<MRT_MODE_GUIDED>), sg, dim3(SHADE_BLOCK), 0, st, s, a, b, pb,
-                                   pb.q_o[qi], pb.q_d[qi], cnt + b, pb.q_o[qi ^ 1], pb.q_d[qi ^ 1], cnt + b + 1,
-                                   scnt + b);
+                                   pb.q_o[qi], pb.q_d[qi], pb.q_t[qi], pb.q_r[qi], cnt + b, pb.q_o[qi ^ 1],
+                                   pb.q_d[qi ^ 1], pb.q_t[qi ^ 1], pb.q_r[qi ^ 1], cnt + b + 1, scnt + b);
             } else {
                 hipLaunchKernelGGL(HIP_KERNEL_NAME(k_shade<MRT_MODE_BSDF>), sg, dim3(SHADE_BLOCK), 0, st, s, a, b, pb,
-                                   pb.q_o[qi], pb.q_d[qi], cnt + b, pb.q_o[qi ^ 1], pb.q_d[qi ^ 1], cnt + b + 1,
-                                   scnt + b);
+                                   pb.q_o[qi], pb.q_d[qi], pb.q_t[qi], pb.q_r[qi], cnt + b, pb.q_o[qi ^ 1],
+                                   pb.q_d[qi ^ 1], pb.q_t[qi ^ 1], pb.q_r[qi ^ 1], cnt + b + 1, scnt + b);
             }
             mark(1);
             if (nee && b + 1 < L && err == hipSuccess) {

@@ -166,6 +166,8 @@ hipError_t carve_job(Slot& sl, const Job& j, const JobShape& sh) {
     for (int k = 0; k < 2; k++) {
         b.pb.q_o[k] = static_cast<float4*>(A(16 * P));
         b.pb.q_d[k] = static_cast<float4*>(A(16 * P));
+        b.pb.q_t[k] = static_cast<float4*>(A(16 * P));
+        b.pb.q_r[k] = static_cast<float4*>(A(16 * P));
     }
     b.pb.hits = static_cast<float4*>(A(16 * P));
     b.pb.aux = static_cast<float*>(A(4 * P));

@@ -184,7 +184,7 @@ inline JobLayout job_layout(const JobShape& s) {
     b += 2 * round256(16 * s.npix);               // gbuffer, irradiance
     b += round256(32 * l.paths);                  // path state
     b += round256(4 * l.paths);                   // signature
-    b += 4 * round256(16 * l.paths);              // two ray queues (origin, direction)
+    b += 8 * round256(16 * l.paths);              // two ray queues (origin, direction, path state)
     b += round256(16 * l.paths);                  // hits
     b += round256(4 * l.paths);                   // aux (pdf of the last direction)
     if (s.nee) b += 3 * round256(16 * l.paths);   // shadow queue
