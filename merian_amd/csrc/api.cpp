@@ -76,7 +76,7 @@ hipStream_t pick_stream(mrt_ctx* c, void* stream) {
 void free_path_buffers(mrt_ctx* c) {
     PathBuffers& p = c->paths;
     for (void* q : {(void*)p.state, (void*)p.sig, (void*)p.q_o[0], (void*)p.q_o[1], (void*)p.q_d[0],
-                    (void*)p.q_d[1], (void*)p.q_t[0], (void*)p.q_t[1], (void*)p.q_r[0], (void*)p.q_r[1], (void*)p.hits, (void*)p.aux, (void*)p.sq_o, (void*)p.sq_d, (void*)p.sq_c,
+                    (void*)p.q_d[1], (void*)p.q_t[0], (void*)p.q_t[1], (void*)p.q_r[0], (void*)p.q_r[1], (void*)p.hits, (void*)p.sq_o, (void*)p.sq_d, (void*)p.sq_c,
                     (void*)p.counters, (void*)p.accum, (void*)p.demod, (void*)p.sig_accum})
         dev_free(c, q);
     p = PathBuffers{};
@@ -103,7 +103,6 @@ mrt_status ensure_path_buffers(mrt_ctx* c, uint64_t paths, uint64_t npix, uint64
         p.q_r[k] = static_cast<float4*>(A(16 * paths));
     }
     p.hits = static_cast<float4*>(A(16 * paths));
-    p.aux = static_cast<float*>(A(4 * paths));
     p.sq_o = static_cast<float4*>(A(16 * paths));
     p.sq_d = static_cast<float4*>(A(16 * paths));
     p.sq_c = static_cast<float4*>(A(16 * paths));
@@ -703,7 +702,9 @@ mrt_status render_frame_dev(mrt_ctx* c, mrt_scene* s, const mrt_pt_params* prm, 
     const uint64_t spp = uint64_t(prm->spp);
     const uint64_t max_paths = s->tuning.max_paths ? s->tuning.max_paths : c->max_paths;
     uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(spp ? spp : 1, max_paths / npix));
-    if (npix * chunk > 0xFFFFFFFFull) chunk = 0xFFFFFFFFull / npix;
+    const uint64_t path_limit = prm->mode == MRT_MODE_NEE && !mcpg ? NEE_MAX_PATHS : 0xFFFFFFFFull;
+    if (npix * chunk > path_limit) chunk = path_limit / npix;
+    if (chunk == 0) return fail(c, MRT_ERR_UNSUPPORTED, "image too large for one sample per pass");
     if (mcpg) chunk = 1; // the samples of a pixel run one after another
     const uint64_t n_chunks = spp ? (spp + chunk - 1) / chunk : 1;
     st = ensure_path_buffers(c, npix * chunk, npix, n_chunks * uint64_t(prm->max_path_length + 1));

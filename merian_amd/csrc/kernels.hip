@@ -140,19 +140,25 @@ struct SrcShadow { // NEE shadow queue: (origin, path id), (direction, -); [0, 1
         tmax = SCENE_RAY_TMAX;
     }
 };
+// Where a shadow ray's contribution goes (the w of its origin): the radiance of the path's
+// continuation ray in the next queue (its slot), or, for a path that ended at the vertex, the
+// path's own record (SHADOW_TO_PATH | path index).
+constexpr uint32_t SHADOW_TO_PATH = 0x80000000u;
 struct SinkShadow { // unoccluded: add the queued contribution to the path's radiance
     const float4* so;
     const float4* sc;
     float4* state; // PathBuffers.state
+    float4* qr;    // radiance column of the queue the shaded vertices' continuation rays went to
     __device__ __forceinline__ void store(uint32_t i, const HitRec& h) const {
         if (h.id != TRAVERSAL_DONE) return;
-        const uint32_t pid = __float_as_uint(so[i].w);
+        const uint32_t to = __float_as_uint(so[i].w);
         const float4 c = ld_stream(sc + i);
-        float4 b = state[2 * size_t(pid) + 1];
+        float4* const p = (to & SHADOW_TO_PATH) ? state + 2 * size_t(to & ~SHADOW_TO_PATH) + 1 : qr + to;
+        float4 b = *p;
         b.x = b.x + c.x;
         b.y = b.y + c.y;
         b.z = b.z + c.z;
-        state[2 * size_t(pid) + 1] = b;
+        *p = b;
     }
 };
 
@@ -517,7 +523,6 @@ __global__ void k_pt_init(DeviceScene S, PtArgs A, const mrt_packed_hit* __restr
         st_stream(st + k, make_float4(1.f, 1.f, 1.f, 1.f));
         st_stream(sr + k, make_float4(0.f, 0.f, 0.f, __uint_as_float(pcg4d16(x, y, A.frame, s))));
         if (A.record_signature) P.sig[i] = sig_step(0x9E3779B9u * (s + 1u), gid);
-        if (A.mode != MRT_MODE_BSDF) P.aux[i] = 0.f;
     }
 }
 
@@ -536,11 +541,13 @@ struct RayStage {
     uint32_t* meta; // [0] staged count, [1] reserved queue base, [2] count being flushed
 };
 
+// Returns the ray's index among the staged rays (the flush writes it to the reserved base plus
+// that index); meaningless for a lane that does not push.
 template <bool STATE = false>
-__device__ __forceinline__ void stage_push(const RayStage& st, bool push, float4 o, float4 d, float4 t = float4{},
-                                           float4 r = float4{}) {
+__device__ __forceinline__ uint32_t stage_push(const RayStage& st, bool push, float4 o, float4 d, float4 t = float4{},
+                                               float4 r = float4{}) {
     const uint64_t b = __ballot(push);
-    if (!b) return;
+    if (!b) return 0u;
     const uint32_t lane = lane_id();
     const uint32_t leader = uint32_t(__ffsll((unsigned long long)b) - 1);
     uint32_t slot = 0;
@@ -554,6 +561,7 @@ __device__ __forceinline__ void stage_push(const RayStage& st, bool push, float4
             st.r[slot] = r;
         }
     }
+    return slot;
 }
 
 // Block-wide append without staging: slots within the block from an LDS counter, the block's
@@ -577,9 +585,10 @@ __device__ __forceinline__ uint32_t block_append(bool push, uint32_t* __restrict
 }
 
 // Every thread of the block calls this; at most SHADE_TILE rays may be staged between calls.
-// nt, nr: the queue's path-state arrays (STATE stages only).
+// nt, nr: the queue's path-state arrays (STATE stages only). Returns the queue index of the first
+// flushed ray.
 template <bool STATE = false>
-__device__ __forceinline__ void stage_flush(const RayStage& st, float4* __restrict__ no, float4* __restrict__ nd,
+__device__ __forceinline__ uint32_t stage_flush(const RayStage& st, float4* __restrict__ no, float4* __restrict__ nd,
                                             uint32_t* __restrict__ count_out, float4* __restrict__ nt = nullptr,
                                             float4* __restrict__ nr = nullptr) {
     __syncthreads();
@@ -600,6 +609,7 @@ __device__ __forceinline__ void stage_flush(const RayStage& st, float4* __restri
         }
     }
     __syncthreads();
+    return base;
 }
 
 // Groups the tile's paths [tile, tile + SHADE_TILE) by shading class (counting sort in LDS,
@@ -728,6 +738,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(MOD
             const uint32_t i = sort ? s_order[slot] : tile + slot;
             bool push = false, shadow = false, record = false;
             f3 next_o{0, 0, 0}, next_d{0, 0, 0};
+            float next_pdf = 0.f; // NEE: rides in the continuation ray's direction.w
             f3 rec_pos{0, 0, 0}, rec_rad{0, 0, 0}, rd_keep{0, 0, 0}, thr_keep{0, 0, 0};
             float rec_pdf = 0.f;
             f3 sh_o{0, 0, 0}, sh_d{0, 0, 0}, sh_c{0, 0, 0};
@@ -749,7 +760,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(MOD
                 f3 emission = sp.emission;
                 if (nee && bounce > 0 && sp.env) {
                     // the environment was also reachable by NEE from the previous vertex
-                    emission = emission * power_heuristic(P.aux[pid], env_pdf(S, rd));
+                    emission = emission * power_heuristic(d4.w, env_pdf(S, rd));
                 }
                 if (bounce == 0) {
                     rad = A.emission_on_primary ? thr * emission : f3{0.f, 0.f, 0.f};
@@ -837,7 +848,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(MOD
                     } else {
                         float pdf = 0.f;
                         push = bsdf_continue(sp, wi_local, ng, wi_below, rng, thr, ior, next_o, next_d, pdf);
-                        if (nee && push) P.aux[pid] = pdf;
+                        if (nee && push) next_pdf = pdf;
                     }
                 }
                 out_t = make_float4(thr.x, thr.y, thr.z, ior);
@@ -845,8 +856,9 @@ __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(MOD
                 if (!queued) st_stream(P.state + 2 * size_t(pid), out_t);
                 if (!queued || !push) st_stream(P.state + 2 * size_t(pid) + 1, out_r);
             }
-            stage_push<queued>(stage, push, make_float4(next_o.x, next_o.y, next_o.z, __uint_as_float(pid)),
-                               make_float4(next_d.x, next_d.y, next_d.z, 0.f), out_t, out_r);
+            const uint32_t staged =
+                stage_push<queued>(stage, push, make_float4(next_o.x, next_o.y, next_o.z, __uint_as_float(pid)),
+                                   make_float4(next_d.x, next_d.y, next_d.z, next_pdf), out_t, out_r);
             if (guided) {
                 const uint32_t r = block_append(record, P.grec_count, s_app);
                 if (record && r < P.grec_cap) {
@@ -859,15 +871,20 @@ __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(MOD
                 }
             }
             if (nee) {
+                // this round's continuation rays are flushed first, so that a shadow ray can name
+                // the queue slot its path went to
+                const uint32_t base = stage_flush<queued>(stage, no, nd, count_out, nt, nr);
+                const uint32_t to = push ? base + staged : (SHADOW_TO_PATH | pid);
                 const uint32_t q = block_append(shadow, shadow_count, s_app);
                 if (shadow) {
-                    st_stream(P.sq_o + q, make_float4(sh_o.x, sh_o.y, sh_o.z, __uint_as_float(pid)));
+                    st_stream(P.sq_o + q, make_float4(sh_o.x, sh_o.y, sh_o.z, __uint_as_float(to)));
                     st_stream(P.sq_d + q, make_float4(sh_d.x, sh_d.y, sh_d.z, 0.f));
                     st_stream(P.sq_c + q, make_float4(sh_c.x, sh_c.y, sh_c.z, 0.f));
                 }
             }
         }
-        stage_flush<queued>(stage, no, nd, count_out, nt, nr); // also keeps s_order intact until every round read it
+        // (NEE flushed every round) the flush also keeps s_order intact until every round read it
+        if (!nee) stage_flush<queued>(stage, no, nd, count_out, nt, nr);
     }
 }
 
@@ -1418,6 +1435,7 @@ hipError_t launch_pt_frame(const DeviceScene& s, const TraceScratch& ts, PtArgs 
     const uint32_t chunk = mc ? 1u : (uint32_t(pb.capacity / a.npix) < spp ? uint32_t(pb.capacity / a.npix) : spp);
     if (mc && pb.capacity < a.npix) return hipErrorInvalidValue;
     if (chunk == 0) return hipErrorInvalidValue;
+    if (a.mode == MRT_MODE_NEE && uint64_t(a.npix) * chunk > NEE_MAX_PATHS) return hipErrorInvalidValue;
     const uint32_t n_chunks = (spp + chunk - 1) / chunk;
     const uint32_t per_chunk = uint32_t(L + 1);
     // timing: one event before and after every launch, read once the frame has completed
@@ -1511,7 +1529,7 @@ hipError_t launch_pt_frame(const DeviceScene& s, const TraceScratch& ts, PtArgs 
             if (nee && b + 1 < L && err == hipSuccess) {
                 mark(-1);
                 err = launch_trace<false, SrcShadow, SinkShadow, true>(
-                    s, ts, SrcShadow{pb.sq_o, pb.sq_d}, SinkShadow{pb.sq_o, pb.sq_c, pb.state}, scnt + b, a.n_paths,
+                    s, ts, SrcShadow{pb.sq_o, pb.sq_d}, SinkShadow{pb.sq_o, pb.sq_c, pb.state, pb.q_r[qi ^ 1]}, scnt + b, a.n_paths,
                     swork + b, a.instance_mask, nullptr, st);
                 mark(0);
             }

@@ -97,15 +97,15 @@ struct PtArgs {
 struct PathBuffers {
     float4* state;     // per path 32 B: [2 pid] = throughput.xyz, current ior; [2 pid + 1] = radiance.xyz,
                        // PCG state bits. Modes whose state travels with the queue (state_in_queue) keep
-                       // only [2 pid + 1] here, written once when the path ends.
+                       // only [2 pid + 1] here: the radiance of a path that has ended.
     uint32_t* sig;     // hit-sequence hash per path
     float4* q_o[2];    // ray queue: origin.xyz, path index bits
-    float4* q_d[2];    // ray queue: direction.xyz
+    float4* q_d[2];    // ray queue: direction.xyz, NEE: pdf of the sampled direction (MIS weight of an
+                       // escaping ray)
     float4* q_t[2];    // state of the queued path (state_in_queue modes): throughput.xyz, current ior
     float4* q_r[2];    //   radiance.xyz, PCG state bits
     float4* hits;      // t, u, v, triangle id bits; indexed like the queue being shaded
     uint32_t* counters;// per chunk and bounce: queue sizes, then the trace kernels' ray cursors
-    float* aux;        // pdf of the last direction sampled (MIS weight of an escaping ray)
     float4* sq_o;      // shadow queue: origin.xyz, path index bits
     float4* sq_d;      // shadow queue: direction.xyz
     float4* sq_c;      // shadow queue: unoccluded contribution.rgb
@@ -122,9 +122,11 @@ struct PathBuffers {
 };
 
 // Modes whose per-path state rides in the ray queue beside the ray (q_t, q_r), so the shading
-// kernels read and write it in queue order instead of gathering it by path index. NEE keeps it
-// per path: its shadow rays add to a path's radiance after the path has been requeued.
-__host__ __device__ constexpr bool state_in_queue(int mode) { return mode == MRT_MODE_BSDF || mode == MRT_MODE_GUIDED; }
+// kernels read and write it in queue order instead of gathering it by path index. MCPG keeps it
+// per path: its passes continue a pixel's RNG stream from one sample to the next.
+__host__ __device__ constexpr bool state_in_queue(int mode) { return mode != MRT_MODE_MCPG; }
+// Paths per pass in NEE mode: a shadow ray names its path's queue slot or path index in 31 bits.
+constexpr uint64_t NEE_MAX_PATHS = 0x7FFFFFFFull;
 
 struct GuideNet;
 

@@ -115,7 +115,7 @@ JobShape shape_of(const mrt_server* s, const mrt_job_desc& d) {
     sh.L = uint32_t(d.params.max_path_length);
     sh.nee = d.params.mode == MRT_MODE_NEE;
     sh.demod = d.params.demodulate_albedo != 0;
-    sh.max_paths = s->cfg.max_paths_per_job;
+    sh.max_paths = sh.nee ? std::min<uint64_t>(s->cfg.max_paths_per_job, NEE_MAX_PATHS) : s->cfg.max_paths_per_job;
     sh.trace_threads = s->trace_threads;
     const uint32_t need = d.scene->ds.stack_need;
     sh.stack_entries = need > uint32_t(TRACE_LDS_STACK) ? need - TRACE_LDS_STACK : 1u;
@@ -170,7 +170,6 @@ hipError_t carve_job(Slot& sl, const Job& j, const JobShape& sh) {
         b.pb.q_r[k] = static_cast<float4*>(A(16 * P));
     }
     b.pb.hits = static_cast<float4*>(A(16 * P));
-    b.pb.aux = static_cast<float*>(A(4 * P));
     if (sh.nee) {
         b.pb.sq_o = static_cast<float4*>(A(16 * P));
         b.pb.sq_d = static_cast<float4*>(A(16 * P));

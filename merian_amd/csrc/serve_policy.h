@@ -186,7 +186,6 @@ inline JobLayout job_layout(const JobShape& s) {
     b += round256(4 * l.paths);                   // signature
     b += 8 * round256(16 * l.paths);              // two ray queues (origin, direction, path state)
     b += round256(16 * l.paths);                  // hits
-    b += round256(4 * l.paths);                   // aux (pdf of the last direction)
     if (s.nee) b += 3 * round256(16 * l.paths);   // shadow queue
     b += round256(4 * l.counters);                // queue counters
     b += round256(16 * s.npix) + round256(4 * s.npix); // accumulation, signature sum
