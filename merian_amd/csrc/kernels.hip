@@ -700,7 +700,7 @@ __device__ __forceinline__ bool guide_eligible(const mrt_material& m) {
 // In the state_in_queue modes the path's state is read from the input queue's slot (qt, qr) and
 // written with the continuation ray to the output queue's (nt, nr); a path that ends writes its
 // radiance to P.state once. NEE reads and writes the state by path index.
-// BSDF mode fits 168 VGPRs without spills: 3 waves per SIMD instead of 2.
+// BSDF mode fits 168 VGPRs (one spilled, no measurable cost): 3 waves per SIMD instead of 2.
 template <int MODE>
 __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(MODE == MRT_MODE_BSDF ? 3 : 1))) void k_shade(DeviceScene S, PtArgs A, int bounce, PathBuffers P,
                                                        const float4* __restrict__ qo, const float4* __restrict__ qd,
