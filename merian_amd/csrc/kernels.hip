@@ -129,7 +129,7 @@ struct SinkGBuffer { // PackedHitInfo (gbuffer.slang:175-196); one pretransforme
     }
 };
 
-struct SrcShadow { // NEE shadow queue: (origin, path id), (direction, -); [0, 1e7] like a continuation ray
+struct SrcShadow { // NEE shadow queue: (origin, radiance target), (direction, -); [0, 1e7] like a continuation ray
     const float4* so;
     const float4* sd;
     __device__ __forceinline__ void load(uint32_t i, f3& o, f3& d, float& tmin, float& tmax) const {
@@ -697,9 +697,11 @@ __device__ __forceinline__ bool guide_eligible(const mrt_material& m) {
 // the guide's lobes with probability A.guide_fraction, else the BSDF, and weighs the direction
 // by f / (a q + (1 - a) p_bsdf) (one-sample MIS); sample-0 paths of the first chunk append a
 // training record per guided vertex.
-// In the state_in_queue modes the path's state is read from the input queue's slot (qt, qr) and
-// written with the continuation ray to the output queue's (nt, nr); a path that ends writes its
-// radiance to P.state once. NEE reads and writes the state by path index.
+// The path's state is read from the input queue's slot (qt, qr) and written with the
+// continuation ray to the output queue's (nt, nr); a path that ends writes its radiance to
+// P.state once (every mode this kernel runs keeps its state in the queue). A NEE shadow ray names
+// where its contribution goes: its path's slot in the output queue (each round's continuation
+// rays are flushed before the round's shadow rays are appended) or its path's record.
 // BSDF mode fits 168 VGPRs (one spilled, no measurable cost): 3 waves per SIMD instead of 2.
 template <int MODE>
 __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(MODE == MRT_MODE_BSDF ? 3 : 1))) void k_shade(DeviceScene S, PtArgs A, int bounce, PathBuffers P,
