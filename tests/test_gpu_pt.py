@@ -176,6 +176,33 @@ def test_frame_split_into_passes(ctx, small):
     _compare(split, sig2, ref, ref_sig)
 
 
+@pytest.mark.parametrize("mode", [abi.MODE_BSDF, abi.MODE_NEE])
+def test_wide_shading_tiles_match_narrow(ctx, small, mode):
+    # passes of 2^23 paths or more shade wider tiles (three paths per thread instead of two);
+    # nothing else depends on the pass size, so a frame run as one pass of 8.4 M paths equals
+    # the same frame in two passes of 4.2 M bit for bit
+    sc, ds, osc = small
+    W = H = 1024
+    cam = ctx.camera(sc.camera, 9)
+    p = pt_params(spp=8, max_path_length=4, mode=mode)
+    t = ds.tuning()
+    saved = t.max_paths
+    try:
+        t.max_paths = 8 * W * H
+        ds.set_tuning(t)
+        wide, sig1, _, st1 = ctx.render(ds, p, cam, 9, W, H)
+        t = ds.tuning()
+        t.max_paths = 4 * W * H
+        ds.set_tuning(t)
+        narrow, sig2, _, st2 = ctx.render(ds, p, cam, 9, W, H)
+    finally:
+        t = ds.tuning()
+        t.max_paths = saved
+        ds.set_tuning(t)
+    assert int(st1.continuation_segments) == int(st2.continuation_segments) > 0
+    assert wide.tobytes() == narrow.tobytes() and sig1.tobytes() == sig2.tobytes()
+
+
 @pytest.mark.parametrize("which", ["default", "side"])
 def test_frames_on_torch_streams(ctx, small, which):
     # frames enqueued on a torch stream (the default one, handle 0 = the HIP null stream, or a
