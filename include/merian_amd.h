@@ -203,6 +203,9 @@ typedef struct mrt_packed_hit {
  * Render parameters: the "Render (Path-traced)" properties
  * (src/merian-graph/nodes/render_pt/render_pt.cpp:147-187) plus build-side modes.
  * ---------------------------------------------------------------------------------- */
+/* A frame runs in passes of whole samples that fit the paths in flight (mrt_tuning.max_paths);
+ * an MRT_MODE_NEE pass holds at most 2^31 - 1 paths (its shadow rays name their path in 31
+ * bits), so an NEE image of 2^31 pixels or more is refused with MRT_ERR_UNSUPPORTED. */
 enum { MRT_MODE_BSDF = 0, MRT_MODE_NEE = 1, MRT_MODE_GUIDED = 2, MRT_MODE_MCPG = 3 };
 
 typedef struct mrt_pt_params {
