@@ -1068,10 +1068,10 @@ __device__ __forceinline__ void mcpg_complete(const McpgArgs& M, McpgCarry& c, f
 // vertex's cache and chain updates. A vertex that scatters picks a lobe by RIS over
 // M.samples chain lookups, samples it with probability p_guide (else the BSDF), weighs the
 // direction by the stochastic-MIS pdf and carries its update inputs to the next bounce.
-// Three waves per SIMD (168 VGPRs, 56 B of spills per lane) instead of the unconstrained two
-// (197 VGPRs): more gathers in flight take 11 % off the kernel on 1080p frames and add 3 % on
-// 480x270 ones, whose launches are too small to fill the device; four waves spill 240 B and lose.
-__global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void k_shade_mcpg(
+// Two waves per SIMD, without spills: with the path-state columns in the kernel's arguments, three
+// waves spill 15 registers and the 1080p MCPG frame takes 533 ms instead of 529 (an earlier
+// version, with fewer spills at three waves, measured the opposite: 11 % off the kernel).
+__global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(2))) void k_shade_mcpg(
                                                             DeviceScene S, PtArgs A, McpgArgs M, int bounce,
                                                             PathBuffers P, const float4* __restrict__ qo,
                                                             const float4* __restrict__ qd,
